@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused G1 step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 250] [--warmup 50] [--nenv 16384]
+
+Workload (synthetic, as the randomized-actuation configuration): 16,384 envs
+of the 29-DoF G1, dt = 2e-3, initial base tilt U(+-0.05) rad, base velocity
+U(+-0.5) m/s in x/y, joint qd ~ N(0, 0.3), PD targets stand + U(+-0.3) rad
+resampled every 10 steps (all target blocks are generated beforehand and are
+resident in HBM).  One launch of the step kernel advances all envs by the 10
+steps that share a target block (nsub = 10); the timed region is `--steps`
+physics steps after `--warmup` untimed ones.
+
+Prints one JSON line: throughput in env-steps/s, the roofline of the step
+kernel (algorithmic FP32 FLOP of the reference formulation, 41,491 per
+env-step, over the kernel's average launch time from HIP events on the launch
+stream), and the CPU baseline (the reference physics core compiled from the
+reference sources, float, OpenMP over the host cores) timed on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "env-steps/s @16k envs: FP32 ABA vs surrogate; max qacc rel err vs FP64"
+FLOP_PER_ENV_STEP = 41491          # reference formulation, stance (8 contacts)
+BYTES_PER_ENV_LAUNCH = 892         # SoA loads + stores of one launch (nsub independent)
+PEAK_FP32 = 157.3e12               # MI355X FP32 (vector and matrix), dense
+PUBLISHED_16K = 12.8e6             # reference README.md:38-42, RTX 3080 Laptop, 16,384 envs
+CTRL_PERIOD = 10
+
+
+def cpu_baseline(nenv_gpu):
+    """Reference physics core (src/g1_core.h, float, OpenMP one env per
+    iteration) on the host cores, on a bounded sample of the workload."""
+    from g1sim_amd import workloads as W
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    try:
+        fn = O.ref_bench().g1ref_bench_f32
+        kind = "reference"
+    except (OSError, FileNotFoundError):
+        fn = O.lib32().g1o_bench_f32
+        kind = "port"
+    res = {}
+    for threads, nenv, steps in ((cores, 16384, 50), (1, 2048, 25)):
+        qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=1)
+        ct = np.ascontiguousarray(cb[0])
+        t = fn(nenv, steps, 2e-3, O.fptr(qp), O.fptr(qv), O.fptr(an), O.fptr(ct), threads)
+        res[threads] = (nenv * steps / t, nenv, steps)
+    v_all, n_all, s_all = res[cores]
+    v_one, n_one, s_one = res[1]
+    return {"value": v_all, "unit": "env-steps/s", "cores": cores, "kind": kind,
+            "sample": f"{n_all} envs x {s_all} steps of the same workload (dt 2e-3, fixed targets) "
+                      f"on {cores} threads; 1-thread rate from {n_one} envs x {s_one} steps",
+            "value_1core": v_one}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=250)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nenv", type=int, default=16384)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the step shards envs with no exchange, so N GPUs run N "
+              "independent copies of this benchmark; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from g1sim_amd import _lib, workloads as W
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nenv = args.nenv
+    nblocks = (args.warmup + args.steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
+    qpos = torch.as_tensor(qp, device=dev)
+    qvel = torch.as_tensor(qv, device=dev)
+    anchor = torch.as_tensor(an, device=dev)
+    ctrl = torch.as_tensor(cb, device=dev)                 # all target blocks resident in HBM
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+
+    def launch(step0, nsteps):
+        """Launches covering physics steps [step0, step0 + nsteps)."""
+        s = step0
+        while s < step0 + nsteps:
+            n = min(CTRL_PERIOD - s % CTRL_PERIOD, step0 + nsteps - s)
+            rc = lib.g1_hip_step_ex(nenv, n, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                    ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None,
+                                    0, sptr)
+            if rc:
+                raise RuntimeError(_lib.last_error())
+            yield n
+            s += n
+
+    for _ in launch(0, args.warmup):
+        pass
+    torch.cuda.synchronize()
+
+    # timed region: wall clock between synchronisations, plus a HIP event pair
+    # around every launch on the launch stream for the kernel's own duration
+    starts, ends, subs = [], [], []
+    gen = launch(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        try:
+            n = next(gen)
+        except StopIteration:
+            break
+        e1.record(stream)
+        starts.append(e0); ends.append(e1); subs.append(n)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch_ms = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+    sub_arr = np.array(subs)
+    h = qpos[2].cpu().numpy()
+    ok = bool(np.isfinite(qpos.cpu().numpy()).all() and ((h > 0.2) & (h < 1.2)).mean() > 0.99)
+
+    env_steps = nenv * args.steps
+    value = env_steps / wall
+    full = sub_arr == CTRL_PERIOD
+    avg_ms = float(per_launch_ms[full].mean()) if full.any() else float(per_launch_ms.mean())
+    nsub_avg = CTRL_PERIOD if full.any() else float(sub_arr.mean())
+    flop_per_launch = FLOP_PER_ENV_STEP * nenv * nsub_avg
+    achieved = flop_per_launch / (avg_ms * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            t = json.load(f)
+        if t.get("nenv") == nenv and t.get("nsub") == CTRL_PERIOD:
+            traffic = t.get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "env-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_16K,
+        "vs_baseline_ref": "12.8e6 env-steps/s, reference fused CUDA step, 16,384 envs, RTX 3080 Laptop (README.md:38-42)",
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "randomized actuation: 16,384 envs, dt 2e-3, PD targets stand+U(+-0.3) "
+                               "resampled every 10 steps, FP32 ABA step",
+                   "nenv": nenv, "dt": 2e-3, "nsub_per_launch": CTRL_PERIOD, "path": "FP32 ABA"},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
+                     "kernel": "g1k::step_kernel<false,false>",
+                     "flop_per_launch": flop_per_launch, "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": BYTES_PER_ENV_LAUNCH * nenv},
+        "sanity_standing": ok,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(nenv)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

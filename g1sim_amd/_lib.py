@@ -1,0 +1,58 @@
+"""ctypes binding of libg1sim_amd.so (C-ABI in include/g1sim_amd.h).
+
+The library is built in-tree by g1sim_amd.build.  There is no fallback: if
+it is missing or cannot be loaded, load() raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libg1sim_amd.so")
+
+G1_FLAG_F64_SOLVE = 1
+G1_FLAG_F64_BASE = 2
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m g1sim_amd.build` "
+                           "(the HIP step engine has no CPU fallback)")
+    # torch's HIP runtime must be the one in the process when torch tensors and
+    # streams are passed; importing torch first makes the loader reuse it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    vp, i, u, f = C.c_void_p, C.c_int, C.c_uint, C.c_float
+    lib.g1_cuda_step.restype = None
+    lib.g1_cuda_step.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.g1_cuda_reset_done.restype = None
+    lib.g1_cuda_reset_done.argtypes = [i, vp, vp, vp, vp, u, f, f, vp]
+    lib.g1_hip_step_ex.restype = i
+    lib.g1_hip_step_ex.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
+    lib.g1_hip_last_error.restype = C.c_char_p
+    lib.g1_hip_last_error.argtypes = []
+    lib.g1_hip_version.restype = i
+    lib.g1_hip_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
+            "g1_hip_version"]
+
+
+def last_error():
+    return load().g1_hip_last_error().decode()
+
+
+def check(rc, what="g1_hip_step_ex"):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")

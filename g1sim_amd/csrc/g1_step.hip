@@ -1,0 +1,951 @@
+// g1_step.hip -- fused G1 physics step for MI355X (gfx950).
+//
+// One step = kinematics and body twists -> 8 foot-sphere spring-damper
+// contacts with stick-slip anchors -> PD servos with implicit damping and soft
+// joint limits -> Featherstone ABA (packed 21-float articulated inertias,
+// floating-base 6x6 Cholesky) -> semi-implicit Euler.  The physics is that of
+// src/g1_core.h:29-232; the data path is built for CDNA4:
+//
+// * Work split.  A workgroup holds 64 envs (lane = env) in 4 wave64s.  Each
+//   wave owns one limb chain of the tree: wave 0 the left leg (bodies 1-6)
+//   and contacts 0-3, wave 1 the right leg (7-12) and contacts 4-7, wave 2
+//   the left arm (16-22), wave 3 the right arm (23-29).  Chain code is
+//   specialised per wave (uniform branch on the wave index), so every model
+//   constant is an instruction literal and every hinge rotation is a planar
+//   rotation about a unit axis.  16,384 envs fill 1,024 waves, one per SIMD
+//   of the 256 CUs, where one lane per env would leave 3 of 4 SIMDs idle.
+// * One LDS exchange per substep.  Each chain reduces to a 27-float
+//   articulated inertia + bias force in its root's parent frame; the four
+//   contributions go through LDS (double-buffered, one barrier per substep)
+//   and every wave then runs the short pelvis/waist part (bodies 0, 13-15,
+//   the 6x6 base solve) with identical code, so no second barrier is needed.
+// * Registers only.  No per-env workspace in memory: the forward pass keeps
+//   7 floats per joint (U/D and u/D) plus the joint's cos/sin and 4-float
+//   velocity-product term.
+// * Gravity enters as a uniform acceleration offset of the base (the classic
+//   ABA identity: bias forces without gravity, a0 = a0' + [0; R0^T g]), so no
+//   body needs its world rotation except the two feet chains.
+// * HBM traffic is the state itself, coalesced SoA loads/stores
+//   ([dim][nenv], env fastest, as src/g1_kernel.cu:9-10), once per launch;
+//   nsub substeps run from registers.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdarg.h>
+
+#include <type_traits>
+
+#include "g1_model_kernel.h"
+#include "../../include/g1sim_amd.h"
+
+namespace g1k {
+
+#define G1K_DEV __device__ __forceinline__
+
+struct Cfg {
+  float dt;
+  float kn, dn, kt, dtt, inv_kt;
+  float limit_kp, limit_kd;
+  float gz;
+  float mu;
+};
+
+// src/g1_types.h:44-55 (dt is a runtime argument here)
+static Cfg default_cfg(float dt) {
+  Cfg c;
+  c.dt = dt;
+  c.kn = 2e4f; c.dn = 280.f; c.kt = 1e4f; c.dtt = 280.f; c.inv_kt = 1.0f / 1e4f;
+  c.limit_kp = 200.f; c.limit_kd = 5.f;
+  c.gz = (float)G1M_GRAVITY_Z;
+  c.mu = (float)G1M_CONTACT_MU;
+  return c;
+}
+
+constexpr float ANCHOR_FREE = 1e30f;     // src/g1_core.h:108
+
+// ---------------------------------------------------------------- rotations
+// Planar rotation about axis K: plane (I, J) = ((K+1)%3, (K+2)%3) and
+// R_K = identity on K, [[c, -s], [s, c]] on (I, J)  (src/spatial.h:63-70 for a
+// unit axis).
+template <int K> struct Plane { static constexpr int I = (K + 1) % 3, J = (K + 2) % 3; };
+
+template <int K> G1K_DEV void rot(float* v, float c, float s) {          // v <- R_K v
+  constexpr int I = Plane<K>::I, J = Plane<K>::J;
+  const float a = v[I], b = v[J];
+  v[I] = c * a - s * b;
+  v[J] = s * a + c * b;
+}
+template <int K> G1K_DEV void rotT(float* v, float c, float s) {         // v <- R_K^T v
+  constexpr int I = Plane<K>::I, J = Plane<K>::J;
+  const float a = v[I], b = v[J];
+  v[I] = c * a + s * b;
+  v[J] = c * b - s * a;
+}
+// constant rotation of body B's tree frame
+template <int B> G1K_DEV void rt_mul(float* v) {                          // v <- RT v
+  constexpr const double* R = kM.b[B].RT;
+  const float x = v[0], y = v[1], z = v[2];
+  v[0] = (float)R[0] * x + (float)R[1] * y + (float)R[2] * z;
+  v[1] = (float)R[3] * x + (float)R[4] * y + (float)R[5] * z;
+  v[2] = (float)R[6] * x + (float)R[7] * y + (float)R[8] * z;
+}
+template <int B> G1K_DEV void rt_mulT(float* v) {                         // v <- RT^T v
+  constexpr const double* R = kM.b[B].RT;
+  const float x = v[0], y = v[1], z = v[2];
+  v[0] = (float)R[0] * x + (float)R[3] * y + (float)R[6] * z;
+  v[1] = (float)R[1] * x + (float)R[4] * y + (float)R[7] * z;
+  v[2] = (float)R[2] * x + (float)R[5] * y + (float)R[8] * z;
+}
+
+// ------------------------------------------------------- spatial transforms
+// X_B = (R, t) with R = RT_B R_K(q): pose of body B in its parent.
+// Motion vector parent -> child (src/spatial.h:102-105).
+template <int B> G1K_DEV void motion_to_child(const float* m, float c, float s, float* out) {
+  constexpr int K = kM.b[B].axis;
+  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
+  float a[3] = {m[0], m[1], m[2]};
+  float l[3] = {m[3] - (t1 * m[2] - t2 * m[1]), m[4] - (t2 * m[0] - t0 * m[2]),
+                m[5] - (t0 * m[1] - t1 * m[0])};
+  if constexpr (!kM.b[B].rt_ident) { rt_mulT<B>(a); rt_mulT<B>(l); }
+  rotT<K>(a, c, s);
+  rotT<K>(l, c, s);
+  out[0] = a[0]; out[1] = a[1]; out[2] = a[2];
+  out[3] = l[0]; out[4] = l[1]; out[5] = l[2];
+}
+// Force vector child -> parent, accumulated (src/spatial.h:107-110).
+template <int B> G1K_DEV void force_to_parent_add(const float* f, float c, float s, float* acc) {
+  constexpr int K = kM.b[B].axis;
+  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
+  float a[3] = {f[0], f[1], f[2]}, l[3] = {f[3], f[4], f[5]};
+  rot<K>(a, c, s);
+  rot<K>(l, c, s);
+  if constexpr (!kM.b[B].rt_ident) { rt_mul<B>(a); rt_mul<B>(l); }
+  acc[0] += a[0] + (t1 * l[2] - t2 * l[1]);
+  acc[1] += a[1] + (t2 * l[0] - t0 * l[2]);
+  acc[2] += a[2] + (t0 * l[1] - t1 * l[0]);
+  acc[3] += l[0]; acc[4] += l[1]; acc[5] += l[2];
+}
+
+// Symmetric 3x3 S <- R_K S R_K^T, with c2 = cos 2q, s2 = sin 2q.
+template <int K> G1K_DEV void rot_sym(float (&S)[3][3], float c, float s, float c2, float s2) {
+  constexpr int I = Plane<K>::I, J = Plane<K>::J;
+  const float ski = S[K][I], skj = S[K][J], sii = S[I][I], sjj = S[J][J], sij = S[I][J];
+  const float nki = c * ski - s * skj, nkj = s * ski + c * skj;
+  const float h = 0.5f * (sii + sjj), d = sii - sjj;
+  const float e = 0.5f * d * c2 - sij * s2;
+  const float nij = 0.5f * d * s2 + sij * c2;
+  S[K][I] = S[I][K] = nki;
+  S[K][J] = S[J][K] = nkj;
+  S[I][I] = h + e;
+  S[J][J] = h - e;
+  S[I][J] = S[J][I] = nij;
+}
+// General 3x3 H <- R_K H R_K^T.
+template <int K> G1K_DEV void rot_gen(float (&H)[3][3], float c, float s) {
+  constexpr int I = Plane<K>::I, J = Plane<K>::J;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float a = H[I][m], b = H[J][m];
+    H[I][m] = c * a - s * b;
+    H[J][m] = s * a + c * b;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float a = H[m][I], b = H[m][J];
+    H[m][I] = c * a - s * b;
+    H[m][J] = s * a + c * b;
+  }
+}
+// A <- RT A RT^T for a constant tree rotation (sym: only the lower triangle is used).
+template <int B, bool SYM> G1K_DEV void rt_sim(float (&A)[3][3]) {
+  constexpr const double* R = kM.b[B].RT;
+  float T[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      T[r][c] = (float)R[3 * r] * A[0][c] + (float)R[3 * r + 1] * A[1][c] + (float)R[3 * r + 2] * A[2][c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (SYM && c > r) continue;
+      A[r][c] = T[r][0] * (float)R[3 * c] + T[r][1] * (float)R[3 * c + 1] + T[r][2] * (float)R[3 * c + 2];
+    }
+  if (SYM) {
+    A[0][1] = A[1][0]; A[0][2] = A[2][0]; A[1][2] = A[2][1];
+  }
+}
+
+// Articulated inertia child -> parent, accumulated: P += X* Ia X^-1
+// (src/spatial.h:219-251): rotate the J/H/M blocks, then shift by t:
+// H_A = H + t^ M, J_A = J + t^ H^T - H_A t^, M_A = M.
+template <int B>
+G1K_DEV void inertia_to_parent_add(const float (&A)[21], float c, float s, float (&P)[21]) {
+  constexpr int K = kM.b[B].axis;
+  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
+  float Jb[3][3], H[3][3], M[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Jb[r][q] = A[pk(r, q)];
+      H[r][q] = A[pk(r, 3 + q)];
+      M[r][q] = A[pk(3 + r, 3 + q)];
+    }
+  const float c2 = c * c - s * s, s2 = 2.0f * c * s;
+  rot_sym<K>(Jb, c, s, c2, s2);
+  rot_sym<K>(M, c, s, c2, s2);
+  rot_gen<K>(H, c, s);
+  if constexpr (!kM.b[B].rt_ident) {
+    rt_sim<B, true>(Jb);
+    rt_sim<B, true>(M);
+    rt_sim<B, false>(H);
+  }
+  const float tx[3][3] = {{0.f, -t2, t1}, {t2, 0.f, -t0}, {-t1, t0, 0.f}};
+  float HA[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      HA[r][q] = H[r][q] + (tx[r][0] * M[0][q] + tx[r][1] * M[1][q] + tx[r][2] * M[2][q]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q <= r; ++q) {
+      const float tht = tx[r][0] * H[q][0] + tx[r][1] * H[q][1] + tx[r][2] * H[q][2];
+      const float hat = HA[r][0] * tx[0][q] + HA[r][1] * tx[1][q] + HA[r][2] * tx[2][q];
+      P[pk(r, q)] += Jb[r][q] + tht - hat;
+      P[pk(3 + r, 3 + q)] += M[r][q];
+    }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) P[pk(r, 3 + q)] += HA[r][q];
+}
+
+// bias = v x* (I v) with body B's constant rigid-body inertia
+// (src/g1_core.h:119 without its gravity term).
+template <int B> G1K_DEV void bias_force(const float* v, float* f) {
+  float h[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc += (float)kM.b[B].I[pk(r, m)] * v[m];
+    h[r] = acc;
+  }
+  // [w x h_a + v_l x h_l ; w x h_l]   (src/spatial.h:116-118)
+  f[0] = (v[1] * h[2] - v[2] * h[1]) + (v[4] * h[5] - v[5] * h[4]);
+  f[1] = (v[2] * h[0] - v[0] * h[2]) + (v[5] * h[3] - v[3] * h[5]);
+  f[2] = (v[0] * h[1] - v[1] * h[0]) + (v[3] * h[4] - v[4] * h[3]);
+  f[3] = v[1] * h[5] - v[2] * h[4];
+  f[4] = v[2] * h[3] - v[0] * h[5];
+  f[5] = v[0] * h[4] - v[1] * h[3];
+}
+
+template <int B> G1K_DEV void init_inertia(float (&A)[21]) {
+#pragma unroll
+  for (int m = 0; m < 21; ++m) A[m] = (float)kM.b[B].I[m];
+}
+
+// ------------------------------------------------------------ per-joint data
+// Kept per hinge from kinematics to the forward pass.
+struct Joint {
+  float c, s;     // cos/sin of the joint angle
+  float cor[4];   // velocity-product acceleration v x (qd S), nonzero (I,J) entries: a_I, a_J, l_I, l_J
+  float bias[6];  // velocity-product bias force (consumed by the backward pass)
+  float W[6];     // U / D
+  float uD;       // u / D
+};
+
+// Kinematics of hinge body B: twist from the parent's (src/g1_core.h:39-53).
+template <int B> G1K_DEV void fk_joint(float q, float qd, const float* vpar, float* v, Joint& jt) {
+  constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+  sincosf(q, &jt.s, &jt.c);
+  motion_to_child<B>(vpar, jt.c, jt.s, v);
+  v[K] += qd;
+  // cross_motion(v, qd S) with S = [e_K; 0] (src/g1_core.h:180)
+  jt.cor[0] = qd * v[J];
+  jt.cor[1] = -qd * v[I];
+  jt.cor[2] = qd * v[3 + J];
+  jt.cor[3] = -qd * v[3 + I];
+  bias_force<B>(v, jt.bias);
+}
+
+// World pose of hinge body B from its parent's (src/g1_core.h:47-50).
+template <int B>
+G1K_DEV void world_pose(const float (&Rp)[9], const float (&pp)[3], float c, float s, float (&R)[9],
+                        float (&p)[3]) {
+  constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) p[r] = Rp[3 * r] * t0 + Rp[3 * r + 1] * t1 + Rp[3 * r + 2] * t2 + pp[r];
+  float T[9];
+  if constexpr (kM.b[B].rt_ident) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) T[m] = Rp[m];
+  } else {
+    constexpr const double* Q = kM.b[B].RT;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        T[3 * r + q] = Rp[3 * r] * (float)Q[q] + Rp[3 * r + 1] * (float)Q[3 + q] + Rp[3 * r + 2] * (float)Q[6 + q];
+  }
+  // columns I, J of T R_K
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float a = T[3 * r + I], b = T[3 * r + J];
+    R[3 * r + K] = T[3 * r + K];
+    R[3 * r + I] = c * a + s * b;
+    R[3 * r + J] = c * b - s * a;
+  }
+}
+
+// Backward step of hinge body B into its parent: PD servo + soft limits +
+// implicit damping (src/g1_core.h:156-186).  A/pA (the body's articulated
+// inertia and bias) are consumed.
+template <int B>
+G1K_DEV void backward_joint(const Cfg& cf, float (&A)[21], float (&pA)[6], float q, float qd, float ctl,
+                            Joint& jt, float (&Ppar)[21], float (&pApar)[6]) {
+  constexpr int h = B - 1, K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+  constexpr float kp = (float)g1m_act_kp[h], kv = (float)g1m_act_kv[h];
+  constexpr float lim = (float)g1m_act_frclim[h];
+  constexpr float lo = (float)g1m_jnt_range[2 * h], hi = (float)g1m_jnt_range[2 * h + 1];
+  constexpr float arm = (float)g1m_armature[h];
+  float t = kp * (ctl - q) - kv * qd;
+  t = fminf(fmaxf(t, -lim), lim);
+  float b = kv;
+  const bool below = q < lo, above = q > hi;
+  const float lt = cf.limit_kp * ((below ? lo : hi) - q) - cf.limit_kd * qd;
+  t += (below || above) ? lt : 0.f;
+  b += (below || above) ? cf.limit_kd : 0.f;
+
+  float U[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) U[m] = A[pk(m, K)];
+  const float D = U[K] + arm + cf.dt * b;
+  const float invD = 1.0f / D;
+  const float u = t - pA[K];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) jt.W[m] = U[m] * invD;
+  jt.uD = u * invD;
+  // Ia = IA - U U^T / D
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int q2 = 0; q2 <= r; ++q2) A[pk(r, q2)] -= U[r] * jt.W[q2];
+  // pa = pA + Ia c + (u/D) U, c nonzero at angular I,J and linear I,J
+  float pa[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+    pa[r] = pA[r] + A[pk(r, I)] * jt.cor[0] + A[pk(r, J)] * jt.cor[1] + A[pk(r, 3 + I)] * jt.cor[2] +
+            A[pk(r, 3 + J)] * jt.cor[3] + jt.uD * U[r];
+  inertia_to_parent_add<B>(A, jt.c, jt.s, Ppar);
+  force_to_parent_add<B>(pa, jt.c, jt.s, pApar);
+}
+
+// Forward step of hinge body B (src/g1_core.h:190-201): acceleration from the
+// parent's, qdd = u/D - (U/D) . ap.
+template <int B> G1K_DEV float forward_joint(const float* apar, const Joint& jt, float* a) {
+  constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+  motion_to_child<B>(apar, jt.c, jt.s, a);
+  a[I] += jt.cor[0];
+  a[J] += jt.cor[1];
+  a[3 + I] += jt.cor[2];
+  a[3 + J] += jt.cor[3];
+  float dot = 0.f;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) dot += jt.W[m] * a[m];
+  const float qdd = jt.uD - dot;
+  a[K] += qdd;
+  return qdd;
+}
+
+// ------------------------------------------------------------------ chains
+// Chain of N bodies FIRST..FIRST+N-1 (each the parent of the next).
+template <int FIRST, int N> struct ChainOps {
+  template <int i = 0> static G1K_DEV void fk(const float* q, const float* qd, const float* vroot, Joint* jt,
+                                              float (&vleaf)[6]) {
+    if constexpr (i < N) {
+      float v[6];
+      fk_joint<FIRST + i>(q[i], qd[i], vroot, v, jt[i]);
+      if constexpr (i + 1 < N) fk<i + 1>(q, qd, v, jt, vleaf);
+      else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) vleaf[m] = v[m];
+      }
+    }
+  }
+  // world poses along the chain (feet only); returns the leaf's
+  template <int i = 0> static G1K_DEV void world(const float (&Rp)[9], const float (&pp)[3], const Joint* jt,
+                                                 float (&Rl)[9], float (&pl)[3]) {
+    if constexpr (i < N) {
+      float R[9], p[3];
+      world_pose<FIRST + i>(Rp, pp, jt[i].c, jt[i].s, R, p);
+      if constexpr (i + 1 < N) world<i + 1>(R, p, jt, Rl, pl);
+      else {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) Rl[m] = R[m];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) pl[m] = p[m];
+      }
+    }
+  }
+  // backward from the leaf, whose articulated quantities start as (A, pA);
+  // the chain root's contribution lands in (Pout, pout) in the root's parent frame.
+  template <int i = N - 1> static G1K_DEV void backward(const Cfg& cf, float (&A)[21], float (&pA)[6],
+                                                        const float* q, const float* qd, const float* ctl,
+                                                        Joint* jt, float (&Pout)[21], float (&pout)[6]) {
+    if constexpr (i > 0) {
+      float Ap[21], pp[6];
+      init_inertia<FIRST + i - 1>(Ap);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) pp[m] = jt[i - 1].bias[m];
+      backward_joint<FIRST + i>(cf, A, pA, q[i], qd[i], ctl[i], jt[i], Ap, pp);
+      backward<i - 1>(cf, Ap, pp, q, qd, ctl, jt, Pout, pout);
+    } else {
+      backward_joint<FIRST>(cf, A, pA, q[0], qd[0], ctl[0], jt[0], Pout, pout);
+    }
+  }
+  template <int i = 0> static G1K_DEV void forward(const float* aroot, const Joint* jt, float* qdd, float (&aleaf)[6]) {
+    if constexpr (i < N) {
+      float a[6];
+      qdd[i] = forward_joint<FIRST + i>(aroot, jt[i], a);
+      if constexpr (i + 1 < N) forward<i + 1>(a, jt, qdd, aleaf);
+      else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) aleaf[m] = a[m];
+      }
+    }
+  }
+};
+
+// ----------------------------------------------------------------- contacts
+// Four foot spheres on body FOOT (contacts K0..K0+3) against the plane z = 0
+// (src/g1_core.h:122-153): normal spring-damper clamped to [0, 500] N,
+// tangential anchor spring-damper projected on the Coulomb cone, the anchor
+// dragged on slip and released on lift-off.  The wrench is summed in the
+// world frame about the body origin and rotated once.
+template <int K0>
+G1K_DEV void foot_contacts(const Cfg& cf, float mu, const float (&R)[9], const float (&p)[3], const float* v,
+                           float* anchor, float* fn_out, float* pA) {
+  float Fw[3] = {0.f, 0.f, 0.f}, Tw[3] = {0.f, 0.f, 0.f};
+  constexpr float radius = (float)G1M_CONTACT_RADIUS;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float r0 = (float)g1m_contact_pos[3 * (K0 + k)], r1 = (float)g1m_contact_pos[3 * (K0 + k) + 1],
+                r2 = (float)g1m_contact_pos[3 * (K0 + k) + 2];
+    const float rw[3] = {R[0] * r0 + R[1] * r1 + R[2] * r2, R[3] * r0 + R[4] * r1 + R[5] * r2,
+                         R[6] * r0 + R[7] * r1 + R[8] * r2};
+    const float pw[3] = {rw[0] + p[0], rw[1] + p[1], rw[2] + p[2]};
+    const float depth = radius - pw[2];
+    // point velocity in the body frame, then world: R (v_l + w x r)
+    const float vb[3] = {v[3] + (v[1] * r2 - v[2] * r1), v[4] + (v[2] * r0 - v[0] * r2),
+                         v[5] + (v[0] * r1 - v[1] * r0)};
+    const float vp[3] = {R[0] * vb[0] + R[1] * vb[1] + R[2] * vb[2], R[3] * vb[0] + R[4] * vb[1] + R[5] * vb[2],
+                         R[6] * vb[0] + R[7] * vb[1] + R[8] * vb[2]};
+    const bool active = depth > 0.f;
+    float fn = cf.kn * depth - cf.dn * vp[2];
+    fn = fminf(fmaxf(fn, 0.f), 500.f);
+    float ax = anchor[2 * k], ay = anchor[2 * k + 1];
+    const bool fresh = ax >= ANCHOR_FREE;
+    ax = fresh ? pw[0] : ax;
+    ay = fresh ? pw[1] : ay;
+    float ftx = -cf.kt * (pw[0] - ax) - cf.dtt * vp[0];
+    float fty = -cf.kt * (pw[1] - ay) - cf.dtt * vp[1];
+    const float ftn = sqrtf(ftx * ftx + fty * fty), fmx = mu * fn;
+    const bool slip = ftn > fmx;
+    const float sc = fmx / (ftn + 1e-12f);
+    ftx = slip ? ftx * sc : ftx;
+    fty = slip ? fty * sc : fty;
+    ax = slip ? pw[0] + ftx * cf.inv_kt : ax;
+    ay = slip ? pw[1] + fty * cf.inv_kt : ay;
+    fn = active ? fn : 0.f;
+    ftx = active ? ftx : 0.f;
+    fty = active ? fty : 0.f;
+    anchor[2 * k] = active ? ax : ANCHOR_FREE;
+    anchor[2 * k + 1] = active ? ay : ANCHOR_FREE;
+    fn_out[k] = fn;
+    Fw[0] += ftx; Fw[1] += fty; Fw[2] += fn;
+    Tw[0] += rw[1] * fn - rw[2] * fty;
+    Tw[1] += rw[2] * ftx - rw[0] * fn;
+    Tw[2] += rw[0] * fty - rw[1] * ftx;
+  }
+  // pA -= [R^T Tw; R^T Fw]
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    pA[m] -= R[m] * Tw[0] + R[3 + m] * Tw[1] + R[6 + m] * Tw[2];
+    pA[3 + m] -= R[m] * Fw[0] + R[3 + m] * Fw[1] + R[6 + m] * Fw[2];
+  }
+}
+
+// ------------------------------------------------------------ base solve
+// IA0 a = -pA0 by packed Cholesky (src/spatial.h:254-278).
+template <typename T> G1K_DEV void base_solve(const float (&A)[21], const float (&pA)[6], float (&a)[6]) {
+  T L[21], y[6];
+#pragma unroll
+  for (int m = 0; m < 21; ++m) L[m] = (T)A[m];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int k = 0; k < j; ++k) L[pk(j, j)] -= L[pk(j, k)] * L[pk(j, k)];
+    L[pk(j, j)] = sqrt(L[pk(j, j)]);
+    const T inv = (T)1 / L[pk(j, j)];
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+#pragma unroll
+      for (int k = 0; k < j; ++k) L[pk(i, j)] -= L[pk(i, k)] * L[pk(j, k)];
+      L[pk(i, j)] *= inv;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    T acc = -(T)pA[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) acc -= L[pk(i, k)] * y[k];
+    y[i] = acc / L[pk(i, i)];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    T acc = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) acc -= L[pk(k, i)] * y[k];
+    y[i] = acc / L[pk(i, i)];
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) a[m] = (float)y[m];
+}
+
+// ------------------------------------------------------------------ kernel
+constexpr int ENVS_PER_WG = 64;
+constexpr int XCH = 27;  // floats per chain contribution (21 + 6)
+
+struct StepArgs {
+  int nenv, nsub;
+  Cfg cfg;
+  float* qpos;
+  float* qvel;
+  float* anchor;
+  const float* ctrl;
+  float* foot_pos;
+  float* foot_vel;
+  float* contact_forces;
+  float* qacc;
+  const float* mu;
+  double* base64;
+};
+
+// Per-lane chain slots (legs use 6 of 7).
+constexpr int MAXJ = 7;
+
+template <int W> struct WaveChain;
+template <> struct WaveChain<0> { static constexpr int first = 1, n = 6, root = 0, foot = 6, k0 = 0; };
+template <> struct WaveChain<1> { static constexpr int first = 7, n = 6, root = 0, foot = 12, k0 = 4; };
+template <> struct WaveChain<2> { static constexpr int first = 16, n = 7, root = 15, foot = -1, k0 = -1; };
+template <> struct WaveChain<3> { static constexpr int first = 23, n = 7, root = 15, foot = -1, k0 = -1; };
+
+using Waist = ChainOps<13, 3>;
+
+// Chain part before the exchange: kinematics, contacts, backward pass.
+template <int W>
+G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const float* ctl, float mu,
+                       const float (&v0)[6], const float (&v15)[6], const float (&R0)[9], const float (&p0)[3],
+                       Joint* jt, float* anchor, float* fn, float (&footp)[3], float (&footv)[3],
+                       float (&Pout)[21], float (&pout)[6]) {
+  using WC = WaveChain<W>;
+  using Ch = ChainOps<WC::first, WC::n>;
+  float vleaf[6];
+  Ch::fk(q, qd, (WC::root == 0) ? v0 : v15, jt, vleaf);
+  float A[21], pA[6];
+  init_inertia<WC::first + WC::n - 1>(A);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) pA[m] = jt[WC::n - 1].bias[m];
+  if constexpr (WC::foot > 0) {
+    float Rf[9], pf[3];
+    Ch::world(R0, p0, jt, Rf, pf);
+    foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anchor, fn, pA);
+    // foot outputs (src/g1_kernel.cu:128-145): origin position, world linear velocity
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      footp[m] = pf[m];
+      footv[m] = Rf[3 * m] * vleaf[3] + Rf[3 * m + 1] * vleaf[4] + Rf[3 * m + 2] * vleaf[5];
+    }
+  }
+  Ch::backward(cf, A, pA, q, qd, ctl, jt, Pout, pout);
+}
+
+template <int W>
+G1K_DEV void chain_post(const float (&a0)[6], const float (&a15)[6], const Joint* jt, float* qdd) {
+  using WC = WaveChain<W>;
+  float aleaf[6];
+  ChainOps<WC::first, WC::n>::forward((WC::root == 0) ? a0 : a15, jt, qdd, aleaf);
+}
+
+template <bool F64_SOLVE, bool F64_BASE>
+__global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
+  __shared__ float xch[2][4][XCH][ENVS_PER_WG];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nenv = args.nenv;
+  const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
+  const bool valid = e_raw < nenv;
+  const int e = valid ? e_raw : nenv - 1;      // tail lanes mirror the last env, never store
+  const size_t N = (size_t)nenv;
+  const Cfg cf = args.cfg;
+
+  // ---- load (all waves: base + waist; own chain)
+  using BaseT = typename std::conditional<F64_BASE, double, float>::type;
+  BaseT bp[3], bq[4];
+  float vw[3], om[3];
+  if constexpr (F64_BASE) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) bp[m] = args.base64[m * N + e];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) bq[m] = args.base64[(3 + m) * N + e];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) bp[m] = args.qpos[m * N + e];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) bq[m] = args.qpos[(3 + m) * N + e];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    vw[m] = args.qvel[m * N + e];
+    om[m] = args.qvel[(3 + m) * N + e];
+  }
+  float wq[3], wqd[3], wct[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    wq[m] = args.qpos[(7 + 12 + m) * N + e];
+    wqd[m] = args.qvel[(6 + 12 + m) * N + e];
+    wct[m] = args.ctrl[(12 + m) * N + e];
+  }
+  // own chain: hinge h0 .. h0+n-1
+  int h0, nj;
+  switch (wave) {
+    case 0: h0 = 0; nj = 6; break;
+    case 1: h0 = 6; nj = 6; break;
+    case 2: h0 = 15; nj = 7; break;
+    default: h0 = 22; nj = 7; break;
+  }
+  float cq[MAXJ], cqd[MAXJ], cct[MAXJ];
+#pragma unroll
+  for (int m = 0; m < MAXJ; ++m) {
+    const int mm = m < nj ? m : nj - 1;
+    cq[m] = args.qpos[(7 + h0 + mm) * N + e];
+    cqd[m] = args.qvel[(6 + h0 + mm) * N + e];
+    cct[m] = args.ctrl[(h0 + mm) * N + e];
+  }
+  float anc[8], fn[4] = {0.f, 0.f, 0.f, 0.f}, footp[3] = {0.f, 0.f, 0.f}, footv[3] = {0.f, 0.f, 0.f};
+  float mu = cf.mu;
+  if (wave < 2) {
+    const int a0i = wave * 8;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) anc[m] = args.anchor[(a0i + m) * N + e];
+    if (args.mu) mu = args.mu[e];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) anc[m] = ANCHOR_FREE;
+  }
+
+  float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[MAXJ];
+#pragma unroll
+  for (int m = 0; m < MAXJ; ++m) qdd_c[m] = 0.f;
+  const float dt = cf.dt;
+  for (int sub = 0; sub < args.nsub; ++sub) {
+    const int buf = sub & 1;
+    // ---- pelvis kinematics (src/g1_core.h:30-37)
+    float R0[9], p0[3], v0[6];
+    {
+      const BaseT n2 = bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2] + bq[3] * bq[3];
+      const BaseT inv = (BaseT)1 / sqrt(n2);
+      const float w = (float)(bq[0] * inv), x = (float)(bq[1] * inv), y = (float)(bq[2] * inv),
+                  z = (float)(bq[3] * inv);
+      R0[0] = 1 - 2 * (y * y + z * z); R0[1] = 2 * (x * y - w * z); R0[2] = 2 * (x * z + w * y);
+      R0[3] = 2 * (x * y + w * z); R0[4] = 1 - 2 * (x * x + z * z); R0[5] = 2 * (y * z - w * x);
+      R0[6] = 2 * (x * z - w * y); R0[7] = 2 * (y * z + w * x); R0[8] = 1 - 2 * (x * x + y * y);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p0[m] = (float)bp[m];
+      v0[0] = om[0]; v0[1] = om[1]; v0[2] = om[2];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
+    }
+    // ---- waist kinematics (bodies 13-15), needed by the arms and the merge
+    Joint wj[3];
+    float v15[6];
+    Waist::fk(wq, wqd, v0, wj, v15);
+
+    // ---- own chain before the exchange
+    Joint cj[MAXJ];
+    float Pc[21], pc[6];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) Pc[m] = 0.f;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) pc[m] = 0.f;
+    switch (wave) {
+      case 0: chain_pre<0>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
+      case 1: chain_pre<1>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
+      case 2: chain_pre<2>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
+      default: chain_pre<3>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
+    }
+#pragma unroll
+    for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pc[m];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) xch[buf][wave][21 + m][lane] = pc[m];
+    __syncthreads();
+
+    // ---- merge: torso <- arms, waist backward, pelvis <- legs + waist
+    float A15[21], p15[6], A0[21], pA0[6];
+    init_inertia<15>(A15);
+    init_inertia<0>(A0);
+#pragma unroll
+    for (int m = 0; m < 21; ++m) {
+      A15[m] += xch[buf][2][m][lane] + xch[buf][3][m][lane];
+      A0[m] += xch[buf][0][m][lane] + xch[buf][1][m][lane];
+    }
+    bias_force<0>(v0, pA0);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      p15[m] = wj[2].bias[m] + xch[buf][2][21 + m][lane] + xch[buf][3][21 + m][lane];
+      pA0[m] += xch[buf][0][21 + m][lane] + xch[buf][1][21 + m][lane];
+    }
+    Waist::backward(cf, A15, p15, wq, wqd, wct, wj, A0, pA0);
+
+    // ---- base: IA0 a0' = -pA0 (gravity-free), forward through the waist
+    float a0[6];
+    base_solve<typename std::conditional<F64_SOLVE, double, float>::type>(A0, pA0, a0);
+    float a15[6];
+    Waist::forward(a0, wj, qdd_w, a15);
+    switch (wave) {
+      case 0: chain_post<0>(a0, a15, cj, qdd_c); break;
+      case 1: chain_post<1>(a0, a15, cj, qdd_c); break;
+      case 2: chain_post<2>(a0, a15, cj, qdd_c); break;
+      default: chain_post<3>(a0, a15, cj, qdd_c); break;
+    }
+    // base qacc in MuJoCo convention (src/g1_core.h:202-206), gravity restored:
+    // world linear = R0 (a0'_l + w x v_l) + g, angular = body frame a0'_a
+    {
+      const float cl[3] = {a0[3] + (v0[1] * v0[5] - v0[2] * v0[4]), a0[4] + (v0[2] * v0[3] - v0[0] * v0[5]),
+                           a0[5] + (v0[0] * v0[4] - v0[1] * v0[3])};
+#pragma unroll
+      for (int m = 0; m < 3; ++m) qacc_base[m] = R0[3 * m] * cl[0] + R0[3 * m + 1] * cl[1] + R0[3 * m + 2] * cl[2];
+      qacc_base[2] += cf.gz;
+      qacc_base[3] = a0[0]; qacc_base[4] = a0[1]; qacc_base[5] = a0[2];
+    }
+    // ---- semi-implicit Euler (src/g1_core.h:212-222)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      vw[m] += dt * qacc_base[m];
+      om[m] += dt * qacc_base[3 + m];
+      bp[m] += (BaseT)dt * (BaseT)vw[m];
+    }
+    {
+      // q <- q (x) exp(w dt / 2) (src/spatial.h:73-79)
+      const float angle = sqrtf(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]) * dt;
+      if (angle >= 1e-12f) {
+        float sh, ch;
+        sincosf(0.5f * angle, &sh, &ch);
+        const float s = sh / (angle / dt);
+        const BaseT dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
+        const BaseT nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
+        const BaseT nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
+        const BaseT ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
+        const BaseT nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
+        const BaseT inv = (BaseT)1 / sqrt(nw * nw + nx * nx + ny * ny + nz * nz);
+        bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      wqd[m] += dt * qdd_w[m];
+      wq[m] += dt * wqd[m];
+    }
+#pragma unroll
+    for (int m = 0; m < MAXJ; ++m) {
+      cqd[m] += dt * qdd_c[m];
+      cq[m] += dt * cqd[m];
+    }
+  }
+
+  // ---- store
+  if (!valid || args.nsub <= 0) return;
+  if (wave == 0) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) args.qpos[m * N + e] = (float)bp[m];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) args.qpos[(3 + m) * N + e] = (float)bq[m];
+    if constexpr (F64_BASE) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.base64[m * N + e] = (double)bp[m];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) args.base64[(3 + m) * N + e] = (double)bq[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      args.qvel[m * N + e] = vw[m];
+      args.qvel[(3 + m) * N + e] = om[m];
+      args.qpos[(7 + 12 + m) * N + e] = wq[m];
+      args.qvel[(6 + 12 + m) * N + e] = wqd[m];
+    }
+    if (args.qacc) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) args.qacc[m * N + e] = qacc_base[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.qacc[(6 + 12 + m) * N + e] = qdd_w[m];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MAXJ; ++m) {
+    if (m < nj) {
+      args.qpos[(7 + h0 + m) * N + e] = cq[m];
+      args.qvel[(6 + h0 + m) * N + e] = cqd[m];
+      if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[m];
+    }
+  }
+  if (wave < 2) {
+    const int a0i = wave * 8;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) args.anchor[(a0i + m) * N + e] = anc[m];
+    if (args.contact_forces) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) args.contact_forces[(wave * 4 + m) * N + e] = fn[m];
+    }
+    if (args.foot_pos) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.foot_pos[(wave * 3 + m) * N + e] = footp[m];
+    }
+    if (args.foot_vel) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.foot_vel[(wave * 3 + m) * N + e] = footv[m];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reset
+// src/g1_kernel.cu:54-62: integer hash and uniform draw.
+G1K_DEV unsigned hash32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352du;
+  x ^= x >> 15; x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// src/g1_kernel.cu:85-103, with every float operation rounded separately
+// (no contraction) so the result is defined by the source arithmetic.
+__global__ __launch_bounds__(256) void reset_done_kernel(int nenv, float* qpos, float* qvel, float* anchor,
+                                                         const uint8_t* done, unsigned seed, float noise,
+                                                         float drop) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nenv || !done[e]) return;
+  const size_t N = (size_t)nenv;
+  unsigned s = hash32(seed ^ (unsigned)(e + 1));
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    float v = (float)g1m_qpos_stand[i];
+    if (i == 2) v = __fadd_rn(v, drop);
+    if (i >= 7) {
+      s = hash32(s + 0x9E3779B9u);
+      const float u = __fmul_rn((float)s, 2.3283064365386963e-10f);
+      const float r = __fsub_rn(__fmul_rn(2.0f, u), 1.0f);
+      v = __fadd_rn(v, __fmul_rn(noise, r));
+    }
+    qpos[i * N + e] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) qvel[i * N + e] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 2 * NC; ++i) anchor[i * N + e] = ANCHOR_FREE;
+}
+
+}  // namespace g1k
+
+// ================================================================== C-ABI
+namespace {
+thread_local char g_err[512] = "";
+
+void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+using KernelFn = void (*)(g1k::StepArgs);
+
+int launch_step(const g1k::StepArgs& a, unsigned flags, hipStream_t stream) {
+  const bool f64s = flags & G1_FLAG_F64_SOLVE, f64b = flags & G1_FLAG_F64_BASE;
+  KernelFn k = f64b ? (f64s ? g1k::step_kernel<true, true> : g1k::step_kernel<false, true>)
+                    : (f64s ? g1k::step_kernel<true, false> : g1k::step_kernel<false, false>);
+  const int nblocks = (a.nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG;
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, stream, a);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_err("step launch failed: %s", hipGetErrorString(err));
+    return -2;
+  }
+  return 0;
+}
+
+int validate(int nenv, int nsub, float dt, const void* qpos, const void* qvel, const void* anchor,
+             const void* ctrl) {
+  if (nenv < 0 || nsub < 0) { set_err("nenv (%d) and nsub (%d) must be >= 0", nenv, nsub); return -1; }
+  if (!(dt > 0.f) || !isfinite(dt)) { set_err("dt must be finite and > 0 (got %g)", (double)dt); return -1; }
+  if (nenv > 0 && (!qpos || !qvel || !anchor || !ctrl)) {
+    set_err("qpos, qvel, anchor and ctrl are required");
+    return -1;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int g1_hip_version(void) { return 100; }
+
+const char* g1_hip_last_error(void) { return g_err; }
+
+int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                   float* foot_pos, float* foot_vel, float* contact_forces, float* qacc_out, const float* mu,
+                   double* base_pose64, unsigned flags, void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE)) { set_err("unknown flag bits 0x%x", flags); return -1; }
+  if (((flags & G1_FLAG_F64_BASE) != 0) != (base_pose64 != nullptr)) {
+    set_err("base_pose64 must be given exactly when G1_FLAG_F64_BASE is set");
+    return -1;
+  }
+  if (nenv == 0 || nsub == 0) return 0;
+  g1k::StepArgs a;
+  a.nenv = nenv; a.nsub = nsub;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.qacc = qacc_out; a.mu = mu; a.base64 = base_pose64;
+  return launch_step(a, flags, (hipStream_t)stream);
+}
+
+// src/g1_kernel.cu:154-163: dt = 1e-3 (g1_default_config), model friction.
+void g1_cuda_step(int nenv, int nsub, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                  float* foot_pos, float* foot_vel, float* contact_forces, void* stream) {
+  g1_hip_step_ex(nenv, nsub, 1e-3f, qpos, qvel, anchor, ctrl, foot_pos, foot_vel, contact_forces, nullptr,
+                 nullptr, nullptr, 0u, stream);
+}
+
+void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const uint8_t* done, unsigned seed,
+                        float noise, float drop, void* stream) {
+  g_err[0] = 0;
+  if (nenv <= 0) return;
+  if (!qpos || !qvel || !anchor || !done) { set_err("reset: qpos, qvel, anchor and done are required"); return; }
+  const int tpb = 256, nb = (nenv + tpb - 1) / tpb;
+  hipLaunchKernelGGL(g1k::reset_done_kernel, dim3(nb), dim3(tpb), 0, (hipStream_t)stream, nenv, qpos, qvel,
+                     anchor, done, seed, noise, drop);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) set_err("reset launch failed: %s", hipGetErrorString(err));
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+"""C-ABI of the product library and the oracle: they load and export every
+symbol their headers declare.  No compute call (there is no GPU here); the
+argument checks of g1_hip_step_ex return before any HIP call."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from g1sim_amd import _lib
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def declared(header):
+    txt = open(os.path.join(ROOT, header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_product_exports_every_declared_symbol():
+    names = declared("include/g1sim_amd.h")
+    assert {"g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex"} <= set(names)
+    lib = _lib.load()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(_lib.exported_symbols())
+    assert lib.g1_hip_version() >= 100
+
+
+def test_oracle_exports_every_declared_symbol():
+    from oracle import oracle as O
+    names = declared("oracle/g1_oracle.h")
+    for lib in (O.lib64(), O.lib32()):
+        for n in names:
+            assert hasattr(lib, n), n
+
+
+def test_reference_host_abi_names_kept():
+    """The oracle keeps the reference host C-ABI (src/host_lib.cpp:7-53)."""
+    names = declared("oracle/g1_oracle.h")
+    assert {"g1_sizes", "g1_c_fk", "g1_c_fd", "g1_c_step"} <= set(names)
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((-1, 1, 2e-3), "must be >= 0"),
+    ((4, -2, 2e-3), "must be >= 0"),
+    ((4, 1, 0.0), "dt must be"),
+    ((4, 1, float("nan")), "dt must be"),
+])
+def test_step_ex_rejects_bad_arguments(args, msg):
+    lib = _lib.load()
+    nenv, nsub, dt = args
+    dummy = C.c_void_p(16)
+    rc = lib.g1_hip_step_ex(nenv, nsub, dt, dummy, dummy, dummy, dummy, None, None, None, None, None,
+                            None, 0, None)
+    assert rc == -1
+    assert msg in _lib.last_error()
+
+
+def test_step_ex_requires_state_pointers_and_consistent_flags():
+    lib = _lib.load()
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, None, None, None, None, None, None, None, None, None, None, 0, None)
+    assert rc == -1 and "required" in _lib.last_error()
+    d = C.c_void_p(16)
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None,
+                            _lib.G1_FLAG_F64_BASE, None)
+    assert rc == -1 and "base_pose64" in _lib.last_error()
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None, 64, None)
+    assert rc == -1 and "flag" in _lib.last_error()
+
+
+def test_zero_envs_is_a_no_op():
+    lib = _lib.load()
+    assert lib.g1_hip_step_ex(0, 5, 2e-3, None, None, None, None, None, None, None, None, None,
+                              None, 0, None) == 0
