@@ -100,7 +100,7 @@ def main():
             n = min(CTRL_PERIOD - s % CTRL_PERIOD, step0 + nsteps - s)
             rc = lib.g1_hip_step_ex(nenv, n, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
                                     ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None,
-                                    0, sptr)
+                                    None, 0, sptr)
             if rc:
                 raise RuntimeError(_lib.last_error())
             yield n

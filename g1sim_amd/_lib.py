@@ -35,7 +35,7 @@ def load():
     lib.g1_cuda_reset_done.restype = None
     lib.g1_cuda_reset_done.argtypes = [i, vp, vp, vp, vp, u, f, f, vp]
     lib.g1_hip_step_ex.restype = i
-    lib.g1_hip_step_ex.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
+    lib.g1_hip_step_ex.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_last_error.restype = C.c_char_p
     lib.g1_hip_last_error.argtypes = []
     lib.g1_hip_version.restype = i

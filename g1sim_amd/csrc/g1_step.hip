@@ -128,6 +128,13 @@ template <int B> G1K_DEV void force_to_parent_add(const float* f, float c, float
   acc[3] += l[0]; acc[4] += l[1]; acc[5] += l[2];
 }
 
+template <int B> G1K_DEV void force_to_parent(float (&f)[6], float c, float s) {
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  force_to_parent_add<B>(f, c, s, acc);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) f[m] = acc[m];
+}
+
 // Symmetric 3x3 S <- R_K S R_K^T, with c2 = cos 2q, s2 = sin 2q.
 template <int K> G1K_DEV void rot_sym(float (&S)[3][3], float c, float s, float c2, float s2) {
   constexpr int I = Plane<K>::I, J = Plane<K>::J;
@@ -259,6 +266,7 @@ struct Joint {
   float bias[6];  // velocity-product bias force (consumed by the backward pass)
   float W[6];     // U / D
   float uD;       // u / D
+  float tau;      // servo + joint-limit torque (src/g1_core.h:166-174)
 };
 
 // Kinematics of hinge body B: twist from the parent's (src/g1_core.h:39-53).
@@ -323,6 +331,7 @@ G1K_DEV void backward_joint(const Cfg& cf, float (&A)[21], float (&pA)[6], float
   const float lt = cf.limit_kp * ((below ? lo : hi) - q) - cf.limit_kd * qd;
   t += (below || above) ? lt : 0.f;
   b += (below || above) ? cf.limit_kd : 0.f;
+  jt.tau = t;
 
   float U[6];
 #pragma unroll
@@ -411,6 +420,15 @@ template <int FIRST, int N> struct ChainOps {
       backward_joint<FIRST>(cf, A, pA, q[0], qd[0], ctl[0], jt[0], Pout, pout);
     }
   }
+  // generalized force of a body-frame wrench F on the leaf: tau_i = S_i^T F_i
+  // up the chain; F ends in the root's parent frame (J^T f of src/g1_core.h:146-148)
+  template <int i = N - 1> static G1K_DEV void wrench_to_joints(float (&F)[6], const Joint* jt, float* tc) {
+    if constexpr (i >= 0) {
+      tc[i] = F[kM.b[FIRST + i].axis];
+      force_to_parent<FIRST + i>(F, jt[i].c, jt[i].s);
+      wrench_to_joints<i - 1>(F, jt, tc);
+    }
+  }
   template <int i = 0> static G1K_DEV void forward(const float* aroot, const Joint* jt, float* qdd, float (&aleaf)[6]) {
     if constexpr (i < N) {
       float a[6];
@@ -432,7 +450,7 @@ template <int FIRST, int N> struct ChainOps {
 // world frame about the body origin and rotated once.
 template <int K0>
 G1K_DEV void foot_contacts(const Cfg& cf, float mu, const float (&R)[9], const float (&p)[3], const float* v,
-                           float* anchor, float* fn_out, float* pA) {
+                           float* anchor, float* fn_out, float (&Fb)[6]) {
   float Fw[3] = {0.f, 0.f, 0.f}, Tw[3] = {0.f, 0.f, 0.f};
   constexpr float radius = (float)G1M_CONTACT_RADIUS;
 #pragma unroll
@@ -475,11 +493,11 @@ G1K_DEV void foot_contacts(const Cfg& cf, float mu, const float (&R)[9], const f
     Tw[1] += rw[2] * ftx - rw[0] * fn;
     Tw[2] += rw[0] * fty - rw[1] * ftx;
   }
-  // pA -= [R^T Tw; R^T Fw]
+  // body-frame wrench [R^T Tw; R^T Fw]
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    pA[m] -= R[m] * Tw[0] + R[3 + m] * Tw[1] + R[6 + m] * Tw[2];
-    pA[3 + m] -= R[m] * Fw[0] + R[3 + m] * Fw[1] + R[6 + m] * Fw[2];
+    Fb[m] = R[m] * Tw[0] + R[3 + m] * Tw[1] + R[6 + m] * Tw[2];
+    Fb[3 + m] = R[m] * Fw[0] + R[3 + m] * Fw[1] + R[6 + m] * Fw[2];
   }
 }
 
@@ -535,6 +553,7 @@ struct StepArgs {
   float* foot_vel;
   float* contact_forces;
   float* qacc;
+  float* tau;
   const float* mu;
   double* base64;
 };
@@ -555,7 +574,7 @@ template <int W>
 G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const float* ctl, float mu,
                        const float (&v0)[6], const float (&v15)[6], const float (&R0)[9], const float (&p0)[3],
                        Joint* jt, float* anchor, float* fn, float (&footp)[3], float (&footv)[3],
-                       float (&Pout)[21], float (&pout)[6]) {
+                       float (&Pout)[21], float (&pout)[6], bool want_tau, float* tc, float (&Fpel)[6]) {
   using WC = WaveChain<W>;
   using Ch = ChainOps<WC::first, WC::n>;
   float vleaf[6];
@@ -564,10 +583,13 @@ G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const flo
   init_inertia<WC::first + WC::n - 1>(A);
 #pragma unroll
   for (int m = 0; m < 6; ++m) pA[m] = jt[WC::n - 1].bias[m];
+  float Fb[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if constexpr (WC::foot > 0) {
     float Rf[9], pf[3];
     Ch::world(R0, p0, jt, Rf, pf);
-    foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anchor, fn, pA);
+    foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anchor, fn, Fb);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) pA[m] -= Fb[m];
     // foot outputs (src/g1_kernel.cu:128-145): origin position, world linear velocity
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -576,6 +598,11 @@ G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const flo
     }
   }
   Ch::backward(cf, A, pA, q, qd, ctl, jt, Pout, pout);
+  if constexpr (WC::foot > 0) {
+    if (want_tau) Ch::wrench_to_joints(Fb, jt, tc);    // Fb ends in the pelvis frame
+#pragma unroll
+    for (int m = 0; m < 6; ++m) Fpel[m] = Fb[m];
+  }
 }
 
 template <int W>
@@ -588,6 +615,7 @@ G1K_DEV void chain_post(const float (&a0)[6], const float (&a15)[6], const Joint
 template <bool F64_SOLVE, bool F64_BASE>
 __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
   __shared__ float xch[2][4][XCH][ENVS_PER_WG];
+  __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nenv = args.nenv;
@@ -653,8 +681,11 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
   }
 
   float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[MAXJ];
+  // total generalized force of the last substep (servo + limits + J^T f)
+  const bool want_tau = args.tau != nullptr;
+  float tau_c[MAXJ], tau_w[3] = {0.f, 0.f, 0.f}, tau_b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int m = 0; m < MAXJ; ++m) qdd_c[m] = 0.f;
+  for (int m = 0; m < MAXJ; ++m) { qdd_c[m] = 0.f; tau_c[m] = 0.f; }
   const float dt = cf.dt;
   for (int sub = 0; sub < args.nsub; ++sub) {
     const int buf = sub & 1;
@@ -681,21 +712,27 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
 
     // ---- own chain before the exchange
     Joint cj[MAXJ];
-    float Pc[21], pc[6];
+    float Pc[21], pc[6], tc[MAXJ], Fpel[6];
+#pragma unroll
+    for (int m = 0; m < MAXJ; ++m) tc[m] = 0.f;
 #pragma unroll
     for (int m = 0; m < 21; ++m) Pc[m] = 0.f;
 #pragma unroll
     for (int m = 0; m < 6; ++m) pc[m] = 0.f;
     switch (wave) {
-      case 0: chain_pre<0>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
-      case 1: chain_pre<1>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
-      case 2: chain_pre<2>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
-      default: chain_pre<3>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc); break;
+      case 0: chain_pre<0>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 1: chain_pre<1>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 2: chain_pre<2>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      default: chain_pre<3>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
     }
 #pragma unroll
     for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pc[m];
 #pragma unroll
     for (int m = 0; m < 6; ++m) xch[buf][wave][21 + m][lane] = pc[m];
+    if (want_tau && wave < 2) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) fcx[buf][wave][m][lane] = Fpel[m];
+    }
     __syncthreads();
 
     // ---- merge: torso <- arms, waist backward, pelvis <- legs + waist
@@ -725,6 +762,21 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       case 1: chain_post<1>(a0, a15, cj, qdd_c); break;
       case 2: chain_post<2>(a0, a15, cj, qdd_c); break;
       default: chain_post<3>(a0, a15, cj, qdd_c); break;
+    }
+    if (want_tau) {
+#pragma unroll
+      for (int m = 0; m < MAXJ; ++m) tau_c[m] = (m < nj ? cj[m].tau : 0.f) + tc[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) tau_w[m] = wj[m].tau;
+      // base: world force R0 F.l, body torque F.a (convention of g1_c_fd's tau)
+      float F[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) F[m] = fcx[buf][0][m][lane] + fcx[buf][1][m][lane];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        tau_b[m] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
+        tau_b[3 + m] = F[m];
+      }
     }
     // base qacc in MuJoCo convention (src/g1_core.h:202-206), gravity restored:
     // world linear = R0 (a0'_l + w x v_l) + g, angular = body frame a0'_a
@@ -791,6 +843,12 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       args.qpos[(7 + 12 + m) * N + e] = wq[m];
       args.qvel[(6 + 12 + m) * N + e] = wqd[m];
     }
+    if (want_tau) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) args.tau[m * N + e] = tau_b[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.tau[(6 + 12 + m) * N + e] = tau_w[m];
+    }
     if (args.qacc) {
 #pragma unroll
       for (int m = 0; m < 6; ++m) args.qacc[m * N + e] = qacc_base[m];
@@ -804,6 +862,7 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       args.qpos[(7 + h0 + m) * N + e] = cq[m];
       args.qvel[(6 + h0 + m) * N + e] = cqd[m];
       if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[m];
+      if (want_tau) args.tau[(6 + h0 + m) * N + e] = tau_c[m];
     }
   }
   if (wave < 2) {
@@ -909,8 +968,8 @@ int g1_hip_version(void) { return 100; }
 const char* g1_hip_last_error(void) { return g_err; }
 
 int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
-                   float* foot_pos, float* foot_vel, float* contact_forces, float* qacc_out, const float* mu,
-                   double* base_pose64, unsigned flags, void* stream) {
+                   float* foot_pos, float* foot_vel, float* contact_forces, float* qacc_out, float* tau_out,
+                   const float* mu, double* base_pose64, unsigned flags, void* stream) {
   g_err[0] = 0;
   int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
   if (rc) return rc;
@@ -925,7 +984,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   a.cfg = g1k::default_cfg(dt);
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl;
   a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
-  a.qacc = qacc_out; a.mu = mu; a.base64 = base_pose64;
+  a.qacc = qacc_out; a.tau = tau_out; a.mu = mu; a.base64 = base_pose64;
   return launch_step(a, flags, (hipStream_t)stream);
 }
 
@@ -933,7 +992,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
 void g1_cuda_step(int nenv, int nsub, float* qpos, float* qvel, float* anchor, const float* ctrl,
                   float* foot_pos, float* foot_vel, float* contact_forces, void* stream) {
   g1_hip_step_ex(nenv, nsub, 1e-3f, qpos, qvel, anchor, ctrl, foot_pos, foot_vel, contact_forces, nullptr,
-                 nullptr, nullptr, 0u, stream);
+                 nullptr, nullptr, nullptr, 0u, stream);
 }
 
 void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const uint8_t* done, unsigned seed,

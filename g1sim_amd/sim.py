@@ -100,7 +100,7 @@ class G1Sim:
         rc = self.lib.g1_hip_step_ex(
             self.nenv, nsub, self.dt, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
             self.ctrl.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(),
-            self.contact_forces.data_ptr(), None if self.qacc is None else self.qacc.data_ptr(),
+            self.contact_forces.data_ptr(), None if self.qacc is None else self.qacc.data_ptr(), None,
             None if self.mu is None else self.mu.data_ptr(), None, 0, stream)
         _lib.check(rc)
 

@@ -48,6 +48,12 @@ void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const
 /* g1_cuda_step with a runtime dt and extra outputs.  Returns 0 or a negative
  * error code (-1 invalid argument, -2 launch failure; see g1_hip_last_error).
  *   qacc_out     [35][nenv] float32, qacc of the last substep (nullable)
+ *   tau_out      [35][nenv] float32, total generalized force of the last
+ *                substep: servo + joint-limit torques on the hinges plus J^T f
+ *                of the foot contacts on every dof (base force in the world
+ *                frame, base torque in the body frame, like g1_c_fd's tau);
+ *                with it, the step is qacc = (M + dt B)^-1 (tau - C(q, qd))
+ *                (nullable)
  *   mu           [nenv] per-env foot friction (nullable: the model's 1.0)
  *   base_pose64  [7][nenv] float64 base position + quaternion, read and
  *                written instead of qpos[0:7] for accumulation when
@@ -55,8 +61,8 @@ void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const
  *                rounded pose); must be null otherwise. */
 int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor,
                    const float* ctrl, float* foot_pos, float* foot_vel, float* contact_forces,
-                   float* qacc_out, const float* mu, double* base_pose64, unsigned flags,
-                   void* stream);
+                   float* qacc_out, float* tau_out, const float* mu, double* base_pose64,
+                   unsigned flags, void* stream);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char* g1_hip_last_error(void);

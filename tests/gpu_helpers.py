@@ -26,26 +26,27 @@ class GpuState:
         self.ctrl = to_dev(ctrl.astype(np.float32), device)
         self.mu = None if mu is None else to_dev(np.asarray(mu, np.float32), device)
         z = lambda d: torch.zeros((d, self.n), dtype=torch.float32, device=device)  # noqa: E731
-        self.foot_pos, self.foot_vel, self.fn, self.qacc = z(6), z(6), z(8), z(35)
+        self.foot_pos, self.foot_vel, self.fn, self.qacc, self.tau = z(6), z(6), z(8), z(35), z(35)
         self.base64 = None
 
     def enable_f64_base(self):
         self.base64 = self.qpos[:7].double().contiguous()
 
-    def step(self, nsub=1, dt=2e-3, flags=0, qacc=True):
+    def step(self, nsub=1, dt=2e-3, flags=0, qacc=True, tau=False):
         lib = _lib.load()
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = lib.g1_hip_step_ex(
             self.n, nsub, dt, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
             self.ctrl.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(), self.fn.data_ptr(),
-            self.qacc.data_ptr() if qacc else None, None if self.mu is None else self.mu.data_ptr(),
+            self.qacc.data_ptr() if qacc else None, self.tau.data_ptr() if tau else None,
+            None if self.mu is None else self.mu.data_ptr(),
             None if self.base64 is None else self.base64.data_ptr(), flags, stream)
         _lib.check(rc)
 
     def host(self):
         torch.cuda.synchronize(self.device)
         return {k: getattr(self, k).cpu().numpy() for k in
-                ("qpos", "qvel", "anchor", "ctrl", "foot_pos", "foot_vel", "fn", "qacc")}
+                ("qpos", "qvel", "anchor", "ctrl", "foot_pos", "foot_vel", "fn", "qacc", "tau")}
 
 
 def rel_err(a, ref):

@@ -129,7 +129,7 @@ def test_partial_workgroups(device, n):
     lib = _lib.load()
     stream = torch.cuda.current_stream(device).cuda_stream
     rc = lib.g1_hip_step_ex(n, 3, 2e-3, big.data_ptr(), part.qvel.data_ptr(), part.anchor.data_ptr(),
-                            part.ctrl.data_ptr(), None, None, None, None, None, None, 0, stream)
+                            part.ctrl.data_ptr(), None, None, None, None, None, None, None, 0, stream)
     _lib.check(rc)
     torch.cuda.synchronize()
     hf = full.host()
@@ -209,3 +209,24 @@ def test_per_env_friction(device):
     ok = ~branch_flips(after["fn"], ref["fn"], after["anchor"], ref["anchor"])
     assert ok.mean() > 0.98
     assert rel_err(after["qacc"][:, ok], ref["qacc"][:, ok]).max() < QACC_TOL
+
+
+def test_total_generalized_force(device):
+    """tau_out: servo + limit torques and J^T f of the contacts, against the
+    oracle's total generalized force on the same input (pinned on the CPU by
+    (M + dt B) qacc + C = tau, tests/test_oracle.py)."""
+    qp, qv, an, cb = W.randomized_actuation(2048, seed=8, nblocks=1)
+    st = GpuState(qp, qv, an, cb[0], device)
+    st.step(40, 2e-3)
+    before = st.host()
+    st.step(1, 2e-3, tau=True)
+    after = st.host()
+    ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], 2e-3,
+                           want=("tau", "fn", "anchor"))
+    ok = ~branch_flips(after["fn"], ref["fn"], after["anchor"], ref["anchor"])
+    scale = np.maximum(np.abs(ref["tau"][:, ok]), 1.0)
+    err = (np.abs(after["tau"][:, ok] - ref["tau"][:, ok]) / scale).max()
+    write_metrics("tau_total_2048", {"rel_err": float(err), "ok_frac": float(ok.mean())})
+    assert ok.mean() > 0.98
+    assert err < 1e-3
+    assert np.abs(after["tau"][:6]).max() > 1.0          # contacts load the base

@@ -53,24 +53,24 @@ def test_step_ex_rejects_bad_arguments(args, msg):
     nenv, nsub, dt = args
     dummy = C.c_void_p(16)
     rc = lib.g1_hip_step_ex(nenv, nsub, dt, dummy, dummy, dummy, dummy, None, None, None, None, None,
-                            None, 0, None)
+                            None, None, 0, None)
     assert rc == -1
     assert msg in _lib.last_error()
 
 
 def test_step_ex_requires_state_pointers_and_consistent_flags():
     lib = _lib.load()
-    rc = lib.g1_hip_step_ex(4, 1, 2e-3, None, None, None, None, None, None, None, None, None, None, 0, None)
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, None, None, None, None, None, None, None, None, None, None, None, 0, None)
     assert rc == -1 and "required" in _lib.last_error()
     d = C.c_void_p(16)
-    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None,
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None, None,
                             _lib.G1_FLAG_F64_BASE, None)
     assert rc == -1 and "base_pose64" in _lib.last_error()
-    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None, 64, None)
+    rc = lib.g1_hip_step_ex(4, 1, 2e-3, d, d, d, d, None, None, None, None, None, None, None, 64, None)
     assert rc == -1 and "flag" in _lib.last_error()
 
 
 def test_zero_envs_is_a_no_op():
     lib = _lib.load()
     assert lib.g1_hip_step_ex(0, 5, 2e-3, None, None, None, None, None, None, None, None, None,
-                              None, 0, None) == 0
+                              None, None, 0, None) == 0
