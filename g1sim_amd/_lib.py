@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libg1sim_amd.so")
 
 G1_FLAG_F64_SOLVE = 1
 G1_FLAG_F64_BASE = 2
+G1_FLAG_F64_CONTACT = 4
 
 _lib = None
 
@@ -36,6 +37,20 @@ def load():
     lib.g1_cuda_reset_done.argtypes = [i, vp, vp, vp, vp, u, f, f, vp]
     lib.g1_hip_step_ex.restype = i
     lib.g1_hip_step_ex.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
+    lib.g1_mlp_gemm.restype = i
+    lib.g1_mlp_gemm.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp]
+    lib.g1_mlp_mse_grad.restype = i
+    lib.g1_mlp_mse_grad.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
+    lib.g1_mlp_colsum.restype = i
+    lib.g1_mlp_colsum.argtypes = [vp, i, i, vp, vp]
+    lib.g1_mlp_adam.restype = i
+    lib.g1_mlp_adam.argtypes = [vp, vp, vp, vp, vp, vp, i, f, f, f, f, i, f, vp]
+    lib.g1_mlp_fill.restype = i
+    lib.g1_mlp_fill.argtypes = [vp, f, C.c_longlong, vp]
+    lib.g1_mlp_to_bf16.restype = i
+    lib.g1_mlp_to_bf16.argtypes = [vp, vp, C.c_longlong, vp]
+    lib.g1_mlp_gather.restype = i
+    lib.g1_mlp_gather.argtypes = [vp, vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
     lib.g1_hip_last_error.restype = C.c_char_p
     lib.g1_hip_last_error.argtypes = []
     lib.g1_hip_version.restype = i
@@ -46,7 +61,8 @@ def load():
 
 def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
-            "g1_hip_version"]
+            "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_adam",
+            "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather"]
 
 
 def last_error():

@@ -1,0 +1,327 @@
+// g1_mlp.hip -- BF16 MFMA kernels of the learned forward-dynamics surrogate.
+//
+// One tiled GEMM, C[I x J] = sum_r A(i, r) B(j, r), with bf16 operands, fp32
+// accumulation on v_mfma_f32_16x16x32_bf16 (gfx950) and fused epilogues, plus
+// the small elementwise kernels of training (loss gradient, bias gradient,
+// Adam) and of inference (feature gather/normalisation).  All activations are
+// sample-major ([batch][features], features contiguous); the three GEMMs of a
+// dense layer are then
+//   forward        Z  = H W^T          A = H  (row),  B = W  (row)
+//   data grad      dH = dZ W           A = dZ (row),  B = W  (transposed)
+//   weight grad    dW = dZ^T H         A = dZ (trans), B = H (transposed)
+// selected by the A_T / B_T template flags (transposed operands are turned
+// into the [row][r] LDS layout while they are staged).
+//
+// Tiles: 128 x 128 per 256-thread workgroup (2 x 2 waves of 64 x 64, each 4 x 4
+// MFMA tiles of 16 x 16), r-steps of 32, double-buffered LDS with a 16-byte row
+// pad.  The weight-gradient GEMM reduces over the batch, so it splits r across
+// gridDim.z and adds partial tiles with fp32 atomics.  Dimensions are padded by
+// the host: I, J multiples of 128, R a multiple of 32 (times the split).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace g1mlp {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BI = 128, BJ = 128, BR = 32, PADR = 8, LDR = BR + PADR;
+
+enum Epi : int {
+  EPI_F32 = 0,         // C (fp32) = acc
+  EPI_F32_ATOMIC = 1,  // C (fp32) += acc   (split-r partial tiles)
+  EPI_BIAS_SILU = 2,   // Z (fp32) = acc + b; H (bf16) = silu(Z)     (training forward)
+  EPI_BIAS_SILU_H = 3, // H (bf16) = silu(acc + b)                   (inference forward)
+  EPI_BIAS_F32 = 4,    // C (fp32) = acc + b                          (output layer)
+  EPI_SILU_BWD = 5,    // dZ = acc * silu'(Z): fp32 to C, bf16 to H   (data grad through SiLU)
+};
+
+struct GemmArgs {
+  const bf16* A;
+  const bf16* B;
+  int lda, ldb;          // leading dimensions of A, B as stored
+  int I, J, R;           // problem size (padded)
+  int r_per_split;       // r extent of one gridDim.z slice
+  float* C;              // fp32 output (EPI_F32*, Z of BIAS_SILU, dZ of SILU_BWD, out of BIAS_F32)
+  bf16* H;               // bf16 output (BIAS_SILU*, SILU_BWD)
+  int ldc;               // leading dimension of C and H
+  const float* bias;     // [J]
+  const float* Zin;      // pre-activation for SILU_BWD, leading dimension ldc
+};
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+template <bool TRANS>
+__device__ __forceinline__ void stage(bf16 (*s)[LDR], const bf16* g, int ld, int row0, int r0, int tid) {
+  if constexpr (!TRANS) {
+    // rows row0.., r contiguous: 128 x 32 = 512 chunks of 8
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int c = tid + 256 * t;
+      const int row = c >> 2, rc = (c & 3) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(g + (size_t)(row0 + row) * ld + r0 + rc);
+      *reinterpret_cast<bf16x8*>(&s[row][rc]) = v;
+    }
+  } else {
+    // stored [r][row], row contiguous: 32 x 128 = 512 chunks of 8 rows
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int c = tid + 256 * t;
+      const int rr = c >> 4, rowc = (c & 15) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(g + (size_t)(r0 + rr) * ld + row0 + rowc);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[rowc + q][rr] = v[q];
+    }
+  }
+}
+
+template <bool A_T, bool B_T, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][BI][LDR];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][BJ][LDR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int i0 = blockIdx.x * BI, j0 = blockIdx.y * BJ;
+  const int rb = blockIdx.z * g.r_per_split;
+  const int nsteps = g.r_per_split / BR;
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  stage<A_T>(sA[0], g.A, g.lda, i0, rb, tid);
+  stage<B_T>(sB[0], g.B, g.ldb, j0, rb, tid);
+  __syncthreads();
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < nsteps) {
+      stage<A_T>(sA[cur ^ 1], g.A, g.lda, i0, rb + (st + 1) * BR, tid);
+      stage<B_T>(sB[cur ^ 1], g.B, g.ldb, j0, rb + (st + 1) * BR, tid);
+    }
+    bf16x8 fa[4], fb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fa[a] = *reinterpret_cast<const bf16x8*>(&sA[cur][wi * 64 + a * 16 + lr][lk]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) fb[b] = *reinterpret_cast<const bf16x8*>(&sB[cur][wj * 64 + b * 16 + lr][lk]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+
+  // D fragment: lane holds rows 4*(lane/16)+q, column lane%16 of each 16x16 tile
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + wi * 64 + a * 16 + (lane >> 4) * 4 + q;
+        const int j = j0 + wj * 64 + b * 16 + lr;
+        const size_t o = (size_t)i * g.ldc + j;
+        const float v = acc[a][b][q];
+        if constexpr (EPI == EPI_F32) {
+          g.C[o] = v;
+        } else if constexpr (EPI == EPI_F32_ATOMIC) {
+          atomicAdd(&g.C[o], v);
+        } else if constexpr (EPI == EPI_BIAS_SILU) {
+          const float z = v + g.bias[j];
+          g.C[o] = z;
+          g.H[o] = (bf16)(z * sigmoidf(z));
+        } else if constexpr (EPI == EPI_BIAS_SILU_H) {
+          const float z = v + g.bias[j];
+          g.H[o] = (bf16)(z * sigmoidf(z));
+        } else if constexpr (EPI == EPI_BIAS_F32) {
+          g.C[o] = v + g.bias[j];
+        } else if constexpr (EPI == EPI_SILU_BWD) {
+          const float z = g.Zin[o];
+          const float sg = sigmoidf(z);
+          const float d = v * sg * (1.0f + z * (1.0f - sg));
+          g.C[o] = d;
+          g.H[o] = (bf16)d;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- training
+// dY = 2 (pred - target) / (B * n_out) on the first n_out columns, 0 on padding;
+// also accumulates the summed squared error into *sse (fp32 atomics).
+__global__ void mse_grad_kernel(const float* pred, const float* target, int B, int ld, int n_out, bf16* dY16,
+                                float* dY32, float* sse) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * ld) return;
+  const int j = idx % ld;
+  float d = 0.f;
+  if (j < n_out) {
+    const float e = pred[idx] - target[idx];
+    d = 2.0f * e / (float)(B * n_out);
+    atomicAdd(sse, e * e);
+  }
+  dY16[idx] = (bf16)d;
+  if (dY32) dY32[idx] = d;
+}
+
+// db[j] = sum_i dZ[i][j]; one thread per column, rows strided over blockIdx.y
+__global__ void colsum_kernel(const float* dZ, int B, int ld, int rows_per_block, float* db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ld) return;
+  const int r0 = blockIdx.y * rows_per_block;
+  const int r1 = min(B, r0 + rows_per_block);
+  float s = 0.f;
+  for (int i = r0; i < r1; ++i) s += dZ[(size_t)i * ld + j];
+  atomicAdd(&db[j], s);
+}
+
+// Adam on fp32 master weights; refreshes the bf16 copy used by the GEMMs.
+// Entries with mask == 0 (padding) stay exactly zero.
+__global__ void adam_kernel(float* p, float* m, float* v, const float* grad, bf16* p16, const uint8_t* mask,
+                            int n, float lr, float b1, float b2, float eps, float bc1, float bc2, float wd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mask && !mask[i]) {
+    if (p16) p16[i] = (bf16)0.f;
+    return;
+  }
+  const float gi = grad[i] + wd * p[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float upd = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  const float pi = p[i] - upd;
+  p[i] = pi;
+  if (p16) p16[i] = (bf16)pi;
+}
+
+__global__ void fill_kernel(float* x, float v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void f32_to_bf16_kernel(const float* x, bf16* y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (bf16)x[i];
+}
+
+// ------------------------------------------------------------ features
+// Sample b takes (step, env) = (idx[2b], idx[2b+1]) of a dataset of SoA blocks
+// X[step][nfeat][nenv], Y[step][nout][nenv]; writes the z-scored features as
+// bf16 row b of Xn [B][ldx] (zero padding) and the z-scored targets as fp32
+// row b of Yn [B][ldy].  With idx == null, sample b is env b of step 0.
+__global__ void gather_kernel(const float* X, const float* Y, const int* idx, int B, int nfeat, int nout,
+                              int nenv, const float* xmean, const float* xinv, const float* ymean,
+                              const float* yinv, bf16* Xn, int ldx, float* Yn, int ldy) {
+  const int b = blockIdx.x;
+  if (b >= B) return;
+  const size_t step = idx ? (size_t)idx[2 * b] : 0;
+  const size_t env = idx ? (size_t)idx[2 * b + 1] : (size_t)b;
+  for (int f = threadIdx.x; f < ldx; f += blockDim.x) {
+    float v = 0.f;
+    if (f < nfeat) v = (X[(step * nfeat + f) * nenv + env] - xmean[f]) * xinv[f];
+    Xn[(size_t)b * ldx + f] = (bf16)v;
+  }
+  if (Yn) {
+    for (int f = threadIdx.x; f < ldy; f += blockDim.x) {
+      float v = 0.f;
+      if (f < nout) v = (Y[(step * nout + f) * nenv + env] - ymean[f]) * yinv[f];
+      Yn[(size_t)b * ldy + f] = v;
+    }
+  }
+}
+
+}  // namespace g1mlp
+
+// ================================================================== C-ABI
+namespace {
+using namespace g1mlp;
+template <bool AT, bool BT, int E>
+int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
+  dim3 grid(a.I / BI, a.J / BJ, split);
+  hipLaunchKernelGGL((gemm_kernel<AT, BT, E>), grid, dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+}  // namespace
+
+extern "C" {
+
+// C-ABI of the MLP kernels; see include/g1sim_amd.h for the contract.
+int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
+                int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, void* stream) {
+  if (I <= 0 || J <= 0 || R <= 0 || split <= 0 || I % BI || J % BJ || R % (BR * split)) return -1;
+  if (!A || !B) return -1;
+  if ((epi != EPI_BIAS_SILU_H && !C) || ((epi == EPI_BIAS_SILU || epi == EPI_BIAS_SILU_H || epi == EPI_SILU_BWD) && !H))
+    return -1;
+  if ((epi >= EPI_BIAS_SILU && epi <= EPI_BIAS_F32 && !bias) || (epi == EPI_SILU_BWD && !Zin)) return -1;
+  if (split > 1 && epi != EPI_F32_ATOMIC) return -1;
+  GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin};
+  hipStream_t s = (hipStream_t)stream;
+  const int key = (a_trans ? 1 : 0) | (b_trans ? 2 : 0);
+  switch (key * 8 + epi) {
+    case 0 * 8 + EPI_BIAS_SILU: return launch_gemm<false, false, EPI_BIAS_SILU>(g, split, s);
+    case 0 * 8 + EPI_BIAS_SILU_H: return launch_gemm<false, false, EPI_BIAS_SILU_H>(g, split, s);
+    case 0 * 8 + EPI_BIAS_F32: return launch_gemm<false, false, EPI_BIAS_F32>(g, split, s);
+    case 0 * 8 + EPI_F32: return launch_gemm<false, false, EPI_F32>(g, split, s);
+    case 2 * 8 + EPI_SILU_BWD: return launch_gemm<false, true, EPI_SILU_BWD>(g, split, s);
+    case 2 * 8 + EPI_F32: return launch_gemm<false, true, EPI_F32>(g, split, s);
+    case 3 * 8 + EPI_F32_ATOMIC: return launch_gemm<true, true, EPI_F32_ATOMIC>(g, split, s);
+    case 3 * 8 + EPI_F32: return launch_gemm<true, true, EPI_F32>(g, split, s);
+    default: return -1;
+  }
+}
+
+int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16, float* dY32,
+                    float* sse, void* stream) {
+  if (B <= 0 || ld <= 0 || n_out > ld) return -1;
+  const int n = B * ld;
+  hipLaunchKernelGGL(mse_grad_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, pred, target, B,
+                     ld, n_out, (bf16*)dY16, dY32, sse);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream) {
+  if (B <= 0 || ld <= 0) return -1;
+  const int rpb = 256;
+  hipLaunchKernelGGL(colsum_kernel, dim3((ld + 255) / 256, (B + rpb - 1) / rpb), dim3(256), 0, (hipStream_t)stream,
+                     dZ, B, ld, rpb, db);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask, int n, float lr,
+                float b1, float b2, float eps, int t, float wd, void* stream) {
+  if (n <= 0 || t <= 0) return -1;
+  const float bc1 = 1.f - powf(b1, (float)t), bc2 = 1.f - powf(b2, (float)t);
+  hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, m, v, grad,
+                     (bf16*)p16, mask, n, lr, b1, b2, eps, bc1, bc2, wd);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_fill(float* x, float v, long long n, void* stream) {
+  if (n <= 0) return n == 0 ? 0 : -1;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, v,
+                     (size_t)n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_to_bf16(const float* x, void* y, long long n, void* stream) {
+  if (n <= 0) return n == 0 ? 0 : -1;
+  hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x,
+                     (bf16*)y, (size_t)n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfeat, int nout, int nenv,
+                  const float* xmean, const float* xinv, const float* ymean, const float* yinv, void* Xn, int ldx,
+                  float* Yn, int ldy, void* stream) {
+  if (B <= 0 || nfeat > ldx || (Yn && nout > ldy) || (!idx && B > nenv)) return -1;
+  hipLaunchKernelGGL(gather_kernel, dim3(B), dim3(128), 0, (hipStream_t)stream, X, Y, idx, B, nfeat, nout, nenv,
+                     xmean, xinv, ymean, yinv, (bf16*)Xn, ldx, Yn, ldy);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // extern "C"

@@ -284,14 +284,13 @@ template <int B> G1K_DEV void fk_joint(float q, float qd, const float* vpar, flo
 }
 
 // World pose of hinge body B from its parent's (src/g1_core.h:47-50).
-template <int B>
-G1K_DEV void world_pose(const float (&Rp)[9], const float (&pp)[3], float c, float s, float (&R)[9],
-                        float (&p)[3]) {
+template <int B, typename T_>
+G1K_DEV void world_pose(const T_ (&Rp)[9], const T_ (&pp)[3], T_ c, T_ s, T_ (&R)[9], T_ (&p)[3]) {
   constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
-  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
+  constexpr T_ t0 = (T_)kM.b[B].t[0], t1 = (T_)kM.b[B].t[1], t2 = (T_)kM.b[B].t[2];
 #pragma unroll
   for (int r = 0; r < 3; ++r) p[r] = Rp[3 * r] * t0 + Rp[3 * r + 1] * t1 + Rp[3 * r + 2] * t2 + pp[r];
-  float T[9];
+  T_ T[9];
   if constexpr (kM.b[B].rt_ident) {
 #pragma unroll
     for (int m = 0; m < 9; ++m) T[m] = Rp[m];
@@ -301,7 +300,7 @@ G1K_DEV void world_pose(const float (&Rp)[9], const float (&pp)[3], float c, flo
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        T[3 * r + q] = Rp[3 * r] * (float)Q[q] + Rp[3 * r + 1] * (float)Q[3 + q] + Rp[3 * r + 2] * (float)Q[6 + q];
+        T[3 * r + q] = Rp[3 * r] * (T_)Q[q] + Rp[3 * r + 1] * (T_)Q[3 + q] + Rp[3 * r + 2] * (T_)Q[6 + q];
   }
   // columns I, J of T R_K
 #pragma unroll
@@ -390,12 +389,18 @@ template <int FIRST, int N> struct ChainOps {
     }
   }
   // world poses along the chain (feet only); returns the leaf's
-  template <int i = 0> static G1K_DEV void world(const float (&Rp)[9], const float (&pp)[3], const Joint* jt,
-                                                 float (&Rl)[9], float (&pl)[3]) {
+  // with T_ = double the joint rotations are re-evaluated from q in float64
+  template <typename T_, int i = 0> static G1K_DEV void world(const T_ (&Rp)[9], const T_ (&pp)[3], const Joint* jt,
+                                                             const float* q, T_ (&Rl)[9], T_ (&pl)[3]) {
     if constexpr (i < N) {
-      float R[9], p[3];
-      world_pose<FIRST + i>(Rp, pp, jt[i].c, jt[i].s, R, p);
-      if constexpr (i + 1 < N) world<i + 1>(R, p, jt, Rl, pl);
+      T_ R[9], p[3], c, s;
+      if constexpr (sizeof(T_) == sizeof(float)) {
+        c = jt[i].c; s = jt[i].s;
+      } else {
+        sincos((double)q[i], &s, &c);
+      }
+      world_pose<FIRST + i>(Rp, pp, c, s, R, p);
+      if constexpr (i + 1 < N) world<T_, i + 1>(R, p, jt, q, Rl, pl);
       else {
 #pragma unroll
         for (int m = 0; m < 9; ++m) Rl[m] = R[m];
@@ -448,22 +453,32 @@ template <int FIRST, int N> struct ChainOps {
 // tangential anchor spring-damper projected on the Coulomb cone, the anchor
 // dragged on slip and released on lift-off.  The wrench is summed in the
 // world frame about the body origin and rotated once.
-template <int K0>
-G1K_DEV void foot_contacts(const Cfg& cf, float mu, const float (&R)[9], const float (&p)[3], const float* v,
+// With T_ = double the contact-point geometry (world position, penetration
+// depth, anchor offsets) is evaluated in float64: the depth is a small
+// difference of ~0.75 m terms and the contact stiffness makes qacc sensitive
+// to it at the 1e-2 level per float32 ulp of the base height.
+template <int K0, typename T_>
+G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ (&p)[3], const float* v,
                            float* anchor, float* fn_out, float (&Fb)[6]) {
   float Fw[3] = {0.f, 0.f, 0.f}, Tw[3] = {0.f, 0.f, 0.f};
-  constexpr float radius = (float)G1M_CONTACT_RADIUS;
+  float R[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) R[m] = (float)Rd[m];
+  constexpr T_ radius = (T_)G1M_CONTACT_RADIUS;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float r0 = (float)g1m_contact_pos[3 * (K0 + k)], r1 = (float)g1m_contact_pos[3 * (K0 + k) + 1],
-                r2 = (float)g1m_contact_pos[3 * (K0 + k) + 2];
-    const float rw[3] = {R[0] * r0 + R[1] * r1 + R[2] * r2, R[3] * r0 + R[4] * r1 + R[5] * r2,
-                         R[6] * r0 + R[7] * r1 + R[8] * r2};
-    const float pw[3] = {rw[0] + p[0], rw[1] + p[1], rw[2] + p[2]};
-    const float depth = radius - pw[2];
+    const T_ r0 = (T_)g1m_contact_pos[3 * (K0 + k)], r1 = (T_)g1m_contact_pos[3 * (K0 + k) + 1],
+             r2 = (T_)g1m_contact_pos[3 * (K0 + k) + 2];
+    const T_ rwd[3] = {Rd[0] * r0 + Rd[1] * r1 + Rd[2] * r2, Rd[3] * r0 + Rd[4] * r1 + Rd[5] * r2,
+                       Rd[6] * r0 + Rd[7] * r1 + Rd[8] * r2};
+    const T_ pwd[3] = {rwd[0] + p[0], rwd[1] + p[1], rwd[2] + p[2]};
+    const float rw[3] = {(float)rwd[0], (float)rwd[1], (float)rwd[2]};
+    const float pw[3] = {(float)pwd[0], (float)pwd[1], (float)pwd[2]};
+    const float depth = (float)(radius - pwd[2]);
     // point velocity in the body frame, then world: R (v_l + w x r)
-    const float vb[3] = {v[3] + (v[1] * r2 - v[2] * r1), v[4] + (v[2] * r0 - v[0] * r2),
-                         v[5] + (v[0] * r1 - v[1] * r0)};
+    const float f0 = (float)r0, f1 = (float)r1, f2 = (float)r2;
+    const float vb[3] = {v[3] + (v[1] * f2 - v[2] * f1), v[4] + (v[2] * f0 - v[0] * f2),
+                         v[5] + (v[0] * f1 - v[1] * f0)};
     const float vp[3] = {R[0] * vb[0] + R[1] * vb[1] + R[2] * vb[2], R[3] * vb[0] + R[4] * vb[1] + R[5] * vb[2],
                          R[6] * vb[0] + R[7] * vb[1] + R[8] * vb[2]};
     const bool active = depth > 0.f;
@@ -473,8 +488,8 @@ G1K_DEV void foot_contacts(const Cfg& cf, float mu, const float (&R)[9], const f
     const bool fresh = ax >= ANCHOR_FREE;
     ax = fresh ? pw[0] : ax;
     ay = fresh ? pw[1] : ay;
-    float ftx = -cf.kt * (pw[0] - ax) - cf.dtt * vp[0];
-    float fty = -cf.kt * (pw[1] - ay) - cf.dtt * vp[1];
+    float ftx = -cf.kt * (float)(pwd[0] - (T_)ax) - cf.dtt * vp[0];
+    float fty = -cf.kt * (float)(pwd[1] - (T_)ay) - cf.dtt * vp[1];
     const float ftn = sqrtf(ftx * ftx + fty * fty), fmx = mu * fn;
     const bool slip = ftn > fmx;
     const float sc = fmx / (ftn + 1e-12f);
@@ -570,9 +585,9 @@ template <> struct WaveChain<3> { static constexpr int first = 23, n = 7, root =
 using Waist = ChainOps<13, 3>;
 
 // Chain part before the exchange: kinematics, contacts, backward pass.
-template <int W>
+template <int W, typename G>
 G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const float* ctl, float mu,
-                       const float (&v0)[6], const float (&v15)[6], const float (&R0)[9], const float (&p0)[3],
+                       const float (&v0)[6], const float (&v15)[6], const G (&R0)[9], const G (&p0)[3],
                        Joint* jt, float* anchor, float* fn, float (&footp)[3], float (&footv)[3],
                        float (&Pout)[21], float (&pout)[6], bool want_tau, float* tc, float (&Fpel)[6]) {
   using WC = WaveChain<W>;
@@ -585,16 +600,16 @@ G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const flo
   for (int m = 0; m < 6; ++m) pA[m] = jt[WC::n - 1].bias[m];
   float Fb[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if constexpr (WC::foot > 0) {
-    float Rf[9], pf[3];
-    Ch::world(R0, p0, jt, Rf, pf);
+    G Rf[9], pf[3];
+    Ch::world(R0, p0, jt, q, Rf, pf);
     foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anchor, fn, Fb);
 #pragma unroll
     for (int m = 0; m < 6; ++m) pA[m] -= Fb[m];
     // foot outputs (src/g1_kernel.cu:128-145): origin position, world linear velocity
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      footp[m] = pf[m];
-      footv[m] = Rf[3 * m] * vleaf[3] + Rf[3 * m + 1] * vleaf[4] + Rf[3 * m + 2] * vleaf[5];
+      footp[m] = (float)pf[m];
+      footv[m] = (float)Rf[3 * m] * vleaf[3] + (float)Rf[3 * m + 1] * vleaf[4] + (float)Rf[3 * m + 2] * vleaf[5];
     }
   }
   Ch::backward(cf, A, pA, q, qd, ctl, jt, Pout, pout);
@@ -612,7 +627,7 @@ G1K_DEV void chain_post(const float (&a0)[6], const float (&a15)[6], const Joint
   ChainOps<WC::first, WC::n>::forward((WC::root == 0) ? a0 : a15, jt, qdd, aleaf);
 }
 
-template <bool F64_SOLVE, bool F64_BASE>
+template <bool F64_SOLVE, bool F64_BASE, bool F64_CONTACT>
 __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
   __shared__ float xch[2][4][XCH][ENVS_PER_WG];
   __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame
@@ -705,6 +720,24 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
     }
+    // pelvis pose for the foot-contact geometry (float64 when F64_CONTACT)
+    using GeoT = typename std::conditional<F64_CONTACT, double, float>::type;
+    GeoT R0g[9], p0g[3];
+    if constexpr (F64_CONTACT) {
+      const double q0 = bq[0], q1 = bq[1], q2 = bq[2], q3 = bq[3];
+      const double inv = 1.0 / sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+      const double w = q0 * inv, x = q1 * inv, y = q2 * inv, z = q3 * inv;
+      R0g[0] = 1 - 2 * (y * y + z * z); R0g[1] = 2 * (x * y - w * z); R0g[2] = 2 * (x * z + w * y);
+      R0g[3] = 2 * (x * y + w * z); R0g[4] = 1 - 2 * (x * x + z * z); R0g[5] = 2 * (y * z - w * x);
+      R0g[6] = 2 * (x * z - w * y); R0g[7] = 2 * (y * z + w * x); R0g[8] = 1 - 2 * (x * x + y * y);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p0g[m] = (double)bp[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) R0g[m] = R0[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p0g[m] = p0[m];
+    }
     // ---- waist kinematics (bodies 13-15), needed by the arms and the merge
     Joint wj[3];
     float v15[6];
@@ -720,10 +753,10 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) pc[m] = 0.f;
     switch (wave) {
-      case 0: chain_pre<0>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 1: chain_pre<1>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 2: chain_pre<2>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      default: chain_pre<3>(cf, cq, cqd, cct, mu, v0, v15, R0, p0, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 0: chain_pre<0, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 1: chain_pre<1, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 2: chain_pre<2, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      default: chain_pre<3, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
     }
 #pragma unroll
     for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pc[m];
@@ -893,8 +926,10 @@ G1K_DEV unsigned hash32(unsigned x) {
   return x;
 }
 
-// src/g1_kernel.cu:85-103, with every float operation rounded separately
-// (no contraction) so the result is defined by the source arithmetic.
+// src/g1_kernel.cu:85-103.  The noise term is one fused multiply-add,
+// v = fma(noise, 2u - 1, v) (2u - 1 is exact), which is what a CUDA build of
+// the reference source produces by default (contraction of a*b + c); the
+// oracle evaluates the same expression, so the reset is bit-exact.
 __global__ __launch_bounds__(256) void reset_done_kernel(int nenv, float* qpos, float* qvel, float* anchor,
                                                          const uint8_t* done, unsigned seed, float noise,
                                                          float drop) {
@@ -908,9 +943,9 @@ __global__ __launch_bounds__(256) void reset_done_kernel(int nenv, float* qpos, 
     if (i == 2) v = __fadd_rn(v, drop);
     if (i >= 7) {
       s = hash32(s + 0x9E3779B9u);
-      const float u = __fmul_rn((float)s, 2.3283064365386963e-10f);
-      const float r = __fsub_rn(__fmul_rn(2.0f, u), 1.0f);
-      v = __fadd_rn(v, __fmul_rn(noise, r));
+      const float u = (float)s * 2.3283064365386963e-10f;   // exact scaling by 2^-32
+      const float r = 2.0f * u - 1.0f;                          // exact for u in [0, 1]
+      v = __builtin_fmaf(noise, r, v);
     }
     qpos[i * N + e] = v;
   }
@@ -935,10 +970,15 @@ void set_err(const char* fmt, ...) {
 
 using KernelFn = void (*)(g1k::StepArgs);
 
+template <bool C>
+KernelFn pick(bool f64s, bool f64b) {
+  return f64b ? (f64s ? g1k::step_kernel<true, true, C> : g1k::step_kernel<false, true, C>)
+              : (f64s ? g1k::step_kernel<true, false, C> : g1k::step_kernel<false, false, C>);
+}
+
 int launch_step(const g1k::StepArgs& a, unsigned flags, hipStream_t stream) {
   const bool f64s = flags & G1_FLAG_F64_SOLVE, f64b = flags & G1_FLAG_F64_BASE;
-  KernelFn k = f64b ? (f64s ? g1k::step_kernel<true, true> : g1k::step_kernel<false, true>)
-                    : (f64s ? g1k::step_kernel<true, false> : g1k::step_kernel<false, false>);
+  KernelFn k = (flags & G1_FLAG_F64_CONTACT) ? pick<true>(f64s, f64b) : pick<false>(f64s, f64b);
   const int nblocks = (a.nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG;
   hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, stream, a);
   hipError_t err = hipGetLastError();
@@ -973,7 +1013,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   g_err[0] = 0;
   int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
   if (rc) return rc;
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE)) { set_err("unknown flag bits 0x%x", flags); return -1; }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE | G1_FLAG_F64_CONTACT)) { set_err("unknown flag bits 0x%x", flags); return -1; }
   if (((flags & G1_FLAG_F64_BASE) != 0) != (base_pose64 != nullptr)) {
     set_err("base_pose64 must be given exactly when G1_FLAG_F64_BASE is set");
     return -1;

@@ -1,0 +1,202 @@
+"""Learned forward-dynamics surrogate: an MLP qacc = f(q, qd, tau_total).
+
+Network: 106 inputs (qpos 36, qvel 35, total generalized force 35), four
+hidden layers of 512 with SiLU, 35 outputs; inputs and outputs z-scored;
+bf16 weights and activations on MFMA with fp32 accumulation; fp32 master
+weights and Adam moments.  Every GEMM, activation, loss-gradient, bias-gradient
+and optimizer step runs in the project's own HIP kernels
+(csrc/g1_mlp.hip); torch only allocates device memory and provides the
+stream.  Padded shapes: inputs to 128 columns, outputs to 128 rows, batch to
+a multiple of 128.
+
+Training data come from the ABA step kernel itself (g1_hip_step_ex with the
+qacc and tau outputs) on randomized-actuation rollouts.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import NQ, NV
+
+NIN, NOUT, HID, NHID = NQ + NV + NV, NV, 512, 4
+PIN, POUT = 128, 128
+EPI = dict(F32=0, F32_ATOMIC=1, BIAS_SILU=2, BIAS_SILU_H=3, BIAS_F32=4, SILU_BWD=5)
+
+
+def _pad128(n):
+    return (n + 127) // 128 * 128
+
+
+class MLP:
+    """Parameters, optimizer state and the kernel sequence of the surrogate."""
+
+    def __init__(self, device, seed=0, hidden=HID, nhid=NHID):
+        self.lib = _lib.load()
+        self.dev = torch.device(device)
+        self.dims = [(hidden, PIN, NIN)] + [(hidden, hidden, hidden)] * (nhid - 1) + [(POUT, hidden, hidden)]
+        # (rows padded, cols padded, true fan-in); true fan-out of the last layer is NOUT
+        rng = np.random.default_rng(seed)
+        f32 = dict(dtype=torch.float32, device=self.dev)
+        self.W, self.b, self.W16, self.mW, self.vW, self.mb, self.vb, self.maskW, self.maskb = ([] for _ in range(9))
+        for li, (rows, cols, fan_in) in enumerate(self.dims):
+            out_true = NOUT if li == len(self.dims) - 1 else rows
+            bound = math.sqrt(6.0 / fan_in) if li < len(self.dims) - 1 else math.sqrt(1.0 / fan_in)
+            w = np.zeros((rows, cols), np.float32)
+            w[:out_true, :fan_in] = rng.uniform(-bound, bound, (out_true, fan_in))
+            mask = np.zeros((rows, cols), np.uint8)
+            mask[:out_true, :fan_in] = 1
+            mb = np.zeros(rows, np.uint8)
+            mb[:out_true] = 1
+            self.W.append(torch.as_tensor(w, device=self.dev))
+            self.b.append(torch.zeros(rows, **f32))
+            self.W16.append(self.W[-1].to(torch.bfloat16))
+            self.mW.append(torch.zeros_like(self.W[-1])); self.vW.append(torch.zeros_like(self.W[-1]))
+            self.mb.append(torch.zeros_like(self.b[-1])); self.vb.append(torch.zeros_like(self.b[-1]))
+            self.maskW.append(torch.as_tensor(mask, device=self.dev))
+            self.maskb.append(torch.as_tensor(mb, device=self.dev))
+        self.t = 0
+        self._ws = {}
+        # z-score statistics (set by fit_normalization)
+        self.xmean = torch.zeros(NIN, **f32); self.xinv = torch.ones(NIN, **f32)
+        self.ymean = torch.zeros(NOUT, **f32); self.yinv = torch.ones(NOUT, **f32)
+
+    @property
+    def stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {_lib.last_error()}")
+
+    def gemm(self, a_t, b_t, epi, A, lda, B, ldb, I, J, R, C=None, H=None, ldc=0, bias=None, Z=None, split=1):
+        rc = self.lib.g1_mlp_gemm(a_t, b_t, EPI[epi], A.data_ptr(), lda, B.data_ptr(), ldb, I, J, R, split,
+                                  None if C is None else C.data_ptr(), None if H is None else H.data_ptr(), ldc,
+                                  None if bias is None else bias.data_ptr(), None if Z is None else Z.data_ptr(),
+                                  self.stream)
+        self._check(rc, f"gemm {epi}")
+
+    def workspace(self, B):
+        """Per-batch-size activation buffers (sample-major)."""
+        if B in self._ws:
+            return self._ws[B]
+        f32 = dict(dtype=torch.float32, device=self.dev)
+        L = len(self.dims)
+        ws = {
+            "X16": torch.zeros((B, PIN), dtype=torch.bfloat16, device=self.dev),
+            "T": torch.zeros((B, POUT), **f32),
+            "dY32": torch.zeros((B, POUT), **f32),
+            "Z": [torch.zeros((B, self.dims[l][0]), **f32) for l in range(L - 1)],
+            "H16": [torch.zeros((B, self.dims[l][0]), dtype=torch.bfloat16, device=self.dev) for l in range(L - 1)],
+            "pred": torch.zeros((B, POUT), **f32),
+            "dY16": torch.zeros((B, POUT), dtype=torch.bfloat16, device=self.dev),
+            "dZ": [torch.zeros((B, self.dims[l][0]), **f32) for l in range(L - 1)],
+            "dZ16": [torch.zeros((B, self.dims[l][0]), dtype=torch.bfloat16, device=self.dev) for l in range(L - 1)],
+            "gW": [torch.zeros_like(w) for w in self.W],
+            "gb": [torch.zeros_like(b) for b in self.b],
+            "sse": torch.zeros(1, **f32),
+        }
+        self._ws[B] = ws
+        return ws
+
+    # ------------------------------------------------------------ forward
+    def forward(self, X16, B, train=True, out=None):
+        """X16: bf16 [B][128] z-scored inputs.  Returns the fp32 [B][128]
+        z-scored prediction (first 35 columns meaningful)."""
+        assert B % 128 == 0
+        ws = self.workspace(B)
+        L = len(self.dims)
+        h = X16
+        for l in range(L - 1):
+            rows, cols, _ = self.dims[l]
+            if train:
+                self.gemm(0, 0, "BIAS_SILU", h, cols, self.W16[l], cols, B, rows, cols,
+                          C=ws["Z"][l], H=ws["H16"][l], ldc=rows, bias=self.b[l])
+            else:
+                self.gemm(0, 0, "BIAS_SILU_H", h, cols, self.W16[l], cols, B, rows, cols,
+                          H=ws["H16"][l], ldc=rows, bias=self.b[l])
+            h = ws["H16"][l]
+        rows, cols, _ = self.dims[-1]
+        pred = ws["pred"] if out is None else out
+        self.gemm(0, 0, "BIAS_F32", h, cols, self.W16[-1], cols, B, rows, cols, C=pred, ldc=rows, bias=self.b[-1])
+        return pred
+
+    # ------------------------------------------------------------ backward
+    def backward(self, X16, B, dY16):
+        """Gradients of the loss into ws['gW'], ws['gb'] given dL/dpred (bf16)."""
+        ws = self.workspace(B)
+        L = len(self.dims)
+        for g in ws["gW"] + ws["gb"]:
+            self._check(self.lib.g1_mlp_fill(g.data_ptr(), 0.0, g.numel(), self.stream), "fill")
+        split = max(1, min(16, B // 512))
+        while B % (32 * split):
+            split //= 2
+        d16, dfp = dY16, None
+        for l in range(L - 1, -1, -1):
+            rows, cols, _ = self.dims[l]
+            hin = X16 if l == 0 else ws["H16"][l - 1]
+            # dW = d^T h: reduction over the batch, split across gridDim.z
+            self.gemm(1, 1, "F32_ATOMIC", d16, rows, hin, cols, rows, cols, B, C=ws["gW"][l], ldc=cols, split=split)
+            dsrc = ws["dY32"] if l == L - 1 else dfp
+            self._check(self.lib.g1_mlp_colsum(dsrc.data_ptr(), B, rows, ws["gb"][l].data_ptr(), self.stream),
+                        "colsum")
+            if l > 0:
+                prow = self.dims[l - 1][0]
+                # dH_{l-1} = d W_l, through SiLU'(Z_{l-1})
+                self.gemm(0, 1, "SILU_BWD", d16, rows, self.W16[l], cols, B, cols, rows,
+                          C=ws["dZ"][l - 1], H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1])
+                d16, dfp = ws["dZ16"][l - 1], ws["dZ"][l - 1]
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, B=None):
+        ws = self.workspace(B)
+        self.t += 1
+        for l in range(len(self.dims)):
+            self._check(self.lib.g1_mlp_adam(self.W[l].data_ptr(), self.mW[l].data_ptr(), self.vW[l].data_ptr(),
+                                             ws["gW"][l].data_ptr(), self.W16[l].data_ptr(),
+                                             self.maskW[l].data_ptr(), self.W[l].numel(), lr, b1, b2, eps,
+                                             self.t, wd, self.stream), "adam W")
+            self._check(self.lib.g1_mlp_adam(self.b[l].data_ptr(), self.mb[l].data_ptr(), self.vb[l].data_ptr(),
+                                             ws["gb"][l].data_ptr(), None, self.maskb[l].data_ptr(),
+                                             self.b[l].numel(), lr, b1, b2, eps, self.t, 0.0, self.stream),
+                        "adam b")
+
+    def loss_grad(self, pred, target, B):
+        """MSE on the 35 meaningful outputs: dY as bf16 (GEMM operand) and fp32
+        (output bias gradient); returns the device SSE tensor."""
+        ws = self.workspace(B)
+        self._check(self.lib.g1_mlp_fill(ws["sse"].data_ptr(), 0.0, 1, self.stream), "fill")
+        self._check(self.lib.g1_mlp_mse_grad(pred.data_ptr(), target.data_ptr(), B, POUT, NOUT,
+                                             ws["dY16"].data_ptr(), ws["dY32"].data_ptr(), ws["sse"].data_ptr(),
+                                             self.stream), "mse")
+        return ws["sse"]
+
+    def train_step(self, X16, T, B, lr):
+        """One minibatch: forward, MSE gradient, backward, Adam.  Returns the
+        device SSE (sum of squared z-scored errors)."""
+        pred = self.forward(X16, B, train=True)
+        sse = self.loss_grad(pred, T, B)
+        self.backward(X16, B, self.workspace(B)["dY16"])
+        self.adam(lr, B=B)
+        return sse
+
+    def set_normalization(self, xmean, xstd, ymean, ystd):
+        self.xmean.copy_(torch.as_tensor(xmean, dtype=torch.float32))
+        self.xinv.copy_(1.0 / torch.as_tensor(xstd, dtype=torch.float32).clamp_min(1e-6))
+        self.ymean.copy_(torch.as_tensor(ymean, dtype=torch.float32))
+        self.yinv.copy_(1.0 / torch.as_tensor(ystd, dtype=torch.float32).clamp_min(1e-6))
+
+    def gather(self, X, Y, idx, B, nenv, X16, Yn=None):
+        """z-scored bf16 inputs (and fp32 targets) of samples idx (int32 [B][2]:
+        step, env) from SoA blocks X [steps][106][nenv], Y [steps][35][nenv]."""
+        rc = self.lib.g1_mlp_gather(X.data_ptr(), None if Y is None else Y.data_ptr(),
+                                    None if idx is None else idx.data_ptr(), B, NIN, NOUT, nenv,
+                                    self.xmean.data_ptr(), self.xinv.data_ptr(), self.ymean.data_ptr(),
+                                    self.yinv.data_ptr(), X16.data_ptr(), PIN,
+                                    None if Yn is None else Yn.data_ptr(), POUT, self.stream)
+        self._check(rc, "gather")
+
+    def state_dict(self):
+        return {"W": [w.cpu() for w in self.W], "b": [b.cpu() for b in self.b],
+                "xmean": self.xmean.cpu(), "xinv": self.xinv.cpu(), "ymean": self.ymean.cpu(),
+                "yinv": self.yinv.cpu()}

@@ -44,6 +44,9 @@ void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const
 #define G1_FLAG_F64_SOLVE 1u   /* floating-base 6x6 Cholesky solve in float64 */
 #define G1_FLAG_F64_BASE  2u   /* base position/quaternion accumulated in float64
                                   (requires base_pose64) */
+#define G1_FLAG_F64_CONTACT 4u /* foot-contact geometry (pelvis and leg world
+                                  poses, contact points, penetration depth) in
+                                  float64 */
 
 /* g1_cuda_step with a runtime dt and extra outputs.  Returns 0 or a negative
  * error code (-1 invalid argument, -2 launch failure; see g1_hip_last_error).
@@ -63,6 +66,51 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
                    const float* ctrl, float* foot_pos, float* foot_vel, float* contact_forces,
                    float* qacc_out, float* tau_out, const float* mu, double* base_pose64,
                    unsigned flags, void* stream);
+
+/* ---- learned forward-dynamics surrogate: BF16 MFMA kernels (no reference
+ * counterpart; the reference has no learned component) ------------------ */
+
+/* Epilogues of g1_mlp_gemm. */
+#define G1_MLP_EPI_F32        0  /* C = acc (fp32)                                  */
+#define G1_MLP_EPI_F32_ATOMIC 1  /* C += acc (fp32 atomics; split > 1 allowed)      */
+#define G1_MLP_EPI_BIAS_SILU  2  /* C = acc + bias (fp32), H = silu(C) (bf16)       */
+#define G1_MLP_EPI_BIAS_SILU_H 3 /* H = silu(acc + bias) (bf16)                     */
+#define G1_MLP_EPI_BIAS_F32   4  /* C = acc + bias (fp32)                           */
+#define G1_MLP_EPI_SILU_BWD   5  /* C = acc * silu'(Zin) (fp32), H = same (bf16)    */
+
+/* C[I][J] = sum_r A(i, r) B(j, r) with bf16 A, B and fp32 accumulation on
+ * gfx950 MFMA.  A(i, r) = a_trans ? A[r*lda + i] : A[i*lda + r], likewise B.
+ * I and J must be multiples of 128 and R a multiple of 32*split; split > 1
+ * (gridDim.z partial sums) only with G1_MLP_EPI_F32_ATOMIC.  C and H use the
+ * leading dimension ldc.  Supported (a_trans, b_trans, epi): (0,0, BIAS_SILU
+ * | BIAS_SILU_H | BIAS_F32 | F32), (0,1, SILU_BWD | F32), (1,1, F32_ATOMIC |
+ * F32).  Returns 0, -1 (bad arguments) or -2 (launch failure). */
+int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
+                int I, int J, int R, int split, float* C, void* H, int ldc, const float* bias,
+                const float* Zin, void* stream);
+/* dY = 2 (pred - target) / (B * n_out) on columns < n_out, 0 beyond, as
+ * bf16 (dY16) and fp32 (dY32, nullable); *sse += sum of squared errors.
+ * pred, target: fp32 [B][ld]. */
+int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16,
+                    float* dY32, float* sse, void* stream);
+/* db[j] += sum_i dZ[i][j] over fp32 dZ [B][ld]. */
+int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream);
+/* One Adam step (step number t >= 1, L2 weight decay wd) on fp32 master
+ * weights p with moments m, v; writes the bf16 copy p16 (nullable).  mask
+ * (nullable) marks trainable entries; masked-out entries stay zero. */
+int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask,
+                int n, float lr, float b1, float b2, float eps, int t, float wd, void* stream);
+/* x[0..n) = v (a fill kernel, usable where memset nodes are not). */
+int g1_mlp_fill(float* x, float v, long long n, void* stream);
+/* y (bf16) = x (fp32), n elements. */
+int g1_mlp_to_bf16(const float* x, void* y, long long n, void* stream);
+/* Builds z-scored samples from SoA blocks X[step][nfeat][nenv] and
+ * Y[step][nout][nenv]: sample b is (step, env) = (idx[2b], idx[2b+1]), or env b
+ * of step 0 when idx is null.  Xn: bf16 [B][ldx] (zero padded); Yn (nullable):
+ * fp32 [B][ldy].  xinv / yinv are reciprocal standard deviations. */
+int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfeat, int nout, int nenv,
+                  const float* xmean, const float* xinv, const float* ymean, const float* yinv,
+                  void* Xn, int ldx, float* Yn, int ldy, void* stream);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char* g1_hip_last_error(void);

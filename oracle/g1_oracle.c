@@ -688,7 +688,9 @@ unsigned g1o_hash(unsigned x) {
   return x;
 }
 
-/* src/g1_kernel.cu:59-62 and 85-103, in float32 with separate multiply and add */
+/* src/g1_kernel.cu:59-62 and 85-103 in float32.  The noise term is one
+ * fused multiply-add, v = fma(noise, 2u - 1, v), the contraction a CUDA build
+ * of the reference source applies by default (2u - 1 itself is exact). */
 void g1o_reset_env_f32(int env, unsigned seed, float noise, float drop, float* qpos,
                        float* qvel, float* anchor) {
   unsigned s = g1o_hash(seed ^ (unsigned)(env + 1));
@@ -700,8 +702,7 @@ void g1o_reset_env_f32(int env, unsigned seed, float noise, float drop, float* q
       volatile float u = (float)s * 2.3283064365386963e-10f;
       volatile float two_u = 2.0f * u;
       volatile float r = two_u - 1.0f;
-      volatile float n = noise * r;
-      v = v + n;
+      v = fmaf(noise, r, v);
     }
     qpos[i] = v;
   }

@@ -64,8 +64,8 @@ void g1o_quat_integrate(double* q4, const double* w3, double dt);
 
 /* Integer hash + uniform draw of the reset kernel (src/g1_kernel.cu:54-62). */
 unsigned g1o_hash(unsigned x);
-/* Reset of one env (src/g1_kernel.cu:85-103) evaluated in float32, exactly
- * as the device kernel computes it: writes qpos[36], qvel[35], anchor[16]. */
+/* Reset of one env (src/g1_kernel.cu:85-103) evaluated in float32 with the
+ * noise term as one fused multiply-add: writes qpos[36], qvel[35], anchor[16]. */
 void g1o_reset_env_f32(int env, unsigned seed, float noise, float drop,
                        float* qpos, float* qvel, float* anchor);
 

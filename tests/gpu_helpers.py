@@ -75,6 +75,19 @@ def teacher_forced_step(st, dt, mu=None):
     return before, after, ref
 
 
+def f32_reference_step(before, dt, mu=None):
+    """The reference algorithm evaluated in float32 on the CPU (the oracle
+    source built with real = float) from the same state: the yardstick for
+    what float32 arithmetic can reach on these states."""
+    n = before["qpos"].shape[1]
+    out = {"qacc": np.zeros((35, n)), "fn": np.zeros((8, n)), "anchor": np.zeros((16, n))}
+    args = [np.ascontiguousarray(before[k], np.float32) for k in ("qpos", "qvel", "anchor", "ctrl")]
+    m = None if mu is None else np.ascontiguousarray(mu, np.float32)
+    O.lib32().g1o_step_batch_f32(n, dt, *(O.fptr(a) for a in args), O.fptr(m), O.dptr(out["qacc"]), None, None,
+                                 O.dptr(out["anchor"]), O.dptr(out["fn"]), None, None, None, 0)
+    return out
+
+
 def write_metrics(name, data):
     d = os.environ.get("G1SIM_METRICS_DIR")
     print(f"[metrics] {name}: {json.dumps(data)}", flush=True)
