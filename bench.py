@@ -70,6 +70,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--nenv", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--flags", type=int, default=0,
+                    help="g1_hip_step_ex flags: 1 float64 base solve, 2 float64 base pose, 4 float64 "
+                         "foot-contact geometry")
+    ap.add_argument("--nsub", type=int, default=CTRL_PERIOD,
+                    help="substeps per launch (must divide the 10-step target period)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
+                         "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step shards envs with no exchange, so N GPUs run N "
@@ -84,6 +92,11 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     nenv = args.nenv
+    if CTRL_PERIOD % args.nsub:
+        raise SystemExit("--nsub must divide 10")
+    base64 = None
+    if args.flags & _lib.G1_FLAG_F64_BASE:
+        base64 = torch.empty((7, nenv), dtype=torch.float64, device=dev)
     nblocks = (args.warmup + args.steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
     qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
     qpos = torch.as_tensor(qp, device=dev)
@@ -97,15 +110,17 @@ def main():
         """Launches covering physics steps [step0, step0 + nsteps)."""
         s = step0
         while s < step0 + nsteps:
-            n = min(CTRL_PERIOD - s % CTRL_PERIOD, step0 + nsteps - s)
+            n = min(args.nsub - s % args.nsub, step0 + nsteps - s)
             rc = lib.g1_hip_step_ex(nenv, n, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
                                     ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None,
-                                    None, 0, sptr)
+                                    None if base64 is None else base64.data_ptr(), args.flags, sptr)
             if rc:
                 raise RuntimeError(_lib.last_error())
             yield n
             s += n
 
+    if base64 is not None:
+        base64.copy_(qpos[:7])
     for _ in launch(0, args.warmup):
         pass
     torch.cuda.synchronize()
@@ -129,15 +144,20 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     per_launch_ms = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     sub_arr = np.array(subs)
     h = qpos[2].cpu().numpy()
     ok = bool(np.isfinite(qpos.cpu().numpy()).all() and ((h > 0.2) & (h < 1.2)).mean() > 0.99)
 
     env_steps = nenv * args.steps
     value = env_steps / wall
-    full = sub_arr == CTRL_PERIOD
+    full = sub_arr == args.nsub
     avg_ms = float(per_launch_ms[full].mean()) if full.any() else float(per_launch_ms.mean())
-    nsub_avg = CTRL_PERIOD if full.any() else float(sub_arr.mean())
+    nsub_avg = args.nsub if full.any() else float(sub_arr.mean())
     flop_per_launch = FLOP_PER_ENV_STEP * nenv * nsub_avg
     achieved = flop_per_launch / (avg_ms * 1e-3)
     traffic = None
@@ -145,7 +165,7 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
-        if t.get("nenv") == nenv and t.get("nsub") == CTRL_PERIOD:
+        if t.get("nenv") == nenv and t.get("nsub") == args.nsub and t.get("flags", 0) == args.flags:
             traffic = t.get("bytes_per_launch")
     out = {
         "metric": METRIC,
@@ -163,10 +183,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "randomized actuation: 16,384 envs, dt 2e-3, PD targets stand+U(+-0.3) "
                                "resampled every 10 steps, FP32 ABA step",
-                   "nenv": nenv, "dt": 2e-3, "nsub_per_launch": CTRL_PERIOD, "path": "FP32 ABA"},
+                   "nenv": nenv, "dt": 2e-3, "nsub_per_launch": args.nsub, "path": "FP32 ABA",
+                   "flags": args.flags},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
-                     "kernel": "g1k::step_kernel<false,false>",
+                     "kernel": "g1k::step_kernel<%s>" % ",".join(
+                         "true" if args.flags & b else "false" for b in (1, 2, 4)),
                      "flop_per_launch": flop_per_launch, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_ENV_LAUNCH * nenv},
         "sanity_standing": ok,

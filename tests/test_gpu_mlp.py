@@ -1,0 +1,202 @@
+"""BF16 MFMA kernels of the surrogate MLP against CPU references.
+
+GEMMs: every supported (a_trans, b_trans, epilogue) against a float64 CPU
+product of the same bf16-rounded operands (tolerance: fp32 accumulation over
+R terms, ~1e-5 relative).  Training: the full forward/backward of a small
+batch against a float64 CPU reference that applies the same bf16 roundings
+(activations and gradients are rounded to bf16 where the kernels store them
+as GEMM operands), and one Adam step against its closed form.
+"""
+import numpy as np
+import pytest
+import torch
+
+from g1sim_amd import _lib
+from g1sim_amd.surrogate import EPI, MLP, NIN, NOUT, PIN, POUT
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    return torch.as_tensor(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float32).numpy().astype(np.float64)
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def run_gemm(device, a_t, b_t, epi, A, B, I, J, R, split=1, bias=None, Z=None, C0=None):
+    lib = _lib.load()
+    At = torch.as_tensor(A.astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    Bt = torch.as_tensor(B.astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    C = torch.zeros((I, J), dtype=torch.float32, device=device) if C0 is None else \
+        torch.as_tensor(C0.astype(np.float32), device=device)
+    H = torch.zeros((I, J), dtype=torch.bfloat16, device=device)
+    bt = None if bias is None else torch.as_tensor(bias.astype(np.float32), device=device)
+    Zt = None if Z is None else torch.as_tensor(Z.astype(np.float32), device=device)
+    lda = A.shape[1]
+    ldb = B.shape[1]
+    rc = lib.g1_mlp_gemm(a_t, b_t, EPI[epi], At.data_ptr(), lda, Bt.data_ptr(), ldb, I, J, R, split, C.data_ptr(),
+                         H.data_ptr(), J, None if bt is None else bt.data_ptr(), None if Zt is None else Zt.data_ptr(),
+                         torch.cuda.current_stream(device).cuda_stream)
+    assert rc == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    return C.cpu().numpy().astype(np.float64), H.to(torch.float32).cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("a_t,b_t,epi,split", [
+    (0, 0, "F32", 1), (0, 0, "BIAS_F32", 1), (0, 0, "BIAS_SILU", 1), (0, 0, "BIAS_SILU_H", 1),
+    (0, 1, "F32", 1), (0, 1, "SILU_BWD", 1), (1, 1, "F32", 1), (1, 1, "F32_ATOMIC", 4)])
+def test_gemm_variants(device, a_t, b_t, epi, split):
+    rng = np.random.default_rng(100 * a_t + 10 * b_t + EPI[epi])
+    I, J, R = 256, 384, 256
+    Aop = rng.normal(size=(I, R))
+    Bop = rng.normal(size=(J, R))
+    A = Aop.T.copy() if a_t else Aop          # stored layouts
+    B = Bop.T.copy() if b_t else Bop
+    bias = rng.normal(size=J)
+    Z = rng.normal(size=(I, J))
+    C0 = rng.normal(size=(I, J)) if epi == "F32_ATOMIC" else None
+    C, H = run_gemm(device, a_t, b_t, epi, A, B, I, J, R, split=split, bias=bias, Z=Z, C0=C0)
+    acc = bf16(Aop) @ bf16(Bop).T
+    tol = 1e-5 * np.abs(bf16(Aop)) @ np.abs(bf16(Bop)).T + 1e-6
+    if epi == "F32":
+        assert (np.abs(C - acc) <= tol).all()
+    elif epi == "F32_ATOMIC":
+        assert (np.abs(C - (C0.astype(np.float32) + acc)) <= tol + 1e-5).all()
+    elif epi == "BIAS_F32":
+        assert (np.abs(C - (acc + bias.astype(np.float32))) <= tol + 1e-6).all()
+    elif epi in ("BIAS_SILU", "BIAS_SILU_H"):
+        z = acc + bias.astype(np.float32)
+        if epi == "BIAS_SILU":
+            assert (np.abs(C - z) <= tol + 1e-6).all()
+        assert np.allclose(H, bf16(silu(z)), rtol=1e-2, atol=1e-2)
+        assert (np.abs(H - silu(z)) <= 8e-3 * np.abs(silu(z)) + tol + 1e-3).all()
+    elif epi == "SILU_BWD":
+        d = acc * dsilu(Z.astype(np.float32))
+        assert (np.abs(C - d) <= tol * 1.2 + 1e-6).all()
+        assert (np.abs(H - d) <= 8e-3 * np.abs(d) + tol + 1e-6).all()
+
+
+def test_gemm_rejects_bad_shapes(device):
+    lib = _lib.load()
+    d = torch.zeros(16, device=device)
+    rc = lib.g1_mlp_gemm(0, 0, EPI["F32"], d.data_ptr(), 32, d.data_ptr(), 32, 100, 128, 32, 1, d.data_ptr(),
+                         None, 128, None, None, None)
+    assert rc == -1
+    rc = lib.g1_mlp_gemm(0, 0, EPI["F32"], d.data_ptr(), 32, d.data_ptr(), 32, 128, 128, 32, 2, d.data_ptr(),
+                         None, 128, None, None, None)
+    assert rc == -1          # split only with atomic accumulation
+
+
+def _cpu_reference(mlp, X, T):
+    """float64 forward/backward with the kernels' bf16 roundings."""
+    W = [bf16(w.cpu().numpy()) for w in mlp.W]
+    b = [bb.cpu().numpy().astype(np.float64) for bb in mlp.b]
+    B = X.shape[0]
+    h = [bf16(X)]
+    zs = []
+    for l in range(len(W) - 1):
+        z = h[-1] @ W[l].T + b[l]
+        zs.append(z)
+        h.append(bf16(silu(z)))
+    pred = h[-1] @ W[-1].T + b[-1]
+    d = np.zeros_like(pred)
+    d[:, :NOUT] = 2.0 * (pred[:, :NOUT] - T[:, :NOUT]) / (B * NOUT)
+    gW, gb = [None] * len(W), [None] * len(W)
+    d16 = bf16(d)
+    gb[-1] = d.sum(0)
+    for l in range(len(W) - 1, -1, -1):
+        gW[l] = d16.T @ h[l]
+        if l > 0:
+            dz = (d16 @ W[l]) * dsilu(zs[l - 1])
+            gb[l - 1] = dz.sum(0)
+            d16 = bf16(dz)
+    return pred, gW, gb
+
+
+def test_forward_backward_against_cpu(device):
+    mlp = MLP(device, seed=3)
+    B = 256
+    rng = np.random.default_rng(0)
+    X = np.zeros((B, PIN)); X[:, :NIN] = rng.normal(size=(B, NIN))
+    T = np.zeros((B, POUT)); T[:, :NOUT] = rng.normal(size=(B, NOUT))
+    ws = mlp.workspace(B)
+    ws["X16"].copy_(torch.as_tensor(X.astype(np.float32), device=device).to(torch.bfloat16))
+    ws["T"].copy_(torch.as_tensor(T.astype(np.float32), device=device))
+    pred = mlp.forward(ws["X16"], B, train=True)
+    mlp.loss_grad(pred, ws["T"], B)
+    mlp.backward(ws["X16"], B, ws["dY16"])
+    torch.cuda.synchronize()
+    rpred, rgW, rgb = _cpu_reference(mlp, X, T)
+    assert np.abs(pred.cpu().numpy()[:, :NOUT] - rpred[:, :NOUT]).max() < 2e-3 * max(1, np.abs(rpred).max())
+    for l in range(len(mlp.W)):
+        g = ws["gW"][l].cpu().numpy()
+        scale = np.abs(rgW[l]).max()
+        assert np.abs(g - rgW[l]).max() < 2e-2 * scale, l
+        gb = ws["gb"][l].cpu().numpy()
+        assert np.abs(gb - rgb[l]).max() < 2e-2 * max(np.abs(rgb[l]).max(), 1e-8), l
+        # padding stays exactly zero
+        if l == 0:
+            assert (g[:, NIN:] == 0).all()
+        if l == len(mlp.W) - 1:
+            assert (g[NOUT:] == 0).all() and (gb[NOUT:] == 0).all()
+
+
+def test_adam_step_closed_form(device):
+    lib = _lib.load()
+    n = 1000
+    rng = np.random.default_rng(1)
+    p = rng.normal(size=n).astype(np.float32)
+    g = rng.normal(size=n).astype(np.float32)
+    mask = (rng.random(n) < 0.9).astype(np.uint8)
+    p[mask == 0] = 0
+    t = {k: torch.as_tensor(v, device=device) for k, v in (("p", p), ("g", g), ("mask", mask))}
+    m = torch.zeros(n, device=device); v = torch.zeros(n, device=device)
+    p16 = torch.zeros(n, dtype=torch.bfloat16, device=device)
+    for step in (1, 2):
+        rc = lib.g1_mlp_adam(t["p"].data_ptr(), m.data_ptr(), v.data_ptr(), t["g"].data_ptr(), p16.data_ptr(),
+                             t["mask"].data_ptr(), n, 1e-3, 0.9, 0.999, 1e-8, step, 0.0,
+                             torch.cuda.current_stream(device).cuda_stream)
+        assert rc == 0
+    torch.cuda.synchronize()
+    # closed form after two identical gradients
+    m1 = 0.1 * g; v1 = 0.001 * g * g
+    p1 = p - 1e-3 * (m1 / 0.1) / (np.sqrt(v1 / 0.001) + 1e-8)
+    m2 = 0.9 * m1 + 0.1 * g; v2 = 0.999 * v1 + 0.001 * g * g
+    p2 = p1 - 1e-3 * (m2 / (1 - 0.81)) / (np.sqrt(v2 / (1 - 0.999 ** 2)) + 1e-8)
+    p2[mask == 0] = 0
+    assert np.allclose(t["p"].cpu().numpy(), p2, rtol=1e-5, atol=1e-7)
+    assert (p16.to(torch.float32).cpu().numpy()[mask == 0] == 0).all()
+
+
+def test_gather_normalises(device):
+    lib = _lib.load()
+    rng = np.random.default_rng(2)
+    steps, nenv, B = 3, 200, 128
+    X = rng.normal(size=(steps, NIN, nenv)).astype(np.float32)
+    Y = rng.normal(size=(steps, NOUT, nenv)).astype(np.float32)
+    idx = np.stack([rng.integers(0, steps, B), rng.integers(0, nenv, B)], 1).astype(np.int32)
+    xm, xs = rng.normal(size=NIN).astype(np.float32), rng.uniform(0.5, 2, NIN).astype(np.float32)
+    ym, ys = rng.normal(size=NOUT).astype(np.float32), rng.uniform(0.5, 2, NOUT).astype(np.float32)
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=device)  # noqa: E731
+    Xn = torch.zeros((B, PIN), dtype=torch.bfloat16, device=device)
+    Yn = torch.zeros((B, POUT), device=device)
+    tX, tY, ti = d(X), d(Y), d(idx)
+    txm, txi, tym, tyi = d(xm), d(1 / xs), d(ym), d(1 / ys)
+    rc = lib.g1_mlp_gather(tX.data_ptr(), tY.data_ptr(), ti.data_ptr(), B, NIN, NOUT, nenv, txm.data_ptr(),
+                           txi.data_ptr(), tym.data_ptr(), tyi.data_ptr(), Xn.data_ptr(), PIN, Yn.data_ptr(), POUT,
+                           torch.cuda.current_stream(device).cuda_stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    ex = (X[idx[:, 0], :, idx[:, 1]] - xm) / xs
+    ey = (Y[idx[:, 0], :, idx[:, 1]] - ym) / ys
+    got = Xn.to(torch.float32).cpu().numpy()
+    assert np.allclose(got[:, :NIN], bf16(ex), rtol=0, atol=1e-6) or np.abs(got[:, :NIN] - ex).max() < 2e-2
+    assert (got[:, NIN:] == 0).all()
+    assert np.allclose(Yn.cpu().numpy()[:, :NOUT], ey, rtol=1e-5, atol=1e-5)
