@@ -71,15 +71,15 @@ constexpr float ANCHOR_FREE = 1e30f;     // src/g1_core.h:108
 // unit axis).
 template <int K> struct Plane { static constexpr int I = (K + 1) % 3, J = (K + 2) % 3; };
 
-template <int K> G1K_DEV void rot(float* v, float c, float s) {          // v <- R_K v
+template <int K, typename T_ = float> G1K_DEV void rot(T_* v, T_ c, T_ s) {          // v <- R_K v
   constexpr int I = Plane<K>::I, J = Plane<K>::J;
-  const float a = v[I], b = v[J];
+  const T_ a = v[I], b = v[J];
   v[I] = c * a - s * b;
   v[J] = s * a + c * b;
 }
-template <int K> G1K_DEV void rotT(float* v, float c, float s) {         // v <- R_K^T v
+template <int K, typename T_ = float> G1K_DEV void rotT(T_* v, T_ c, T_ s) {         // v <- R_K^T v
   constexpr int I = Plane<K>::I, J = Plane<K>::J;
-  const float a = v[I], b = v[J];
+  const T_ a = v[I], b = v[J];
   v[I] = c * a + s * b;
   v[J] = c * b - s * a;
 }
@@ -91,26 +91,26 @@ template <int B> G1K_DEV void rt_mul(float* v) {                          // v <
   v[1] = (float)R[3] * x + (float)R[4] * y + (float)R[5] * z;
   v[2] = (float)R[6] * x + (float)R[7] * y + (float)R[8] * z;
 }
-template <int B> G1K_DEV void rt_mulT(float* v) {                         // v <- RT^T v
+template <int B, typename T_ = float> G1K_DEV void rt_mulT(T_* v) {      // v <- RT^T v
   constexpr const double* R = kM.b[B].RT;
-  const float x = v[0], y = v[1], z = v[2];
-  v[0] = (float)R[0] * x + (float)R[3] * y + (float)R[6] * z;
-  v[1] = (float)R[1] * x + (float)R[4] * y + (float)R[7] * z;
-  v[2] = (float)R[2] * x + (float)R[5] * y + (float)R[8] * z;
+  const T_ x = v[0], y = v[1], z = v[2];
+  v[0] = (T_)R[0] * x + (T_)R[3] * y + (T_)R[6] * z;
+  v[1] = (T_)R[1] * x + (T_)R[4] * y + (T_)R[7] * z;
+  v[2] = (T_)R[2] * x + (T_)R[5] * y + (T_)R[8] * z;
 }
 
 // ------------------------------------------------------- spatial transforms
 // X_B = (R, t) with R = RT_B R_K(q): pose of body B in its parent.
 // Motion vector parent -> child (src/spatial.h:102-105).
-template <int B> G1K_DEV void motion_to_child(const float* m, float c, float s, float* out) {
+template <int B, typename T_ = float> G1K_DEV void motion_to_child(const T_* m, T_ c, T_ s, T_* out) {
   constexpr int K = kM.b[B].axis;
-  constexpr float t0 = (float)kM.b[B].t[0], t1 = (float)kM.b[B].t[1], t2 = (float)kM.b[B].t[2];
-  float a[3] = {m[0], m[1], m[2]};
-  float l[3] = {m[3] - (t1 * m[2] - t2 * m[1]), m[4] - (t2 * m[0] - t0 * m[2]),
-                m[5] - (t0 * m[1] - t1 * m[0])};
-  if constexpr (!kM.b[B].rt_ident) { rt_mulT<B>(a); rt_mulT<B>(l); }
-  rotT<K>(a, c, s);
-  rotT<K>(l, c, s);
+  constexpr T_ t0 = (T_)kM.b[B].t[0], t1 = (T_)kM.b[B].t[1], t2 = (T_)kM.b[B].t[2];
+  T_ a[3] = {m[0], m[1], m[2]};
+  T_ l[3] = {m[3] - (t1 * m[2] - t2 * m[1]), m[4] - (t2 * m[0] - t0 * m[2]),
+             m[5] - (t0 * m[1] - t1 * m[0])};
+  if constexpr (!kM.b[B].rt_ident) { rt_mulT<B, T_>(a); rt_mulT<B, T_>(l); }
+  rotT<K, T_>(a, c, s);
+  rotT<K, T_>(l, c, s);
   out[0] = a[0]; out[1] = a[1]; out[2] = a[2];
   out[3] = l[0]; out[4] = l[1]; out[5] = l[2];
 }
@@ -389,23 +389,29 @@ template <int FIRST, int N> struct ChainOps {
     }
   }
   // world poses along the chain (feet only); returns the leaf's
-  // with T_ = double the joint rotations are re-evaluated from q in float64
-  template <typename T_, int i = 0> static G1K_DEV void world(const T_ (&Rp)[9], const T_ (&pp)[3], const Joint* jt,
-                                                             const float* q, T_ (&Rl)[9], T_ (&pl)[3]) {
+  // World poses along the chain (feet only) and, for T_ = double, the body
+  // twists again in float64 with the joint rotations re-evaluated from q.
+  template <typename T_, int i = 0>
+  static G1K_DEV void world(const T_ (&Rp)[9], const T_ (&pp)[3], const T_ (&vp)[6], const Joint* jt, const float* q,
+                            const float* qd, T_ (&Rl)[9], T_ (&pl)[3], T_ (&vl)[6]) {
     if constexpr (i < N) {
-      T_ R[9], p[3], c, s;
+      T_ R[9], p[3], v[6], c, s;
       if constexpr (sizeof(T_) == sizeof(float)) {
         c = jt[i].c; s = jt[i].s;
       } else {
         sincos((double)q[i], &s, &c);
+        motion_to_child<FIRST + i, T_>(vp, c, s, v);
+        v[kM.b[FIRST + i].axis] += (T_)qd[i];
       }
       world_pose<FIRST + i>(Rp, pp, c, s, R, p);
-      if constexpr (i + 1 < N) world<T_, i + 1>(R, p, jt, q, Rl, pl);
+      if constexpr (i + 1 < N) world<T_, i + 1>(R, p, v, jt, q, qd, Rl, pl, vl);
       else {
 #pragma unroll
         for (int m = 0; m < 9; ++m) Rl[m] = R[m];
 #pragma unroll
         for (int m = 0; m < 3; ++m) pl[m] = p[m];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) vl[m] = v[m];
       }
     }
   }
@@ -453,12 +459,14 @@ template <int FIRST, int N> struct ChainOps {
 // tangential anchor spring-damper projected on the Coulomb cone, the anchor
 // dragged on slip and released on lift-off.  The wrench is summed in the
 // world frame about the body origin and rotated once.
-// With T_ = double the contact-point geometry (world position, penetration
-// depth, anchor offsets) is evaluated in float64: the depth is a small
-// difference of ~0.75 m terms and the contact stiffness makes qacc sensitive
-// to it at the 1e-2 level per float32 ulp of the base height.
+// With T_ = double the contact-point kinematics (world position, penetration
+// depth, anchor offsets, point velocity) are evaluated in float64: the depth
+// is a small difference of ~0.75 m terms, the contact stiffness makes qacc
+// sensitive to it at the 1e-2 level per float32 ulp of the base height, and
+// at impact the damping term dn * v_z (~250 N) needs the velocity to better
+// than float32's accumulated ~1e-6.
 template <int K0, typename T_>
-G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ (&p)[3], const float* v,
+G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ (&p)[3], const T_* v,
                            float* anchor, float* fn_out, float (&Fb)[6]) {
   float Fw[3] = {0.f, 0.f, 0.f}, Tw[3] = {0.f, 0.f, 0.f};
   float R[9];
@@ -476,11 +484,11 @@ G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ 
     const float pw[3] = {(float)pwd[0], (float)pwd[1], (float)pwd[2]};
     const float depth = (float)(radius - pwd[2]);
     // point velocity in the body frame, then world: R (v_l + w x r)
-    const float f0 = (float)r0, f1 = (float)r1, f2 = (float)r2;
-    const float vb[3] = {v[3] + (v[1] * f2 - v[2] * f1), v[4] + (v[2] * f0 - v[0] * f2),
-                         v[5] + (v[0] * f1 - v[1] * f0)};
-    const float vp[3] = {R[0] * vb[0] + R[1] * vb[1] + R[2] * vb[2], R[3] * vb[0] + R[4] * vb[1] + R[5] * vb[2],
-                         R[6] * vb[0] + R[7] * vb[1] + R[8] * vb[2]};
+    const T_ vb[3] = {v[3] + (v[1] * r2 - v[2] * r1), v[4] + (v[2] * r0 - v[0] * r2),
+                      v[5] + (v[0] * r1 - v[1] * r0)};
+    const float vp[3] = {(float)(Rd[0] * vb[0] + Rd[1] * vb[1] + Rd[2] * vb[2]),
+                         (float)(Rd[3] * vb[0] + Rd[4] * vb[1] + Rd[5] * vb[2]),
+                         (float)(Rd[6] * vb[0] + Rd[7] * vb[1] + Rd[8] * vb[2])};
     const bool active = depth > 0.f;
     float fn = cf.kn * depth - cf.dn * vp[2];
     fn = fminf(fmaxf(fn, 0.f), 500.f);
@@ -588,6 +596,7 @@ using Waist = ChainOps<13, 3>;
 template <int W, typename G>
 G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const float* ctl, float mu,
                        const float (&v0)[6], const float (&v15)[6], const G (&R0)[9], const G (&p0)[3],
+                       const G (&v0g)[6],
                        Joint* jt, float* anchor, float* fn, float (&footp)[3], float (&footv)[3],
                        float (&Pout)[21], float (&pout)[6], bool want_tau, float* tc, float (&Fpel)[6]) {
   using WC = WaveChain<W>;
@@ -600,9 +609,13 @@ G1K_DEV void chain_pre(const Cfg& cf, const float* q, const float* qd, const flo
   for (int m = 0; m < 6; ++m) pA[m] = jt[WC::n - 1].bias[m];
   float Fb[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if constexpr (WC::foot > 0) {
-    G Rf[9], pf[3];
-    Ch::world(R0, p0, jt, q, Rf, pf);
-    foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anchor, fn, Fb);
+    G Rf[9], pf[3], vf[6];
+    Ch::world(R0, p0, v0g, jt, q, qd, Rf, pf, vf);
+    if constexpr (sizeof(G) == sizeof(float)) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) vf[m] = vleaf[m];
+    }
+    foot_contacts<WC::k0>(cf, mu, Rf, pf, vf, anchor, fn, Fb);
 #pragma unroll
     for (int m = 0; m < 6; ++m) pA[m] -= Fb[m];
     // foot outputs (src/g1_kernel.cu:128-145): origin position, world linear velocity
@@ -722,7 +735,7 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     }
     // pelvis pose for the foot-contact geometry (float64 when F64_CONTACT)
     using GeoT = typename std::conditional<F64_CONTACT, double, float>::type;
-    GeoT R0g[9], p0g[3];
+    GeoT R0g[9], p0g[3], v0g[6];
     if constexpr (F64_CONTACT) {
       const double q0 = bq[0], q1 = bq[1], q2 = bq[2], q3 = bq[3];
       const double inv = 1.0 / sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
@@ -732,11 +745,18 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       R0g[6] = 2 * (x * z - w * y); R0g[7] = 2 * (y * z + w * x); R0g[8] = 1 - 2 * (x * x + y * y);
 #pragma unroll
       for (int m = 0; m < 3; ++m) p0g[m] = (double)bp[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        v0g[m] = (double)om[m];
+        v0g[3 + m] = R0g[m] * (double)vw[0] + R0g[3 + m] * (double)vw[1] + R0g[6 + m] * (double)vw[2];
+      }
     } else {
 #pragma unroll
       for (int m = 0; m < 9; ++m) R0g[m] = R0[m];
 #pragma unroll
       for (int m = 0; m < 3; ++m) p0g[m] = p0[m];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v0g[m] = v0[m];
     }
     // ---- waist kinematics (bodies 13-15), needed by the arms and the merge
     Joint wj[3];
@@ -753,10 +773,10 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) pc[m] = 0.f;
     switch (wave) {
-      case 0: chain_pre<0, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 1: chain_pre<1, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 2: chain_pre<2, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      default: chain_pre<3, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 0: chain_pre<0, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 1: chain_pre<1, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      case 2: chain_pre<2, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
+      default: chain_pre<3, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
     }
 #pragma unroll
     for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pc[m];

@@ -44,9 +44,9 @@ void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const
 #define G1_FLAG_F64_SOLVE 1u   /* floating-base 6x6 Cholesky solve in float64 */
 #define G1_FLAG_F64_BASE  2u   /* base position/quaternion accumulated in float64
                                   (requires base_pose64) */
-#define G1_FLAG_F64_CONTACT 4u /* foot-contact geometry (pelvis and leg world
-                                  poses, contact points, penetration depth) in
-                                  float64 */
+#define G1_FLAG_F64_CONTACT 4u /* foot-contact kinematics in float64: pelvis and
+                                  leg world poses and twists, contact points,
+                                  penetration depth and contact-point velocity */
 
 /* g1_cuda_step with a runtime dt and extra outputs.  Returns 0 or a negative
  * error code (-1 invalid argument, -2 launch failure; see g1_hip_last_error).
