@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "libg1sim_amd.so")
 G1_FLAG_F64_SOLVE = 1
 G1_FLAG_F64_BASE = 2
 G1_FLAG_F64_CONTACT = 4
+G1_FLAG_WAVES4 = 8
+G1_FLAG_WAVES2 = 16
 
 _lib = None
 

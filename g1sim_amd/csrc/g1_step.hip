@@ -65,6 +65,61 @@ static Cfg default_cfg(float dt) {
 
 constexpr float ANCHOR_FREE = 1e30f;     // src/g1_core.h:108
 
+// ------------------------------------------------------------ fast math
+// Hardware reciprocal / square root / reciprocal square root (1 ulp) refined
+// by one Newton step, in place of the IEEE division and square-root
+// sequences (about 10 instructions each).  Arguments are positive and
+// normal where these are used (joint inertias, Cholesky pivots, norms).
+G1K_DEV float frcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+G1K_DEV float frsq(float x) {
+  float r = __builtin_amdgcn_rsqf(x);
+  const float h = 0.5f * x * r;                    // Newton on 1/sqrt: r (1.5 - 0.5 x r^2)
+  return __builtin_fmaf(r, __builtin_fmaf(-h, r, 0.5f), r);
+}
+G1K_DEV float fsqrt(float x) {                    // x >= 0; sqrt(0) = 0
+  const float r = frsq(x);
+  return x > 0.f ? x * r : 0.f;
+}
+G1K_DEV double frcp(double x) { return 1.0 / x; }
+G1K_DEV double fsqrt(double x) { return sqrt(x); }
+G1K_DEV double frsq(double x) { return 1.0 / sqrt(x); }
+
+// sin and cos of a joint angle: Cody-Waite reduction by pi/2 (three-part
+// constant, exact products for |x| < 2^12) and Taylor polynomials on
+// |r| <= pi/4 (truncation below 2e-9).  Larger arguments take the library
+// routine.
+G1K_DEV void fsincos(float x, float* sp, float* cp) {
+  if (__builtin_expect(fabsf(x) > 4096.f, 0)) {
+    sincosf(x, sp, cp);
+    return;
+  }
+  const float k = __builtin_rintf(x * 0.63661977236758134f);     // 2/pi
+  float r = __builtin_fmaf(-k, 1.5707963705062866f, x);          // pi/2 = P1 + P2 + P3
+  r = __builtin_fmaf(-k, -4.3711388286737929e-08f, r);
+  r = __builtin_fmaf(-k, -1.7151245100059206e-15f, r);
+  const float r2 = r * r;
+  // sin r = r (1 - r^2/3! + r^4/5! - r^6/7! + r^8/9!)
+  float ps = __builtin_fmaf(r2, 2.7557319223985888e-06f, -1.9841269841269841e-04f);
+  ps = __builtin_fmaf(ps, r2, 8.3333333333333333e-03f);
+  ps = __builtin_fmaf(ps, r2, -1.6666666666666666e-01f);
+  const float sn = __builtin_fmaf(ps * r2, r, r);
+  // cos r = 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10!
+  float pc = __builtin_fmaf(r2, -2.7557319223985888e-07f, 2.4801587301587302e-05f);
+  pc = __builtin_fmaf(pc, r2, -1.3888888888888889e-03f);
+  pc = __builtin_fmaf(pc, r2, 4.1666666666666667e-02f);
+  pc = __builtin_fmaf(pc, r2, -0.5f);
+  const float cs = __builtin_fmaf(pc, r2, 1.0f);
+  const int quad = (int)k & 3;
+  const float s0 = (quad & 1) ? cs : sn;
+  const float c0 = (quad & 1) ? sn : cs;
+  *sp = (quad & 2) ? -s0 : s0;
+  *cp = ((quad + 1) & 2) ? -c0 : c0;
+}
+
 // ---------------------------------------------------------------- rotations
 // Planar rotation about axis K: plane (I, J) = ((K+1)%3, (K+2)%3) and
 // R_K = identity on K, [[c, -s], [s, c]] on (I, J)  (src/spatial.h:63-70 for a
@@ -272,7 +327,7 @@ struct Joint {
 // Kinematics of hinge body B: twist from the parent's (src/g1_core.h:39-53).
 template <int B> G1K_DEV void fk_joint(float q, float qd, const float* vpar, float* v, Joint& jt) {
   constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
-  sincosf(q, &jt.s, &jt.c);
+  fsincos(q, &jt.s, &jt.c);
   motion_to_child<B>(vpar, jt.c, jt.s, v);
   v[K] += qd;
   // cross_motion(v, qd S) with S = [e_K; 0] (src/g1_core.h:180)
@@ -336,7 +391,7 @@ G1K_DEV void backward_joint(const Cfg& cf, float (&A)[21], float (&pA)[6], float
 #pragma unroll
   for (int m = 0; m < 6; ++m) U[m] = A[pk(m, K)];
   const float D = U[K] + arm + cf.dt * b;
-  const float invD = 1.0f / D;
+  const float invD = frcp(D);
   const float u = t - pA[K];
 #pragma unroll
   for (int m = 0; m < 6; ++m) jt.W[m] = U[m] * invD;
@@ -498,9 +553,9 @@ G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ 
     ay = fresh ? pw[1] : ay;
     float ftx = -cf.kt * (float)(pwd[0] - (T_)ax) - cf.dtt * vp[0];
     float fty = -cf.kt * (float)(pwd[1] - (T_)ay) - cf.dtt * vp[1];
-    const float ftn = sqrtf(ftx * ftx + fty * fty), fmx = mu * fn;
+    const float ftn = fsqrt(ftx * ftx + fty * fty), fmx = mu * fn;
     const bool slip = ftn > fmx;
-    const float sc = fmx / (ftn + 1e-12f);
+    const float sc = fmx * frcp(ftn + 1e-12f);
     ftx = slip ? ftx * sc : ftx;
     fty = slip ? fty * sc : fty;
     ax = slip ? pw[0] + ftx * cf.inv_kt : ax;
@@ -534,8 +589,9 @@ template <typename T> G1K_DEV void base_solve(const float (&A)[21], const float 
   for (int j = 0; j < 6; ++j) {
 #pragma unroll
     for (int k = 0; k < j; ++k) L[pk(j, j)] -= L[pk(j, k)] * L[pk(j, k)];
-    L[pk(j, j)] = sqrt(L[pk(j, j)]);
-    const T inv = (T)1 / L[pk(j, j)];
+    const T d = fsqrt(L[pk(j, j)]);
+    const T inv = frcp(d);
+    L[pk(j, j)] = inv;                         // the diagonal is kept inverted
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
 #pragma unroll
@@ -548,14 +604,14 @@ template <typename T> G1K_DEV void base_solve(const float (&A)[21], const float 
     T acc = -(T)pA[i];
 #pragma unroll
     for (int k = 0; k < i; ++k) acc -= L[pk(i, k)] * y[k];
-    y[i] = acc / L[pk(i, i)];
+    y[i] = acc * L[pk(i, i)];
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {
     T acc = y[i];
 #pragma unroll
     for (int k = i + 1; k < 6; ++k) acc -= L[pk(k, i)] * y[k];
-    y[i] = acc / L[pk(i, i)];
+    y[i] = acc * L[pk(i, i)];
   }
 #pragma unroll
   for (int m = 0; m < 6; ++m) a[m] = (float)y[m];
@@ -579,6 +635,7 @@ struct StepArgs {
   float* tau;
   const float* mu;
   double* base64;
+  int f64_solve;
 };
 
 // Per-lane chain slots (legs use 6 of 7).
@@ -640,20 +697,65 @@ G1K_DEV void chain_post(const float (&a0)[6], const float (&a15)[6], const Joint
   ChainOps<WC::first, WC::n>::forward((WC::root == 0) ? a0 : a15, jt, qdd, aleaf);
 }
 
-template <bool F64_SOLVE, bool F64_BASE, bool F64_CONTACT>
-__global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
-  __shared__ float xch[2][4][XCH][ENVS_PER_WG];
-  __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N_-1.
+template <int N_, typename F> G1K_DEV void static_for(F&& f) {
+  if constexpr (N_ > 0) {
+    static_for<N_ - 1>(f);
+    f(std::integral_constant<int, N_ - 1>{});
+  }
+}
+
+// Limb chains: 0 left leg, 1 right leg, 2 left arm, 3 right arm.
+G1K_DEV int chain_h0(int c) { return c == 0 ? 0 : c == 1 ? 6 : c == 2 ? 15 : 22; }
+G1K_DEV int chain_n(int c) { return c < 2 ? 6 : 7; }
+
+template <typename GeoT>
+G1K_DEV void dispatch_pre(int chain, const Cfg& cf, const float* q, const float* qd, const float* ctl, float mu,
+                          const float (&v0)[6], const float (&v15)[6], const GeoT (&R0g)[9], const GeoT (&p0g)[3],
+                          const GeoT (&v0g)[6], Joint* jt, float* anc, float* fn, float (&footp)[3],
+                          float (&footv)[3], float (&P)[21], float (&p)[6], bool want_tau, float* tc,
+                          float (&Fpel)[6]) {
+  switch (chain) {
+    case 0: chain_pre<0, GeoT>(cf, q, qd, ctl, mu, v0, v15, R0g, p0g, v0g, jt, anc, fn, footp, footv, P, p, want_tau, tc, Fpel); break;
+    case 1: chain_pre<1, GeoT>(cf, q, qd, ctl, mu, v0, v15, R0g, p0g, v0g, jt, anc, fn, footp, footv, P, p, want_tau, tc, Fpel); break;
+    case 2: chain_pre<2, GeoT>(cf, q, qd, ctl, mu, v0, v15, R0g, p0g, v0g, jt, anc, fn, footp, footv, P, p, want_tau, tc, Fpel); break;
+    default: chain_pre<3, GeoT>(cf, q, qd, ctl, mu, v0, v15, R0g, p0g, v0g, jt, anc, fn, footp, footv, P, p, want_tau, tc, Fpel); break;
+  }
+}
+
+G1K_DEV void dispatch_post(int chain, const float (&a0)[6], const float (&a15)[6], const Joint* jt, float* qdd) {
+  switch (chain) {
+    case 0: chain_post<0>(a0, a15, jt, qdd); break;
+    case 1: chain_post<1>(a0, a15, jt, qdd); break;
+    case 2: chain_post<2>(a0, a15, jt, qdd); break;
+    default: chain_post<3>(a0, a15, jt, qdd); break;
+  }
+}
+
+// SPLIT waves per workgroup of 64 envs: with 4, wave w owns chain w; with 2,
+// wave 0 owns both legs (and all 8 contacts) and wave 1 both arms, which
+// halves the pelvis/waist work every wave repeats when there are enough envs
+// to fill the chip anyway.
+//
+// ROLE >= 0 compiles the body for one wave index (the 2-wave split, where
+// each role keeps two chains in registers); ROLE < 0 dispatches the chain
+// code on the wave index at run time and keeps a single copy of the shared
+// pelvis/waist code (the 4-wave split).
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE>
+G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PER_WG],
+                       float (&fcx)[2][2][6][ENVS_PER_WG]) {
+  constexpr int CPW = 4 / SPLIT;                 // chains per wave
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nenv = args.nenv;
   const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
   const bool valid = e_raw < nenv;
   const int e = valid ? e_raw : nenv - 1;      // tail lanes mirror the last env, never store
   const size_t N = (size_t)nenv;
   const Cfg cf = args.cfg;
+  const bool legs = wave * CPW < 2;            // this wave owns the leg chains
 
-  // ---- load (all waves: base + waist; own chain)
+  // ---- load (all waves: base + waist; own chains)
   using BaseT = typename std::conditional<F64_BASE, double, float>::type;
   BaseT bp[3], bq[4];
   float vw[3], om[3];
@@ -680,40 +782,39 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     wqd[m] = args.qvel[(6 + 12 + m) * N + e];
     wct[m] = args.ctrl[(12 + m) * N + e];
   }
-  // own chain: hinge h0 .. h0+n-1
-  int h0, nj;
-  switch (wave) {
-    case 0: h0 = 0; nj = 6; break;
-    case 1: h0 = 6; nj = 6; break;
-    case 2: h0 = 15; nj = 7; break;
-    default: h0 = 22; nj = 7; break;
-  }
-  float cq[MAXJ], cqd[MAXJ], cct[MAXJ];
-#pragma unroll
-  for (int m = 0; m < MAXJ; ++m) {
-    const int mm = m < nj ? m : nj - 1;
-    cq[m] = args.qpos[(7 + h0 + mm) * N + e];
-    cqd[m] = args.qvel[(6 + h0 + mm) * N + e];
-    cct[m] = args.ctrl[(h0 + mm) * N + e];
-  }
-  float anc[8], fn[4] = {0.f, 0.f, 0.f, 0.f}, footp[3] = {0.f, 0.f, 0.f}, footv[3] = {0.f, 0.f, 0.f};
+  float cq[CPW][MAXJ], cqd[CPW][MAXJ], cct[CPW][MAXJ];
+  float anc[CPW][8];
   float mu = cf.mu;
-  if (wave < 2) {
-    const int a0i = wave * 8;
+  if (legs && args.mu) mu = args.mu[e];
+  static_for<CPW>([&](auto C) {
+    constexpr int c = decltype(C)::value;
+    const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) anc[m] = args.anchor[(a0i + m) * N + e];
-    if (args.mu) mu = args.mu[e];
-  } else {
+    for (int m = 0; m < MAXJ; ++m) {
+      const int mm = m < nj ? m : nj - 1;
+      cq[c][m] = args.qpos[(7 + h0 + mm) * N + e];
+      cqd[c][m] = args.qvel[(6 + h0 + mm) * N + e];
+      cct[c][m] = args.ctrl[(h0 + mm) * N + e];
+    }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) anc[m] = ANCHOR_FREE;
-  }
+    for (int m = 0; m < 8; ++m) anc[c][m] = legs ? args.anchor[(ch * 8 + m) * N + e] : ANCHOR_FREE;
+  });
 
-  float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[MAXJ];
+  float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[CPW][MAXJ];
+  float fn[CPW][4], footp[CPW][3], footv[CPW][3];
   // total generalized force of the last substep (servo + limits + J^T f)
   const bool want_tau = args.tau != nullptr;
-  float tau_c[MAXJ], tau_w[3] = {0.f, 0.f, 0.f}, tau_b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const bool f64_solve = args.f64_solve;
+  float tau_c[CPW][MAXJ], tau_w[3] = {0.f, 0.f, 0.f}, tau_b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int m = 0; m < MAXJ; ++m) { qdd_c[m] = 0.f; tau_c[m] = 0.f; }
+  for (int c = 0; c < CPW; ++c) {
+#pragma unroll
+    for (int m = 0; m < MAXJ; ++m) { qdd_c[c][m] = 0.f; tau_c[c][m] = 0.f; }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) fn[c][m] = 0.f;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) { footp[c][m] = 0.f; footv[c][m] = 0.f; }
+  }
   const float dt = cf.dt;
   for (int sub = 0; sub < args.nsub; ++sub) {
     const int buf = sub & 1;
@@ -721,7 +822,7 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     float R0[9], p0[3], v0[6];
     {
       const BaseT n2 = bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2] + bq[3] * bq[3];
-      const BaseT inv = (BaseT)1 / sqrt(n2);
+      const BaseT inv = frsq(n2);
       const float w = (float)(bq[0] * inv), x = (float)(bq[1] * inv), y = (float)(bq[2] * inv),
                   z = (float)(bq[3] * inv);
       R0[0] = 1 - 2 * (y * y + z * z); R0[1] = 2 * (x * y - w * z); R0[2] = 2 * (x * z + w * y);
@@ -733,7 +834,7 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
     }
-    // pelvis pose for the foot-contact geometry (float64 when F64_CONTACT)
+    // pelvis pose for the foot-contact kinematics (float64 when F64_CONTACT)
     using GeoT = typename std::conditional<F64_CONTACT, double, float>::type;
     GeoT R0g[9], p0g[3], v0g[6];
     if constexpr (F64_CONTACT) {
@@ -763,28 +864,38 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     float v15[6];
     Waist::fk(wq, wqd, v0, wj, v15);
 
-    // ---- own chain before the exchange
-    Joint cj[MAXJ];
-    float Pc[21], pc[6], tc[MAXJ], Fpel[6];
+    // ---- own chains before the exchange; contributions summed per wave
+    Joint cj[CPW][MAXJ];
+    float Pw[21], pw[6], tc[CPW][MAXJ], Fw[6];
 #pragma unroll
-    for (int m = 0; m < MAXJ; ++m) tc[m] = 0.f;
+    for (int m = 0; m < 21; ++m) Pw[m] = 0.f;
 #pragma unroll
-    for (int m = 0; m < 21; ++m) Pc[m] = 0.f;
+    for (int m = 0; m < 6; ++m) { pw[m] = 0.f; Fw[m] = 0.f; }
+    static_for<CPW>([&](auto C) {
+      constexpr int c = decltype(C)::value;
+      float Pc[21], pc[6], Fpel[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) pc[m] = 0.f;
-    switch (wave) {
-      case 0: chain_pre<0, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 1: chain_pre<1, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      case 2: chain_pre<2, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-      default: chain_pre<3, GeoT>(cf, cq, cqd, cct, mu, v0, v15, R0g, p0g, v0g, cj, anc, fn, footp, footv, Pc, pc, want_tau, tc, Fpel); break;
-    }
+      for (int m = 0; m < MAXJ; ++m) tc[c][m] = 0.f;
 #pragma unroll
-    for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pc[m];
+      for (int m = 0; m < 21; ++m) Pc[m] = 0.f;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) xch[buf][wave][21 + m][lane] = pc[m];
-    if (want_tau && wave < 2) {
+      for (int m = 0; m < 6; ++m) { pc[m] = 0.f; Fpel[m] = 0.f; }
+      dispatch_pre<GeoT>(wave * CPW + c, cf, cq[c], cqd[c], cct[c], mu, v0, v15, R0g, p0g, v0g, cj[c], anc[c],
+                         fn[c], footp[c], footv[c], Pc, pc, want_tau, tc[c], Fpel);
 #pragma unroll
-      for (int m = 0; m < 6; ++m) fcx[buf][wave][m][lane] = Fpel[m];
+      for (int m = 0; m < 21; ++m) Pw[m] += Pc[m];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) { pw[m] += pc[m]; Fw[m] += Fpel[m]; }
+    });
+#pragma unroll
+    for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pw[m];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) xch[buf][wave][21 + m][lane] = pw[m];
+    if constexpr (SPLIT == 4) {
+      if (want_tau && legs) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) fcx[buf][wave][m][lane] = Fw[m];
+      }
     }
     __syncthreads();
 
@@ -792,39 +903,51 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     float A15[21], p15[6], A0[21], pA0[6];
     init_inertia<15>(A15);
     init_inertia<0>(A0);
-#pragma unroll
-    for (int m = 0; m < 21; ++m) {
-      A15[m] += xch[buf][2][m][lane] + xch[buf][3][m][lane];
-      A0[m] += xch[buf][0][m][lane] + xch[buf][1][m][lane];
-    }
     bias_force<0>(v0, pA0);
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      p15[m] = wj[2].bias[m] + xch[buf][2][21 + m][lane] + xch[buf][3][21 + m][lane];
-      pA0[m] += xch[buf][0][21 + m][lane] + xch[buf][1][21 + m][lane];
+    for (int m = 0; m < 6; ++m) p15[m] = wj[2].bias[m];
+#pragma unroll
+    for (int w = 0; w < SPLIT; ++w) {
+      const bool to_pelvis = w * CPW < 2;
+#pragma unroll
+      for (int m = 0; m < 21; ++m) {
+        if (to_pelvis) A0[m] += xch[buf][w][m][lane];
+        else A15[m] += xch[buf][w][m][lane];
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        if (to_pelvis) pA0[m] += xch[buf][w][21 + m][lane];
+        else p15[m] += xch[buf][w][21 + m][lane];
+      }
     }
     Waist::backward(cf, A15, p15, wq, wqd, wct, wj, A0, pA0);
 
     // ---- base: IA0 a0' = -pA0 (gravity-free), forward through the waist
     float a0[6];
-    base_solve<typename std::conditional<F64_SOLVE, double, float>::type>(A0, pA0, a0);
+    if (f64_solve) base_solve<double>(A0, pA0, a0);
+    else base_solve<float>(A0, pA0, a0);
     float a15[6];
     Waist::forward(a0, wj, qdd_w, a15);
-    switch (wave) {
-      case 0: chain_post<0>(a0, a15, cj, qdd_c); break;
-      case 1: chain_post<1>(a0, a15, cj, qdd_c); break;
-      case 2: chain_post<2>(a0, a15, cj, qdd_c); break;
-      default: chain_post<3>(a0, a15, cj, qdd_c); break;
-    }
+    static_for<CPW>([&](auto C) {
+      constexpr int c = decltype(C)::value;
+      dispatch_post(wave * CPW + c, a0, a15, cj[c], qdd_c[c]);
+    });
     if (want_tau) {
+      static_for<CPW>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const int nj = chain_n(wave * CPW + c);
 #pragma unroll
-      for (int m = 0; m < MAXJ; ++m) tau_c[m] = (m < nj ? cj[m].tau : 0.f) + tc[m];
+        for (int m = 0; m < MAXJ; ++m) tau_c[c][m] = (m < nj ? cj[c][m].tau : 0.f) + tc[c][m];
+      });
 #pragma unroll
       for (int m = 0; m < 3; ++m) tau_w[m] = wj[m].tau;
       // base: world force R0 F.l, body torque F.a (convention of g1_c_fd's tau)
       float F[6];
 #pragma unroll
-      for (int m = 0; m < 6; ++m) F[m] = fcx[buf][0][m][lane] + fcx[buf][1][m][lane];
+      for (int m = 0; m < 6; ++m) {
+        if constexpr (SPLIT == 4) F[m] = fcx[buf][0][m][lane] + fcx[buf][1][m][lane];
+        else F[m] = Fw[m];            // wave 0 holds both legs (only wave 0 stores it)
+      }
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         tau_b[m] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
@@ -841,6 +964,7 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       qacc_base[2] += cf.gz;
       qacc_base[3] = a0[0]; qacc_base[4] = a0[1]; qacc_base[5] = a0[2];
     }
+
     // ---- semi-implicit Euler (src/g1_core.h:212-222)
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -850,17 +974,18 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
     }
     {
       // q <- q (x) exp(w dt / 2) (src/spatial.h:73-79)
-      const float angle = sqrtf(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]) * dt;
+      const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+      const float angle = wn * dt;
       if (angle >= 1e-12f) {
         float sh, ch;
-        sincosf(0.5f * angle, &sh, &ch);
-        const float s = sh / (angle / dt);
+        fsincos(0.5f * angle, &sh, &ch);
+        const float s = sh * frcp(angle / dt);
         const BaseT dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
         const BaseT nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
         const BaseT nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
         const BaseT ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
         const BaseT nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
-        const BaseT inv = (BaseT)1 / sqrt(nw * nw + nx * nx + ny * ny + nz * nz);
+        const BaseT inv = frsq(nw * nw + nx * nx + ny * ny + nz * nz);
         bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
       }
     }
@@ -870,10 +995,12 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       wq[m] += dt * wqd[m];
     }
 #pragma unroll
-    for (int m = 0; m < MAXJ; ++m) {
-      cqd[m] += dt * qdd_c[m];
-      cq[m] += dt * cqd[m];
-    }
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+      for (int m = 0; m < MAXJ; ++m) {
+        cqd[c][m] += dt * qdd_c[c][m];
+        cq[c][m] += dt * cqd[c][m];
+      }
   }
 
   // ---- store
@@ -909,31 +1036,46 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs args) {
       for (int m = 0; m < 3; ++m) args.qacc[(6 + 12 + m) * N + e] = qdd_w[m];
     }
   }
+  static_for<CPW>([&](auto C) {
+    constexpr int c = decltype(C)::value;
+    const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
 #pragma unroll
-  for (int m = 0; m < MAXJ; ++m) {
-    if (m < nj) {
-      args.qpos[(7 + h0 + m) * N + e] = cq[m];
-      args.qvel[(6 + h0 + m) * N + e] = cqd[m];
-      if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[m];
-      if (want_tau) args.tau[(6 + h0 + m) * N + e] = tau_c[m];
+    for (int m = 0; m < MAXJ; ++m) {
+      if (m < nj) {
+        args.qpos[(7 + h0 + m) * N + e] = cq[c][m];
+        args.qvel[(6 + h0 + m) * N + e] = cqd[c][m];
+        if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
+        if (want_tau) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
+      }
     }
-  }
-  if (wave < 2) {
-    const int a0i = wave * 8;
+    if (legs) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) args.anchor[(a0i + m) * N + e] = anc[m];
-    if (args.contact_forces) {
+      for (int m = 0; m < 8; ++m) args.anchor[(ch * 8 + m) * N + e] = anc[c][m];
+      if (args.contact_forces) {
 #pragma unroll
-      for (int m = 0; m < 4; ++m) args.contact_forces[(wave * 4 + m) * N + e] = fn[m];
+        for (int m = 0; m < 4; ++m) args.contact_forces[(ch * 4 + m) * N + e] = fn[c][m];
+      }
+      if (args.foot_pos) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) args.foot_pos[(ch * 3 + m) * N + e] = footp[c][m];
+      }
+      if (args.foot_vel) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) args.foot_vel[(ch * 3 + m) * N + e] = footv[c][m];
+      }
     }
-    if (args.foot_pos) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m) args.foot_pos[(wave * 3 + m) * N + e] = footp[m];
-    }
-    if (args.foot_vel) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m) args.foot_vel[(wave * 3 + m) * N + e] = footv[m];
-    }
+  });
+}
+
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
+  __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
+  __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
+  if constexpr (SPLIT == 4) {
+    step_body<F64_BASE, F64_CONTACT, SPLIT, -1>(args, xch, fcx);
+  } else {
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) step_body<F64_BASE, F64_CONTACT, SPLIT, 0>(args, xch, fcx);
+    else step_body<F64_BASE, F64_CONTACT, SPLIT, 1>(args, xch, fcx);
   }
 }
 
@@ -990,17 +1132,26 @@ void set_err(const char* fmt, ...) {
 
 using KernelFn = void (*)(g1k::StepArgs);
 
-template <bool C>
-KernelFn pick(bool f64s, bool f64b) {
-  return f64b ? (f64s ? g1k::step_kernel<true, true, C> : g1k::step_kernel<false, true, C>)
-              : (f64s ? g1k::step_kernel<true, false, C> : g1k::step_kernel<false, false, C>);
+// Work split: 4 waves (one limb chain each) fill the chip from 16,384 envs
+// (1,024 waves, one per SIMD); from 65,536 envs 2 waves (legs, arms) per 64
+// envs still give 2,048+ waves and repeat the pelvis/waist part half as often.
+constexpr int kSplit2MinEnvs = 65536;
+
+template <int SPLIT>
+KernelFn pick(bool f64b, bool f64c) {
+  return f64b ? (f64c ? g1k::step_kernel<true, true, SPLIT> : g1k::step_kernel<true, false, SPLIT>)
+              : (f64c ? g1k::step_kernel<false, true, SPLIT> : g1k::step_kernel<false, false, SPLIT>);
 }
 
-int launch_step(const g1k::StepArgs& a, unsigned flags, hipStream_t stream) {
-  const bool f64s = flags & G1_FLAG_F64_SOLVE, f64b = flags & G1_FLAG_F64_BASE;
-  KernelFn k = (flags & G1_FLAG_F64_CONTACT) ? pick<true>(f64s, f64b) : pick<false>(f64s, f64b);
+int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream) {
+  const bool f64b = flags & G1_FLAG_F64_BASE, f64c = flags & G1_FLAG_F64_CONTACT;
+  a.f64_solve = (flags & G1_FLAG_F64_SOLVE) ? 1 : 0;
+  int split = a.nenv >= kSplit2MinEnvs ? 2 : 4;
+  if (flags & G1_FLAG_WAVES4) split = 4;
+  if (flags & G1_FLAG_WAVES2) split = 2;
+  KernelFn k = split == 2 ? pick<2>(f64b, f64c) : pick<4>(f64b, f64c);
   const int nblocks = (a.nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG;
-  hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * split), 0, stream, a);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     set_err("step launch failed: %s", hipGetErrorString(err));
@@ -1033,7 +1184,8 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   g_err[0] = 0;
   int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
   if (rc) return rc;
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE | G1_FLAG_F64_CONTACT)) { set_err("unknown flag bits 0x%x", flags); return -1; }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2) ||
+      ((flags & G1_FLAG_WAVES4) && (flags & G1_FLAG_WAVES2))) { set_err("unknown flag bits 0x%x", flags); return -1; }
   if (((flags & G1_FLAG_F64_BASE) != 0) != (base_pose64 != nullptr)) {
     set_err("base_pose64 must be given exactly when G1_FLAG_F64_BASE is set");
     return -1;
@@ -1045,6 +1197,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl;
   a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
   a.qacc = qacc_out; a.tau = tau_out; a.mu = mu; a.base64 = base_pose64;
+  a.f64_solve = 0;
   return launch_step(a, flags, (hipStream_t)stream);
 }
 

@@ -298,3 +298,24 @@ def test_total_generalized_force(device):
     assert ok.mean() > 0.98
     assert err <= F32_FACTOR * err32
     assert np.abs(after["tau"][:6]).max() > 1.0          # contacts load the base
+
+
+@pytest.mark.parametrize("force", [_lib.G1_FLAG_WAVES2, _lib.G1_FLAG_WAVES4])
+def test_work_splits_agree(device, force):
+    """2 waves (legs | arms) and 4 waves (one chain each) per 64 envs compute
+    the same step: both within the float32 bar of the oracle."""
+    qp, qv, an, cb = W.randomized_actuation(4096, seed=12, nblocks=1)
+    st = GpuState(qp, qv, an, cb[0], device)
+    st.step(30, 2e-3, flags=force)
+    before = st.host()
+    st.step(1, 2e-3, flags=force, tau=True)
+    after = st.host()
+    ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], 2e-3,
+                           want=("qacc", "fn", "anchor", "tau"))
+    f32 = f32_reference_step(before, 2e-3)
+    ok = ~branch_flips(after["fn"], ref["fn"], after["anchor"], ref["anchor"])
+    e = rel_err(after["qacc"][:, ok], ref["qacc"][:, ok]).max()
+    e32 = rel_err(f32["qacc"][:, ok], ref["qacc"][:, ok]).max()
+    write_metrics(f"split_flag{force}", {"qacc_rel_max": float(e), "f32ref_qacc_rel_max": float(e32)})
+    assert e <= F32_FACTOR * e32
+    assert np.abs(after["tau"][:6]).max() > 1.0
