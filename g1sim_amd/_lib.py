@@ -53,6 +53,16 @@ def load():
     lib.g1_mlp_to_bf16.argtypes = [vp, vp, C.c_longlong, vp]
     lib.g1_mlp_gather.restype = i
     lib.g1_mlp_gather.argtypes = [vp, vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
+    lib.g1_mlp_features.restype = i
+    lib.g1_mlp_features.argtypes = [i, i, vp, vp, vp, vp, vp, vp, i, vp]
+    lib.g1_hip_forces.restype = i
+    lib.g1_hip_forces.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.g1_hip_judge_list.restype = i
+    lib.g1_hip_judge_list.argtypes = [i, i, i, vp, vp, vp, vp]
+    lib.g1_hip_step_judge.restype = i
+    lib.g1_hip_step_judge.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
+    lib.g1_hip_surrogate_integrate.restype = i
+    lib.g1_hip_surrogate_integrate.argtypes = [i, f, i, i, f, vp, vp, vp, i, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_hip_last_error.restype = C.c_char_p
     lib.g1_hip_last_error.argtypes = []
     lib.g1_hip_version.restype = i
@@ -64,7 +74,8 @@ def load():
 def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
             "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_adam",
-            "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather"]
+            "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
+            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_surrogate_integrate"]
 
 
 def last_error():

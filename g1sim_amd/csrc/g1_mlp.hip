@@ -235,6 +235,32 @@ __global__ void gather_kernel(const float* X, const float* Y, const int* idx, in
   }
 }
 
+// Network input of every env of a SoA state: z-scored (qpos 36, qvel 35,
+// tau 35) as bf16 row e of Xn [nrows][ldx]; rows >= nenv and columns >= 106
+// are zero.  One thread per row.
+__global__ __launch_bounds__(256) void features_kernel(int nenv, int nrows, const float* qpos, const float* qvel,
+                                                       const float* tau, const float* mean, const float* inv,
+                                                       bf16* Xn, int ldx) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nrows) return;
+  const size_t N = (size_t)nenv;
+  bf16* row = Xn + (size_t)e * ldx;
+  for (int f0 = 0; f0 < ldx; f0 += 8) {
+    bf16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f = f0 + q;
+      float x = 0.f;
+      if (e < nenv && f < 106) {
+        const float raw = f < 36 ? qpos[f * N + e] : f < 71 ? qvel[(f - 36) * N + e] : tau[(f - 71) * N + e];
+        x = (raw - mean[f]) * inv[f];
+      }
+      v[q] = (bf16)x;
+    }
+    *reinterpret_cast<bf16x8*>(row + f0) = v;
+  }
+}
+
 }  // namespace g1mlp
 
 // ================================================================== C-ABI
@@ -321,6 +347,15 @@ int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfe
   if (B <= 0 || nfeat > ldx || (Yn && nout > ldy) || (!idx && B > nenv)) return -1;
   hipLaunchKernelGGL(gather_kernel, dim3(B), dim3(128), 0, (hipStream_t)stream, X, Y, idx, B, nfeat, nout, nenv,
                      xmean, xinv, ymean, yinv, (bf16*)Xn, ldx, Yn, ldy);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, const float* tau, const float* mean,
+                    const float* inv, void* Xn, int ldx, void* stream) {
+  if (nenv < 0 || nrows < nenv || ldx < 106 || ldx % 8 || !qpos || !qvel || !tau || !mean || !inv || !Xn) return -1;
+  if (nrows == 0) return 0;
+  hipLaunchKernelGGL(features_kernel, dim3((nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv, nrows,
+                     qpos, qvel, tau, mean, inv, (bf16*)Xn, ldx);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -370,21 +370,29 @@ G1K_DEV void world_pose(const T_ (&Rp)[9], const T_ (&pp)[3], T_ c, T_ s, T_ (&R
 // Backward step of hinge body B into its parent: PD servo + soft limits +
 // implicit damping (src/g1_core.h:156-186).  A/pA (the body's articulated
 // inertia and bias) are consumed.
-template <int B>
-G1K_DEV void backward_joint(const Cfg& cf, float (&A)[21], float (&pA)[6], float q, float qd, float ctl,
-                            Joint& jt, float (&Ppar)[21], float (&pApar)[6]) {
-  constexpr int h = B - 1, K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+// Servo + soft-limit torque of hinge h and its velocity coefficient b for
+// the implicit damping term dt * b (src/g1_core.h:166-176).
+template <int h> G1K_DEV float servo_torque(const Cfg& cf, float q, float qd, float ctl, float& b) {
   constexpr float kp = (float)g1m_act_kp[h], kv = (float)g1m_act_kv[h];
   constexpr float lim = (float)g1m_act_frclim[h];
   constexpr float lo = (float)g1m_jnt_range[2 * h], hi = (float)g1m_jnt_range[2 * h + 1];
-  constexpr float arm = (float)g1m_armature[h];
   float t = kp * (ctl - q) - kv * qd;
   t = fminf(fmaxf(t, -lim), lim);
-  float b = kv;
+  b = kv;
   const bool below = q < lo, above = q > hi;
   const float lt = cf.limit_kp * ((below ? lo : hi) - q) - cf.limit_kd * qd;
   t += (below || above) ? lt : 0.f;
   b += (below || above) ? cf.limit_kd : 0.f;
+  return t;
+}
+
+template <int B>
+G1K_DEV void backward_joint(const Cfg& cf, float (&A)[21], float (&pA)[6], float q, float qd, float ctl,
+                            Joint& jt, float (&Ppar)[21], float (&pApar)[6]) {
+  constexpr int h = B - 1, K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
+  constexpr float arm = (float)g1m_armature[h];
+  float b;
+  const float t = servo_torque<h>(cf, q, qd, ctl, b);
   jt.tau = t;
 
   float U[6];
@@ -465,8 +473,10 @@ template <int FIRST, int N> struct ChainOps {
         for (int m = 0; m < 9; ++m) Rl[m] = R[m];
 #pragma unroll
         for (int m = 0; m < 3; ++m) pl[m] = p[m];
+        if constexpr (sizeof(T_) != sizeof(float)) {   // float: the caller's twists stand
 #pragma unroll
-        for (int m = 0; m < 6; ++m) vl[m] = v[m];
+          for (int m = 0; m < 6; ++m) vl[m] = v[m];
+        }
       }
     }
   }
@@ -636,6 +646,11 @@ struct StepArgs {
   const float* mu;
   double* base64;
   int f64_solve;
+  // judge mode: step the envs env_list[0 .. *list_count) of a batch of
+  // `stride` envs once, store only their qacc (state is read, not written)
+  const int* env_list;
+  const int* list_count;
+  int stride;
 };
 
 // Per-lane chain slots (legs use 6 of 7).
@@ -747,11 +762,14 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   const int lane = threadIdx.x & 63;
   const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nenv = args.nenv;
-  const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
-  const bool valid = e_raw < nenv;
-  const int e = valid ? e_raw : nenv - 1;      // tail lanes mirror the last env, never store
-  const size_t N = (size_t)nenv;
+  const bool judge = args.env_list != nullptr;
+  const int nenv = judge ? *args.list_count : args.nenv;
+  if (judge && (int)(blockIdx.x * ENVS_PER_WG) >= nenv) return;   // uniform per workgroup
+  const int i_raw = blockIdx.x * ENVS_PER_WG + lane;
+  const bool valid = i_raw < nenv;
+  const int i_env = valid ? i_raw : nenv - 1;  // tail lanes mirror the last env, never store
+  const int e = judge ? args.env_list[i_env] : i_env;
+  const size_t N = judge ? (size_t)args.stride : (size_t)nenv;
   const Cfg cf = args.cfg;
   const bool legs = wave * CPW < 2;            // this wave owns the leg chains
 
@@ -1005,6 +1023,22 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 
   // ---- store
   if (!valid || args.nsub <= 0) return;
+  if (judge) {
+    if (wave == 0) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) args.qacc[m * N + e] = qacc_base[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) args.qacc[(6 + 12 + m) * N + e] = qdd_w[m];
+    }
+    static_for<CPW>([&](auto C) {
+      constexpr int c = decltype(C)::value;
+      const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
+#pragma unroll
+      for (int m = 0; m < MAXJ; ++m)
+        if (m < nj) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
+    });
+    return;
+  }
   if (wave == 0) {
 #pragma unroll
     for (int m = 0; m < 3; ++m) args.qpos[m * N + e] = (float)bp[m];
@@ -1076,6 +1110,204 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
   } else {
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) step_body<F64_BASE, F64_CONTACT, SPLIT, 0>(args, xch, fcx);
     else step_body<F64_BASE, F64_CONTACT, SPLIT, 1>(args, xch, fcx);
+  }
+}
+
+// ------------------------------------------------------- surrogate support
+// Total generalized force without ABA: pelvis and leg kinematics, the 8 foot
+// contacts (anchors and normal forces updated exactly as by the step),
+// J^T f of the contact wrenches and the servo + limit torques of all 29
+// hinges.  One lane per env.
+struct ForcesArgs {
+  int nenv;
+  Cfg cfg;
+  const float* qpos;
+  const float* qvel;
+  float* anchor;
+  const float* ctrl;
+  const float* mu;
+  float* tau;
+  float* contact_forces;
+};
+
+template <int h0, int n> G1K_DEV void servo_range(const Cfg& cf, const float* q, const float* qd, const float* ct,
+                                                  float* tau) {
+  static_for<n>([&](auto H) {
+    constexpr int i = decltype(H)::value;
+    float b;
+    tau[i] = servo_torque<h0 + i>(cf, q[i], qd[i], ct[i], b);
+  });
+}
+
+__global__ __launch_bounds__(256) void forces_kernel(ForcesArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nenv) return;
+  const size_t N = (size_t)a.nenv;
+  const Cfg cf = a.cfg;
+  float q[NU], qd[NU], ct[NU], tau[NV];
+#pragma unroll
+  for (int h = 0; h < NU; ++h) {
+    q[h] = a.qpos[(7 + h) * N + e];
+    qd[h] = a.qvel[(6 + h) * N + e];
+    ct[h] = a.ctrl[h * N + e];
+  }
+  float bq[4], p0[3], vw[3], om[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    p0[m] = a.qpos[m * N + e];
+    vw[m] = a.qvel[m * N + e];
+    om[m] = a.qvel[(3 + m) * N + e];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) bq[m] = a.qpos[(3 + m) * N + e];
+  const float inv = frsq(bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2] + bq[3] * bq[3]);
+  const float w = bq[0] * inv, x = bq[1] * inv, y = bq[2] * inv, z = bq[3] * inv;
+  float R0[9], v0[6];
+  R0[0] = 1 - 2 * (y * y + z * z); R0[1] = 2 * (x * y - w * z); R0[2] = 2 * (x * z + w * y);
+  R0[3] = 2 * (x * y + w * z); R0[4] = 1 - 2 * (x * x + z * z); R0[5] = 2 * (y * z - w * x);
+  R0[6] = 2 * (x * z - w * y); R0[7] = 2 * (y * z + w * x); R0[8] = 1 - 2 * (x * x + y * y);
+  v0[0] = om[0]; v0[1] = om[1]; v0[2] = om[2];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
+  const float mu = a.mu ? a.mu[e] : cf.mu;
+  servo_range<0, NU>(cf, q, qd, ct, tau + 6);
+  float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  static_for<2>([&](auto L) {
+    constexpr int leg = decltype(L)::value;
+    using Ch = ChainOps<WaveChain<leg>::first, 6>;
+    Joint jt[6];
+    float vleaf[6], Rf[9], pf[3], Fb[6], anc[8], fn[4], tc[6];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) anc[m] = a.anchor[(leg * 8 + m) * N + e];
+    Ch::fk(q + 6 * leg, qd + 6 * leg, v0, jt, vleaf);
+    Ch::world(R0, p0, v0, jt, q + 6 * leg, qd + 6 * leg, Rf, pf, vleaf);   // float: vleaf kept
+    foot_contacts<WaveChain<leg>::k0>(cf, mu, Rf, pf, vleaf, anc, fn, Fb);
+    Ch::wrench_to_joints(Fb, jt, tc);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      tau[6 + 6 * leg + m] += tc[m];
+      F[m] += Fb[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.anchor[(leg * 8 + m) * N + e] = anc[m];
+    if (a.contact_forces) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a.contact_forces[(leg * 4 + m) * N + e] = fn[m];
+    }
+  });
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    tau[m] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
+    tau[3 + m] = F[m];
+  }
+#pragma unroll
+  for (int m = 0; m < NV; ++m) a.tau[m * N + e] = tau[m];
+}
+
+// The envs whose qacc ABA computes this step: every period-th env starting at
+// `phase` (the rotating judge) plus every env in fallback.
+__global__ void judge_list_kernel(int nenv, int phase, int period, const uint8_t* fallback, int* list, int* count) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nenv) return;
+  if ((e % period) == phase || (fallback && fallback[e])) list[atomicAdd(count, 1)] = e;
+}
+
+__global__ void zero_int_kernel(int* x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+// Applies qacc and integrates (semi-implicit Euler as the step kernel):
+// envs in the judge list take ABA's qacc; judged envs compare the surrogate's
+// qacc with ABA's (rel. error |a - b| / max(|b|, 1), max over dofs) and enter
+// or leave fallback on `tol`; all other envs take the surrogate's qacc
+// (pred is the network's z-scored output, row e of [nenv][ldp]).
+// stats[0] judged, stats[1] rejected, stats[2] max judged error (float bits),
+// stats[3] envs in fallback that used ABA.
+struct IntegrateArgs {
+  int nenv, phase, period;
+  float dt, tol;
+  float* qpos;
+  float* qvel;
+  const float* pred;
+  int ldp;
+  const float* ymean;
+  const float* ystd;
+  const float* qacc_aba;
+  uint8_t* fallback;
+  unsigned* stats;
+  float* qacc_out;
+};
+
+__global__ __launch_bounds__(256) void surrogate_integrate_kernel(IntegrateArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nenv) return;
+  const size_t N = (size_t)a.nenv;
+  const bool judged = (e % a.period) == a.phase;
+  const bool fb = a.fallback && a.fallback[e];
+  float qa[NV];
+  float err = 0.f;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const float sur = a.pred[(size_t)e * a.ldp + j] * a.ystd[j] + a.ymean[j];
+    float v = sur;
+    if (judged || fb) {
+      const float ref = a.qacc_aba[j * N + e];
+      err = fmaxf(err, fabsf(sur - ref) * frcp(fmaxf(fabsf(ref), 1.0f)));
+      v = ref;
+    }
+    qa[j] = v;
+  }
+  if (judged) {
+    const bool reject = !(err <= a.tol);
+    if (a.fallback) a.fallback[e] = reject ? 1 : 0;
+    atomicAdd(&a.stats[0], 1u);
+    if (reject) atomicAdd(&a.stats[1], 1u);
+    atomicMax(&a.stats[2], __float_as_uint(err));
+  } else if (fb) {
+    atomicAdd(&a.stats[3], 1u);
+  }
+  if (a.qacc_out) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a.qacc_out[j * N + e] = qa[j];
+  }
+  const float dt = a.dt;
+  float vw[3], om[3], bp[3], bq[4];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    vw[m] = a.qvel[m * N + e] + dt * qa[m];
+    om[m] = a.qvel[(3 + m) * N + e] + dt * qa[3 + m];
+    bp[m] = a.qpos[m * N + e] + dt * vw[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) bq[m] = a.qpos[(3 + m) * N + e];
+  const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+  const float angle = wn * dt;
+  if (angle >= 1e-12f) {
+    float sh, ch;
+    fsincos(0.5f * angle, &sh, &ch);
+    const float s = sh * frcp(angle / dt);
+    const float dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
+    const float nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
+    const float nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
+    const float ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
+    const float nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
+    const float inv = frsq(nw * nw + nx * nx + ny * ny + nz * nz);
+    bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    a.qvel[m * N + e] = vw[m];
+    a.qvel[(3 + m) * N + e] = om[m];
+    a.qpos[m * N + e] = bp[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) a.qpos[(3 + m) * N + e] = bq[m];
+#pragma unroll
+  for (int h = 0; h < NU; ++h) {
+    const float v = a.qvel[(6 + h) * N + e] + dt * qa[6 + h];
+    a.qvel[(6 + h) * N + e] = v;
+    a.qpos[(7 + h) * N + e] = a.qpos[(7 + h) * N + e] + dt * v;
   }
 }
 
@@ -1198,6 +1430,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
   a.qacc = qacc_out; a.tau = tau_out; a.mu = mu; a.base64 = base_pose64;
   a.f64_solve = 0;
+  a.env_list = nullptr; a.list_count = nullptr; a.stride = nenv;
   return launch_step(a, flags, (hipStream_t)stream);
 }
 
@@ -1206,6 +1439,86 @@ void g1_cuda_step(int nenv, int nsub, float* qpos, float* qvel, float* anchor, c
                   float* foot_pos, float* foot_vel, float* contact_forces, void* stream) {
   g1_hip_step_ex(nenv, nsub, 1e-3f, qpos, qvel, anchor, ctrl, foot_pos, foot_vel, contact_forces, nullptr,
                  nullptr, nullptr, nullptr, 0u, stream);
+}
+
+int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor, const float* ctrl,
+                  const float* mu, float* tau_out, float* contact_forces, void* stream) {
+  g_err[0] = 0;
+  if (nenv < 0 || !(dt > 0.f)) { set_err("forces: bad nenv/dt"); return -1; }
+  if (nenv == 0) return 0;
+  if (!qpos || !qvel || !anchor || !ctrl || !tau_out) { set_err("forces: qpos, qvel, anchor, ctrl, tau required"); return -1; }
+  g1k::ForcesArgs a;
+  a.nenv = nenv; a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl; a.mu = mu; a.tau = tau_out;
+  a.contact_forces = contact_forces;
+  hipLaunchKernelGGL(g1k::forces_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) { set_err("forces launch failed: %s", hipGetErrorString(err)); return -2; }
+  return 0;
+}
+
+int g1_hip_judge_list(int nenv, int phase, int period, const uint8_t* fallback, int* list, int* count,
+                      void* stream) {
+  g_err[0] = 0;
+  if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !list || !count) {
+    set_err("judge_list: bad arguments");
+    return -1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(g1k::zero_int_kernel, dim3(1), dim3(64), 0, s, count, 1);
+  if (nenv > 0)
+    hipLaunchKernelGGL(g1k::judge_list_kernel, dim3((nenv + 255) / 256), dim3(256), 0, s, nenv, phase, period,
+                       fallback, list, count);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) { set_err("judge_list launch failed: %s", hipGetErrorString(err)); return -2; }
+  return 0;
+}
+
+int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, const float* anchor,
+                      const float* ctrl, const float* mu, const int* list, const int* count, float* qacc_out,
+                      unsigned flags, void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (!list || !count || !qacc_out) { set_err("step_judge: list, count and qacc are required"); return -1; }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("step_judge: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  g1k::StepArgs a;
+  a.nenv = nenv; a.nsub = 1;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = const_cast<float*>(qpos); a.qvel = const_cast<float*>(qvel); a.anchor = const_cast<float*>(anchor);
+  a.ctrl = ctrl;
+  a.foot_pos = nullptr; a.foot_vel = nullptr; a.contact_forces = nullptr;
+  a.qacc = qacc_out; a.tau = nullptr; a.mu = mu; a.base64 = nullptr;
+  a.f64_solve = 0;
+  a.env_list = list; a.list_count = count; a.stride = nenv;
+  return launch_step(a, flags, (hipStream_t)stream);
+}
+
+int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
+                               const float* pred, int ldp, const float* ymean, const float* ystd,
+                               const float* qacc_aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                               void* stream) {
+  g_err[0] = 0;
+  if (nenv < 0 || !(dt > 0.f) || period <= 0 || phase < 0 || phase >= period || ldp < 35) {
+    set_err("surrogate_integrate: bad arguments");
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  if (!qpos || !qvel || !pred || !ymean || !ystd || !qacc_aba || !stats) {
+    set_err("surrogate_integrate: null pointer");
+    return -1;
+  }
+  g1k::IntegrateArgs a{nenv, phase, period, dt, tol, qpos, qvel, pred, ldp, ymean, ystd, qacc_aba, fallback,
+                       stats, qacc_out};
+  hipLaunchKernelGGL(g1k::surrogate_integrate_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     a);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) { set_err("surrogate_integrate launch failed: %s", hipGetErrorString(err)); return -2; }
+  return 0;
 }
 
 void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const uint8_t* done, unsigned seed,

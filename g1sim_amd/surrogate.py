@@ -200,3 +200,161 @@ class MLP:
         return {"W": [w.cpu() for w in self.W], "b": [b.cpu() for b in self.b],
                 "xmean": self.xmean.cpu(), "xinv": self.xinv.cpu(), "ymean": self.ymean.cpu(),
                 "yinv": self.yinv.cpu()}
+
+
+# ====================================================================== data
+class Dataset:
+    """Transitions of the ABA step kernel: X [steps][106][nenv] = (qpos, qvel,
+    tau_total) before each step, Y [steps][35][nenv] = the step's qacc."""
+
+    def __init__(self, X, Y):
+        self.X, self.Y = X, Y
+        self.steps, _, self.nenv = X.shape
+
+    @property
+    def n(self):
+        return self.steps * self.nenv
+
+
+def generate_dataset(device, nenv=16384, steps=1221, dt=2e-3, seed=7, reset_below=0.35):
+    """Randomized-actuation rollouts (targets stand + U(+-0.3) resampled every
+    10 steps) stepped by g1_hip_step_ex one step at a time with its qacc and
+    tau outputs; envs whose pelvis drops below `reset_below` m are reset by
+    g1_cuda_reset_done with fresh noise."""
+    from . import workloads as W
+    lib = _lib.load()
+    dev = torch.device(device)
+    nblocks = (steps + 9) // 10
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=seed, nblocks=nblocks)
+    qpos = torch.as_tensor(qp, device=dev)
+    qvel = torch.as_tensor(qv, device=dev)
+    anchor = torch.as_tensor(an, device=dev)
+    ctrl = torch.as_tensor(cb, device=dev)
+    X = torch.empty((steps, NIN, nenv), dtype=torch.float32, device=dev)
+    Y = torch.empty((steps, NOUT, nenv), dtype=torch.float32, device=dev)
+    done = torch.zeros(nenv, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    for s in range(steps):
+        X[s, :NQ].copy_(qpos)
+        X[s, NQ:NQ + NV].copy_(qvel)
+        rc = lib.g1_hip_step_ex(nenv, 1, dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                ctrl[s // 10].data_ptr(), None, None, None, Y[s].data_ptr(),
+                                X[s, NQ + NV:].data_ptr(), None, None, 0, stream)
+        _lib.check(rc)
+        if s % 50 == 49:
+            torch.lt(qpos[2], reset_below, out=done)
+            lib.g1_cuda_reset_done(nenv, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), done.data_ptr(),
+                                   (seed * 1000003 + s) & 0xFFFFFFFF, 0.3, 0.02, stream)
+    return Dataset(X, Y)
+
+
+def fit_normalization(mlp, ds, sample=2_000_000, seed=0):
+    g = torch.Generator(device=ds.X.device).manual_seed(seed)
+    k = min(sample, ds.n)
+    st = torch.randint(0, ds.steps, (k,), device=ds.X.device, generator=g)
+    en = torch.randint(0, ds.nenv, (k,), device=ds.X.device, generator=g)
+    xs = ds.X[st, :, en].double()
+    ys = ds.Y[st, :, en].double()
+    mlp.set_normalization(xs.mean(0), xs.std(0), ys.mean(0), ys.std(0))
+
+
+def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=None, seed=0, log=None):
+    """Minibatch Adam on the steps [0, steps - holdout); returns per-iteration
+    mean z-scored MSE every 100 iterations."""
+    dev = ds.X.device
+    holdout = holdout_steps if holdout_steps is not None else max(1, ds.steps // 20)
+    train_steps = ds.steps - holdout
+    g = torch.Generator(device=dev).manual_seed(seed)
+    ws = mlp.workspace(batch)
+    idx = torch.empty((batch, 2), dtype=torch.int32, device=dev)
+    hist = []
+    for it in range(iters):
+        idx[:, 0] = torch.randint(0, train_steps, (batch,), device=dev, generator=g, dtype=torch.int32)
+        idx[:, 1] = torch.randint(0, ds.nenv, (batch,), device=dev, generator=g, dtype=torch.int32)
+        mlp.gather(ds.X, ds.Y, idx, batch, ds.nenv, ws["X16"], ws["T"])
+        a = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * it / max(1, iters - 1)))
+        sse = mlp.train_step(ws["X16"], ws["T"], batch, a)
+        if it % 100 == 0 or it == iters - 1:
+            v = float(sse.item()) / (batch * NOUT)
+            hist.append((it, v))
+            if log:
+                log(f"iter {it} lr {a:.2e} mse {v:.4e}")
+    return hist
+
+
+def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
+    """Held-out accuracy in physical units: rel. qacc error |a - b| / max(|b|, 1)
+    (max over dofs) of the surrogate against ABA; returns a dict."""
+    dev = ds.X.device
+    s0 = steps_from if steps_from is not None else ds.steps - max(1, ds.steps // 20)
+    B = (nsamp + 127) // 128 * 128
+    g = torch.Generator(device=dev).manual_seed(seed)
+    idx = torch.empty((B, 2), dtype=torch.int32, device=dev)
+    idx[:, 0] = torch.randint(s0, ds.steps, (B,), device=dev, generator=g, dtype=torch.int32)
+    idx[:, 1] = torch.randint(0, ds.nenv, (B,), device=dev, generator=g, dtype=torch.int32)
+    ws = mlp.workspace(B)
+    mlp.gather(ds.X, ds.Y, idx, B, ds.nenv, ws["X16"], ws["T"])
+    pred = mlp.forward(ws["X16"], B, train=False)[:, :NOUT]
+    ystd = 1.0 / mlp.yinv
+    qs = pred * ystd + mlp.ymean
+    qa = ws["T"][:, :NOUT] * ystd + mlp.ymean
+    err = ((qs - qa).abs() / qa.abs().clamp_min(1.0)).max(dim=1).values
+    e = err.double().cpu().numpy()
+    return {"samples": int(B), "accept_frac": float((e <= tol).mean()), "median_rel_err": float(np.median(e)),
+            "p90_rel_err": float(np.percentile(e, 90)), "max_rel_err": float(e.max()), "tol": tol}
+
+
+# ====================================================================== step
+class SurrogateStepper:
+    """qacc from the network for most envs, ABA as judge on a rotating
+    1/period of the envs every step and as per-env fallback after a rejected
+    judgement (g1_hip_surrogate_integrate)."""
+
+    def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0):
+        self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
+        self.lib = _lib.load()
+        self.dev = torch.device(device)
+        self.rows = (nenv + 127) // 128 * 128
+        f32 = dict(dtype=torch.float32, device=self.dev)
+        self.tau = torch.zeros((NV, nenv), **f32)
+        self.qacc_aba = torch.zeros((NV, nenv), **f32)
+        self.list = torch.zeros(nenv, dtype=torch.int32, device=self.dev)
+        self.count = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.fallback = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
+        self.stats = torch.zeros(4, dtype=torch.int32, device=self.dev)
+        self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
+        self.pred = torch.zeros((self.rows, POUT), **f32)
+        self.ystd = (1.0 / mlp.yinv).contiguous()
+        self.judge_flags = judge_flags
+        self.t = 0
+
+    def step(self, qpos, qvel, anchor, ctrl, mu=None):
+        lib, s, n = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
+        phase = self.t % self.period
+        ck = _lib.check
+        ck(lib.g1_hip_judge_list(n, phase, self.period, self.fallback.data_ptr(), self.list.data_ptr(),
+                                 self.count.data_ptr(), s), "judge_list")
+        ck(lib.g1_hip_step_judge(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
+                                 None if mu is None else mu.data_ptr(), self.list.data_ptr(), self.count.data_ptr(),
+                                 self.qacc_aba.data_ptr(), self.judge_flags, s), "step_judge")
+        ck(lib.g1_hip_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
+                             None if mu is None else mu.data_ptr(), self.tau.data_ptr(), None, s), "forces")
+        rc = lib.g1_mlp_features(n, self.rows, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
+                                 self.mlp.xmean.data_ptr(), self.mlp.xinv.data_ptr(), self.X16.data_ptr(), PIN, s)
+        ck(rc, "features")
+        self.mlp.forward(self.X16, self.rows, train=False, out=self.pred)
+        ck(lib.g1_hip_surrogate_integrate(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
+                                          self.pred.data_ptr(), POUT, self.mlp.ymean.data_ptr(), self.ystd.data_ptr(),
+                                          self.qacc_aba.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(),
+                                          None, s), "surrogate_integrate")
+        self.t += 1
+
+    def summary(self):
+        st = self.stats.cpu().numpy().astype(np.uint32)
+        judged, rejected, fb = int(st[0]), int(st[1]), int(st[3])
+        maxerr = float(np.array([st[2]], np.uint32).view(np.float32)[0])
+        total = self.t * self.nenv
+        return {"env_steps": total, "judged": judged, "rejected": rejected,
+                "reject_frac": rejected / max(judged, 1), "fallback_env_steps": fb,
+                "aba_frac": (judged + fb) / max(total, 1), "max_judged_rel_err": maxerr,
+                "fallback_envs_now": int(self.fallback.sum().item())}

@@ -115,6 +115,48 @@ int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfe
                   const float* xmean, const float* xinv, const float* ymean, const float* yinv,
                   void* Xn, int ldx, float* Yn, int ldy, void* stream);
 
+/* Network input of every env: z-scored (qpos, qvel, tau) as bf16 rows of
+ * Xn [nrows][ldx] (nrows >= nenv, padding rows and columns >= 106 zero;
+ * ldx a multiple of 8).  mean / inv: [106] feature means and reciprocal
+ * standard deviations. */
+int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, const float* tau,
+                    const float* mean, const float* inv, void* Xn, int ldx, void* stream);
+
+/* ---- surrogate step with ABA judge and fallback (no reference counterpart) */
+
+/* Total generalized force (servo + limits + contact J^T f, as tau_out of
+ * g1_hip_step_ex) and the contact update (anchors, normal forces) of one step,
+ * without ABA and without integration: the surrogate's input.  mu and
+ * contact_forces are nullable. */
+int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor,
+                  const float* ctrl, const float* mu, float* tau_out, float* contact_forces,
+                  void* stream);
+/* Device list of the envs that ABA steps this step: e % period == phase (the
+ * rotating judge) or fallback[e] != 0 (nullable).  list: int [nenv]; *count
+ * is reset and filled on the device. */
+int g1_hip_judge_list(int nenv, int phase, int period, const uint8_t* fallback, int* list, int* count,
+                      void* stream);
+/* ABA qacc (one substep at dt, state read-only) of the listed envs of a batch
+ * of nenv, written to qacc_out [35][nenv] at their columns.  The contact
+ * anchors are read, not updated.  flags: G1_FLAG_F64_SOLVE,
+ * G1_FLAG_F64_CONTACT, G1_FLAG_WAVES4/2. */
+int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, const float* anchor,
+                      const float* ctrl, const float* mu, const int* list, const int* count, float* qacc_out,
+                      unsigned flags, void* stream);
+/* Applies qacc and integrates one step (semi-implicit Euler as the fused
+ * step).  pred: the network's z-scored output, row e of [>= nenv][ldp];
+ * qacc = pred * ystd + ymean, except for envs judged this step
+ * (e % period == phase) or in fallback, which take qacc_aba.  A judged env
+ * whose surrogate qacc differs from ABA's by more than tol (max over dofs of
+ * |a - b| / max(|b|, 1)) enters fallback (ABA every step) until a later judge
+ * accepts it.  stats (unsigned [4], accumulated): judged, rejected, max
+ * judged error (float bits), fallback env-steps.  fallback and qacc_out are
+ * nullable. */
+int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos,
+                               float* qvel, const float* pred, int ldp, const float* ymean,
+                               const float* ystd, const float* qacc_aba, uint8_t* fallback,
+                               unsigned* stats, float* qacc_out, void* stream);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* g1_hip_last_error(void);
 

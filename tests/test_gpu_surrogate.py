@@ -1,0 +1,173 @@
+"""The learned forward-dynamics surrogate on the GPU: its force kernel and
+ABA judge against the fused step, the integrate/fallback logic, and what a
+trained network gains over the untrained one and over ABA alone."""
+import numpy as np
+import pytest
+import torch
+
+from g1sim_amd import _lib, workloads as W
+from g1sim_amd.surrogate import (MLP, NOUT, POUT, SurrogateStepper, evaluate, fit_normalization,
+                                 generate_dataset, train)
+
+from .gpu_helpers import GpuState, write_metrics
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(device, n, seed, steps=40):
+    qp, qv, an, cb = W.randomized_actuation(n, seed=seed, nblocks=1)
+    st = GpuState(qp, qv, an, cb[0], device)
+    st.step(steps, 2e-3)
+    torch.cuda.synchronize()
+    return st
+
+
+def test_forces_kernel_matches_step(device):
+    """g1_hip_forces gives the step's tau_total, anchors and normal forces."""
+    st = _state(device, 4096, 21)
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    anchor2 = st.anchor.clone()
+    tau = torch.zeros((35, st.n), device=device)
+    fn = torch.zeros((8, st.n), device=device)
+    rc = lib.g1_hip_forces(st.n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), anchor2.data_ptr(),
+                           st.ctrl.data_ptr(), None, tau.data_ptr(), fn.data_ptr(), s)
+    _lib.check(rc)
+    st.step(1, 2e-3, tau=True)
+    h = st.host()
+    t = tau.cpu().numpy()
+    scale = np.maximum(np.abs(h["tau"]), 1.0)
+    assert (np.abs(t - h["tau"]) / scale).max() < 1e-3
+    assert np.abs(fn.cpu().numpy() - h["fn"]).max() < 1e-2
+    a2 = anchor2.cpu().numpy()
+    assert np.array_equal(a2 >= 1e29, h["anchor"] >= 1e29)
+    fin = h["anchor"] < 1e29
+    assert np.abs(a2[fin] - h["anchor"][fin]).max(initial=0) < 1e-5
+
+
+def test_judge_matches_step_qacc(device):
+    """The judge mode steps exactly the listed envs, reads the state and
+    writes only their qacc."""
+    st = _state(device, 5000, 22)
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    fallback = torch.zeros(st.n, dtype=torch.uint8, device=device)
+    fallback[::97] = 1
+    lst = torch.zeros(st.n, dtype=torch.int32, device=device)
+    cnt = torch.zeros(1, dtype=torch.int32, device=device)
+    _lib.check(lib.g1_hip_judge_list(st.n, 3, 16, fallback.data_ptr(), lst.data_ptr(), cnt.data_ptr(), s))
+    qacc = torch.full((35, st.n), 1e9, device=device)
+    before = st.host()
+    _lib.check(lib.g1_hip_step_judge(st.n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
+                                     st.ctrl.data_ptr(), None, lst.data_ptr(), cnt.data_ptr(), qacc.data_ptr(), 0, s))
+    torch.cuda.synchronize()
+    after = st.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(before[k], after[k]), k
+    c = int(cnt.item())
+    e = np.sort(lst[:c].cpu().numpy())
+    expect = np.sort(np.union1d(np.arange(3, st.n, 16), np.nonzero(fallback.cpu().numpy())[0]))
+    assert np.array_equal(e, expect)
+    st.step(1, 2e-3)
+    ref = st.host()["qacc"]
+    q = qacc.cpu().numpy()
+    assert np.array_equal(q[:, e], ref[:, e])
+    others = np.setdiff1d(np.arange(st.n), e)
+    assert (q[:, others] == 1e9).all()
+
+
+def _stepper_with_oracle_pred(device, n, seed, corrupt=None):
+    """A stepper whose 'network' output is ABA's own qacc (z-scored), or a
+    corrupted copy, to test the integrate/judge/fallback logic alone."""
+    st = _state(device, n, seed)
+    mlp = MLP(device, seed=0)
+    stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e-3)
+    ref = GpuState(st.qpos.cpu().numpy(), st.qvel.cpu().numpy(), st.anchor.cpu().numpy(), st.ctrl.cpu().numpy(),
+                   device)
+    ref.step(1, 2e-3)
+    qa = ref.qacc.clone()
+    if corrupt is not None:
+        qa[:, corrupt] += 1.0
+    stepper.pred[:n, :NOUT] = ((qa - mlp.ymean[:, None]) * mlp.yinv[:, None]).T
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    _lib.check(lib.g1_hip_judge_list(n, 0, 16, stepper.fallback.data_ptr(), stepper.list.data_ptr(),
+                                     stepper.count.data_ptr(), s))
+    _lib.check(lib.g1_hip_step_judge(n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
+                                     st.ctrl.data_ptr(), None, stepper.list.data_ptr(), stepper.count.data_ptr(),
+                                     stepper.qacc_aba.data_ptr(), 0, s))
+    _lib.check(lib.g1_hip_surrogate_integrate(n, 2e-3, 0, 16, 1e-3, st.qpos.data_ptr(), st.qvel.data_ptr(),
+                                              stepper.pred.data_ptr(), POUT, mlp.ymean.data_ptr(),
+                                              stepper.ystd.data_ptr(), stepper.qacc_aba.data_ptr(),
+                                              stepper.fallback.data_ptr(), stepper.stats.data_ptr(), None, s))
+    torch.cuda.synchronize()
+    return st, ref, stepper
+
+
+def test_integrate_with_exact_prediction(device):
+    st, ref, stepper = _stepper_with_oracle_pred(device, 4096, 23)
+    a, b = st.host(), ref.host()
+    assert np.abs(a["qpos"] - b["qpos"]).max() < 1e-5
+    assert np.abs(a["qvel"] - b["qvel"]).max() < 1e-3
+    sm = stepper.stats.cpu().numpy()
+    assert sm[0] == 4096 // 16 and sm[1] == 0
+    assert int(stepper.fallback.sum()) == 0
+
+
+def test_judge_rejects_and_falls_back(device):
+    bad = np.arange(0, 4096, 2)                      # every other env predicted wrong
+    st, ref, stepper = _stepper_with_oracle_pred(device, 4096, 24, corrupt=bad)
+    a, b = st.host(), ref.host()
+    fb = stepper.fallback.cpu().numpy().astype(bool)
+    judged = np.arange(0, 4096, 16)
+    assert fb[judged].all()                          # judged envs 0, 16, ... are all in `bad`
+    assert fb.sum() == judged.size
+    # judged envs used ABA's qacc: their state matches the fused step
+    assert np.abs(a["qpos"][:, judged] - b["qpos"][:, judged]).max() < 1e-5
+    sm = stepper.stats.cpu().numpy()
+    assert sm[1] == judged.size
+
+
+def test_trained_network_gain(device):
+    """A network trained by the project's kernels on ABA transitions: its
+    held-out accuracy against the untrained network, and the surrogate +
+    judge path against plain ABA over a short rollout (error stays at the
+    judge's tolerance; throughput and fallback rate are reported)."""
+    ds = generate_dataset(device, nenv=8192, steps=160, seed=31)
+    mlp = MLP(device, seed=1)
+    fit_normalization(mlp, ds)
+    before = evaluate(mlp, ds, nsamp=32768)
+    hist = train(mlp, ds, iters=1500, batch=8192, lr=1e-3)
+    after = evaluate(mlp, ds, nsamp=32768)
+    # rollout: surrogate + rotating judge vs plain ABA from the same start
+    n = 8192
+    qp, qv, an, cb = W.randomized_actuation(n, seed=77, nblocks=5)
+    a = GpuState(qp, qv, an, cb[0], device)
+    b = GpuState(qp, qv, an, cb[0], device)
+    stepper = SurrogateStepper(mlp, n, device)
+    torch.cuda.synchronize()
+    t_sur = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_sur[0].record()
+    for k in range(50):
+        a.ctrl.copy_(torch.as_tensor(cb[k // 10], device=device))
+        stepper.step(a.qpos, a.qvel, a.anchor, a.ctrl)
+    t_sur[1].record()
+    t_aba = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_aba[0].record()
+    for k in range(50):
+        b.ctrl.copy_(torch.as_tensor(cb[k // 10], device=device))
+        b.step(1, 2e-3, qacc=False)
+    t_aba[1].record()
+    torch.cuda.synchronize()
+    summ = stepper.summary()
+    ha, hb = a.host(), b.host()
+    drift = float(np.abs(ha["qpos"] - hb["qpos"]).max())
+    m = {"untrained": before, "trained": after, "loss_first": hist[0][1], "loss_last": hist[-1][1],
+         "rollout": summ, "rollout_qpos_drift_vs_aba": drift,
+         "ms_per_step_surrogate": t_sur[0].elapsed_time(t_sur[1]) / 50,
+         "ms_per_step_aba": t_aba[0].elapsed_time(t_aba[1]) / 50, "nenv": n}
+    write_metrics("surrogate_gain_8192", m)
+    assert hist[-1][1] < 0.2 * hist[0][1]                       # it learns
+    assert after["median_rel_err"] < 0.2 * before["median_rel_err"]
+    assert np.isfinite(ha["qpos"]).all()
+    assert summ["judged"] == 50 * n // 16
