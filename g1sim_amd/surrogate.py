@@ -242,7 +242,7 @@ def generate_dataset(device, nenv=16384, steps=1221, dt=2e-3, seed=7, reset_belo
                                 X[s, NQ + NV:].data_ptr(), None, None, 0, stream)
         _lib.check(rc)
         if s % 50 == 49:
-            torch.lt(qpos[2], reset_below, out=done)
+            done.copy_(qpos[2] < reset_below)
             lib.g1_cuda_reset_done(nenv, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), done.data_ptr(),
                                    (seed * 1000003 + s) & 0xFFFFFFFF, 0.3, 0.02, stream)
     return Dataset(X, Y)
