@@ -63,6 +63,50 @@ def cpu_baseline(nenv_gpu):
             "value_1core": v_one}
 
 
+def surrogate_leg(dev, nenv, steps, warmup, train_iters):
+    """The learned surrogate on the same workload: a 20M-transition dataset
+    from the ABA kernel (16,384 envs x 1,221 steps), training by the
+    project's MFMA kernels, held-out accuracy, then surrogate + rotating 1/16
+    ABA judge + per-env fallback (tolerance 1e-3) stepping `steps` steps."""
+    import torch
+    from g1sim_amd import workloads as W
+    from g1sim_amd.surrogate import MLP, SurrogateStepper, evaluate, fit_normalization, generate_dataset, train
+    t0 = time.perf_counter()
+    ds = generate_dataset(dev, nenv=16384, steps=1221, seed=5)
+    mlp = MLP(dev, seed=0)
+    fit_normalization(mlp, ds)
+    hist = train(mlp, ds, iters=train_iters, batch=8192, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    acc = evaluate(mlp, ds, nsamp=65536, tol=1e-3)
+    del ds
+    torch.cuda.empty_cache()
+    nblocks = (warmup + steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
+    qpos = torch.as_tensor(qp, device=dev)
+    qvel = torch.as_tensor(qv, device=dev)
+    anchor = torch.as_tensor(an, device=dev)
+    ctrl = torch.as_tensor(cb, device=dev)
+    st = SurrogateStepper(mlp, nenv, dev, period=16, tol=1e-3)
+    for s in range(warmup):
+        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    summ = st.summary()
+    return {"value": nenv * steps / wall, "unit": "env-steps/s", "ms_per_step": wall * 1e3 / steps,
+            "judge": "rotating 1/16 of envs per step, per-env fallback at rel. qacc err > 1e-3",
+            "heldout_accept_frac": acc["accept_frac"], "heldout_median_rel_err": acc["median_rel_err"],
+            "heldout_p90_rel_err": acc["p90_rel_err"], "train_iters": train_iters,
+            "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train,
+            "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps",
+            "aba_frac": summ["aba_frac"], "reject_frac": summ["reject_frac"],
+            "fallback_envs_at_end": summ["fallback_envs_now"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +119,8 @@ def main():
                          "foot-contact geometry")
     ap.add_argument("--nsub", type=int, default=CTRL_PERIOD,
                     help="substeps per launch (must divide the 10-step target period)")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -187,12 +233,15 @@ def main():
                    "flags": args.flags},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
-                     "kernel": "g1k::step_kernel<%s>" % ",".join(
-                         "true" if args.flags & b else "false" for b in (1, 2, 4)),
+                     "kernel": "g1k::step_kernel<%s,%s,%d>" % (
+                         "true" if args.flags & 2 else "false", "true" if args.flags & 4 else "false",
+                         4 if args.flags & 8 else 2 if (args.flags & 16 or nenv >= 32768) else 4),
                      "flop_per_launch": flop_per_launch, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_ENV_LAUNCH * nenv},
         "sanity_standing": ok,
     }
+    if not args.no_surrogate and args.flags == 0:
+        out["surrogate"] = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nenv)
     print(json.dumps(out), flush=True)

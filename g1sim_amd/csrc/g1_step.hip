@@ -1365,9 +1365,10 @@ void set_err(const char* fmt, ...) {
 using KernelFn = void (*)(g1k::StepArgs);
 
 // Work split: 4 waves (one limb chain each) fill the chip from 16,384 envs
-// (1,024 waves, one per SIMD); from 65,536 envs 2 waves (legs, arms) per 64
-// envs still give 2,048+ waves and repeat the pelvis/waist part half as often.
-constexpr int kSplit2MinEnvs = 65536;
+// (1,024 waves, one per SIMD); from 32,768 envs 2 waves (legs, arms) per 64
+// envs still give 1,024+ waves and repeat the pelvis/waist part half as often
+// (measured on MI355X: +25 % at 32k, +27 % at 65k, +29 % at 131k).
+constexpr int kSplit2MinEnvs = 32768;
 
 template <int SPLIT>
 KernelFn pick(bool f64b, bool f64c) {
