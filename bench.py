@@ -34,6 +34,9 @@ FLOP_PER_ENV_STEP = 41491          # reference formulation, stance (8 contacts)
 BYTES_PER_ENV_LAUNCH = 892         # SoA loads + stores of one launch (nsub independent)
 PEAK_FP32 = 157.3e12               # MI355X FP32 (vector and matrix), dense
 PUBLISHED_16K = 12.8e6             # reference README.md:38-42, RTX 3080 Laptop, 16,384 envs
+PEAK_BF16 = 2.5e15                 # MI355X dense BF16 MFMA
+SURR_FLOP = 1717248                # 106 -> 512 x4 -> 35 MLP forward, per env-step
+POLICY_FLOP = 433408               # 96 -> 512 -> 256 -> 128 -> 29 actor forward, per env-step
 CTRL_PERIOD = 10
 
 
@@ -63,11 +66,24 @@ def cpu_baseline(nenv_gpu):
             "value_1core": v_one}
 
 
+def _event_avg_ms(fn, reps, stream):
+    """Average duration of fn()'s launches from HIP events on `stream`."""
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+
+
 def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     """The learned surrogate on the same workload: a 20M-transition dataset
     from the ABA kernel (16,384 envs x 1,221 steps), training by the
     project's MFMA kernels, held-out accuracy, then surrogate + rotating 1/16
-    ABA judge + per-env fallback (tolerance 1e-3) stepping `steps` steps."""
+    ABA judge + per-env fallback (tolerance 1e-3) stepping `steps` steps.
+    Returns (result, trained network)."""
     import torch
     from g1sim_amd import workloads as W
     from g1sim_amd.surrogate import MLP, SurrogateStepper, evaluate, fit_normalization, generate_dataset, train
@@ -83,28 +99,93 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     torch.cuda.empty_cache()
     nblocks = (warmup + steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
     qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
-    qpos = torch.as_tensor(qp, device=dev)
-    qvel = torch.as_tensor(qv, device=dev)
-    anchor = torch.as_tensor(an, device=dev)
     ctrl = torch.as_tensor(cb, device=dev)
-    st = SurrogateStepper(mlp, nenv, dev, period=16, tol=1e-3)
-    for s in range(warmup):
-        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for s in range(warmup, warmup + steps):
-        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    summ = st.summary()
+
+    def run(tol):
+        qpos = torch.as_tensor(qp, device=dev)
+        qvel = torch.as_tensor(qv, device=dev)
+        anchor = torch.as_tensor(an, device=dev)
+        st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
+        for s in range(warmup):
+            st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for s in range(warmup, warmup + steps):
+            st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t1, st.summary(), (qpos, qvel, st.tau)
+
+    wall, summ, _ = run(1e-3)
+    # the same pipeline with every judgement accepted: the cost of the
+    # surrogate path itself, independent of the network's accuracy
+    wall_acc, _, (qpos, qvel, tau) = run(1e30)
+    # the fused network kernel alone (features + 5 layers), HIP events
+    pred = torch.empty((nenv, 64), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    fwd = lambda: mlp.forward_state(nenv, qpos, qvel, tau, pred)  # noqa: E731
+    fwd()
+    k_ms = _event_avg_ms(fwd, 50, stream)
+    k_tflops = SURR_FLOP * nenv / (k_ms * 1e-3) / 1e12
     return {"value": nenv * steps / wall, "unit": "env-steps/s", "ms_per_step": wall * 1e3 / steps,
             "judge": "rotating 1/16 of envs per step, per-env fallback at rel. qacc err > 1e-3",
             "heldout_accept_frac": acc["accept_frac"], "heldout_median_rel_err": acc["median_rel_err"],
             "heldout_p90_rel_err": acc["p90_rel_err"], "train_iters": train_iters,
             "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train,
             "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps",
+            "value_all_accepted": nenv * steps / wall_acc, "ms_per_step_all_accepted": wall_acc * 1e3 / steps,
             "aba_frac": summ["aba_frac"], "reject_frac": summ["reject_frac"],
-            "fallback_envs_at_end": summ["fallback_envs_now"]}
+            "fallback_envs_at_end": summ["fallback_envs_now"],
+            "roofline": {"bound": "mfma", "kernel": "g1inf::surrogate_kernel<0> (features + 5 layers)",
+                         "achieved": k_tflops, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                         "frac": k_tflops * 1e12 / PEAK_BF16, "avg_launch_ms": k_ms,
+                         "flop_per_launch": SURR_FLOP * nenv}}, mlp
+
+
+def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
+    """Policy in the loop: a randomly initialised actor (96 -> 512 -> 256 ->
+    128 -> 29, ELU, fused on MFMA) drives nenv envs for iters rollouts of
+    `steps` policy steps, each nsub physics substeps at dt 2e-3, with
+    rewards, terminations, resets, commands (U(+-1) m/s, U(+-1) rad/s) and
+    pushes on the device; FP32 ABA physics, then the surrogate with its
+    judge."""
+    import torch
+    from g1sim_amd.policy import PolicyRollout
+    out = {"nenv": nenv, "steps": steps, "iterations": iters, "nsub": nsub, "dt": 2e-3,
+           "note": "rollouts only: the PPO update between iterations is not part of the loop timed here"}
+    stream = torch.cuda.current_stream(dev)
+    for physics in ("aba", "surrogate"):
+        ro = PolicyRollout(nenv, dev, physics=physics, surrogate=mlp if physics == "surrogate" else None,
+                           seed=11, nsub=nsub, cmd_lin=1.0, cmd_yaw=1.0)
+        ro.rollout(steps)                                  # warmup iteration
+        rs = torch.zeros(steps, dtype=torch.float64, device=dev)
+        dc = torch.zeros(steps, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            ro.rollout(steps, rs, dc)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        psteps = nenv * steps * iters
+        r = {"env_steps_per_s": psteps * nsub / wall, "policy_steps_per_s": psteps / wall,
+             "ms_per_policy_step": wall * 1e3 / (steps * iters),
+             "mean_reward": float(rs.sum().item() / psteps), "done_frac": float(dc.sum().item() / psteps)}
+        if ro.stepper is not None:
+            sm = ro.stepper.summary()
+            r.update({"aba_frac": sm["aba_frac"], "reject_frac": sm["reject_frac"],
+                      "fallback_envs_at_end": sm["fallback_envs_now"]})
+        out[physics] = r
+        if physics == "aba":
+            pol = ro.policy
+            fn = lambda: pol.act(nenv, ro.qpos, ro.qvel, ro.commands, ro.ctrl, ro.ctrl,  # noqa: E731
+                                 actions=ro.actions, seed=11, t=0)
+            k_ms = _event_avg_ms(fn, 50, stream)
+            tf = POLICY_FLOP * nenv / (k_ms * 1e-3) / 1e12
+            out["policy_kernel"] = {"bound": "mfma", "kernel": "g1inf::policy_kernel", "avg_launch_ms": k_ms,
+                                    "achieved": tf, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                                    "frac": tf * 1e12 / PEAK_BF16}
+        del ro
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -121,6 +202,7 @@ def main():
                     help="substeps per launch (must divide the 10-step target period)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--train-iters", type=int, default=12000)
+    ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -241,7 +323,9 @@ def main():
         "sanity_standing": ok,
     }
     if not args.no_surrogate and args.flags == 0:
-        out["surrogate"] = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
+        out["surrogate"], mlp = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
+        if not args.no_policy:
+            out["policy_in_the_loop"] = policy_leg(dev, mlp)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nenv)
     print(json.dumps(out), flush=True)

@@ -56,13 +56,26 @@ def load():
     lib.g1_mlp_features.restype = i
     lib.g1_mlp_features.argtypes = [i, i, vp, vp, vp, vp, vp, vp, i, vp]
     lib.g1_hip_forces.restype = i
-    lib.g1_hip_forces.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.g1_hip_forces.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_hip_judge_list.restype = i
     lib.g1_hip_judge_list.argtypes = [i, i, i, vp, vp, vp, vp]
     lib.g1_hip_step_judge.restype = i
     lib.g1_hip_step_judge.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_surrogate_integrate.restype = i
     lib.g1_hip_surrogate_integrate.argtypes = [i, f, i, i, f, vp, vp, vp, i, vp, vp, vp, vp, vp, vp, vp]
+    pa = C.POINTER(C.c_void_p)           # host array of device pointers
+    lib.g1_hip_surrogate_forward.restype = i
+    lib.g1_hip_surrogate_forward.argtypes = [i, vp, vp, vp, vp, vp, pa, pa, vp, i, vp]
+    lib.g1_hip_surrogate_step.restype = i
+    lib.g1_hip_surrogate_step.argtypes = [i, f, i, i, f, vp, vp, vp, vp, vp, pa, pa, vp, vp, vp, vp, vp, vp, vp]
+    lib.g1_hip_get_obs.restype = i
+    lib.g1_hip_get_obs.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
+    lib.g1_policy_act.restype = i
+    lib.g1_policy_act.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, pa, pa, vp, u, i, vp, vp, vp, vp, vp]
+    lib.g1_rollout_post.restype = i
+    lib.g1_rollout_post.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, u, i, vp, vp, vp, vp]
+    lib.g1_rollout_init.restype = i
+    lib.g1_rollout_init.argtypes = [i, f, f, u, vp, vp, vp, vp, vp]
     lib.g1_hip_last_error.restype = C.c_char_p
     lib.g1_hip_last_error.argtypes = []
     lib.g1_hip_version.restype = i
@@ -75,7 +88,16 @@ def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
             "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_adam",
             "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
-            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_surrogate_integrate"]
+            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_surrogate_integrate",
+            "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs", "g1_policy_act",
+            "g1_rollout_post", "g1_rollout_init"]
+
+
+def ptr_array(tensors_or_ptrs):
+    """Host array of device pointers (the W / b arguments of the fused
+    inference entry points)."""
+    vals = [t if isinstance(t, int) else t.data_ptr() for t in tensors_or_ptrs]
+    return (C.c_void_p * len(vals))(*vals)
 
 
 def last_error():

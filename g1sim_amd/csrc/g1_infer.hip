@@ -1,0 +1,641 @@
+// g1_infer.hip -- fused network inference for the step loop (gfx950).
+//
+// One workgroup of 8 wave64s takes 64 envs through a whole MLP: it builds
+// the network input from the SoA state, runs every dense layer on
+// v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32 accumulation) with the
+// activations held in LDS (two 64 x 520 bf16 buffers, 133 KB), and finishes
+// with a fused epilogue, so no activation ever goes to HBM.  Weights are
+// read from L2 (1.7 MB for the surrogate, 0.43 MB for the policy), each
+// lane loading 16 contiguous bytes of a weight row as its B fragment, with a
+// one-step register prefetch.
+//
+// * Surrogate step: z-scored (q, qd, tau_total) -> 512 x4 SiLU -> 35, then
+//   the per-env commit (judge / fallback / semi-implicit Euler) of
+//   g1_commit.h.  The forward-only variant writes the raw output instead.
+// * Policy act: the RL observation (96- or 99-dim, src/g1sim.py:132-170),
+//   normalised and clipped as examples/train.py:69-70, -> 512 -> 256 -> 128
+//   (ELU) -> 29 (examples/train.py:21-29), an optional Gaussian sample, and
+//   ctrl = default + action * scale (examples/train.py:299); the push of
+//   examples/train.py:301-310 is applied to the state in the same kernel.
+//
+// Tiling: a layer of NOUT outputs has NOUT/16 column tiles of 16.  With 8 or
+// more column tiles each wave owns NOUT/128 of them for all 4 row tiles
+// (weight reuse x4 in registers); narrower layers split the 4 x NC tiles so
+// every wave has NC/2 row tiles of one column tile.  The k-order of the
+// accumulation is that of the tiled GEMM in g1_mlp.hip (32 at a time,
+// ascending), so both give the same fp32 results.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "g1_model_kernel.h"
+#include "g1_fastmath.h"
+#include "g1_commit.h"
+
+namespace g1inf {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int ROWS = 64;                 // envs per workgroup
+constexpr int NWAVE = 8;
+constexpr int NTHR = 64 * NWAVE;
+constexpr int LDA = 512 + 8;             // activation row stride (bf16); 16-byte pad
+constexpr int LDO = 65;                  // fp32 output row stride in LDS (odd: no bank conflicts)
+constexpr int NV = 35, NU = 29;
+
+enum Act : int { ACT_SILU = 0, ACT_ELU = 1 };
+
+template <int NOUT> struct Tile {
+  static constexpr int NC = NOUT / 16;
+  static constexpr bool WIDE = NC >= NWAVE;
+  static constexpr int CT = WIDE ? NC / NWAVE : 1;
+  static constexpr int RT = WIDE ? 4 : (4 * NC) / NWAVE;
+  static_assert(NOUT % 16 == 0, "output width must be a multiple of 16");
+  static_assert(WIDE ? NC % NWAVE == 0 : (NWAVE % NC == 0 && (4 * NC) % NWAVE == 0), "unsupported width");
+  __device__ static int r0(int w) { return WIDE ? 0 : (w / NC) * RT; }
+  __device__ static int c0(int w) { return WIDE ? w * CT : w % NC; }
+};
+
+// acc = in[64 x NIN] (LDS) . W[cols][NIN]^T for this wave's tiles.
+template <int NIN, int NOUT>
+__device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf16* __restrict__ W, int w,
+                                          int lane, f32x4 (&acc)[Tile<NOUT>::RT][Tile<NOUT>::CT]) {
+  using T = Tile<NOUT>;
+  constexpr int KS = NIN / 32;
+  static_assert(NIN % 32 == 0, "input width must be a multiple of 32");
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const int r0 = T::r0(w), c0 = T::c0(w);
+#pragma unroll
+  for (int r = 0; r < T::RT; ++r)
+#pragma unroll
+    for (int c = 0; c < T::CT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16* wp = W + (size_t)(c0 * 16 + lr) * NIN + lk;
+  bf16x8 fb[2][T::CT];
+#pragma unroll
+  for (int c = 0; c < T::CT; ++c) fb[0][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN);
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    if (k + 1 < KS) {
+#pragma unroll
+      for (int c = 0; c < T::CT; ++c)
+        fb[(k + 1) & 1][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN + (k + 1) * 32);
+    }
+    bf16x8 fa[T::RT];
+#pragma unroll
+    for (int r = 0; r < T::RT; ++r)
+      fa[r] = *reinterpret_cast<const bf16x8*>(sin + ((r0 + r) * 16 + lr) * LDA + k * 32 + lk);
+#pragma unroll
+    for (int r = 0; r < T::RT; ++r)
+#pragma unroll
+      for (int c = 0; c < T::CT; ++c)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[r], fb[k & 1][c], acc[r][c], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// hidden layer: out = act(in W^T + b) as bf16 into LDS
+template <int NIN, int NOUT, int ACT>
+__device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __restrict__ sout,
+                                      const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane) {
+  using T = Tile<NOUT>;
+  f32x4 acc[T::RT][T::CT];
+  mma_layer<NIN, NOUT>(sin, W, w, lane, acc);
+  const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
+#pragma unroll
+  for (int c = 0; c < T::CT; ++c) {
+    const int col = (c0 + c) * 16 + lr;
+    const float bc = b[col];
+#pragma unroll
+    for (int r = 0; r < T::RT; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = (r0 + r) * 16 + (lane >> 4) * 4 + q;
+        const float z = acc[r][c][q] + bc;
+        float h;
+        if constexpr (ACT == ACT_SILU) h = z * sigmoidf(z);
+        else h = z > 0.f ? z : expm1f(z);
+        sout[row * LDA + col] = (bf16)h;
+      }
+  }
+}
+
+// output layer: out = in W^T + b as fp32 into LDS [64][LDO]
+template <int NIN, int NOUT>
+__device__ __forceinline__ void dense_out(const bf16* __restrict__ sin, float* __restrict__ sout,
+                                          const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane) {
+  using T = Tile<NOUT>;
+  static_assert(NOUT <= LDO, "output tile wider than the LDS row");
+  f32x4 acc[T::RT][T::CT];
+  mma_layer<NIN, NOUT>(sin, W, w, lane, acc);
+  const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
+#pragma unroll
+  for (int c = 0; c < T::CT; ++c) {
+    const int col = (c0 + c) * 16 + lr;
+    const float bc = b[col];
+#pragma unroll
+    for (int r = 0; r < T::RT; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sout[((r0 + r) * 16 + (lane >> 4) * 4 + q) * LDO + col] = acc[r][c][q] + bc;
+  }
+}
+
+// ------------------------------------------------------------- surrogate
+struct SurrogateArgs {
+  int nenv;
+  const float* qpos;
+  const float* qvel;
+  const float* tau;
+  const float* xmean;
+  const float* xinv;
+  const bf16* W[5];      // [512][128], [512][512] x3, [>=64][512]
+  const float* b[5];     // [512] x4, [>=64]
+  // forward-only output (MODE 0)
+  float* pred;
+  int ldp;
+  // commit (MODE 1)
+  int phase, period;
+  float tol, dt;
+  const float* ymean;
+  const float* ystd;
+  const float* qacc_aba;
+  uint8_t* fallback;
+  unsigned* stats;
+  float* qacc_out;
+  float* qpos_rw;
+  float* qvel_rw;
+};
+
+constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
+
+template <int MODE>
+__global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int e0 = blockIdx.x * ROWS;
+  const size_t N = (size_t)a.nenv;
+  {
+    // network input: z-scored (qpos 36, qvel 35, tau 35), zero padding
+    const int r = tid & 63, e = e0 + r;
+    for (int f = w; f < SUR_PIN; f += NWAVE) {
+      float x = 0.f;
+      if (e < a.nenv && f < SUR_NIN) {
+        const float raw = f < 36 ? a.qpos[f * N + e] : f < 71 ? a.qvel[(f - 36) * N + e] : a.tau[(f - 71) * N + e];
+        x = (raw - a.xmean[f]) * a.xinv[f];
+      }
+      buf[0][r * LDA + f] = (bf16)x;
+    }
+  }
+  __syncthreads();
+  dense<SUR_PIN, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[0], a.b[0], w, lane);
+  __syncthreads();
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[1], a.b[1], w, lane);
+  __syncthreads();
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[2], a.b[2], w, lane);
+  __syncthreads();
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[3], a.b[3], w, lane);
+  __syncthreads();
+  float* out = reinterpret_cast<float*>(buf[1]);
+  dense_out<SUR_H, SUR_POUT>(buf[0], out, a.W[4], a.b[4], w, lane);
+  __syncthreads();
+  if constexpr (MODE == 0) {
+    // [64][64] fp32 tile to pred rows; 512 threads, 8 per row
+    for (int i = tid; i < ROWS * SUR_POUT; i += NTHR) {
+      const int r = i / SUR_POUT, c = i % SUR_POUT, e = e0 + r;
+      if (e < a.nenv) a.pred[(size_t)e * a.ldp + c] = out[r * LDO + c];
+    }
+  } else {
+    if (w == 0) {
+      const int e = e0 + lane;
+      const bool live = e < a.nenv;
+      float sur[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) sur[j] = out[lane * LDO + j] * a.ystd[j] + a.ymean[j];
+      g1k::surrogate_commit(live, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback, a.stats,
+                            a.qacc_out, a.qpos_rw, a.qvel_rw);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- policy
+// src/g1sim.py:76-82
+__constant__ float kActionScale[NU] = {
+    0.443200f, 0.274054f, 0.443200f, 0.274054f, 0.365131f, 0.365131f,
+    0.443200f, 0.274054f, 0.443200f, 0.274054f, 0.365131f, 0.365131f,
+    0.443200f, 0.365131f, 0.365131f,
+    0.365131f, 0.365131f, 0.365131f, 0.365131f, 0.365131f, 0.074501f, 0.074501f,
+    0.365131f, 0.365131f, 0.365131f, 0.365131f, 0.365131f, 0.074501f, 0.074501f};
+
+// Counter-based uniform draws for the rollout: stream k of env e at step t.
+__device__ __forceinline__ unsigned mix32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352du;
+  x ^= x >> 15; x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ unsigned draw_bits(unsigned seed, int t, int e, int k) {
+  unsigned h = mix32(seed ^ 0x68bc21ebu);
+  h = mix32(h + (unsigned)t * 0x9E3779B9u);
+  h = mix32(h ^ ((unsigned)e * 0x85EBCA6Bu));
+  return mix32(h + (unsigned)k * 0xC2B2AE35u);
+}
+// uniform in [0, 1) with 24 random bits
+__device__ __forceinline__ float draw_u01(unsigned seed, int t, int e, int k) {
+  return (float)(draw_bits(seed, t, e, k) >> 8) * 5.9604644775390625e-08f;
+}
+
+// v rotated by the inverse of the unit quaternion (w, x, y, z), written as
+// src/g1sim.py:149-156 (u = -xyz; v + 2 (w (u x v) + u x (u x v)))
+__device__ __forceinline__ void quat_rotate_inverse(float qw, float qx, float qy, float qz, const float* v,
+                                                    float* o) {
+  const float ux = -qx, uy = -qy, uz = -qz;
+  const float a0 = uy * v[2] - uz * v[1], a1 = uz * v[0] - ux * v[2], a2 = ux * v[1] - uy * v[0];
+  const float b0 = uy * a2 - uz * a1, b1 = uz * a0 - ux * a2, b2 = ux * a1 - uy * a0;
+  o[0] = v[0] + 2.0f * (qw * a0 + b0);
+  o[1] = v[1] + 2.0f * (qw * a1 + b1);
+  o[2] = v[2] + 2.0f * (qw * a2 + b2);
+}
+
+// Feature f of env e of the observation.  layout 0: the 99-dim G1Sim.get_obs
+// order (commands, projected gravity, base linear velocity in the base frame,
+// base angular velocity, q - default, qd, last ctrl); layout 1: the 96-dim
+// order (base angular velocity, projected gravity, commands, q - default, qd,
+// last ctrl).
+__device__ float obs_feature(int layout, int f, size_t N, int e, const float* qpos, const float* qvel,
+                             const float* commands, const float* last_ctrl) {
+  int kind, i;   // 0 cmd, 1 gravity, 2 lin vel, 3 ang vel, 4 joint pos, 5 joint vel, 6 last ctrl
+  if (layout == 0) {
+    if (f < 12) { kind = f / 3 == 0 ? 0 : f / 3 == 1 ? 1 : f / 3 == 2 ? 2 : 3; i = f % 3; }
+    else { kind = 4 + (f - 12) / NU; i = (f - 12) % NU; }
+  } else {
+    if (f < 9) { kind = f / 3 == 0 ? 3 : f / 3 == 1 ? 1 : 0; i = f % 3; }
+    else { kind = 4 + (f - 9) / NU; i = (f - 9) % NU; }
+  }
+  switch (kind) {
+    case 0: return commands[i * N + e];
+    case 3: return qvel[(3 + i) * N + e];
+    case 4: return qpos[(7 + i) * N + e] - (float)g1m_qpos_stand[7 + i];
+    case 5: return qvel[(6 + i) * N + e];
+    case 6: return last_ctrl[i * N + e];
+    default: break;
+  }
+  const float qw = qpos[3 * N + e], qx = qpos[4 * N + e], qy = qpos[5 * N + e], qz = qpos[6 * N + e];
+  float v[3], o[3];
+  if (kind == 1) { v[0] = 0.f; v[1] = 0.f; v[2] = -1.f; }
+  else { v[0] = qvel[e]; v[1] = qvel[N + e]; v[2] = qvel[2 * N + e]; }
+  quat_rotate_inverse(qw, qx, qy, qz, v, o);
+  return o[i];
+}
+
+struct PolicyArgs {
+  int nenv, layout, nobs;
+  const float* qpos;
+  float* qvel;                 // read for the observation; pushes add to it
+  const float* commands;
+  const float* last_ctrl;
+  const float* obs_mean;
+  const float* obs_inv;        // 1 / sqrt(var + 1e-8)
+  float clip;
+  const bf16* W[4];            // [512][128], [256][512], [128][256], [>=32][128]
+  const float* b[4];
+  const float* log_std;        // [29] or null (deterministic mean action)
+  unsigned seed;
+  int t;
+  float* ctrl;                 // [29][N]
+  float* actions;              // [29][N] or null
+  float* obs_out;              // [N][nobs] or null
+  int* push_interval;          // [N] or null (no pushes)
+};
+
+constexpr int POL_PIN = 128;
+
+__global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int e0 = blockIdx.x * ROWS;
+  const size_t N = (size_t)a.nenv;
+  {
+    const int r = tid & 63, e = e0 + r;
+    for (int f = w; f < POL_PIN; f += NWAVE) {
+      float x = 0.f;
+      if (e < a.nenv && f < a.nobs) {
+        const float o = obs_feature(a.layout, f, N, e, a.qpos, a.qvel, a.commands, a.last_ctrl);
+        if (a.obs_out) a.obs_out[(size_t)e * a.nobs + f] = o;
+        x = fminf(fmaxf((o - a.obs_mean[f]) * a.obs_inv[f], -a.clip), a.clip);
+      }
+      buf[0][r * LDA + f] = (bf16)x;
+    }
+  }
+  __syncthreads();
+  if (a.push_interval && w == 0) {
+    // examples/train.py:301-310: countdown; on expiry add a random base
+    // velocity kick and draw the next interval from [50, 150)
+    const int e = e0 + lane;
+    if (e < a.nenv) {
+      int pi = a.push_interval[e] - 1;
+      if (pi <= 0) {
+        const float half[6] = {0.5f, 0.5f, 0.4f, 0.52f, 0.52f, 0.78f};
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          const float u = draw_u01(a.seed, a.t, e, 64 + m);
+          a.qvel[m * N + e] += u * (2.0f * half[m]) - half[m];
+        }
+        pi = 50 + (int)(draw_u01(a.seed, a.t, e, 70) * 100.0f);
+      }
+      a.push_interval[e] = pi;
+    }
+  }
+  dense<POL_PIN, 512, ACT_ELU>(buf[0], buf[1], a.W[0], a.b[0], w, lane);
+  __syncthreads();
+  dense<512, 256, ACT_ELU>(buf[1], buf[0], a.W[1], a.b[1], w, lane);
+  __syncthreads();
+  dense<256, 128, ACT_ELU>(buf[0], buf[1], a.W[2], a.b[2], w, lane);
+  __syncthreads();
+  float* out = reinterpret_cast<float*>(buf[0]);
+  dense_out<128, 32>(buf[1], out, a.W[3], a.b[3], w, lane);
+  __syncthreads();
+  // epilogue: thread (r, group) -> env r, actions j = group, group + 8, ...
+  const int r = tid & 63, e = e0 + r;
+  if (e < a.nenv) {
+    for (int j = w; j < NU; j += NWAVE) {
+      float act = out[r * LDO + j];
+      if (a.log_std) {
+        // Box-Muller: u1 in (0, 1], u2 in [0, 1)
+        const float u1 = 1.0f - draw_u01(a.seed, a.t, e, 2 * j);
+        const float u2 = draw_u01(a.seed, a.t, e, 2 * j + 1);
+        const float z = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+        act = fmaf(expf(a.log_std[j]), z, act);
+      }
+      a.ctrl[j * N + e] = (float)g1m_qpos_stand[7 + j] + act * kActionScale[j];
+      if (a.actions) a.actions[j * N + e] = act;
+    }
+  }
+}
+
+// ------------------------------------------------------------ observation
+// G1Sim.get_obs as one kernel: obs [N][99] (src/g1sim.py:132-170).
+__global__ __launch_bounds__(256) void obs_kernel(int nenv, int layout, int nobs, const float* qpos,
+                                                  const float* qvel, const float* commands,
+                                                  const float* last_ctrl, float* obs) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nenv * nobs) return;
+  const int e = (int)(i / nobs), f = (int)(i % nobs);
+  obs[i] = obs_feature(layout, f, (size_t)nenv, e, qpos, qvel, commands, last_ctrl);
+}
+
+// ---------------------------------------------------------------- rollout
+// examples/train.py:256-270: per-joint posture tolerances while walking / running
+__constant__ float kStdWalking[NU] = {
+    0.3f, 0.15f, 0.15f, 0.35f, 0.25f, 0.1f, 0.3f, 0.15f, 0.15f, 0.35f, 0.25f, 0.1f, 0.2f, 0.08f, 0.1f,
+    0.15f, 0.15f, 0.1f, 0.15f, 0.3f, 0.3f, 0.3f, 0.15f, 0.15f, 0.1f, 0.15f, 0.3f, 0.3f, 0.3f};
+__constant__ float kStdRunning[NU] = {
+    0.5f, 0.2f, 0.2f, 0.6f, 0.35f, 0.15f, 0.5f, 0.2f, 0.2f, 0.6f, 0.35f, 0.15f, 0.3f, 0.08f, 0.2f,
+    0.5f, 0.2f, 0.15f, 0.35f, 0.3f, 0.3f, 0.3f, 0.5f, 0.2f, 0.15f, 0.35f, 0.3f, 0.3f, 0.3f};
+
+// draw streams of the bookkeeping kernels (the policy kernel uses 0..70)
+constexpr int K_RESET = 128, K_TIMEOUT = 136, K_INIT = 144;
+
+// examples/train.py:227-233 with the ranges as arguments (the reference
+// uses lin 1.0, yaw 0.5)
+__device__ __forceinline__ void resample_command(float* commands, size_t N, int e, unsigned seed, int t, int k,
+                                                 float lin, float yaw) {
+  commands[e] = draw_u01(seed, t, e, k) * (2.0f * lin) - lin;
+  commands[N + e] = draw_u01(seed, t, e, k + 1) * (2.0f * lin) - lin;
+  commands[2 * N + e] = draw_u01(seed, t, e, k + 2) * (2.0f * yaw) - yaw;
+}
+// integer in [lo, lo + n)
+__device__ __forceinline__ int draw_int(unsigned seed, int t, int e, int k, int lo, int n) {
+  return lo + (int)(draw_u01(seed, t, e, k) * (float)n);
+}
+
+struct PostArgs {
+  int nenv;
+  const float* qpos;
+  const float* qvel;
+  const float* foot_pos;
+  const float* foot_vel;
+  const float* contact_forces;
+  const float* actions;
+  float* commands;
+  float* prev_actions;
+  int* command_timeout;
+  int* episode_length;
+  int* push_interval;
+  int max_episode_length;
+  float cmd_lin, cmd_yaw;
+  unsigned seed;
+  int t;
+  float* reward;
+  uint8_t* done;
+  uint8_t* reset;
+};
+
+// Reward and termination of examples/train.py:143-225 on the post-step state,
+// then the per-step bookkeeping of examples/train.py:319-351: previous
+// actions, command timer, episode length, timeouts, the reset mask and, for
+// reset envs, new commands / timers / push interval.
+__global__ __launch_bounds__(256) void post_step_kernel(PostArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nenv) return;
+  const size_t N = (size_t)a.nenv;
+  const float c0 = a.commands[e], c1 = a.commands[N + e], c2 = a.commands[2 * N + e];
+  const float qw = a.qpos[3 * N + e], qx = a.qpos[4 * N + e], qy = a.qpos[5 * N + e], qz = a.qpos[6 * N + e];
+  const float down[3] = {0.f, 0.f, -1.f};
+  const float vw[3] = {a.qvel[e], a.qvel[N + e], a.qvel[2 * N + e]};
+  float g[3], v[3];
+  quat_rotate_inverse(qw, qx, qy, qz, down, g);
+  quat_rotate_inverse(qw, qx, qy, qz, vw, v);
+  const float w0 = a.qvel[3 * N + e], w1 = a.qvel[4 * N + e], w2 = a.qvel[5 * N + e];
+  // 1. velocity tracking
+  const float lin_err = (c0 - v[0]) * (c0 - v[0]) + (c1 - v[1]) * (c1 - v[1]) + v[2] * v[2];
+  const float rew_lin = expf(-lin_err / 0.25f);
+  const float ang_err = (c2 - w2) * (c2 - w2) + (w0 * w0 + w1 * w1);
+  const float rew_ang = expf(-ang_err / 0.5f);
+  // 2. orientation and speed-dependent posture
+  const float rew_orient = expf(-(g[0] * g[0] + g[1] * g[1]) / 0.2f);
+  const float speed = sqrtf(c0 * c0 + c1 * c1) + fabsf(c2);
+  const bool standing = speed < 0.05f, running = speed >= 1.5f;
+  float pose = 0.f, dof = 0.f;
+#pragma unroll
+  for (int j = 0; j < NU; ++j) {
+    const float q = a.qpos[(7 + j) * N + e];
+    const float err = q - (float)g1m_qpos_stand[7 + j];
+    const float sd = standing ? 0.05f : running ? kStdRunning[j] : kStdWalking[j];
+    pose += err * err / (sd * sd);
+    const float lo = (float)g1m_jnt_range[2 * j], hi = (float)g1m_jnt_range[2 * j + 1];
+    const float range = hi - lo;
+    const float slo = lo + 0.05f * range, shi = hi - 0.05f * range;
+    dof += fmaxf(q - shi, 0.f) + fmaxf(slo - q, 0.f);
+  }
+  const float rew_pose = expf(-(pose / (float)NU));
+  // 3. penalties
+  float rate = 0.f;
+#pragma unroll
+  for (int j = 0; j < NU; ++j) {
+    const float d = a.actions[j * N + e] - a.prev_actions[j * N + e];
+    rate += d * d;
+  }
+  const float body_ang = w0 * w0 + w1 * w1;
+  // 4. feet
+  const float lz = a.foot_pos[2 * N + e], rz = a.foot_pos[5 * N + e];
+  const float lv2 = a.foot_vel[e] * a.foot_vel[e] + a.foot_vel[N + e] * a.foot_vel[N + e];
+  const float rv2 = a.foot_vel[3 * N + e] * a.foot_vel[3 * N + e] + a.foot_vel[4 * N + e] * a.foot_vel[4 * N + e];
+  const float active = speed > 0.05f ? 1.f : 0.f;
+  const float clearance =
+      (fabsf(lz - 0.1f) * sqrtf(lv2 + 1e-6f) + fabsf(rz - 0.1f) * sqrtf(rv2 + 1e-6f)) * active;
+  const float lf = a.contact_forces[e] + a.contact_forces[N + e] + a.contact_forces[2 * N + e] +
+                   a.contact_forces[3 * N + e];
+  const float rf = a.contact_forces[4 * N + e] + a.contact_forces[5 * N + e] + a.contact_forces[6 * N + e] +
+                   a.contact_forces[7 * N + e];
+  const float slip = (lv2 * (lf > 1.f ? 1.f : 0.f) + rv2 * (rf > 1.f ? 1.f : 0.f)) * active;
+  a.reward[e] = 2.0f * rew_lin + 2.0f * rew_ang + rew_orient + rew_pose - dof - 2.0f * clearance - 0.5f * slip -
+                0.05f * rate - 0.05f * body_ang;
+  // termination: tilt beyond 70 degrees (the reference's 3.14159)
+  const float tilt = fabsf(acosf(fminf(fmaxf(-g[2], -1.f), 1.f)));
+  const bool done = tilt > (float)(70.0 * 3.14159 / 180.0);
+  a.done[e] = done ? 1 : 0;
+  // bookkeeping
+  int ct = a.command_timeout[e] - 1;
+  int el = a.episode_length[e] + 1;
+  const bool rs = done || el >= a.max_episode_length;
+#pragma unroll
+  for (int j = 0; j < NU; ++j) a.prev_actions[j * N + e] = rs ? 0.f : a.actions[j * N + e];
+  if (rs) {
+    resample_command(a.commands, N, e, a.seed, a.t, K_RESET, a.cmd_lin, a.cmd_yaw);
+    ct = draw_int(a.seed, a.t, e, K_RESET + 3, 150, 100);
+    el = 0;
+    if (a.push_interval) a.push_interval[e] = draw_int(a.seed, a.t, e, K_RESET + 4, 50, 100);
+  }
+  if (ct <= 0) {
+    resample_command(a.commands, N, e, a.seed, a.t, K_TIMEOUT, a.cmd_lin, a.cmd_yaw);
+    ct = draw_int(a.seed, a.t, e, K_TIMEOUT + 3, 150, 100);
+  }
+  a.command_timeout[e] = ct;
+  a.episode_length[e] = el;
+  a.reset[e] = rs ? 1 : 0;
+}
+
+// Rollout start (examples/train.py:237-276): commands for every env, timers
+// at zero, push intervals in [50, 150).
+__global__ __launch_bounds__(256) void rollout_init_kernel(int nenv, float lin, float yaw, unsigned seed,
+                                                           float* commands, int* command_timeout,
+                                                           int* episode_length, int* push_interval) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nenv) return;
+  resample_command(commands, (size_t)nenv, e, seed, -1, K_INIT, lin, yaw);
+  command_timeout[e] = 0;
+  episode_length[e] = 0;
+  if (push_interval) push_interval[e] = draw_int(seed, -1, e, K_INIT + 3, 50, 100);
+}
+
+}  // namespace g1inf
+
+// ================================================================== C-ABI
+namespace {
+using namespace g1inf;
+int launch_status() { return hipGetLastError() == hipSuccess ? 0 : -2; }
+
+bool fill_surrogate_weights(SurrogateArgs& a, const void* const* W, const float* const* b) {
+  if (!W || !b) return false;
+  for (int l = 0; l < 5; ++l) {
+    if (!W[l] || !b[l]) return false;
+    a.W[l] = (const bf16*)W[l];
+    a.b[l] = b[l];
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, const float* tau, const float* xmean,
+                             const float* xinv, const void* const* W, const float* const* b, float* pred, int ldp,
+                             void* stream) {
+  if (nenv < 0 || !qpos || !qvel || !tau || !xmean || !xinv || !pred || ldp < SUR_POUT) return -1;
+  SurrogateArgs a{};
+  if (!fill_surrogate_weights(a, W, b)) return -1;
+  if (nenv == 0) return 0;
+  a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
+  a.pred = pred; a.ldp = ldp;
+  hipLaunchKernelGGL(surrogate_kernel<0>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
+                          const float* tau, const float* xmean, const float* xinv, const void* const* W,
+                          const float* const* b, const float* ymean, const float* ystd, const float* qacc_aba,
+                          uint8_t* fallback, unsigned* stats, float* qacc_out, void* stream) {
+  if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !(dt > 0.f)) return -1;
+  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba) return -1;
+  SurrogateArgs a{};
+  if (!fill_surrogate_weights(a, W, b)) return -1;
+  if (nenv == 0) return 0;
+  a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
+  a.phase = phase; a.period = period; a.tol = tol; a.dt = dt; a.ymean = ymean; a.ystd = ystd;
+  a.qacc_aba = qacc_aba; a.fallback = fallback; a.stats = stats; a.qacc_out = qacc_out;
+  a.qpos_rw = qpos; a.qvel_rw = qvel;
+  hipLaunchKernelGGL(surrogate_kernel<1>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const float* commands,
+                  const float* last_ctrl, const float* obs_mean, const float* obs_inv, float clip,
+                  const void* const* W, const float* const* b, const float* log_std, unsigned seed, int t,
+                  float* ctrl, float* actions, float* obs_out, int* push_interval, void* stream) {
+  if (nenv < 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !obs_mean ||
+      !obs_inv || !ctrl || !W || !b || !(clip > 0.f))
+    return -1;
+  PolicyArgs a{};
+  for (int l = 0; l < 4; ++l) {
+    if (!W[l] || !b[l]) return -1;
+    a.W[l] = (const bf16*)W[l];
+    a.b[l] = b[l];
+  }
+  if (nenv == 0) return 0;
+  a.nenv = nenv; a.layout = layout; a.nobs = layout == 0 ? 99 : 96;
+  a.qpos = qpos; a.qvel = qvel; a.commands = commands; a.last_ctrl = last_ctrl;
+  a.obs_mean = obs_mean; a.obs_inv = obs_inv; a.clip = clip; a.log_std = log_std; a.seed = seed; a.t = t;
+  a.ctrl = ctrl; a.actions = actions; a.obs_out = obs_out; a.push_interval = push_interval;
+  hipLaunchKernelGGL(policy_kernel, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+int g1_hip_get_obs(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
+                   const float* last_ctrl, float* obs, void* stream) {
+  if (nenv < 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !obs) return -1;
+  if (nenv == 0) return 0;
+  const int nobs = layout == 0 ? 99 : 96;
+  const size_t n = (size_t)nenv * nobs;
+  hipLaunchKernelGGL(obs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nenv, layout,
+                     nobs, qpos, qvel, commands, last_ctrl, obs);
+  return launch_status();
+}
+
+int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float* foot_pos, const float* foot_vel,
+                    const float* contact_forces, const float* actions, float* commands, float* prev_actions,
+                    int* command_timeout, int* episode_length, int* push_interval, int max_episode_length,
+                    float cmd_lin, float cmd_yaw, unsigned seed, int t, float* reward, uint8_t* done,
+                    uint8_t* reset, void* stream) {
+  if (nenv < 0 || max_episode_length <= 0 || !qpos || !qvel || !foot_pos || !foot_vel || !contact_forces ||
+      !actions || !commands || !prev_actions || !command_timeout || !episode_length || !reward || !done || !reset)
+    return -1;
+  if (nenv == 0) return 0;
+  PostArgs a{nenv, qpos, qvel, foot_pos, foot_vel, contact_forces, actions, commands, prev_actions,
+             command_timeout, episode_length, push_interval, max_episode_length, cmd_lin, cmd_yaw, seed, t,
+             reward, done, reset};
+  hipLaunchKernelGGL(post_step_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float* commands, int* command_timeout,
+                    int* episode_length, int* push_interval, void* stream) {
+  if (nenv < 0 || !commands || !command_timeout || !episode_length) return -1;
+  if (nenv == 0) return 0;
+  hipLaunchKernelGGL(rollout_init_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv,
+                     cmd_lin, cmd_yaw, seed, commands, command_timeout, episode_length, push_interval);
+  return launch_status();
+}
+
+}  // extern "C"

@@ -38,11 +38,11 @@
 #include <type_traits>
 
 #include "g1_model_kernel.h"
+#include "g1_fastmath.h"
+#include "g1_commit.h"
 #include "../../include/g1sim_amd.h"
 
 namespace g1k {
-
-#define G1K_DEV __device__ __forceinline__
 
 struct Cfg {
   float dt;
@@ -64,61 +64,6 @@ static Cfg default_cfg(float dt) {
 }
 
 constexpr float ANCHOR_FREE = 1e30f;     // src/g1_core.h:108
-
-// ------------------------------------------------------------ fast math
-// Hardware reciprocal / square root / reciprocal square root (1 ulp) refined
-// by one Newton step, in place of the IEEE division and square-root
-// sequences (about 10 instructions each).  Arguments are positive and
-// normal where these are used (joint inertias, Cholesky pivots, norms).
-G1K_DEV float frcp(float x) {
-  float r = __builtin_amdgcn_rcpf(x);
-  const float e = __builtin_fmaf(-x, r, 1.0f);
-  return __builtin_fmaf(r, e, r);
-}
-G1K_DEV float frsq(float x) {
-  float r = __builtin_amdgcn_rsqf(x);
-  const float h = 0.5f * x * r;                    // Newton on 1/sqrt: r (1.5 - 0.5 x r^2)
-  return __builtin_fmaf(r, __builtin_fmaf(-h, r, 0.5f), r);
-}
-G1K_DEV float fsqrt(float x) {                    // x >= 0; sqrt(0) = 0
-  const float r = frsq(x);
-  return x > 0.f ? x * r : 0.f;
-}
-G1K_DEV double frcp(double x) { return 1.0 / x; }
-G1K_DEV double fsqrt(double x) { return sqrt(x); }
-G1K_DEV double frsq(double x) { return 1.0 / sqrt(x); }
-
-// sin and cos of a joint angle: Cody-Waite reduction by pi/2 (three-part
-// constant, exact products for |x| < 2^12) and Taylor polynomials on
-// |r| <= pi/4 (truncation below 2e-9).  Larger arguments take the library
-// routine.
-G1K_DEV void fsincos(float x, float* sp, float* cp) {
-  if (__builtin_expect(fabsf(x) > 4096.f, 0)) {
-    sincosf(x, sp, cp);
-    return;
-  }
-  const float k = __builtin_rintf(x * 0.63661977236758134f);     // 2/pi
-  float r = __builtin_fmaf(-k, 1.5707963705062866f, x);          // pi/2 = P1 + P2 + P3
-  r = __builtin_fmaf(-k, -4.3711388286737929e-08f, r);
-  r = __builtin_fmaf(-k, -1.7151245100059206e-15f, r);
-  const float r2 = r * r;
-  // sin r = r (1 - r^2/3! + r^4/5! - r^6/7! + r^8/9!)
-  float ps = __builtin_fmaf(r2, 2.7557319223985888e-06f, -1.9841269841269841e-04f);
-  ps = __builtin_fmaf(ps, r2, 8.3333333333333333e-03f);
-  ps = __builtin_fmaf(ps, r2, -1.6666666666666666e-01f);
-  const float sn = __builtin_fmaf(ps * r2, r, r);
-  // cos r = 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10!
-  float pc = __builtin_fmaf(r2, -2.7557319223985888e-07f, 2.4801587301587302e-05f);
-  pc = __builtin_fmaf(pc, r2, -1.3888888888888889e-03f);
-  pc = __builtin_fmaf(pc, r2, 4.1666666666666667e-02f);
-  pc = __builtin_fmaf(pc, r2, -0.5f);
-  const float cs = __builtin_fmaf(pc, r2, 1.0f);
-  const int quad = (int)k & 3;
-  const float s0 = (quad & 1) ? cs : sn;
-  const float c0 = (quad & 1) ? sn : cs;
-  *sp = (quad & 2) ? -s0 : s0;
-  *cp = ((quad + 1) & 2) ? -c0 : c0;
-}
 
 // ---------------------------------------------------------------- rotations
 // Planar rotation about axis K: plane (I, J) = ((K+1)%3, (K+2)%3) and
@@ -1128,6 +1073,8 @@ struct ForcesArgs {
   const float* mu;
   float* tau;
   float* contact_forces;
+  float* foot_pos;
+  float* foot_vel;
 };
 
 template <int h0, int n> G1K_DEV void servo_range(const Cfg& cf, const float* q, const float* qd, const float* ct,
@@ -1194,6 +1141,16 @@ __global__ __launch_bounds__(256) void forces_kernel(ForcesArgs a) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) a.contact_forces[(leg * 4 + m) * N + e] = fn[m];
     }
+    // foot body origin and its world linear velocity, as the step reports them
+    if (a.foot_pos) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) a.foot_pos[(leg * 3 + m) * N + e] = pf[m];
+    }
+    if (a.foot_vel) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        a.foot_vel[(leg * 3 + m) * N + e] = Rf[3 * m] * vleaf[3] + Rf[3 * m + 1] * vleaf[4] + Rf[3 * m + 2] * vleaf[5];
+    }
   });
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -1241,74 +1198,15 @@ struct IntegrateArgs {
 
 __global__ __launch_bounds__(256) void surrogate_integrate_kernel(IntegrateArgs a) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.nenv) return;
+  const bool live = e < a.nenv;
   const size_t N = (size_t)a.nenv;
-  const bool judged = (e % a.period) == a.phase;
-  const bool fb = a.fallback && a.fallback[e];
-  float qa[NV];
-  float err = 0.f;
+  float sur[NV];
+  if (live) {
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    const float sur = a.pred[(size_t)e * a.ldp + j] * a.ystd[j] + a.ymean[j];
-    float v = sur;
-    if (judged || fb) {
-      const float ref = a.qacc_aba[j * N + e];
-      err = fmaxf(err, fabsf(sur - ref) * frcp(fmaxf(fabsf(ref), 1.0f)));
-      v = ref;
-    }
-    qa[j] = v;
+    for (int j = 0; j < NV; ++j) sur[j] = a.pred[(size_t)e * a.ldp + j] * a.ystd[j] + a.ymean[j];
   }
-  if (judged) {
-    const bool reject = !(err <= a.tol);
-    if (a.fallback) a.fallback[e] = reject ? 1 : 0;
-    atomicAdd(&a.stats[0], 1u);
-    if (reject) atomicAdd(&a.stats[1], 1u);
-    atomicMax(&a.stats[2], __float_as_uint(err));
-  } else if (fb) {
-    atomicAdd(&a.stats[3], 1u);
-  }
-  if (a.qacc_out) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) a.qacc_out[j * N + e] = qa[j];
-  }
-  const float dt = a.dt;
-  float vw[3], om[3], bp[3], bq[4];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    vw[m] = a.qvel[m * N + e] + dt * qa[m];
-    om[m] = a.qvel[(3 + m) * N + e] + dt * qa[3 + m];
-    bp[m] = a.qpos[m * N + e] + dt * vw[m];
-  }
-#pragma unroll
-  for (int m = 0; m < 4; ++m) bq[m] = a.qpos[(3 + m) * N + e];
-  const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
-  const float angle = wn * dt;
-  if (angle >= 1e-12f) {
-    float sh, ch;
-    fsincos(0.5f * angle, &sh, &ch);
-    const float s = sh * frcp(angle / dt);
-    const float dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
-    const float nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
-    const float nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
-    const float ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
-    const float nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
-    const float inv = frsq(nw * nw + nx * nx + ny * ny + nz * nz);
-    bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
-  }
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    a.qvel[m * N + e] = vw[m];
-    a.qvel[(3 + m) * N + e] = om[m];
-    a.qpos[m * N + e] = bp[m];
-  }
-#pragma unroll
-  for (int m = 0; m < 4; ++m) a.qpos[(3 + m) * N + e] = bq[m];
-#pragma unroll
-  for (int h = 0; h < NU; ++h) {
-    const float v = a.qvel[(6 + h) * N + e] + dt * qa[6 + h];
-    a.qvel[(6 + h) * N + e] = v;
-    a.qpos[(7 + h) * N + e] = a.qpos[(7 + h) * N + e] + dt * v;
-  }
+  surrogate_commit(live, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback, a.stats, a.qacc_out,
+                   a.qpos, a.qvel);
 }
 
 // ------------------------------------------------------------------ reset
@@ -1443,7 +1341,8 @@ void g1_cuda_step(int nenv, int nsub, float* qpos, float* qvel, float* anchor, c
 }
 
 int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor, const float* ctrl,
-                  const float* mu, float* tau_out, float* contact_forces, void* stream) {
+                  const float* mu, float* tau_out, float* contact_forces, float* foot_pos, float* foot_vel,
+                  void* stream) {
   g_err[0] = 0;
   if (nenv < 0 || !(dt > 0.f)) { set_err("forces: bad nenv/dt"); return -1; }
   if (nenv == 0) return 0;
@@ -1451,7 +1350,7 @@ int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, floa
   g1k::ForcesArgs a;
   a.nenv = nenv; a.cfg = g1k::default_cfg(dt);
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl; a.mu = mu; a.tau = tau_out;
-  a.contact_forces = contact_forces;
+  a.contact_forces = contact_forces; a.foot_pos = foot_pos; a.foot_vel = foot_vel;
   hipLaunchKernelGGL(g1k::forces_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) { set_err("forces launch failed: %s", hipGetErrorString(err)); return -2; }

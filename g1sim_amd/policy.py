@@ -1,0 +1,185 @@
+"""Policy-in-the-loop rollouts on the device.
+
+The loop of examples/train.py:284-355 (without the PPO update), with every
+per-step piece a HIP kernel on one stream and no host synchronisation:
+
+    g1_policy_act     observation -> normalise/clip -> actor MLP (BF16 MFMA,
+                      fused, activations in LDS) -> sampled action -> ctrl;
+                      the push countdown and kick
+    physics           g1_hip_step_ex (FP32 ABA, nsub substeps), or the
+                      learned surrogate with its rotating ABA judge
+    g1_rollout_post   reward, termination, timers, reset mask, commands
+    g1_cuda_reset_done  resets the flagged envs (mask on the device)
+
+The reference computes `resets.any()` and index lists on the host every step
+(examples/train.py:334-344); here the reset kernel takes the device mask.
+
+Network: examples/train.py:21-29 (actor), nn.Linear's default initialisation
+U(+-1/sqrt(fan_in)) for weights and biases, log-std 0 (std 1,
+examples/train.py:42), observation normaliser at its initial state (mean 0,
+variance 1, clip 5; examples/train.py:44-47, 69-70).
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import MODEL, NC, NQ, NU, NV
+
+OBS_DIM = {0: 99, 1: 96}
+LAYERS = [(512, 128), (256, 512), (128, 256), (32, 128)]     # padded (out, in) of the fused kernel
+HIDDEN = [512, 256, 128]
+
+
+class Policy:
+    """Actor MLP obs -> 512 -> 256 -> 128 -> 29 (ELU), bf16 weights for the
+    fused kernel and fp32 copies for checking."""
+
+    def __init__(self, device, layout=1, seed=0, log_std=0.0, clip=5.0):
+        self.lib = _lib.load()
+        self.dev = torch.device(device)
+        self.layout = layout
+        self.nobs = OBS_DIM[layout]
+        rng = np.random.default_rng(seed)
+        true = [(HIDDEN[0], self.nobs), (HIDDEN[1], HIDDEN[0]), (HIDDEN[2], HIDDEN[1]), (NU, HIDDEN[2])]
+        self.W32, self.b32, self.W16, self.b = [], [], [], []
+        for (rows, cols), (out, fan_in) in zip(LAYERS, true):
+            bound = 1.0 / math.sqrt(fan_in)
+            w = np.zeros((rows, cols), np.float32)
+            w[:out, :fan_in] = rng.uniform(-bound, bound, (out, fan_in))
+            b = np.zeros(rows, np.float32)
+            b[:out] = rng.uniform(-bound, bound, out)
+            self.W32.append(w[:out, :fan_in].copy())
+            self.b32.append(b[:out].copy())
+            self.W16.append(torch.as_tensor(w, device=self.dev).to(torch.bfloat16).contiguous())
+            self.b.append(torch.as_tensor(b, device=self.dev))
+        self.obs_mean = torch.zeros(128, dtype=torch.float32, device=self.dev)
+        self.obs_var = torch.ones(128, dtype=torch.float32, device=self.dev)
+        self.clip = float(clip)
+        self.log_std = None if log_std is None else torch.full((NU,), float(log_std), dtype=torch.float32,
+                                                                device=self.dev)
+        self._ptrs = (_lib.ptr_array(self.W16), _lib.ptr_array(self.b))
+        self.update_normalizer(self.obs_mean, self.obs_var)
+
+    def update_normalizer(self, mean, var):
+        self.obs_mean.copy_(torch.as_tensor(mean, dtype=torch.float32))
+        self.obs_var.copy_(torch.as_tensor(var, dtype=torch.float32))
+        self.obs_inv = (1.0 / torch.sqrt(self.obs_var.double() + 1e-8)).float().contiguous()
+
+    def act(self, nenv, qpos, qvel, commands, last_ctrl, ctrl, actions=None, obs_out=None, push_interval=None,
+            seed=0, t=0, sample=True):
+        W, b = self._ptrs
+        s = torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.g1_policy_act(nenv, self.layout, qpos.data_ptr(), qvel.data_ptr(), commands.data_ptr(),
+                                    last_ctrl.data_ptr(), self.obs_mean.data_ptr(), self.obs_inv.data_ptr(),
+                                    self.clip, W, b,
+                                    self.log_std.data_ptr() if (sample and self.log_std is not None) else None,
+                                    seed & 0xFFFFFFFF, t, ctrl.data_ptr(),
+                                    None if actions is None else actions.data_ptr(),
+                                    None if obs_out is None else obs_out.data_ptr(),
+                                    None if push_interval is None else push_interval.data_ptr(), s)
+        _lib.check(rc, "g1_policy_act")
+
+
+class PolicyRollout:
+    """nenv envs driven by a policy for `steps`-step rollouts (one policy
+    step = nsub physics substeps at dt).  physics: "aba" (the fused FP32 step)
+    or "surrogate" (a trained surrogate.MLP with the rotating ABA judge)."""
+
+    def __init__(self, nenv, device, policy=None, physics="aba", surrogate=None, nsub=10, dt=2e-3, seed=0,
+                 max_episode_length=1000, cmd_lin=1.0, cmd_yaw=1.0, sample=True, pushes=True, period=16,
+                 tol=1e-3):
+        self.lib = _lib.load()
+        self.dev = torch.device(device)
+        self.nenv, self.nsub, self.dt, self.seed = nenv, nsub, float(dt), seed
+        self.policy = policy or Policy(device, layout=1, seed=seed)
+        self.physics = physics
+        self.max_episode_length, self.cmd_lin, self.cmd_yaw = max_episode_length, cmd_lin, cmd_yaw
+        self.sample, self.pushes = sample, pushes
+        f32 = dict(dtype=torch.float32, device=self.dev)
+        i32 = dict(dtype=torch.int32, device=self.dev)
+        u8 = dict(dtype=torch.uint8, device=self.dev)
+        self.qpos = torch.zeros((NQ, nenv), **f32)
+        self.qvel = torch.zeros((NV, nenv), **f32)
+        self.anchor = torch.full((2 * NC, nenv), 1e30, **f32)
+        self.ctrl = torch.zeros((NU, nenv), **f32)
+        self.actions = torch.zeros((NU, nenv), **f32)
+        self.prev_actions = torch.zeros((NU, nenv), **f32)
+        self.commands = torch.zeros((3, nenv), **f32)
+        self.command_timeout = torch.zeros(nenv, **i32)
+        self.episode_length = torch.zeros(nenv, **i32)
+        self.push_interval = torch.zeros(nenv, **i32)
+        self.reward = torch.zeros(nenv, **f32)
+        self.done = torch.zeros(nenv, **u8)
+        self.reset = torch.ones(nenv, **u8)
+        self.foot_pos = torch.zeros((6, nenv), **f32)
+        self.foot_vel = torch.zeros((6, nenv), **f32)
+        self.contact_forces = torch.zeros((8, nenv), **f32)
+        self.stepper = None
+        if physics == "surrogate":
+            from .surrogate import SurrogateStepper
+            if surrogate is None:
+                raise ValueError("physics='surrogate' needs a trained surrogate.MLP")
+            self.stepper = SurrogateStepper(surrogate, nenv, device, dt=dt, period=period, tol=tol)
+            self.foot_pos, self.foot_vel = self.stepper.foot_pos, self.stepper.foot_vel
+            self.contact_forces = self.stepper.contact_forces
+        elif physics != "aba":
+            raise ValueError(f"physics must be 'aba' or 'surrogate', not {physics!r}")
+        self.t = 0
+        s = self.stream
+        # examples/train.py:243, 248, 274-276
+        self.lib.g1_cuda_reset_done(nenv, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
+                                    self.reset.data_ptr(), self._seed(-1), 0.1, 0.02, s)
+        _lib.check(self.lib.g1_rollout_init(nenv, cmd_lin, cmd_yaw, seed & 0xFFFFFFFF, self.commands.data_ptr(),
+                                            self.command_timeout.data_ptr(), self.episode_length.data_ptr(),
+                                            self.push_interval.data_ptr() if pushes else None, s),
+                   "g1_rollout_init")
+
+    @property
+    def stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _seed(self, t):
+        return (self.seed * 0x9E3779B1 + (t + 2) * 0x85EBCA77) & 0xFFFFFFFF
+
+    def step(self):
+        """One policy step: act, nsub physics substeps, reward/termination,
+        bookkeeping and resets."""
+        lib, s, n, t = self.lib, self.stream, self.nenv, self.t
+        self.policy.act(n, self.qpos, self.qvel, self.commands, self.ctrl, self.ctrl, actions=self.actions,
+                        push_interval=self.push_interval if self.pushes else None, seed=self.seed, t=t,
+                        sample=self.sample)
+        if self.stepper is None:
+            rc = lib.g1_hip_step_ex(n, self.nsub, self.dt, self.qpos.data_ptr(), self.qvel.data_ptr(),
+                                    self.anchor.data_ptr(), self.ctrl.data_ptr(), self.foot_pos.data_ptr(),
+                                    self.foot_vel.data_ptr(), self.contact_forces.data_ptr(), None, None, None,
+                                    None, 0, s)
+            _lib.check(rc)
+        else:
+            for _ in range(self.nsub):
+                self.stepper.step(self.qpos, self.qvel, self.anchor, self.ctrl)
+        rc = lib.g1_rollout_post(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.foot_pos.data_ptr(),
+                                 self.foot_vel.data_ptr(), self.contact_forces.data_ptr(), self.actions.data_ptr(),
+                                 self.commands.data_ptr(), self.prev_actions.data_ptr(),
+                                 self.command_timeout.data_ptr(), self.episode_length.data_ptr(),
+                                 self.push_interval.data_ptr() if self.pushes else None, self.max_episode_length,
+                                 self.cmd_lin, self.cmd_yaw, self._seed(t) ^ 0x5bd1e995, t,
+                                 self.reward.data_ptr(), self.done.data_ptr(), self.reset.data_ptr(), s)
+        _lib.check(rc, "g1_rollout_post")
+        lib.g1_cuda_reset_done(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
+                               self.reset.data_ptr(), self._seed(t), 0.1, 0.02, s)
+        self.t += 1
+
+    def rollout(self, steps=24, reward_sums=None, done_counts=None):
+        """examples/train.py:284-355 without the buffer writes: prev_actions
+        start at zero; per-step reward sums / done counts (device tensors of
+        length >= steps, optional) are accumulated on the stream."""
+        _lib.check(self.lib.g1_mlp_fill(self.prev_actions.data_ptr(), 0.0, self.prev_actions.numel(), self.stream),
+                   "fill")
+        for k in range(steps):
+            self.step()
+            if reward_sums is not None:
+                reward_sums[k] += self.reward.sum()
+            if done_counts is not None:
+                done_counts[k] += self.done.sum()

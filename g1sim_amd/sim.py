@@ -108,25 +108,11 @@ class G1Sim:
         """The 99-dim RL observation (src/g1sim.py:132-170): commands (3),
         projected gravity (3), base linear velocity in the base frame (3), base
         angular velocity (3), joint position error (29), joint velocities (29),
-        previous actions (29).  Returns (nenv, 99)."""
-        q = self.qpos[3:7, :]
-        qw = q[0]
-        q_vec_inv = -q[1:4]
-
-        def quat_rotate_inverse(v):
-            uv = torch.cross(q_vec_inv, v, dim=0)
-            uuv = torch.cross(q_vec_inv, uv, dim=0)
-            return v + 2.0 * (qw * uv + uuv)
-
-        proj_gravity = quat_rotate_inverse(self.global_gravity)
-        base_lin_vel = quat_rotate_inverse(self.qvel[0:3, :])
-        obs = torch.cat([
-            self.commands,
-            proj_gravity,
-            base_lin_vel,
-            self.qvel[3:6, :],
-            self.qpos[7:36, :] - self.default_joint_pos,
-            self.qvel[6:35, :],
-            self.last_actions,
-        ], dim=0).T
+        previous actions (29).  Returns (nenv, 99), built by one HIP kernel
+        (g1_hip_get_obs) on the current stream."""
+        obs = torch.empty((self.nenv, 99), dtype=torch.float32, device=self.device)
+        rc = self.lib.g1_hip_get_obs(self.nenv, 0, self.qpos.data_ptr(), self.qvel.data_ptr(),
+                                     self.commands.data_ptr(), self.last_actions.data_ptr(), obs.data_ptr(),
+                                     _stream_ptr(self.device))
+        _lib.check(rc, "g1_hip_get_obs")
         return obs

@@ -180,6 +180,29 @@ class MLP:
         self.adam(lr, B=B)
         return sse
 
+    # ------------------------------------------------------ fused inference
+    @property
+    def fusable(self):
+        return [d[:2] for d in self.dims] == [(HID, PIN), (HID, HID), (HID, HID), (HID, HID), (POUT, HID)]
+
+    def _wb(self):
+        # the bf16 weights and the biases are updated in place, so their
+        # device pointers are fixed for the life of the network
+        if getattr(self, "_wb_cache", None) is None:
+            self._wb_cache = (_lib.ptr_array(self.W16), _lib.ptr_array(self.b))
+        return self._wb_cache
+
+    def forward_state(self, nenv, qpos, qvel, tau, pred):
+        """Fused forward of every env of a SoA state (features built in the
+        kernel): pred fp32 [>= nenv][ldp >= 64], z-scored (35 meaningful)."""
+        assert self.fusable, "the fused kernel is built for 106 -> 512 x4 -> 35"
+        W, b = self._wb()
+        rc = self.lib.g1_hip_surrogate_forward(nenv, qpos.data_ptr(), qvel.data_ptr(), tau.data_ptr(),
+                                               self.xmean.data_ptr(), self.xinv.data_ptr(), W, b, pred.data_ptr(),
+                                               pred.shape[1], self.stream)
+        self._check(rc, "surrogate_forward")
+        return pred
+
     def set_normalization(self, xmean, xstd, ymean, ystd):
         self.xmean.copy_(torch.as_tensor(xmean, dtype=torch.float32))
         self.xinv.copy_(1.0 / torch.as_tensor(xstd, dtype=torch.float32).clamp_min(1e-6))
@@ -310,8 +333,9 @@ class SurrogateStepper:
     1/period of the envs every step and as per-env fallback after a rejected
     judgement (g1_hip_surrogate_integrate)."""
 
-    def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0):
+    def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True):
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
+        self.fused = fused and mlp.fusable
         self.lib = _lib.load()
         self.dev = torch.device(device)
         self.rows = (nenv + 127) // 128 * 128
@@ -326,6 +350,9 @@ class SurrogateStepper:
         self.pred = torch.zeros((self.rows, POUT), **f32)
         self.ystd = (1.0 / mlp.yinv).contiguous()
         self.judge_flags = judge_flags
+        self.foot_pos = torch.zeros((6, nenv), **f32)
+        self.foot_vel = torch.zeros((6, nenv), **f32)
+        self.contact_forces = torch.zeros((8, nenv), **f32)
         self.t = 0
 
     def step(self, qpos, qvel, anchor, ctrl, mu=None):
@@ -338,15 +365,26 @@ class SurrogateStepper:
                                  None if mu is None else mu.data_ptr(), self.list.data_ptr(), self.count.data_ptr(),
                                  self.qacc_aba.data_ptr(), self.judge_flags, s), "step_judge")
         ck(lib.g1_hip_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
-                             None if mu is None else mu.data_ptr(), self.tau.data_ptr(), None, s), "forces")
-        rc = lib.g1_mlp_features(n, self.rows, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
-                                 self.mlp.xmean.data_ptr(), self.mlp.xinv.data_ptr(), self.X16.data_ptr(), PIN, s)
-        ck(rc, "features")
-        self.mlp.forward(self.X16, self.rows, train=False, out=self.pred)
-        ck(lib.g1_hip_surrogate_integrate(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
-                                          self.pred.data_ptr(), POUT, self.mlp.ymean.data_ptr(), self.ystd.data_ptr(),
-                                          self.qacc_aba.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(),
-                                          None, s), "surrogate_integrate")
+                             None if mu is None else mu.data_ptr(), self.tau.data_ptr(),
+                             self.contact_forces.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(), s),
+           "forces")
+        m = self.mlp
+        if self.fused:
+            W, b = m._wb()
+            ck(lib.g1_hip_surrogate_step(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
+                                         self.tau.data_ptr(), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b,
+                                         m.ymean.data_ptr(), self.ystd.data_ptr(), self.qacc_aba.data_ptr(),
+                                         self.fallback.data_ptr(), self.stats.data_ptr(), None, s), "surrogate_step")
+        else:
+            rc = lib.g1_mlp_features(n, self.rows, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
+                                     m.xmean.data_ptr(), m.xinv.data_ptr(), self.X16.data_ptr(), PIN, s)
+            ck(rc, "features")
+            m.forward(self.X16, self.rows, train=False, out=self.pred)
+            ck(lib.g1_hip_surrogate_integrate(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(),
+                                              qvel.data_ptr(), self.pred.data_ptr(), POUT, m.ymean.data_ptr(),
+                                              self.ystd.data_ptr(), self.qacc_aba.data_ptr(),
+                                              self.fallback.data_ptr(), self.stats.data_ptr(), None, s),
+               "surrogate_integrate")
         self.t += 1
 
     def summary(self):

@@ -126,11 +126,13 @@ int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, c
 
 /* Total generalized force (servo + limits + contact J^T f, as tau_out of
  * g1_hip_step_ex) and the contact update (anchors, normal forces) of one step,
- * without ABA and without integration: the surrogate's input.  mu and
- * contact_forces are nullable. */
+ * without ABA and without integration: the surrogate's input.  foot_pos /
+ * foot_vel ([6][nenv], bodies 6 and 12, world frame, as g1_cuda_step reports
+ * them) are the feet before the step.  mu, contact_forces, foot_pos and
+ * foot_vel are nullable. */
 int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor,
                   const float* ctrl, const float* mu, float* tau_out, float* contact_forces,
-                  void* stream);
+                  float* foot_pos, float* foot_vel, void* stream);
 /* Device list of the envs that ABA steps this step: e % period == phase (the
  * rotating judge) or fallback[e] != 0 (nullable).  list: int [nenv]; *count
  * is reset and filled on the device. */
@@ -156,6 +158,74 @@ int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float 
                                float* qvel, const float* pred, int ldp, const float* ymean,
                                const float* ystd, const float* qacc_aba, uint8_t* fallback,
                                unsigned* stats, float* qacc_out, void* stream);
+
+/* ---- fused network inference (no reference counterpart) ------------------
+ * One 512-thread workgroup per 64 envs runs a whole MLP with its activations
+ * in LDS.  Weights are bf16 row-major [out][in] device arrays, biases fp32;
+ * W and b are HOST arrays of device pointers. */
+
+/* Surrogate forward: features as g1_mlp_features (106 z-scored inputs), then
+ * 128 -> 512 x4 (SiLU) -> 64 with W = {[512][128], [512][512] x3,
+ * [>=64][512]} (input columns >= 106 and output rows >= 35 zero) and
+ * b = {[512] x4, [>=64]}.  pred: fp32 [nenv][ldp], ldp >= 64; the same
+ * values as the layer-by-layer GEMMs (g1_mlp_gemm). */
+int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, const float* tau,
+                             const float* xmean, const float* xinv, const void* const* W,
+                             const float* const* b, float* pred, int ldp, void* stream);
+/* Surrogate step: g1_hip_surrogate_forward followed by the commit of
+ * g1_hip_surrogate_integrate (same judge, fallback, statistics and
+ * integration), in one kernel; qpos / qvel are updated in place. */
+int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
+                          const float* tau, const float* xmean, const float* xinv, const void* const* W,
+                          const float* const* b, const float* ymean, const float* ystd,
+                          const float* qacc_aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                          void* stream);
+
+/* RL observation of every env (replaces the torch code of
+ * src/g1sim.py:132-170).  layout 0: 99 values per env in the reference order
+ * (commands 3, projected gravity 3, base linear velocity in the base frame 3,
+ * base angular velocity 3, q - stand 29, qd 29, last ctrl 29); layout 1: 96
+ * values (base angular velocity, projected gravity, commands, q - stand, qd,
+ * last ctrl).  commands [3][nenv], last_ctrl [29][nenv]; obs fp32
+ * [nenv][99 or 96]. */
+int g1_hip_get_obs(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
+                   const float* last_ctrl, float* obs, void* stream);
+
+/* Policy act (examples/train.py:21-29, 69-70, 86-95, 284-310): observation
+ * (layout as g1_hip_get_obs), z-scored with obs_mean / obs_inv
+ * (1 / sqrt(var + 1e-8), >= 99 entries) and clipped to +-clip, then
+ * 128 -> 512 -> 256 -> 128 (ELU) -> 32 with W = {[512][128], [256][512],
+ * [128][256], [>=32][128]}; action j = out_j (+ exp(log_std_j) N(0, 1) when
+ * log_std is non-null, drawn from (seed, t, env)); ctrl = stand + action *
+ * action_scale ([29][nenv]).  actions ([29][nenv]) and obs_out
+ * ([nenv][nobs], the raw observation) are nullable.  push_interval
+ * (nullable, int [nenv]) counts down; at <= 0 the base velocity qvel[0:6]
+ * gets a uniform kick of +-(0.5, 0.5, 0.4, 0.52, 0.52, 0.78) and a new
+ * interval in [50, 150) (examples/train.py:301-310).  last_ctrl may alias
+ * ctrl. */
+int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const float* commands,
+                  const float* last_ctrl, const float* obs_mean, const float* obs_inv, float clip,
+                  const void* const* W, const float* const* b, const float* log_std, unsigned seed, int t,
+                  float* ctrl, float* actions, float* obs_out, int* push_interval, void* stream);
+
+/* Reward and termination (examples/train.py:143-225) of the post-step state,
+ * then the rollout bookkeeping (examples/train.py:319-351): prev_actions <-
+ * actions (0 for reset envs), command_timeout - 1, episode_length + 1,
+ * reset = done | episode_length >= max_episode_length; reset envs get new
+ * commands (x, y ~ U(+-cmd_lin), yaw ~ U(+-cmd_yaw)), a timer in [150, 250),
+ * episode_length 0 and a push interval in [50, 150) (push_interval
+ * nullable); envs whose timer ran out get new commands and timer.  reward
+ * fp32 [nenv], done and reset uint8 [nenv]; the caller resets the envs
+ * flagged in `reset` (g1_cuda_reset_done). */
+int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float* foot_pos,
+                    const float* foot_vel, const float* contact_forces, const float* actions, float* commands,
+                    float* prev_actions, int* command_timeout, int* episode_length, int* push_interval,
+                    int max_episode_length, float cmd_lin, float cmd_yaw, unsigned seed, int t,
+                    float* reward, uint8_t* done, uint8_t* reset, void* stream);
+/* Rollout start: commands for every env, timers 0, push intervals in
+ * [50, 150) (push_interval nullable). */
+int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float* commands,
+                    int* command_timeout, int* episode_length, int* push_interval, void* stream);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char* g1_hip_last_error(void);

@@ -23,18 +23,23 @@ def _state(device, n, seed, steps=40):
 
 
 def test_forces_kernel_matches_step(device):
-    """g1_hip_forces gives the step's tau_total, anchors and normal forces."""
+    """g1_hip_forces gives the step's tau_total, anchors, normal forces and
+    foot kinematics."""
     st = _state(device, 4096, 21)
     lib = _lib.load()
     s = torch.cuda.current_stream(device).cuda_stream
     anchor2 = st.anchor.clone()
     tau = torch.zeros((35, st.n), device=device)
     fn = torch.zeros((8, st.n), device=device)
+    fp = torch.zeros((6, st.n), device=device)
+    fv = torch.zeros((6, st.n), device=device)
     rc = lib.g1_hip_forces(st.n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), anchor2.data_ptr(),
-                           st.ctrl.data_ptr(), None, tau.data_ptr(), fn.data_ptr(), s)
+                           st.ctrl.data_ptr(), None, tau.data_ptr(), fn.data_ptr(), fp.data_ptr(), fv.data_ptr(), s)
     _lib.check(rc)
     st.step(1, 2e-3, tau=True)
     h = st.host()
+    np.testing.assert_allclose(fp.cpu().numpy(), h["foot_pos"], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(fv.cpu().numpy(), h["foot_vel"], rtol=1e-5, atol=2e-5)
     t = tau.cpu().numpy()
     scale = np.maximum(np.abs(h["tau"]), 1.0)
     assert (np.abs(t - h["tau"]) / scale).max() < 1e-3
@@ -171,3 +176,62 @@ def test_trained_network_gain(device):
     assert after["median_rel_err"] < 0.2 * before["median_rel_err"]
     assert np.isfinite(ha["qpos"]).all()
     assert summ["judged"] == 50 * n // 16
+
+
+def _trained_small(device, seed=4):
+    ds = generate_dataset(device, nenv=4096, steps=40, seed=seed)
+    mlp = MLP(device, seed=seed)
+    fit_normalization(mlp, ds)
+    train(mlp, ds, iters=100, batch=4096)
+    return mlp
+
+
+def test_fused_forward_equals_layer_gemms(device):
+    """The fused kernel (features + 5 layers, activations in LDS) gives the
+    same fp32 outputs as the layer-by-layer MFMA GEMMs: same bf16 operands,
+    same accumulation order."""
+    mlp = _trained_small(device)
+    st = _state(device, 1000, 25)                  # 1000 envs: a partial last workgroup
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    tau = torch.zeros((35, st.n), device=device)
+    _lib.check(lib.g1_hip_forces(st.n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.clone().data_ptr(),
+                                 st.ctrl.data_ptr(), None, tau.data_ptr(), None, None, None, s))
+    rows = 1024
+    X16 = torch.zeros((rows, 128), dtype=torch.bfloat16, device=device)
+    _lib.check(lib.g1_mlp_features(st.n, rows, st.qpos.data_ptr(), st.qvel.data_ptr(), tau.data_ptr(),
+                                   mlp.xmean.data_ptr(), mlp.xinv.data_ptr(), X16.data_ptr(), 128, s))
+    ref = mlp.forward(X16, rows, train=False).clone()
+    fused = torch.full((st.n, 64), 7.0, device=device)
+    mlp.forward_state(st.n, st.qpos, st.qvel, tau, fused)
+    torch.cuda.synchronize()
+    a, b = fused.cpu().numpy(), ref[:st.n, :64].cpu().numpy()
+    diff = np.abs(a - b)
+    write_metrics("surrogate_fused_forward", {"max_abs_diff": float(diff.max()), "equal_frac": float((a == b).mean())})
+    assert (a == b).mean() > 0.999
+    assert diff.max() <= 1e-5 * max(1.0, np.abs(b).max())
+
+
+def test_fused_step_equals_unfused(device):
+    """Surrogate steps through the fused kernel and through features + GEMMs +
+    integrate agree on state, fallback flags and statistics."""
+    mlp = _trained_small(device, seed=6)
+    n = 3000
+    qp, qv, an, cb = W.randomized_actuation(n, seed=26, nblocks=1)
+    runs = []
+    for fused in (True, False):
+        st = GpuState(qp, qv, an, cb[0], device)
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=0.5, fused=fused)
+        assert stepper.fused == fused
+        for _ in range(5):
+            stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
+        torch.cuda.synchronize()
+        runs.append((st.host(), stepper.fallback.cpu().numpy(), stepper.stats.cpu().numpy()))
+    (ha, fa, sa), (hb, fb, sb) = runs
+    for k in ("qpos", "qvel", "anchor"):
+        d = np.abs(ha[k] - hb[k])
+        fin = np.isfinite(hb[k])
+        assert np.array_equal(np.isfinite(ha[k]), fin), k
+        assert d[fin].max(initial=0) <= 1e-4 * max(1.0, np.abs(hb[k][fin]).max(initial=0)), k
+    assert (fa != fb).sum() <= 2
+    assert abs(int(sa[0]) - int(sb[0])) == 0 and abs(int(sa[1]) - int(sb[1])) <= 2

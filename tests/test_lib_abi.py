@@ -74,3 +74,32 @@ def test_zero_envs_is_a_no_op():
     lib = _lib.load()
     assert lib.g1_hip_step_ex(0, 5, 2e-3, None, None, None, None, None, None, None, None, None,
                               None, None, 0, None) == 0
+
+
+def test_fused_inference_entry_points_validate_arguments():
+    """Argument checks of the fused inference and rollout entry points run on
+    the host, before any launch."""
+    lib = _lib.load()
+    d = C.c_void_p(16)
+    ws = _lib.ptr_array([16] * 5)
+    wmiss = _lib.ptr_array([16, 16, 0, 16, 16])
+    # ldp below the 64 output columns, a missing weight, a bad period
+    assert lib.g1_hip_surrogate_forward(8, d, d, d, d, d, ws, ws, d, 35, None) == -1
+    assert lib.g1_hip_surrogate_forward(8, d, d, d, d, d, wmiss, ws, d, 64, None) == -1
+    assert lib.g1_hip_surrogate_forward(0, d, d, d, d, d, ws, ws, d, 64, None) == 0
+    assert lib.g1_hip_surrogate_step(8, 2e-3, 3, 2, 1e-3, d, d, d, d, d, ws, ws, d, d, d, None, None, None,
+                                     None) == -1
+    assert lib.g1_hip_surrogate_step(8, 0.0, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, None, None, None,
+                                     None) == -1
+    # observation layout, policy clip and weights
+    assert lib.g1_hip_get_obs(8, 2, d, d, d, d, d, None) == -1
+    assert lib.g1_hip_get_obs(0, 0, d, d, d, d, d, None) == 0
+    w4 = _lib.ptr_array([16] * 4)
+    assert lib.g1_policy_act(8, 1, d, d, d, d, d, d, 0.0, w4, w4, None, 0, 0, d, None, None, None, None) == -1
+    assert lib.g1_policy_act(8, 3, d, d, d, d, d, d, 5.0, w4, w4, None, 0, 0, d, None, None, None, None) == -1
+    assert lib.g1_policy_act(8, 1, d, d, d, d, d, d, 5.0, _lib.ptr_array([16, 0, 16, 16]), w4, None, 0, 0, d,
+                             None, None, None, None) == -1
+    # rollout bookkeeping needs every buffer except push_interval, and a positive episode length
+    assert lib.g1_rollout_post(8, d, d, d, d, d, d, d, d, d, d, None, 0, 1.0, 0.5, 0, 0, d, d, d, None) == -1
+    assert lib.g1_rollout_post(8, d, d, d, d, d, None, d, d, d, d, None, 10, 1.0, 0.5, 0, 0, d, d, d, None) == -1
+    assert lib.g1_rollout_init(8, 1.0, 0.5, 0, None, d, d, None, None) == -1

@@ -1,0 +1,155 @@
+"""Pins of the RL oracle (oracle/rl_oracle.py) on the CPU: each check uses
+an independent formulation or a closed form, so a dropped term, a wrong sign
+or a transposed operand in the oracle fails one of them."""
+import numpy as np
+import pytest
+import torch
+
+from g1sim_amd.model import MODEL
+from oracle import rl_oracle as R
+
+DEFAULT = MODEL.qpos_stand[7:36]
+
+
+def quat(axis, angle):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    return np.concatenate([[np.cos(angle / 2)], np.sin(angle / 2) * axis])
+
+
+def rotmat(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_quat_rotate_inverse_is_transpose_rotation():
+    rng = np.random.default_rng(0)
+    q = rng.normal(size=(4, 50)); q /= np.linalg.norm(q, axis=0)
+    v = rng.normal(size=(3, 50))
+    got = R.quat_rotate_inverse(q, v)
+    for k in range(50):
+        np.testing.assert_allclose(got[:, k], rotmat(q[:, k]).T @ v[:, k], atol=1e-12)
+    # 90 degrees of yaw: world +x is body -y
+    q90 = quat([0, 0, 1], np.pi / 2).reshape(4, 1)
+    np.testing.assert_allclose(R.quat_rotate_inverse(q90, np.array([[1.0], [0], [0]]))[:, 0], [0, -1, 0],
+                               atol=1e-12)
+
+
+def _state(n, rng):
+    qpos = np.zeros((36, n)); qpos[2] = 0.75; qpos[3] = 1.0; qpos[7:] = DEFAULT[:, None]
+    qvel = np.zeros((35, n))
+    return qpos, qvel
+
+
+def test_get_obs_layouts_and_identity_pose():
+    rng = np.random.default_rng(1)
+    n = 7
+    qpos, qvel = _state(n, rng)
+    qvel[:] = rng.normal(size=qvel.shape)
+    qpos[7:] += rng.normal(size=(29, n)) * 0.1
+    cmd = rng.normal(size=(3, n)); last = rng.normal(size=(29, n))
+    o99 = R.get_obs(qpos, qvel, cmd, last, DEFAULT, layout=0)
+    o96 = R.get_obs(qpos, qvel, cmd, last, DEFAULT, layout=1)
+    assert o99.shape == (n, 99) and o96.shape == (n, 96)
+    # identity orientation: gravity straight down, body velocity = world velocity
+    np.testing.assert_allclose(o99[:, 3:6], np.tile([0, 0, -1.0], (n, 1)))
+    np.testing.assert_allclose(o99[:, 6:9], qvel[0:3].T)
+    np.testing.assert_allclose(o99[:, 0:3], cmd.T)
+    np.testing.assert_allclose(o99[:, 9:12], qvel[3:6].T)
+    np.testing.assert_allclose(o99[:, 12:41], (qpos[7:] - DEFAULT[:, None]).T)
+    np.testing.assert_allclose(o99[:, 41:70], qvel[6:].T)
+    np.testing.assert_allclose(o99[:, 70:99], last.T)
+    # the 96 layout is a permutation of the 99 layout without linear velocity
+    perm = list(range(9, 12)) + list(range(3, 6)) + list(range(0, 3)) + list(range(12, 99))
+    np.testing.assert_allclose(o96, o99[:, perm])
+
+
+def test_reward_closed_form_standing_still():
+    """Zero command, upright, at rest in the stand pose, both feet loaded at
+    clearance: every tracking term is 1 and every penalty 0 -> reward 6."""
+    n = 3
+    qpos, qvel = _state(n, None)
+    cmd = np.zeros((3, n)); last = np.zeros((29, n))
+    obs = R.get_obs(qpos, qvel, cmd, last, DEFAULT)
+    foot_pos = np.zeros((6, n)); foot_vel = np.zeros((6, n)); cf = np.full((8, n), 40.0)
+    a = np.zeros((n, 29))
+    r, d = R.reward_and_done(obs, qpos, foot_pos, foot_vel, cf, a, a, MODEL.jnt_lo, MODEL.jnt_hi)
+    np.testing.assert_allclose(r, 6.0, atol=1e-12)
+    assert not d.any()
+
+
+def test_reward_terms_individually():
+    n = 1
+    qpos, qvel = _state(n, None)
+    cmd = np.array([[0.5], [0.0], [0.0]])            # walking (speed 0.5), base at rest
+    last = np.zeros((29, n))
+    foot_pos = np.zeros((6, n)); foot_pos[2] = 0.1; foot_pos[5] = 0.1     # clearance term zero
+    foot_vel = np.zeros((6, n)); foot_vel[0] = 0.3                        # left foot sliding in x
+    cf = np.zeros((8, n)); cf[0:4] = 1.0                                  # left contact (sum 4 > 1)
+    a = np.full((n, 29), 0.1); pa = np.zeros((n, 29))
+    obs = R.get_obs(qpos, qvel, cmd, last, DEFAULT)
+    r, d = R.reward_and_done(obs, qpos, foot_pos, foot_vel, cf, a, pa, MODEL.jnt_lo, MODEL.jnt_hi)
+    expect = (2 * np.exp(-0.25 / 0.25) + 2 * 1.0 + 1.0 + 1.0       # lin vel error 0.5^2
+              - 0.5 * 0.09                                         # slip: lv2 = 0.09, in contact, active
+              - 0.05 * 29 * 0.01)                                  # action rate
+    np.testing.assert_allclose(r, expect, atol=1e-12)
+    # a joint beyond its soft upper limit costs its overshoot
+    j = 3
+    lo, hi = MODEL.jnt_lo[j], MODEL.jnt_hi[j]
+    qpos2 = qpos.copy(); qpos2[7 + j] = hi - 0.05 * (hi - lo) + 0.2
+    obs2 = R.get_obs(qpos2, qvel, cmd, last, DEFAULT)
+    r2, _ = R.reward_and_done(obs2, qpos2, foot_pos, foot_vel, cf, a, pa, MODEL.jnt_lo, MODEL.jnt_hi)
+    pose_err = (qpos2[7 + j, 0] - DEFAULT[j]) ** 2 / R.STD_WALKING[j] ** 2 / 29
+    np.testing.assert_allclose(r - r2, 0.2 + (1.0 - np.exp(-pose_err)), atol=1e-12)
+
+
+def test_termination_at_70_degrees():
+    n = 4
+    qpos, qvel = _state(n, None)
+    lim = 70.0 * 3.14159 / 180.0
+    for k, ang in enumerate([0.0, lim - 1e-3, lim + 1e-3, np.pi / 2]):
+        qpos[3:7, k] = quat([1, 0, 0], ang)
+    obs = R.get_obs(qpos, qvel, np.zeros((3, n)), np.zeros((29, n)), DEFAULT)
+    z = np.zeros((6, n))
+    _, d = R.reward_and_done(obs, qpos, z, z, np.zeros((8, n)), np.zeros((n, 29)), np.zeros((n, 29)),
+                             MODEL.jnt_lo, MODEL.jnt_hi)
+    assert list(d) == [False, False, True, True]
+
+
+def test_policy_forward_matches_torch_modules():
+    """The oracle's actor against torch.nn (float64) with the same weights."""
+    rng = np.random.default_rng(3)
+    dims = [96, 512, 256, 128, 29]
+    Ws = [rng.normal(size=(dims[i + 1], dims[i])) / np.sqrt(dims[i]) for i in range(4)]
+    bs = [rng.normal(size=dims[i + 1]) * 0.1 for i in range(4)]
+    x = rng.normal(size=(16, 96))
+    net = torch.nn.Sequential(torch.nn.Linear(96, 512), torch.nn.ELU(), torch.nn.Linear(512, 256), torch.nn.ELU(),
+                              torch.nn.Linear(256, 128), torch.nn.ELU(), torch.nn.Linear(128, 29)).double()
+    with torch.no_grad():
+        for lin, W, b in zip([net[0], net[2], net[4], net[6]], Ws, bs):
+            lin.weight.copy_(torch.as_tensor(W)); lin.bias.copy_(torch.as_tensor(b))
+        ref = net(torch.as_tensor(x)).numpy()
+    np.testing.assert_allclose(R.policy_mean(x, Ws, bs), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_normalize_and_ctrl():
+    obs = np.array([[0.0, 10.0, -10.0]])
+    np.testing.assert_allclose(R.normalize_obs(obs, np.zeros(3), np.ones(3)), [[0.0, 5.0, -5.0]])
+    a = np.zeros((2, 29)); a[1, 0] = 1.0
+    c = R.ctrl_from_action(a, DEFAULT)
+    np.testing.assert_allclose(c[:, 0], DEFAULT)
+    np.testing.assert_allclose(c[0, 1], DEFAULT[0] + R.ACTION_SCALE[0])
+
+
+@pytest.mark.parametrize("done,ct,el,expect_reset,expect_resample", [
+    (False, 5, 3, False, False),      # nothing happens
+    (True, 5, 3, True, True),         # died
+    (False, 5, 999, True, True),      # episode timeout (max 1000)
+    (False, 1, 3, False, True),       # command timer ran out
+])
+def test_bookkeeping_cases(done, ct, el, expect_reset, expect_resample):
+    r, ct2, el2, rs = R.bookkeeping(np.array([done]), np.array([ct]), np.array([el]), 1000)
+    assert r[0] == expect_reset and rs[0] == expect_resample
+    assert ct2[0] == ct - 1
+    assert el2[0] == (0 if expect_reset else el + 1)
