@@ -40,6 +40,31 @@ POLICY_FLOP = 433408               # 96 -> 512 -> 256 -> 128 -> 29 actor forward
 CTRL_PERIOD = 10
 
 
+def accuracy_vs_fp64(sample):
+    """qacc of one GPU step against the FP64 oracle from the same float32
+    state (teacher forcing): per env the max over dofs of |a - b| / max(|b|, 1)
+    (tests/test_vs_mujoco.py:74-75).  The reference algorithm evaluated in
+    float32 on the CPU is scored the same way: the yardstick of what float32
+    arithmetic reaches on these states."""
+    from oracle import oracle as O
+    before = sample["before"]
+    ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], 2e-3,
+                           want=("qacc",))["qacc"]
+    n = ref.shape[1]
+    q32 = np.zeros((35, n))
+    args = [np.ascontiguousarray(before[k], np.float32) for k in ("qpos", "qvel", "anchor", "ctrl")]
+    O.lib32().g1o_step_batch_f32(n, 2e-3, *(O.fptr(a) for a in args), None, O.dptr(q32), None, None, None, None,
+                                 None, None, None, 0)
+
+    def score(q):
+        e = (np.abs(q - ref) / np.maximum(np.abs(ref), 1.0)).max(axis=0)
+        return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "median": float(np.median(e))}
+
+    return {"metric": "max over dofs of |qacc - qacc_fp64| / max(|qacc_fp64|, 1), per env",
+            "sample": f"{n} envs after the timed region, one step at dt 2e-3 (teacher forced)",
+            "gpu_fp32": score(sample["qacc"]), "reference_algorithm_in_float32": score(q32)}
+
+
 def cpu_baseline(nenv_gpu):
     """Reference physics core (src/g1_core.h, float, OpenMP one env per
     iteration) on the host cores, on a bounded sample of the workload."""
@@ -119,6 +144,36 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     # the same pipeline with every judgement accepted: the cost of the
     # surrogate path itself, independent of the network's accuracy
     wall_acc, _, (qpos, qvel, tau) = run(1e30)
+    # the surrogate's own evaluation configuration: 65,536 envs x 1,000 steps,
+    # 4,096 judged per step, state drift against plain ABA from the same start
+    n4 = 65536
+    qp4, qv4, an4, cb4 = W.randomized_actuation(n4, seed=4, nblocks=100)
+    ctrl4 = torch.as_tensor(cb4, device=dev)
+    sq, sv, sa = (torch.as_tensor(a, device=dev) for a in (qp4, qv4, an4))
+    aq, av, aa = (torch.as_tensor(a, device=dev) for a in (qp4, qv4, an4))
+    st4 = SurrogateStepper(mlp, n4, dev, period=16, tol=1e-3)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    for s in range(1000):
+        st4.step(sq, sv, sa, ctrl4[s // CTRL_PERIOD])
+    torch.cuda.synchronize()
+    wall4 = time.perf_counter() - t4
+    from g1sim_amd import _lib
+    lib = _lib.load()
+    for blk in range(100):
+        rc = lib.g1_hip_step_ex(n4, CTRL_PERIOD, 2e-3, aq.data_ptr(), av.data_ptr(), aa.data_ptr(),
+                                ctrl4[blk].data_ptr(), None, None, None, None, None, None, None, 0,
+                                torch.cuda.current_stream(dev).cuda_stream)
+        if rc:
+            raise RuntimeError(_lib.last_error())
+    torch.cuda.synchronize()
+    s4 = st4.summary()
+    dpos = (sq[0:3] - aq[0:3]).norm(dim=0).cpu().numpy()
+    eval4 = {"nenv": n4, "steps": 1000, "judged_per_step": n4 // 16, "env_steps_per_s": n4 * 1000 / wall4,
+             "aba_frac": s4["aba_frac"], "reject_frac": s4["reject_frac"],
+             "fallback_envs_at_end": s4["fallback_envs_now"],
+             "base_pos_drift_vs_aba_m": {"median": float(np.median(dpos)), "max": float(dpos.max())}}
+    del sq, sv, sa, aq, av, aa, ctrl4, st4
     # the fused network kernel alone (features + 5 layers), HIP events
     pred = torch.empty((nenv, 64), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -134,7 +189,7 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
             "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps",
             "value_all_accepted": nenv * steps / wall_acc, "ms_per_step_all_accepted": wall_acc * 1e3 / steps,
             "aba_frac": summ["aba_frac"], "reject_frac": summ["reject_frac"],
-            "fallback_envs_at_end": summ["fallback_envs_now"],
+            "fallback_envs_at_end": summ["fallback_envs_now"], "evaluation_65536x1000": eval4,
             "roofline": {"bound": "mfma", "kernel": "g1inf::surrogate_kernel<0> (features + 5 layers)",
                          "achieved": k_tflops, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
                          "frac": k_tflops * 1e12 / PEAK_BF16, "avg_launch_ms": k_ms,
@@ -278,6 +333,20 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     sub_arr = np.array(subs)
+    # accuracy sample: one more step of the final state with the qacc output
+    acc_sample = None
+    if not args.no_cpu_baseline:
+        blk = ctrl[(args.warmup + args.steps) // CTRL_PERIOD]
+        before = {"qpos": qpos.cpu().numpy(), "qvel": qvel.cpu().numpy(), "anchor": anchor.cpu().numpy(),
+                  "ctrl": blk.cpu().numpy()}
+        qp2, qv2, an2 = qpos.clone(), qvel.clone(), anchor.clone()
+        qacc = torch.zeros((35, nenv), dtype=torch.float32, device=dev)
+        rc = lib.g1_hip_step_ex(nenv, 1, 2e-3, qp2.data_ptr(), qv2.data_ptr(), an2.data_ptr(), blk.data_ptr(),
+                                None, None, None, qacc.data_ptr(), None, None, None,
+                                args.flags & ~_lib.G1_FLAG_F64_BASE, sptr)
+        if rc:
+            raise RuntimeError(_lib.last_error())
+        acc_sample = {"before": before, "qacc": qacc.cpu().numpy().astype(np.float64)}
     h = qpos[2].cpu().numpy()
     ok = bool(np.isfinite(qpos.cpu().numpy()).all() and ((h > 0.2) & (h < 1.2)).mean() > 0.99)
 
@@ -328,6 +397,7 @@ def main():
             out["policy_in_the_loop"] = policy_leg(dev, mlp)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nenv)
+        out["max_qacc_rel_err_vs_fp64"] = accuracy_vs_fp64(acc_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -6,8 +6,8 @@
 // activations held in LDS (two 64 x 520 bf16 buffers, 133 KB), and finishes
 // with a fused epilogue, so no activation ever goes to HBM.  Weights are
 // read from L2 (1.7 MB for the surrogate, 0.43 MB for the policy), each
-// lane loading 16 contiguous bytes of a weight row as its B fragment, with a
-// one-step register prefetch.
+// lane loading 16 contiguous bytes of a weight row as its B fragment, two
+// k-steps ahead of its use.
 //
 // * Surrogate step: z-scored (q, qd, tau_total) -> 512 x4 SiLU -> 35, then
 //   the per-env commit (judge / fallback / semi-implicit Euler) of
@@ -31,6 +31,7 @@
 #include "g1_model_kernel.h"
 #include "g1_fastmath.h"
 #include "g1_commit.h"
+#include "g1_act.h"
 
 namespace g1inf {
 
@@ -72,29 +73,40 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
 #pragma unroll
     for (int c = 0; c < T::CT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const bf16* wp = W + (size_t)(c0 * 16 + lr) * NIN + lk;
-  bf16x8 fb[2][T::CT];
+  const bf16* ap = sin + (r0 * 16 + lr) * LDA + lk;
+  // Software pipeline: weight fragments (L2) run PF k-steps ahead in a ring
+  // of PF + 1, activation fragments (LDS) one step ahead; a scheduling
+  // barrier per k-step keeps the compiler from sinking the prefetches next
+  // to their uses.
+  constexpr int PF = KS >= 3 ? 2 : 1, NB = PF + 1;
+  bf16x8 fb[NB][T::CT], fa[2][T::RT];
 #pragma unroll
-  for (int c = 0; c < T::CT; ++c) fb[0][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN);
+  for (int k = 0; k < PF && k < KS; ++k)
+#pragma unroll
+    for (int c = 0; c < T::CT; ++c)
+      fb[k][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN + k * 32);
+#pragma unroll
+  for (int r = 0; r < T::RT; ++r) fa[0][r] = *reinterpret_cast<const bf16x8*>(ap + r * 16 * LDA);
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
+    if (k + PF < KS) {
+#pragma unroll
+      for (int c = 0; c < T::CT; ++c)
+        fb[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN + (k + PF) * 32);
+    }
     if (k + 1 < KS) {
 #pragma unroll
-      for (int c = 0; c < T::CT; ++c)
-        fb[(k + 1) & 1][c] = *reinterpret_cast<const bf16x8*>(wp + (size_t)c * 16 * NIN + (k + 1) * 32);
+      for (int r = 0; r < T::RT; ++r)
+        fa[(k + 1) & 1][r] = *reinterpret_cast<const bf16x8*>(ap + r * 16 * LDA + (k + 1) * 32);
     }
-    bf16x8 fa[T::RT];
-#pragma unroll
-    for (int r = 0; r < T::RT; ++r)
-      fa[r] = *reinterpret_cast<const bf16x8*>(sin + ((r0 + r) * 16 + lr) * LDA + k * 32 + lk);
 #pragma unroll
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[r], fb[k & 1][c], acc[r][c], 0, 0, 0);
+        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], fb[k % NB][c], acc[r][c], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
-
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
 
 // hidden layer: out = act(in W^T + b) as bf16 into LDS
 template <int NIN, int NOUT, int ACT>
@@ -115,8 +127,8 @@ __device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __rest
         const int row = (r0 + r) * 16 + (lane >> 4) * 4 + q;
         const float z = acc[r][c][q] + bc;
         float h;
-        if constexpr (ACT == ACT_SILU) h = z * sigmoidf(z);
-        else h = z > 0.f ? z : expm1f(z);
+        if constexpr (ACT == ACT_SILU) h = g1act::silu(z);
+        else h = g1act::elu(z);
         sout[row * LDA + col] = (bf16)h;
       }
   }

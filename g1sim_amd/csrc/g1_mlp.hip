@@ -21,6 +21,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "g1_act.h"
+
 namespace g1mlp {
 
 typedef __bf16 bf16;
@@ -50,8 +52,6 @@ struct GemmArgs {
   const float* bias;     // [J]
   const float* Zin;      // pre-activation for SILU_BWD, leading dimension ldc
 };
-
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
 
 template <bool TRANS>
 __device__ __forceinline__ void stage(bf16 (*s)[LDR], const bf16* g, int ld, int row0, int r0, int tid) {
@@ -133,16 +133,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         } else if constexpr (EPI == EPI_BIAS_SILU) {
           const float z = v + g.bias[j];
           g.C[o] = z;
-          g.H[o] = (bf16)(z * sigmoidf(z));
+          g.H[o] = (bf16)g1act::silu(z);
         } else if constexpr (EPI == EPI_BIAS_SILU_H) {
           const float z = v + g.bias[j];
-          g.H[o] = (bf16)(z * sigmoidf(z));
+          g.H[o] = (bf16)g1act::silu(z);
         } else if constexpr (EPI == EPI_BIAS_F32) {
           g.C[o] = v + g.bias[j];
         } else if constexpr (EPI == EPI_SILU_BWD) {
-          const float z = g.Zin[o];
-          const float sg = sigmoidf(z);
-          const float d = v * sg * (1.0f + z * (1.0f - sg));
+          const float d = v * g1act::silu_grad(g.Zin[o]);
           g.C[o] = d;
           g.H[o] = (bf16)d;
         }

@@ -1163,10 +1163,20 @@ __global__ __launch_bounds__(256) void forces_kernel(ForcesArgs a) {
 
 // The envs whose qacc ABA computes this step: every period-th env starting at
 // `phase` (the rotating judge) plus every env in fallback.
+// One atomic per wave: the wave's selected envs take consecutive slots in
+// ascending env order (their rank among the wave's set bits), so the list is
+// sorted within each wave's run and the judge step's loads stay coalesced.
 __global__ void judge_list_kernel(int nenv, int phase, int period, const uint8_t* fallback, int* list, int* count) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nenv) return;
-  if ((e % period) == phase || (fallback && fallback[e])) list[atomicAdd(count, 1)] = e;
+  const bool sel = e < nenv && ((e % period) == phase || (fallback && fallback[e]));
+  const unsigned long long m = __ballot(sel);
+  if (m == 0ull) return;                                   // uniform across the wave
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (sel) list[base + __popcll(m & ((1ull << lane) - 1ull))] = e;
 }
 
 __global__ void zero_int_kernel(int* x, int n) {

@@ -1,0 +1,106 @@
+"""Contact-stress rollout (131,072 envs x 1,000 steps at dt 2e-3, targets
+resampled every 10 steps, a horizontal push every 100 steps, per-env friction
+U(0.4, 1.2)): all-FP32 against the FP32 step with the float64 islands, each
+measured for drift against FP64 oracle rollouts of a 256-env sample from the
+same float32 start, and for throughput."""
+import numpy as np
+import pytest
+import torch
+
+from g1sim_amd import _lib, workloads as W
+from oracle import oracle as O
+
+from .gpu_helpers import GpuState, write_metrics
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, NSAMPLE = 131072, 1000, 256
+MODES = {"fp32": 0,
+         "fp64_solve+base": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE,
+         "fp64_solve+base+contact": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE | _lib.G1_FLAG_F64_CONTACT}
+
+
+def _quat_angle(q, p):
+    d = np.abs(np.sum(q * p, axis=0)) / (np.linalg.norm(q, axis=0) * np.linalg.norm(p, axis=0))
+    return 2.0 * np.arccos(np.clip(d, 0.0, 1.0))
+
+
+def _fp64_rollouts(qp, qv, an, cb, mu, pushes, envs):
+    out = np.zeros((36, envs.size))
+    for i, e in enumerate(envs):
+        q, v, a = qp[:, e].astype(np.float64), qv[:, e].astype(np.float64), an[:, e].astype(np.float64)
+        for blk in range(STEPS // 10):
+            s = blk * 10
+            if s % 100 == 0:
+                v[0:2] += pushes[s // 100][:, e]
+            r = O.step_full(q, v, cb[blk][:, e], a, 10, 2e-3, mu=float(mu[e]))
+            q, v, a = r["qpos"], r["qvel"], r["anchor"]
+        out[:, i] = q
+    return out
+
+
+def test_contact_stress_drift_and_throughput(device):
+    qp, qv, an, cb, mu, pushes = W.contact_stress(N, seed=2, nblocks=STEPS // 10, npush=STEPS // 100)
+    envs = np.arange(0, N, N // NSAMPLE)
+    ref = _fp64_rollouts(qp, qv, an, cb, mu, pushes, envs)
+    ctrl_dev = torch.as_tensor(cb, device=device)
+    push_dev = torch.as_tensor(pushes, device=device)
+    lib = _lib.load()
+    # load every kernel variant once so the timed runs exclude module loading
+    for flags in MODES.values():
+        warm = GpuState(qp[:, :128], qv[:, :128], an[:, :128], cb[0][:, :128], device, mu=mu[:128])
+        if flags & _lib.G1_FLAG_F64_BASE:
+            warm.enable_f64_base()
+        warm.step(1, 2e-3, flags=flags)
+    torch.cuda.synchronize()
+    metrics = {"nenv": N, "steps": STEPS, "sample": int(envs.size)}
+    finals = {}
+    def rollout(flags):
+        st = GpuState(qp, qv, an, cb[0], device, mu=mu)
+        if flags & _lib.G1_FLAG_F64_BASE:
+            st.enable_f64_base()
+        stream = torch.cuda.current_stream(device)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for blk in range(STEPS // 10):
+            s = blk * 10
+            if s % 100 == 0:
+                st.qvel[0:2] += push_dev[s // 100]
+            rc = lib.g1_hip_step_ex(N, 10, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
+                                    ctrl_dev[blk].data_ptr(), None, None, None, None, None, st.mu.data_ptr(),
+                                    None if st.base64 is None else st.base64.data_ptr(), flags, stream.cuda_stream)
+            _lib.check(rc)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return st, e0.elapsed_time(e1)
+
+    for name, flags in MODES.items():
+        rollout(flags)                  # the GPU idled through the CPU rollouts: bring its clocks up first
+        st, ms = rollout(flags)
+        q = st.qpos.cpu().numpy()
+        assert np.isfinite(q).all(), name
+        z = q[2]
+        qs = q[:, envs].astype(np.float64)
+        pos = np.linalg.norm(qs[0:3] - ref[0:3], axis=0)
+        ang = _quat_angle(qs[3:7], ref[3:7])
+        jnt = np.abs(qs[7:] - ref[7:]).max(axis=0)
+        finals[name] = qs
+        metrics[name] = {
+            "env_steps_per_s": N * STEPS / (ms * 1e-3), "ms": ms,
+            "standing_frac": float(((z > 0.5) & (z < 1.2)).mean()),
+            "drift_base_pos_m": {"median": float(np.median(pos)), "p90": float(np.percentile(pos, 90)),
+                                 "max": float(pos.max())},
+            "drift_base_angle_rad": {"median": float(np.median(ang)), "p90": float(np.percentile(ang, 90))},
+            "drift_joint_max_rad": {"median": float(np.median(jnt)), "p90": float(np.percentile(jnt, 90))},
+        }
+    # how far the FP32 and island runs end apart (rounding-level differences
+    # amplified by contact chaos over 1,000 steps)
+    d_isl = np.linalg.norm(finals["fp32"][0:3] - finals["fp64_solve+base"][0:3], axis=0)
+    metrics["fp32_vs_islands_base_pos_m_median"] = float(np.median(d_isl))
+    write_metrics("contact_stress_131072x1000", metrics)
+    # only the foot spheres collide (src/g1_model.h:339-350), so a fallen
+    # robot's pelvis may end below the plane; the sample must still track the
+    # FP64 rollouts for most envs
+    for name in MODES:
+        assert metrics[name]["drift_base_pos_m"]["median"] < 0.05, name
