@@ -33,8 +33,8 @@ HIDDEN = [512, 256, 128]
 
 
 class Policy:
-    """Actor MLP obs -> 512 -> 256 -> 128 -> 29 (ELU), bf16 weights for the
-    fused kernel and fp32 copies for checking."""
+    """Actor MLP obs -> 512 -> 256 -> 128 -> 29 (ELU): bf16 weights (row-major
+    W16 and the fused kernel's k-tiled Wt) and fp32 copies for checking."""
 
     def __init__(self, device, layout=1, seed=0, log_std=0.0, clip=5.0):
         self.lib = _lib.load()
@@ -59,7 +59,9 @@ class Policy:
         self.clip = float(clip)
         self.log_std = None if log_std is None else torch.full((NU,), float(log_std), dtype=torch.float32,
                                                                 device=self.dev)
-        self._ptrs = (_lib.ptr_array(self.W16), _lib.ptr_array(self.b))
+        from .surrogate import k_tiled
+        self.Wt = [k_tiled(w, rows) for w, (rows, _) in zip(self.W16, LAYERS)]
+        self._ptrs = (_lib.ptr_array(self.Wt), _lib.ptr_array(self.b))
         self.update_normalizer(self.obs_mean, self.obs_var)
 
     def update_normalizer(self, mean, var):

@@ -25,6 +25,14 @@ PIN, POUT = 128, 128
 EPI = dict(F32=0, F32_ATOMIC=1, BIAS_SILU=2, BIAS_SILU_H=3, BIAS_F32=4, SILU_BWD=5)
 
 
+def k_tiled(w16, nout):
+    """bf16 [out][in] -> the fused kernels' layout [in/32][nout][32] (first
+    nout rows)."""
+    rows, cols = w16.shape
+    assert nout <= rows and cols % 32 == 0
+    return w16[:nout].reshape(nout, cols // 32, 32).permute(1, 0, 2).contiguous()
+
+
 def _pad128(n):
     return (n + 127) // 128 * 128
 
@@ -186,10 +194,13 @@ class MLP:
         return [d[:2] for d in self.dims] == [(HID, PIN), (HID, HID), (HID, HID), (HID, HID), (POUT, HID)]
 
     def _wb(self):
-        # the bf16 weights and the biases are updated in place, so their
-        # device pointers are fixed for the life of the network
-        if getattr(self, "_wb_cache", None) is None:
-            self._wb_cache = (_lib.ptr_array(self.W16), _lib.ptr_array(self.b))
+        """Weights in the fused kernel's k-tiled layout [in/32][out][32]
+        (rebuilt after every optimizer step) and the bias pointers."""
+        if getattr(self, "_wb_t", None) != self.t:
+            outs = [HID, HID, HID, HID, 64]
+            self._Wt = [k_tiled(w, o) for w, o in zip(self.W16, outs)]
+            self._wb_cache = (_lib.ptr_array(self._Wt), _lib.ptr_array(self.b))
+            self._wb_t = self.t
         return self._wb_cache
 
     def forward_state(self, nenv, qpos, qvel, tau, pred):

@@ -161,12 +161,13 @@ int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float 
 
 /* ---- fused network inference (no reference counterpart) ------------------
  * One 512-thread workgroup per 64 envs runs a whole MLP with its activations
- * in LDS.  Weights are bf16 row-major [out][in] device arrays, biases fp32;
- * W and b are HOST arrays of device pointers. */
+ * in LDS.  Weights are bf16 device arrays in the k-tiled layout
+ * [in / 32][out][32] (element (o, i) at ((i / 32) * out + o) * 32 + i % 32),
+ * biases fp32; W and b are HOST arrays of device pointers. */
 
 /* Surrogate forward: features as g1_mlp_features (106 z-scored inputs), then
- * 128 -> 512 x4 (SiLU) -> 64 with W = {[512][128], [512][512] x3,
- * [>=64][512]} (input columns >= 106 and output rows >= 35 zero) and
+ * 128 -> 512 x4 (SiLU) -> 64 with W of (in, out) = (128, 512), (512, 512) x3,
+ * (512, 64) (input columns >= 106 and output rows >= 35 zero) and
  * b = {[512] x4, [>=64]}.  pred: fp32 [nenv][ldp], ldp >= 64; the same
  * values as the layer-by-layer GEMMs (g1_mlp_gemm). */
 int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, const float* tau,
@@ -194,8 +195,8 @@ int g1_hip_get_obs(int nenv, int layout, const float* qpos, const float* qvel, c
 /* Policy act (examples/train.py:21-29, 69-70, 86-95, 284-310): observation
  * (layout as g1_hip_get_obs), z-scored with obs_mean / obs_inv
  * (1 / sqrt(var + 1e-8), >= 99 entries) and clipped to +-clip, then
- * 128 -> 512 -> 256 -> 128 (ELU) -> 32 with W = {[512][128], [256][512],
- * [128][256], [>=32][128]}; action j = out_j (+ exp(log_std_j) N(0, 1) when
+ * 128 -> 512 -> 256 -> 128 (ELU) -> 32 with W of (in, out) = (128, 512),
+ * (512, 256), (256, 128), (128, 32); action j = out_j (+ exp(log_std_j) N(0, 1) when
  * log_std is non-null, drawn from (seed, t, env)); ctrl = stand + action *
  * action_scale ([29][nenv]).  actions ([29][nenv]) and obs_out
  * ([nenv][nobs], the raw observation) are nullable.  push_interval
