@@ -243,6 +243,38 @@ def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
     return out
 
 
+def kernel_only(kind, nenv, steps, warmup):
+    """One fused inference kernel alone (random weights, randomized-actuation
+    state), timed with HIP events: for profiling runs."""
+    import torch
+    from g1sim_amd import workloads as W
+    dev = torch.device("cuda:0")
+    qp, qv, _, _ = W.randomized_actuation(nenv, seed=1, nblocks=1)
+    qpos, qvel = torch.as_tensor(qp, device=dev), torch.as_tensor(qv, device=dev)
+    if kind == "surrogate":
+        from g1sim_amd.surrogate import MLP
+        mlp = MLP(dev, seed=0)
+        tau = torch.randn((35, nenv), device=dev)
+        pred = torch.empty((nenv, 64), device=dev)
+        fn = lambda: mlp.forward_state(nenv, qpos, qvel, tau, pred)  # noqa: E731
+        flop, name = SURR_FLOP, "g1inf::surrogate_kernel<0>"
+    else:
+        from g1sim_amd.policy import Policy
+        pol = Policy(dev, layout=1, seed=0)
+        cmd = torch.zeros((3, nenv), device=dev)
+        ctrl = torch.zeros((29, nenv), device=dev)
+        act = torch.zeros((29, nenv), device=dev)
+        fn = lambda: pol.act(nenv, qpos, qvel, cmd, ctrl, ctrl, actions=act, seed=1, t=0)  # noqa: E731
+        flop, name = POLICY_FLOP, "g1inf::policy_kernel"
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ms = _event_avg_ms(fn, steps, torch.cuda.current_stream(dev))
+    tf = flop * nenv / (ms * 1e-3) / 1e12
+    print(json.dumps({"kernel": name, "nenv": nenv, "launches": steps, "avg_launch_ms": ms,
+                      "achieved_tflops": tf, "frac_bf16_peak": tf * 1e12 / PEAK_BF16}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +290,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
+    ap.add_argument("--kernel", choices=("surrogate", "policy"),
+                    help="time one fused inference kernel alone at --nenv (for profiling) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -268,6 +302,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.kernel:
+        kernel_only(args.kernel, args.nenv, args.steps, args.warmup)
+        return
     import torch
     from g1sim_amd import _lib, workloads as W
 
