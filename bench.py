@@ -31,7 +31,8 @@ import numpy as np  # noqa: E402
 
 METRIC = "env-steps/s @16k envs: FP32 ABA vs surrogate; max qacc rel err vs FP64"
 FLOP_PER_ENV_STEP = 41491          # reference formulation, stance (8 contacts)
-BYTES_PER_ENV_LAUNCH = 892         # SoA loads + stores of one launch (nsub independent)
+BYTES_PER_ENV_LAUNCH = 812         # SoA loads (qpos, qvel, anchor, ctrl) + stores (qpos, qvel, anchor) of
+                                   # one launch without the optional foot/contact outputs (nsub independent)
 PEAK_FP32 = 157.3e12               # MI355X FP32 (vector and matrix), dense
 PUBLISHED_16K = 12.8e6             # reference README.md:38-42, RTX 3080 Laptop, 16,384 envs
 PEAK_BF16 = 2.5e15                 # MI355X dense BF16 MFMA
