@@ -276,6 +276,37 @@ def kernel_only(kind, nenv, steps, warmup):
                       "achieved_tflops": tf, "frac_bf16_peak": tf * 1e12 / PEAK_BF16}), flush=True)
 
 
+def measure_peaks():
+    """Measured machine peaks on this device: non-packed FP32 FMA, dense BF16
+    MFMA (16x16x32, register operands), and HBM copy bandwidth."""
+    import torch
+    from g1sim_amd import _lib
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    props = torch.cuda.get_device_properties(dev)
+    stream = torch.cuda.current_stream(dev)
+    cus = props.multi_processor_count
+    out = torch.empty(cus * 8 * 256, dtype=torch.float32, device=dev)
+    res = {"device": props.name, "compute_units": cus}
+    # best over 2 and 8 waves per SIMD (4-wave blocks, 2 or 8 per CU)
+    for name, fn, iters, per_wave in (("fp32_fma_tflops", lib.g1_hip_peak_fp32, 20000, 64 * 8 * 2),
+                                      ("bf16_mfma_tflops", lib.g1_hip_peak_bf16, 1000, 16 * 16384)):
+        best = 0.0
+        for blocks in (cus * 2, cus * 8):
+            launch = lambda: _lib.check(fn(blocks, iters, out.data_ptr(), stream.cuda_stream), name)  # noqa: E731
+            launch()
+            ms = _event_avg_ms(launch, 10, stream)
+            best = max(best, blocks * 4 * iters * per_wave / (ms * 1e-3) / 1e12)
+        res[name] = best
+    src = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)     # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ms = _event_avg_ms(lambda: dst.copy_(src), 10, stream)
+    res["hbm_copy_tbps"] = 2 * src.numel() * 4 / (ms * 1e-3) / 1e12
+    res["published"] = {"fp32_tflops": PEAK_FP32 / 1e12, "bf16_mfma_tflops": PEAK_BF16 / 1e12, "hbm_tbps": 8.0}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +322,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
+    ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
     ap.add_argument("--kernel", choices=("surrogate", "policy"),
                     help="time one fused inference kernel alone at --nenv (for profiling) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -305,6 +337,9 @@ def main():
 
     if args.kernel:
         kernel_only(args.kernel, args.nenv, args.steps, args.warmup)
+        return
+    if args.peaks:
+        print(json.dumps(measure_peaks()), flush=True)
         return
     import torch
     from g1sim_amd import _lib, workloads as W
@@ -429,6 +464,9 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_ENV_LAUNCH * nenv},
         "sanity_standing": ok,
     }
+    peaks = measure_peaks()
+    out["measured_peaks"] = peaks
+    out["roofline"]["frac_of_measured_fp32_fma"] = achieved / (peaks["fp32_fma_tflops"] * 1e12)
     if not args.no_surrogate and args.flags == 0:
         out["surrogate"], mlp = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
         if not args.no_policy:

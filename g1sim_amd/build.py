@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libg1sim_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", n) for n in ("g1_step.hip", "g1_mlp.hip", "g1_infer.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("g1_step.hip", "g1_mlp.hip", "g1_infer.hip", "g1_peaks.hip")]
 ARCH = os.environ.get("G1SIM_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

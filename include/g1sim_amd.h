@@ -228,6 +228,13 @@ int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float*
 int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float* commands,
                     int* command_timeout, int* episode_length, int* push_interval, void* stream);
 
+/* ---- machine-peak microbenchmarks (measurement only) -----------------------
+ * Non-packed FP32 FMA (blocks x 256 threads x iters x 8 FMAs) and dense BF16
+ * MFMA (blocks x 4 waves x iters x 16 v_mfma_f32_16x16x32_bf16) on register
+ * operands; out: float [blocks * 256]. */
+int g1_hip_peak_fp32(int blocks, int iters, float* out, void* stream);
+int g1_hip_peak_bf16(int blocks, int iters, float* out, void* stream);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* g1_hip_last_error(void);
 
