@@ -66,9 +66,11 @@ def accuracy_vs_fp64(sample):
             "gpu_fp32": score(sample["qacc"]), "reference_algorithm_in_float32": score(q32)}
 
 
-def cpu_baseline(nenv_gpu):
+def cpu_baseline(nenv_gpu, acc_sample=None):
     """Reference physics core (src/g1_core.h, float, OpenMP one env per
-    iteration) on the host cores, on a bounded sample of the workload."""
+    iteration) on the host cores, on a bounded sample of the workload; with
+    acc_sample, also the FP64 oracle's score of the GPU's qacc on it.
+    Returns (baseline, accuracy or None)."""
     from g1sim_amd import workloads as W
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0))
@@ -86,10 +88,11 @@ def cpu_baseline(nenv_gpu):
         res[threads] = (nenv * steps / t, nenv, steps)
     v_all, n_all, s_all = res[cores]
     v_one, n_one, s_one = res[1]
-    return {"value": v_all, "unit": "env-steps/s", "cores": cores, "kind": kind,
+    base = {"value": v_all, "unit": "env-steps/s", "cores": cores, "kind": kind,
             "sample": f"{n_all} envs x {s_all} steps of the same workload (dt 2e-3, fixed targets) "
                       f"on {cores} threads; 1-thread rate from {n_one} envs x {s_one} steps",
             "value_1core": v_one}
+    return base, (accuracy_vs_fp64(acc_sample) if acc_sample is not None else None)
 
 
 def _event_avg_ms(fn, reps, stream):
@@ -472,8 +475,7 @@ def main():
         if not args.no_policy:
             out["policy_in_the_loop"] = policy_leg(dev, mlp)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(nenv)
-        out["max_qacc_rel_err_vs_fp64"] = accuracy_vs_fp64(acc_sample)
+        out["cpu_baseline"], out["max_qacc_rel_err_vs_fp64"] = cpu_baseline(nenv, acc_sample)
     print(json.dumps(out), flush=True)
 
 

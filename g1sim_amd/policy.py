@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .model import MODEL, NC, NQ, NU, NV
+from .model import NC, NQ, NU, NV
 
 OBS_DIM = {0: 99, 1: 96}
 LAYERS = [(512, 128), (256, 512), (128, 256), (32, 128)]     # padded (out, in) of the fused kernel
