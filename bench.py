@@ -148,6 +148,12 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     # the same pipeline with every judgement accepted: the cost of the
     # surrogate path itself, independent of the network's accuracy
     wall_acc, _, (qpos, qvel, tau) = run(1e30)
+    # looser acceptance tolerances than the configuration's 1e-3, for context
+    sweep = []
+    for tol in (1e-1, 1.0):
+        w_t, s_t, _ = run(tol)
+        sweep.append({"tol": tol, "env_steps_per_s": nenv * steps / w_t, "aba_frac": s_t["aba_frac"],
+                      "reject_frac": s_t["reject_frac"], "fallback_envs_at_end": s_t["fallback_envs_now"]})
     # the surrogate's own evaluation configuration: 65,536 envs x 1,000 steps,
     # 4,096 judged per step, state drift against plain ABA from the same start
     n4 = 65536
@@ -188,7 +194,8 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     return {"value": nenv * steps / wall, "unit": "env-steps/s", "ms_per_step": wall * 1e3 / steps,
             "judge": "rotating 1/16 of envs per step, per-env fallback at rel. qacc err > 1e-3",
             "heldout_accept_frac": acc["accept_frac"], "heldout_median_rel_err": acc["median_rel_err"],
-            "heldout_p90_rel_err": acc["p90_rel_err"], "train_iters": train_iters,
+            "heldout_p90_rel_err": acc["p90_rel_err"], "heldout_accept_frac_at": acc["accept_frac_at"],
+            "tolerance_sweep": sweep, "train_iters": train_iters,
             "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train,
             "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps",
             "value_all_accepted": nenv * steps / wall_acc, "ms_per_step_all_accepted": wall_acc * 1e3 / steps,

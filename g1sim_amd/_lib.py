@@ -74,6 +74,8 @@ def load():
     lib.g1_policy_act.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, pa, pa, vp, u, i, vp, vp, vp, vp, vp]
     lib.g1_rollout_post.restype = i
     lib.g1_rollout_post.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, u, i, vp, vp, vp, vp]
+    lib.g1_policy_obs_norm_update.restype = i
+    lib.g1_policy_obs_norm_update.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_rollout_init.restype = i
     lib.g1_rollout_init.argtypes = [i, f, f, u, vp, vp, vp, vp, vp]
     for name in ("g1_hip_peak_fp32", "g1_hip_peak_bf16"):
@@ -93,7 +95,7 @@ def exported_symbols():
             "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
             "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_surrogate_integrate",
             "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs", "g1_policy_act",
-            "g1_rollout_post", "g1_rollout_init", "g1_hip_peak_fp32", "g1_hip_peak_bf16"]
+            "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32", "g1_hip_peak_bf16"]
 
 
 def ptr_array(tensors_or_ptrs):

@@ -403,6 +403,70 @@ __global__ __launch_bounds__(256) void obs_kernel(int nenv, int layout, int nobs
   obs[i] = obs_feature(layout, f, (size_t)nenv, e, qpos, qvel, commands, last_ctrl);
 }
 
+// ------------------------------------------------- observation normaliser
+// examples/train.py:49-66 (ActorCritic.update_obs_norm, called by act()
+// before the observation is normalised): batch mean and biased variance over
+// the envs, merged into the running statistics by Chan's formula.  Pass 1:
+// workgroup (f, chunk) sums feature f over one chunk of envs about the
+// running mean, in float64; pass 2 adds the chunks and merges.
+constexpr int NORM_CHUNKS = 64;
+
+__global__ __launch_bounds__(256) void obs_batch_stats_kernel(int nenv, int layout, int chunk, const float* qpos,
+                                                              const float* qvel, const float* commands,
+                                                              const float* last_ctrl, const float* run_mean,
+                                                              double* part) {
+  __shared__ double s1[256], s2[256];
+  const int f = blockIdx.x, ch = blockIdx.y;
+  const double c = run_mean[f];
+  const int e0 = ch * chunk, e1 = min(nenv, e0 + chunk);
+  double a = 0.0, b = 0.0;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const double x = (double)obs_feature(layout, f, (size_t)nenv, e, qpos, qvel, commands, last_ctrl) - c;
+    a += x;
+    b += x * x;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      s1[threadIdx.x] += s1[threadIdx.x + off];
+      s2[threadIdx.x] += s2[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[(size_t)(f * NORM_CHUNKS + ch) * 2] = s1[0];
+    part[(size_t)(f * NORM_CHUNKS + ch) * 2 + 1] = s2[0];
+  }
+}
+
+// mean, var: [nobs] running statistics, count: [1] running count (all fp32,
+// updated in place); inv = 1 / sqrt(var + 1e-8) for the policy kernel.
+__global__ void obs_norm_merge_kernel(int nenv, int nobs, int nchunks, const double* part, float* mean, float* var,
+                                      float* count, float* inv) {
+  const int f = threadIdx.x;
+  const double n = (double)nenv, cnt = (double)count[0], tot = cnt + n;
+  if (f < nobs) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      s1 += part[(size_t)(f * NORM_CHUNKS + ch) * 2];
+      s2 += part[(size_t)(f * NORM_CHUNKS + ch) * 2 + 1];
+    }
+    const double m = mean[f], v = var[f];
+    const double d = s1 / n;                                   // batch mean - running mean
+    const double bvar = fmax(s2 / n - d * d, 0.0);             // biased batch variance
+    const double nm = m + d * n / tot;
+    const double M2 = v * cnt + bvar * n + d * d * cnt * n / tot;
+    const double nv = M2 / tot;
+    mean[f] = (float)nm;
+    var[f] = (float)nv;
+    inv[f] = (float)(1.0 / sqrt((double)(float)nv + 1e-8));
+  }
+  __syncthreads();
+  if (f == 0) count[0] = (float)tot;
+}
+
 // ---------------------------------------------------------------- rollout
 // examples/train.py:256-270: per-joint posture tolerances while walking / running
 __constant__ float kStdWalking[NU] = {
@@ -653,6 +717,22 @@ int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float
   if (nenv == 0) return 0;
   hipLaunchKernelGGL(rollout_init_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv,
                      cmd_lin, cmd_yaw, seed, commands, command_timeout, episode_length, push_interval);
+  return launch_status();
+}
+
+int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
+                              const float* last_ctrl, float* mean, float* var, float* count, float* inv,
+                              double* scratch, void* stream) {
+  if (nenv <= 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !mean || !var ||
+      !count || !inv || !scratch)
+    return -1;
+  const int nobs = layout == 0 ? 99 : 96;
+  const int chunk = (nenv + NORM_CHUNKS - 1) / NORM_CHUNKS;
+  const int nchunks = (nenv + chunk - 1) / chunk;
+  hipLaunchKernelGGL(obs_batch_stats_kernel, dim3(nobs, nchunks), dim3(256), 0, (hipStream_t)stream, nenv, layout,
+                     chunk, qpos, qvel, commands, last_ctrl, mean, scratch);
+  hipLaunchKernelGGL(obs_norm_merge_kernel, dim3(1), dim3(128), 0, (hipStream_t)stream, nenv, nobs, nchunks, scratch,
+                     mean, var, count, inv);
   return launch_status();
 }
 

@@ -16,8 +16,9 @@ The reference computes `resets.any()` and index lists on the host every step
 
 Network: examples/train.py:21-29 (actor), nn.Linear's default initialisation
 U(+-1/sqrt(fan_in)) for weights and biases, log-std 0 (std 1,
-examples/train.py:42), observation normaliser at its initial state (mean 0,
-variance 1, clip 5; examples/train.py:44-47, 69-70).
+examples/train.py:42), observation normaliser starting at mean 0, variance 1,
+count 1e-4 and updated from every batch before it is applied, clip 5
+(examples/train.py:44-47, 49-70, 86-88).
 """
 import math
 
@@ -56,6 +57,9 @@ class Policy:
             self.b.append(torch.as_tensor(b, device=self.dev))
         self.obs_mean = torch.zeros(128, dtype=torch.float32, device=self.dev)
         self.obs_var = torch.ones(128, dtype=torch.float32, device=self.dev)
+        self.obs_inv = torch.ones(128, dtype=torch.float32, device=self.dev)
+        self.obs_count = torch.full((1,), 1e-4, dtype=torch.float32, device=self.dev)   # examples/train.py:46
+        self._norm_scratch = torch.zeros(2 * 64 * 128, dtype=torch.float64, device=self.dev)
         self.clip = float(clip)
         self.log_std = None if log_std is None else torch.full((NU,), float(log_std), dtype=torch.float32,
                                                                 device=self.dev)
@@ -65,9 +69,20 @@ class Policy:
         self.update_normalizer(self.obs_mean, self.obs_var)
 
     def update_normalizer(self, mean, var):
+        """Sets the running statistics (mean, variance) of the observation."""
         self.obs_mean.copy_(torch.as_tensor(mean, dtype=torch.float32))
         self.obs_var.copy_(torch.as_tensor(var, dtype=torch.float32))
-        self.obs_inv = (1.0 / torch.sqrt(self.obs_var.double() + 1e-8)).float().contiguous()
+        self.obs_inv.copy_((1.0 / torch.sqrt(self.obs_var.double() + 1e-8)).float())
+
+    def update_obs_norm(self, nenv, qpos, qvel, commands, last_ctrl):
+        """examples/train.py:49-66 on the device: merges this batch of
+        observations into the running mean / variance / count."""
+        s = torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.g1_policy_obs_norm_update(nenv, self.layout, qpos.data_ptr(), qvel.data_ptr(),
+                                                commands.data_ptr(), last_ctrl.data_ptr(), self.obs_mean.data_ptr(),
+                                                self.obs_var.data_ptr(), self.obs_count.data_ptr(),
+                                                self.obs_inv.data_ptr(), self._norm_scratch.data_ptr(), s)
+        _lib.check(rc, "g1_policy_obs_norm_update")
 
     def act(self, nenv, qpos, qvel, commands, last_ctrl, ctrl, actions=None, obs_out=None, push_interval=None,
             seed=0, t=0, sample=True):
@@ -91,14 +106,14 @@ class PolicyRollout:
 
     def __init__(self, nenv, device, policy=None, physics="aba", surrogate=None, nsub=10, dt=2e-3, seed=0,
                  max_episode_length=1000, cmd_lin=1.0, cmd_yaw=1.0, sample=True, pushes=True, period=16,
-                 tol=1e-3):
+                 tol=1e-3, update_norm=True):
         self.lib = _lib.load()
         self.dev = torch.device(device)
         self.nenv, self.nsub, self.dt, self.seed = nenv, nsub, float(dt), seed
         self.policy = policy or Policy(device, layout=1, seed=seed)
         self.physics = physics
         self.max_episode_length, self.cmd_lin, self.cmd_yaw = max_episode_length, cmd_lin, cmd_yaw
-        self.sample, self.pushes = sample, pushes
+        self.sample, self.pushes, self.update_norm = sample, pushes, update_norm
         f32 = dict(dtype=torch.float32, device=self.dev)
         i32 = dict(dtype=torch.int32, device=self.dev)
         u8 = dict(dtype=torch.uint8, device=self.dev)
@@ -149,6 +164,8 @@ class PolicyRollout:
         """One policy step: act, nsub physics substeps, reward/termination,
         bookkeeping and resets."""
         lib, s, n, t = self.lib, self.stream, self.nenv, self.t
+        if self.update_norm:                      # act(obs) updates the normaliser first (examples/train.py:86-88)
+            self.policy.update_obs_norm(n, self.qpos, self.qvel, self.commands, self.ctrl)
         self.policy.act(n, self.qpos, self.qvel, self.commands, self.ctrl, self.ctrl, actions=self.actions,
                         push_interval=self.push_interval if self.pushes else None, seed=self.seed, t=t,
                         sample=self.sample)

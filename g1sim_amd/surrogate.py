@@ -335,7 +335,8 @@ def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
     err = ((qs - qa).abs() / qa.abs().clamp_min(1.0)).max(dim=1).values
     e = err.double().cpu().numpy()
     return {"samples": int(B), "accept_frac": float((e <= tol).mean()), "median_rel_err": float(np.median(e)),
-            "p90_rel_err": float(np.percentile(e, 90)), "max_rel_err": float(e.max()), "tol": tol}
+            "p90_rel_err": float(np.percentile(e, 90)), "max_rel_err": float(e.max()), "tol": tol,
+            "accept_frac_at": {str(t): float((e <= t).mean()) for t in (1e-3, 1e-2, 1e-1, 1.0)}}
 
 
 # ====================================================================== step

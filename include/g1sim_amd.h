@@ -209,6 +209,16 @@ int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const fl
                   const void* const* W, const float* const* b, const float* log_std, unsigned seed, int t,
                   float* ctrl, float* actions, float* obs_out, int* push_interval, void* stream);
 
+/* Running observation normaliser update of examples/train.py:49-66 (what
+ * ActorCritic.act() does before normalising): batch mean and biased variance
+ * of the observation (layout as g1_hip_get_obs) over the nenv envs, merged
+ * into mean / var ([nobs] fp32) and count ([1] fp32) in place by Chan's
+ * formula; inv ([nobs]) receives 1 / sqrt(var + 1e-8) for g1_policy_act.
+ * scratch: double [2 * 64 * nobs] device workspace. */
+int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
+                              const float* last_ctrl, float* mean, float* var, float* count, float* inv,
+                              double* scratch, void* stream);
+
 /* Reward and termination (examples/train.py:143-225) of the post-step state,
  * then the rollout bookkeeping (examples/train.py:319-351): prev_actions <-
  * actions (0 for reset envs), command_timeout - 1, episode_length + 1,

@@ -8,7 +8,8 @@ Restates, from the reference's Python text:
   * the observation of G1Sim.get_obs            src/g1sim.py:132-170
   * quat_rotate_inverse                          src/g1sim.py:149-156
   * reward and termination                       examples/train.py:143-225
-  * observation normalisation                    examples/train.py:69-70
+  * observation normalisation and its running
+    statistics                                   examples/train.py:49-70
   * the policy MLP (96/99 -> 512 -> 256 -> 128 -> 29, ELU)
                                                  examples/train.py:21-29
   * ctrl from an action                          examples/train.py:299
@@ -101,6 +102,19 @@ def reward_and_done(obs99, qpos, foot_pos, foot_vel, contact_forces, actions, pr
     limit_angle = 70.0 * 3.14159 / 180.0
     tilt = np.abs(np.arccos(np.clip(-proj_gravity[:, 2], -1.0, 1.0)))
     return reward, tilt > limit_angle
+
+
+def update_obs_norm(obs, mean, var, count):
+    """examples/train.py:49-66: merges a batch (n, nobs) into the running
+    (mean, var, count); returns the new triple."""
+    batch_mean = obs.mean(axis=0)
+    batch_var = obs.var(axis=0)
+    n = obs.shape[0]
+    delta = batch_mean - mean
+    tot = count + n
+    new_mean = mean + delta * n / tot
+    M2 = var * count + batch_var * n + delta ** 2 * count * n / tot
+    return new_mean, M2 / tot, tot
 
 
 def normalize_obs(obs, mean, var, clip=5.0):

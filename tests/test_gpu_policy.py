@@ -225,3 +225,23 @@ def test_rollout_with_surrogate(device):
     assert summ["env_steps"] == 6 * 4 * n
     assert summ["judged"] == 6 * 4 * n // 16
     write_metrics("rollout_surrogate_2048", summ)
+
+
+def test_obs_norm_update_matches_oracle(device):
+    """Two consecutive running-statistics updates on the device against the
+    oracle's restatement of examples/train.py:49-66."""
+    pol = Policy(device, layout=1, seed=3)
+    mean = np.zeros(96); var = np.ones(96); count = 1e-4
+    for k in range(2):
+        n = 5000 + 777 * k
+        qp, qv, cmd, last = random_rl_state(n, 60 + k)
+        t = [dev(a, device) for a in (qp, qv, cmd, last)]
+        pol.update_obs_norm(n, *t)
+        torch.cuda.synchronize()
+        obs = R.get_obs(qp.astype(np.float64), qv.astype(np.float64), cmd.astype(np.float64),
+                        last.astype(np.float64), DEFAULT32.astype(np.float64), layout=1)
+        mean, var, count = R.update_obs_norm(obs, mean, var, count)
+        np.testing.assert_allclose(pol.obs_mean.cpu().numpy()[:96], mean, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(pol.obs_var.cpu().numpy()[:96], var, rtol=1e-5, atol=1e-6)
+        assert abs(float(pol.obs_count.item()) - count) <= 1e-6 * count
+        np.testing.assert_allclose(pol.obs_inv.cpu().numpy()[:96], 1.0 / np.sqrt(var + 1e-8), rtol=2e-5)

@@ -153,3 +153,17 @@ def test_bookkeeping_cases(done, ct, el, expect_reset, expect_resample):
     assert r[0] == expect_reset and rs[0] == expect_resample
     assert ct2[0] == ct - 1
     assert el2[0] == (0 if expect_reset else el + 1)
+
+
+def test_obs_norm_update_is_chan_merge():
+    """Merging batches one after another from an empty prior gives the mean
+    and (biased) variance of all samples together."""
+    rng = np.random.default_rng(12)
+    a = rng.normal(2.0, 3.0, size=(500, 7))
+    b = rng.normal(-1.0, 0.5, size=(300, 7))
+    m, v, c = R.update_obs_norm(a, np.zeros(7), np.ones(7), 0.0)
+    m, v, c = R.update_obs_norm(b, m, v, c)
+    allx = np.concatenate([a, b])
+    np.testing.assert_allclose(m, allx.mean(axis=0), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(v, allx.var(axis=0), rtol=1e-12, atol=1e-12)
+    assert c == 800
