@@ -43,6 +43,8 @@ def load():
     lib.g1_mlp_gemm.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp]
     lib.g1_mlp_mse_grad.restype = i
     lib.g1_mlp_mse_grad.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
+    lib.g1_mlp_sum_splits.restype = i
+    lib.g1_mlp_sum_splits.argtypes = [vp, i, C.c_longlong, vp, vp]
     lib.g1_mlp_colsum.restype = i
     lib.g1_mlp_colsum.argtypes = [vp, i, i, vp, vp]
     lib.g1_mlp_adam.restype = i
@@ -91,7 +93,7 @@ def load():
 
 def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
-            "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_adam",
+            "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_sum_splits", "g1_mlp_adam",
             "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
             "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_surrogate_integrate",
             "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs", "g1_policy_act",

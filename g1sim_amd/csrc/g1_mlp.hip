@@ -38,6 +38,7 @@ enum Epi : int {
   EPI_BIAS_SILU_H = 3, // H (bf16) = silu(acc + b)                   (inference forward)
   EPI_BIAS_F32 = 4,    // C (fp32) = acc + b                          (output layer)
   EPI_SILU_BWD = 5,    // dZ = acc * silu'(Z): fp32 to C, bf16 to H   (data grad through SiLU)
+  EPI_F32_SPLIT = 6,   // C[z] (fp32) = acc of r-slice z; C[z] at C + z * I * ldc (summed by g1_mlp_sum_splits)
 };
 
 struct GemmArgs {
@@ -130,6 +131,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
           g.C[o] = v;
         } else if constexpr (EPI == EPI_F32_ATOMIC) {
           atomicAdd(&g.C[o], v);
+        } else if constexpr (EPI == EPI_F32_SPLIT) {
+          g.C[(size_t)blockIdx.z * g.I * g.ldc + o] = v;
         } else if constexpr (EPI == EPI_BIAS_SILU) {
           const float z = v + g.bias[j];
           g.C[o] = z;
@@ -150,30 +153,69 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 // ---------------------------------------------------------------- training
 // dY = 2 (pred - target) / (B * n_out) on the first n_out columns, 0 on padding;
 // also accumulates the summed squared error into *sse (fp32 atomics).
-__global__ void mse_grad_kernel(const float* pred, const float* target, int B, int ld, int n_out, bf16* dY16,
-                                float* dY32, float* sse) {
+__global__ __launch_bounds__(256) void mse_grad_kernel(const float* pred, const float* target, int B, int ld,
+                                                       int n_out, bf16* dY16, float* dY32, float* sse) {
+  __shared__ float wsum[4];
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= B * ld) return;
-  const int j = idx % ld;
-  float d = 0.f;
-  if (j < n_out) {
-    const float e = pred[idx] - target[idx];
-    d = 2.0f * e / (float)(B * n_out);
-    atomicAdd(sse, e * e);
+  float e2 = 0.f;
+  if (idx < B * ld) {
+    const int j = idx % ld;
+    float d = 0.f;
+    if (j < n_out) {
+      const float e = pred[idx] - target[idx];
+      d = 2.0f * e / (float)(B * n_out);
+      e2 = e * e;
+    }
+    dY16[idx] = (bf16)d;
+    if (dY32) dY32[idx] = d;
   }
-  dY16[idx] = (bf16)d;
-  if (dY32) dY32[idx] = d;
+  if (!sse) return;                         // uniform: the loss is only wanted now and then
+  // one atomic per workgroup (per-element atomics on one address serialise)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(sse, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
-// db[j] = sum_i dZ[i][j]; one thread per column, rows strided over blockIdx.y
-__global__ void colsum_kernel(const float* dZ, int B, int ld, int rows_per_block, float* db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ld) return;
+// out[i] = sum over z < split of parts[z * n + i] (the r-slices of a split
+// weight-gradient GEMM), float4 at a time when n is a multiple of 4.
+__global__ void sum_splits_kernel(const float* parts, int split, size_t n, float* out) {
+  const size_t i4 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i4 >= n) return;
+  if (i4 + 4 <= n && (n & 3) == 0) {
+    float4 acc = *reinterpret_cast<const float4*>(parts + i4);
+    for (int z = 1; z < split; ++z) {
+      const float4 v = *reinterpret_cast<const float4*>(parts + (size_t)z * n + i4);
+      acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+    }
+    *reinterpret_cast<float4*>(out + i4) = acc;
+  } else {
+    for (size_t i = i4; i < n && i < i4 + 4; ++i) {
+      float a = 0.f;
+      for (int z = 0; z < split; ++z) a += parts[(size_t)z * n + i];
+      out[i] = a;
+    }
+  }
+}
+
+// db[j] = sum_i dZ[i][j]: a workgroup covers 64 columns x rows_per_block
+// rows; its 4 waves take every 4th row (coalesced 256-byte row segments),
+// reduce through LDS and add one partial per column.
+__global__ __launch_bounds__(256) void colsum_kernel(const float* dZ, int B, int ld, int rows_per_block, float* db) {
+  __shared__ float part[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
   const int r0 = blockIdx.y * rows_per_block;
   const int r1 = min(B, r0 + rows_per_block);
   float s = 0.f;
-  for (int i = r0; i < r1; ++i) s += dZ[(size_t)i * ld + j];
-  atomicAdd(&db[j], s);
+  if (j < ld) {
+#pragma unroll 4
+    for (int i = r0 + w; i < r1; i += 4) s += dZ[(size_t)i * ld + j];
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && j < ld) atomicAdd(&db[j], part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
 }
 
 // Adam on fp32 master weights; refreshes the bf16 copy used by the GEMMs.
@@ -282,7 +324,7 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
   if ((epi != EPI_BIAS_SILU_H && !C) || ((epi == EPI_BIAS_SILU || epi == EPI_BIAS_SILU_H || epi == EPI_SILU_BWD) && !H))
     return -1;
   if ((epi >= EPI_BIAS_SILU && epi <= EPI_BIAS_F32 && !bias) || (epi == EPI_SILU_BWD && !Zin)) return -1;
-  if (split > 1 && epi != EPI_F32_ATOMIC) return -1;
+  if (split > 1 && epi != EPI_F32_ATOMIC && epi != EPI_F32_SPLIT) return -1;
   GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin};
   hipStream_t s = (hipStream_t)stream;
   const int key = (a_trans ? 1 : 0) | (b_trans ? 2 : 0);
@@ -294,6 +336,7 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
     case 2 * 8 + EPI_SILU_BWD: return launch_gemm<false, true, EPI_SILU_BWD>(g, split, s);
     case 2 * 8 + EPI_F32: return launch_gemm<false, true, EPI_F32>(g, split, s);
     case 3 * 8 + EPI_F32_ATOMIC: return launch_gemm<true, true, EPI_F32_ATOMIC>(g, split, s);
+    case 3 * 8 + EPI_F32_SPLIT: return launch_gemm<true, true, EPI_F32_SPLIT>(g, split, s);
     case 3 * 8 + EPI_F32: return launch_gemm<true, true, EPI_F32>(g, split, s);
     default: return -1;
   }
@@ -308,10 +351,19 @@ int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, void* stream) {
+  if (split <= 0 || n < 0 || !parts || !out) return -1;
+  if (n == 0) return 0;
+  const size_t n4 = ((size_t)n + 3) / 4;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, parts,
+                     split, (size_t)n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream) {
   if (B <= 0 || ld <= 0) return -1;
-  const int rpb = 256;
-  hipLaunchKernelGGL(colsum_kernel, dim3((ld + 255) / 256, (B + rpb - 1) / rpb), dim3(256), 0, (hipStream_t)stream,
+  const int rpb = 64;
+  hipLaunchKernelGGL(colsum_kernel, dim3((ld + 63) / 64, (B + rpb - 1) / rpb), dim3(256), 0, (hipStream_t)stream,
                      dZ, B, ld, rpb, db);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

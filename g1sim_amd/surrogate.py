@@ -22,7 +22,7 @@ from .model import NQ, NV
 
 NIN, NOUT, HID, NHID = NQ + NV + NV, NV, 512, 4
 PIN, POUT = 128, 128
-EPI = dict(F32=0, F32_ATOMIC=1, BIAS_SILU=2, BIAS_SILU_H=3, BIAS_F32=4, SILU_BWD=5)
+EPI = dict(F32=0, F32_ATOMIC=1, BIAS_SILU=2, BIAS_SILU_H=3, BIAS_F32=4, SILU_BWD=5, F32_SPLIT=6)
 
 
 def k_tiled(w16, nout):
@@ -104,6 +104,8 @@ class MLP:
             "gW": [torch.zeros_like(w) for w in self.W],
             "gb": [torch.zeros_like(b) for b in self.b],
             "sse": torch.zeros(1, **f32),
+            # r-slices of the weight-gradient GEMMs (split over the batch), summed by g1_mlp_sum_splits
+            "splits": torch.zeros(self._split(B) * max(r * c for r, c, _ in self.dims), **f32),
         }
         self._ws[B] = ws
         return ws
@@ -131,21 +133,29 @@ class MLP:
         return pred
 
     # ------------------------------------------------------------ backward
+    @staticmethod
+    def _split(B):
+        split = max(1, min(16, B // 512))
+        while B % (32 * split):
+            split //= 2
+        return split
+
     def backward(self, X16, B, dY16):
         """Gradients of the loss into ws['gW'], ws['gb'] given dL/dpred (bf16)."""
         ws = self.workspace(B)
         L = len(self.dims)
-        for g in ws["gW"] + ws["gb"]:
+        for g in ws["gb"]:
             self._check(self.lib.g1_mlp_fill(g.data_ptr(), 0.0, g.numel(), self.stream), "fill")
-        split = max(1, min(16, B // 512))
-        while B % (32 * split):
-            split //= 2
+        split = self._split(B)
         d16, dfp = dY16, None
         for l in range(L - 1, -1, -1):
             rows, cols, _ = self.dims[l]
             hin = X16 if l == 0 else ws["H16"][l - 1]
-            # dW = d^T h: reduction over the batch, split across gridDim.z
-            self.gemm(1, 1, "F32_ATOMIC", d16, rows, hin, cols, rows, cols, B, C=ws["gW"][l], ldc=cols, split=split)
+            # dW = d^T h: reduction over the batch, split across gridDim.z into
+            # r-slices (plain stores), then one pass adds the slices
+            self.gemm(1, 1, "F32_SPLIT", d16, rows, hin, cols, rows, cols, B, C=ws["splits"], ldc=cols, split=split)
+            self._check(self.lib.g1_mlp_sum_splits(ws["splits"].data_ptr(), split, rows * cols,
+                                                   ws["gW"][l].data_ptr(), self.stream), "sum_splits")
             dsrc = ws["dY32"] if l == L - 1 else dfp
             self._check(self.lib.g1_mlp_colsum(dsrc.data_ptr(), B, rows, ws["gb"][l].data_ptr(), self.stream),
                         "colsum")
@@ -169,21 +179,23 @@ class MLP:
                                              self.b[l].numel(), lr, b1, b2, eps, self.t, 0.0, self.stream),
                         "adam b")
 
-    def loss_grad(self, pred, target, B):
+    def loss_grad(self, pred, target, B, want_sse=True):
         """MSE on the 35 meaningful outputs: dY as bf16 (GEMM operand) and fp32
-        (output bias gradient); returns the device SSE tensor."""
+        (output bias gradient); returns the device SSE tensor (None when not
+        wanted: the reduction is skipped)."""
         ws = self.workspace(B)
-        self._check(self.lib.g1_mlp_fill(ws["sse"].data_ptr(), 0.0, 1, self.stream), "fill")
+        if want_sse:
+            self._check(self.lib.g1_mlp_fill(ws["sse"].data_ptr(), 0.0, 1, self.stream), "fill")
         self._check(self.lib.g1_mlp_mse_grad(pred.data_ptr(), target.data_ptr(), B, POUT, NOUT,
-                                             ws["dY16"].data_ptr(), ws["dY32"].data_ptr(), ws["sse"].data_ptr(),
-                                             self.stream), "mse")
-        return ws["sse"]
+                                             ws["dY16"].data_ptr(), ws["dY32"].data_ptr(),
+                                             ws["sse"].data_ptr() if want_sse else None, self.stream), "mse")
+        return ws["sse"] if want_sse else None
 
-    def train_step(self, X16, T, B, lr):
+    def train_step(self, X16, T, B, lr, want_loss=True):
         """One minibatch: forward, MSE gradient, backward, Adam.  Returns the
-        device SSE (sum of squared z-scored errors)."""
+        device SSE (sum of squared z-scored errors), or None without want_loss."""
         pred = self.forward(X16, B, train=True)
-        sse = self.loss_grad(pred, T, B)
+        sse = self.loss_grad(pred, T, B, want_sse=want_loss)
         self.backward(X16, B, self.workspace(B)["dY16"])
         self.adam(lr, B=B)
         return sse
@@ -307,8 +319,9 @@ def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=N
         idx[:, 1] = torch.randint(0, ds.nenv, (batch,), device=dev, generator=g, dtype=torch.int32)
         mlp.gather(ds.X, ds.Y, idx, batch, ds.nenv, ws["X16"], ws["T"])
         a = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * it / max(1, iters - 1)))
-        sse = mlp.train_step(ws["X16"], ws["T"], batch, a)
-        if it % 100 == 0 or it == iters - 1:
+        log_now = it % 100 == 0 or it == iters - 1
+        sse = mlp.train_step(ws["X16"], ws["T"], batch, a, want_loss=log_now)
+        if log_now:
             v = float(sse.item()) / (batch * NOUT)
             hist.append((it, v))
             if log:

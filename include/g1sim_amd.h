@@ -80,24 +80,30 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
 #define G1_MLP_EPI_BIAS_SILU_H 3 /* H = silu(acc + bias) (bf16)                     */
 #define G1_MLP_EPI_BIAS_F32   4  /* C = acc + bias (fp32)                           */
 #define G1_MLP_EPI_SILU_BWD   5  /* C = acc * silu'(Zin) (fp32), H = same (bf16)    */
+#define G1_MLP_EPI_F32_SPLIT  6  /* C + z*I*ldc = acc of r-slice z (split partials) */
 
 /* C[I][J] = sum_r A(i, r) B(j, r) with bf16 A, B and fp32 accumulation on
  * gfx950 MFMA.  A(i, r) = a_trans ? A[r*lda + i] : A[i*lda + r], likewise B.
  * I and J must be multiples of 128 and R a multiple of 32*split; split > 1
- * (gridDim.z partial sums) only with G1_MLP_EPI_F32_ATOMIC.  C and H use the
- * leading dimension ldc.  Supported (a_trans, b_trans, epi): (0,0, BIAS_SILU
- * | BIAS_SILU_H | BIAS_F32 | F32), (0,1, SILU_BWD | F32), (1,1, F32_ATOMIC |
- * F32).  Returns 0, -1 (bad arguments) or -2 (launch failure). */
+ * (gridDim.z partial sums) only with G1_MLP_EPI_F32_ATOMIC or
+ * G1_MLP_EPI_F32_SPLIT (C then holds split slices of I x ldc).  C and H use
+ * the leading dimension ldc.  Supported (a_trans, b_trans, epi): (0,0,
+ * BIAS_SILU | BIAS_SILU_H | BIAS_F32 | F32), (0,1, SILU_BWD | F32), (1,1,
+ * F32_ATOMIC | F32_SPLIT | F32).  Returns 0, -1 (bad arguments) or -2 (launch
+ * failure). */
 int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
                 int I, int J, int R, int split, float* C, void* H, int ldc, const float* bias,
                 const float* Zin, void* stream);
 /* dY = 2 (pred - target) / (B * n_out) on columns < n_out, 0 beyond, as
- * bf16 (dY16) and fp32 (dY32, nullable); *sse += sum of squared errors.
- * pred, target: fp32 [B][ld]. */
+ * bf16 (dY16) and fp32 (dY32, nullable); *sse += sum of squared errors when
+ * sse is non-null.  pred, target: fp32 [B][ld]. */
 int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16,
                     float* dY32, float* sse, void* stream);
 /* db[j] += sum_i dZ[i][j] over fp32 dZ [B][ld]. */
 int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream);
+/* out[i] = sum_{z < split} parts[z * n + i] (the slices of a G1_MLP_EPI_F32_SPLIT
+ * GEMM). */
+int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, void* stream);
 /* One Adam step (step number t >= 1, L2 weight decay wd) on fp32 master
  * weights p with moments m, v; writes the bf16 copy p16 (nullable).  mask
  * (nullable) marks trainable entries; masked-out entries stay zero. */

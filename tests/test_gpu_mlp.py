@@ -200,3 +200,29 @@ def test_gather_normalises(device):
     assert np.allclose(got[:, :NIN], bf16(ex), rtol=0, atol=1e-6) or np.abs(got[:, :NIN] - ex).max() < 2e-2
     assert (got[:, NIN:] == 0).all()
     assert np.allclose(Yn.cpu().numpy()[:, :NOUT], ey, rtol=1e-5, atol=1e-5)
+
+
+def test_gemm_split_slices_sum_to_product(device):
+    """F32_SPLIT writes the r-slices of the weight-gradient GEMM; summing them
+    (g1_mlp_sum_splits) gives the full product."""
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    I, J, R, split = 256, 128, 1024, 4
+    Aop, Bop = rng.normal(size=(I, R)), rng.normal(size=(J, R))
+    At = torch.as_tensor(Aop.T.copy().astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    Bt = torch.as_tensor(Bop.T.copy().astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    parts = torch.full((split, I, J), 7.0, device=device)
+    s = torch.cuda.current_stream(device).cuda_stream
+    rc = lib.g1_mlp_gemm(1, 1, EPI["F32_SPLIT"], At.data_ptr(), I, Bt.data_ptr(), J, I, J, R, split, parts.data_ptr(),
+                         None, J, None, None, s)
+    assert rc == 0, _lib.last_error()
+    out = torch.full((I, J), -1.0, device=device)
+    assert lib.g1_mlp_sum_splits(parts.data_ptr(), split, I * J, out.data_ptr(), s) == 0
+    torch.cuda.synchronize()
+    acc = bf16(Aop) @ bf16(Bop).T
+    tol = 1e-5 * np.abs(bf16(Aop)) @ np.abs(bf16(Bop)).T + 1e-6
+    assert (np.abs(out.cpu().numpy() - acc) <= tol).all()
+    # each slice is the product over its quarter of r
+    q = R // split
+    p1 = bf16(Aop[:, q:2 * q]) @ bf16(Bop[:, q:2 * q]).T
+    assert np.allclose(parts[1].cpu().numpy(), p1, rtol=1e-5, atol=1e-4)
