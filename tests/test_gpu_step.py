@@ -319,3 +319,31 @@ def test_work_splits_agree(device, force):
     write_metrics(f"split_flag{force}", {"qacc_rel_max": float(e), "f32ref_qacc_rel_max": float(e32)})
     assert e <= F32_FACTOR * e32
     assert np.abs(after["tau"][:6]).max() > 1.0
+
+
+def test_empty_and_zero_substep_calls_leave_state(device):
+    """nsub = 0 and nenv = 0 are no-ops on the device (every entry point
+    accepts them), and a judge list with no entries writes nothing."""
+    qp, qv, an, cb = W.randomized_actuation(300, seed=41, nblocks=1)
+    st = GpuState(qp, qv, an, cb[0], device)
+    before = st.host()
+    st.step(0, 2e-3)
+    after = st.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(before[k], after[k]), k
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    assert lib.g1_hip_step_ex(0, 3, 2e-3, None, None, None, None, None, None, None, None, None, None, None, 0, s) == 0
+    lib.g1_cuda_reset_done(0, None, None, None, None, 1, 0.1, 0.02, s)
+    lst = torch.zeros(300, dtype=torch.int32, device=device)
+    cnt = torch.full((1,), 123, dtype=torch.int32, device=device)
+    fb = torch.zeros(300, dtype=torch.uint8, device=device)
+    # period 400 > nenv and phase 350: nobody is judged
+    _lib.check(lib.g1_hip_judge_list(300, 350, 400, fb.data_ptr(), lst.data_ptr(), cnt.data_ptr(), s))
+    qacc = torch.full((35, 300), 5.0, device=device)
+    _lib.check(lib.g1_hip_step_judge(300, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
+                                     st.ctrl.data_ptr(), None, lst.data_ptr(), cnt.data_ptr(), qacc.data_ptr(), 0, s))
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == 0
+    assert (qacc == 5.0).all()
+    assert np.array_equal(st.host()["qpos"], before["qpos"])
