@@ -219,7 +219,7 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
   dense_out<SUR_H, SUR_POUT>(buf[0], out, a.W[4], a.b[4], w, lane);
   __syncthreads();
   if constexpr (MODE == 0) {
-    // [64][64] fp32 tile to pred rows; 512 threads, 8 per row
+    // [64][64] fp32 tile to pred rows; 512 threads, 8 elements each
     for (int i = tid; i < ROWS * SUR_POUT; i += NTHR) {
       const int r = i / SUR_POUT, c = i % SUR_POUT, e = e0 + r;
       if (e < a.nenv) a.pred[(size_t)e * a.ldp + c] = out[r * LDO + c];
