@@ -10,6 +10,14 @@
 
 namespace g1k {
 
+// The rotating judge works on blocks of JUDGE_BLOCK consecutive envs (one
+// wavefront of the step kernel, one row tile of the fused network): block b
+// is judged at the steps where b % period == phase, so every env is judged
+// once per `period` steps, the judge's state loads stay coalesced and whole
+// network tiles can be skipped.
+constexpr int JUDGE_BLOCK = 64;
+G1K_DEV bool judged_env(int e, int phase, int period) { return ((e / JUDGE_BLOCK) % period) == phase; }
+
 // `live` lanes are envs < nenv; every lane of the wave must call this (the
 // statistics are reduced across the wave).  sur[35] is the network's qacc in
 // physical units.  stats: [0] judged, [1] rejected, [2] max judged rel. error
@@ -19,7 +27,7 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
                               const float* sur, const float* qacc_aba, uint8_t* fallback, unsigned* stats,
                               float* qacc_out, float* qpos, float* qvel) {
   constexpr int NV = 35, NU = 29;
-  const bool judged = live && (e % period) == phase;
+  const bool judged = live && judged_env(e, phase, period);
   const bool fb = live && fallback && fallback[e];
   float qa[NV];
   float err = 0.f;

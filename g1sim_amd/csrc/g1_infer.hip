@@ -187,6 +187,7 @@ struct SurrogateArgs {
 };
 
 constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
+static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 
 template <int MODE>
 __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
@@ -194,6 +195,25 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * ROWS;
   const size_t N = (size_t)a.nenv;
+  if constexpr (MODE == 1) {
+    // A tile that is not judged this step and whose live envs are all in
+    // fallback takes ABA's qacc everywhere: commit without the network.
+    // (The tile's rows are exactly one judge block, so `judged` is uniform.)
+    if (a.fallback && !g1k::judged_env(e0, a.phase, a.period)) {
+      const int e = e0 + (tid & 63);
+      const int own = tid < 64 && e < a.nenv && !a.fallback[e];
+      if (!__syncthreads_or(own)) {
+        if (w == 0) {
+          float sur[NV];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) sur[j] = 0.f;
+          g1k::surrogate_commit(e < a.nenv, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback,
+                                a.stats, a.qacc_out, a.qpos_rw, a.qvel_rw);
+        }
+        return;
+      }
+    }
+  }
   {
     // network input: z-scored (qpos 36, qvel 35, tau 35), zero padding
     const int r = tid & 63, e = e0 + r;

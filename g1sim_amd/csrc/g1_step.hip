@@ -596,7 +596,23 @@ struct StepArgs {
   const int* env_list;
   const int* list_count;
   int stride;
+  // block-select judge mode (sel_period > 0): workgroup b steps the envs of
+  // block b that judged_env() or sel_fallback marks, as the list would hold
+  const uint8_t* sel_fallback;
+  int sel_phase, sel_period;
 };
+static_assert(ENVS_PER_WG == JUDGE_BLOCK, "judge blocks are the step's workgroups");
+
+// Position of the k-th set bit (k from 0, k < popcount(m)) of m.
+G1K_DEV int nth_set_bit(unsigned long long m, int k) {
+  int lo = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int c = __popcll((m >> lo) & ((1ull << w) - 1ull));
+    if (c <= k) { k -= c; lo += w; }
+  }
+  return lo;
+}
 
 // Per-lane chain slots (legs use 6 of 7).
 constexpr int MAXJ = 7;
@@ -707,14 +723,34 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   const int lane = threadIdx.x & 63;
   const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool judge = args.env_list != nullptr;
-  const int nenv = judge ? *args.list_count : args.nenv;
-  if (judge && (int)(blockIdx.x * ENVS_PER_WG) >= nenv) return;   // uniform per workgroup
-  const int i_raw = blockIdx.x * ENVS_PER_WG + lane;
-  const bool valid = i_raw < nenv;
-  const int i_env = valid ? i_raw : nenv - 1;  // tail lanes mirror the last env, never store
-  const int e = judge ? args.env_list[i_env] : i_env;
-  const size_t N = judge ? (size_t)args.stride : (size_t)nenv;
+  const bool select = args.sel_period > 0;      // judge by blocks, no list
+  const bool judge = select || args.env_list != nullptr;
+  bool valid;
+  int e;
+  size_t N;
+  if (select) {
+    // this workgroup's block of 64 envs: all of them on the block's judge
+    // step, else its fallback envs, packed into the low lanes in env order.
+    // Every wave reads the same 64 flags, so the mask and the early exit are
+    // uniform per workgroup.
+    const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
+    const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
+    const bool in = e_raw < args.nenv && (blk || (args.sel_fallback && args.sel_fallback[e_raw]));
+    const unsigned long long m = __ballot(in);
+    if (m == 0ull) return;
+    const int cnt = __popcll(m);
+    valid = lane < cnt;
+    e = blockIdx.x * ENVS_PER_WG + nth_set_bit(m, valid ? lane : cnt - 1);
+    N = (size_t)args.nenv;
+  } else {
+    const int nenv = judge ? *args.list_count : args.nenv;
+    if (judge && (int)(blockIdx.x * ENVS_PER_WG) >= nenv) return;   // uniform per workgroup
+    const int i_raw = blockIdx.x * ENVS_PER_WG + lane;
+    valid = i_raw < nenv;
+    const int i_env = valid ? i_raw : nenv - 1;  // tail lanes mirror the last env, never store
+    e = judge ? args.env_list[i_env] : i_env;
+    N = judge ? (size_t)args.stride : (size_t)nenv;
+  }
   const Cfg cf = args.cfg;
   const bool legs = wave * CPW < 2;            // this wave owns the leg chains
 
@@ -1062,7 +1098,10 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
 // Total generalized force without ABA: pelvis and leg kinematics, the 8 foot
 // contacts (anchors and normal forces updated exactly as by the step),
 // J^T f of the contact wrenches and the servo + limit torques of all 29
-// hinges.  One lane per env.
+// hinges.  A workgroup holds 64 envs (lane = env) in 2 waves, one per leg:
+// wave L runs leg L's kinematics and 4 contacts and the servos of its leg and
+// of half the upper body; the pelvis wrench of the right leg reaches wave 0
+// through LDS.
 struct ForcesArgs {
   int nenv;
   Cfg cfg;
@@ -1077,27 +1116,72 @@ struct ForcesArgs {
   float* foot_vel;
 };
 
-template <int h0, int n> G1K_DEV void servo_range(const Cfg& cf, const float* q, const float* qd, const float* ct,
-                                                  float* tau) {
+// Servo + limit torques of hinges h0 .. h0+n-1, stored as tau rows 6 + h.
+template <int h0, int n> G1K_DEV void servo_store(const ForcesArgs& a, size_t N, int e, bool live) {
   static_for<n>([&](auto H) {
-    constexpr int i = decltype(H)::value;
+    constexpr int h = h0 + decltype(H)::value;
     float b;
-    tau[i] = servo_torque<h0 + i>(cf, q[i], qd[i], ct[i], b);
+    const float t = servo_torque<h>(a.cfg, a.qpos[(7 + h) * N + e], a.qvel[(6 + h) * N + e], a.ctrl[h * N + e], b);
+    if (live) a.tau[(6 + h) * N + e] = t;
   });
 }
 
-__global__ __launch_bounds__(256) void forces_kernel(ForcesArgs a) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.nenv) return;
-  const size_t N = (size_t)a.nenv;
-  const Cfg cf = a.cfg;
-  float q[NU], qd[NU], ct[NU], tau[NV];
+// Leg L: kinematics, contacts (anchors, normal forces, feet), servo + J^T f
+// of its 6 hinges; returns the legs' contact wrench on the pelvis in Fb.
+template <int L>
+G1K_DEV void leg_forces(const ForcesArgs& a, size_t N, int e, bool live, const float (&R0)[9], const float (&p0)[3],
+                        const float (&v0)[6], float (&Fb)[6]) {
+  using WC = WaveChain<L>;
+  using Ch = ChainOps<WC::first, 6>;
+  constexpr int h0 = WC::first - 1;
+  const Cfg& cf = a.cfg;
+  float q[6], qd[6], ct[6], anc[8], fn[4], tc[6], vleaf[6], Rf[9], pf[3];
 #pragma unroll
-  for (int h = 0; h < NU; ++h) {
-    q[h] = a.qpos[(7 + h) * N + e];
-    qd[h] = a.qvel[(6 + h) * N + e];
-    ct[h] = a.ctrl[h * N + e];
+  for (int m = 0; m < 6; ++m) {
+    q[m] = a.qpos[(7 + h0 + m) * N + e];
+    qd[m] = a.qvel[(6 + h0 + m) * N + e];
+    ct[m] = a.ctrl[(h0 + m) * N + e];
   }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) anc[m] = a.anchor[(L * 8 + m) * N + e];
+  const float mu = a.mu ? a.mu[e] : cf.mu;
+  Joint jt[6];
+  Ch::fk(q, qd, v0, jt, vleaf);
+  Ch::world(R0, p0, v0, jt, q, qd, Rf, pf, vleaf);   // float: vleaf kept
+  foot_contacts<WC::k0>(cf, mu, Rf, pf, vleaf, anc, fn, Fb);
+  Ch::wrench_to_joints(Fb, jt, tc);
+  if (!live) return;
+  static_for<6>([&](auto H) {
+    constexpr int m = decltype(H)::value;
+    float b;
+    a.tau[(6 + h0 + m) * N + e] = servo_torque<h0 + m>(cf, q[m], qd[m], ct[m], b) + tc[m];
+  });
+#pragma unroll
+  for (int m = 0; m < 8; ++m) a.anchor[(L * 8 + m) * N + e] = anc[m];
+  if (a.contact_forces) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a.contact_forces[(L * 4 + m) * N + e] = fn[m];
+  }
+  // foot body origin and its world linear velocity, as the step reports them
+  if (a.foot_pos) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) a.foot_pos[(L * 3 + m) * N + e] = pf[m];
+  }
+  if (a.foot_vel) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      a.foot_vel[(L * 3 + m) * N + e] = Rf[3 * m] * vleaf[3] + Rf[3 * m + 1] * vleaf[4] + Rf[3 * m + 2] * vleaf[5];
+  }
+}
+
+__global__ __launch_bounds__(128) void forces_kernel(ForcesArgs a) {
+  __shared__ float fx[6][ENVS_PER_WG];
+  const int lane = threadIdx.x & 63;
+  const int leg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
+  const bool live = e_raw < a.nenv;
+  const int e = live ? e_raw : a.nenv - 1;        // tail lanes mirror the last env, never store
+  const size_t N = (size_t)a.nenv;
   float bq[4], p0[3], vw[3], om[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -1116,53 +1200,33 @@ __global__ __launch_bounds__(256) void forces_kernel(ForcesArgs a) {
   v0[0] = om[0]; v0[1] = om[1]; v0[2] = om[2];
 #pragma unroll
   for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
-  const float mu = a.mu ? a.mu[e] : cf.mu;
-  servo_range<0, NU>(cf, q, qd, ct, tau + 6);
-  float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  static_for<2>([&](auto L) {
-    constexpr int leg = decltype(L)::value;
-    using Ch = ChainOps<WaveChain<leg>::first, 6>;
-    Joint jt[6];
-    float vleaf[6], Rf[9], pf[3], Fb[6], anc[8], fn[4], tc[6];
+  float Fb[6];
+  if (leg == 0) {
+    leg_forces<0>(a, N, e, live, R0, p0, v0, Fb);
+    servo_store<12, 9>(a, N, e, live);            // waist and left arm
+  } else {
+    leg_forces<1>(a, N, e, live, R0, p0, v0, Fb);
+    servo_store<21, 8>(a, N, e, live);            // right arm
 #pragma unroll
-    for (int m = 0; m < 8; ++m) anc[m] = a.anchor[(leg * 8 + m) * N + e];
-    Ch::fk(q + 6 * leg, qd + 6 * leg, v0, jt, vleaf);
-    Ch::world(R0, p0, v0, jt, q + 6 * leg, qd + 6 * leg, Rf, pf, vleaf);   // float: vleaf kept
-    foot_contacts<WaveChain<leg>::k0>(cf, mu, Rf, pf, vleaf, anc, fn, Fb);
-    Ch::wrench_to_joints(Fb, jt, tc);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      tau[6 + 6 * leg + m] += tc[m];
-      F[m] += Fb[m];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) a.anchor[(leg * 8 + m) * N + e] = anc[m];
-    if (a.contact_forces) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) a.contact_forces[(leg * 4 + m) * N + e] = fn[m];
-    }
-    // foot body origin and its world linear velocity, as the step reports them
-    if (a.foot_pos) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m) a.foot_pos[(leg * 3 + m) * N + e] = pf[m];
-    }
-    if (a.foot_vel) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
-        a.foot_vel[(leg * 3 + m) * N + e] = Rf[3 * m] * vleaf[3] + Rf[3 * m + 1] * vleaf[4] + Rf[3 * m + 2] * vleaf[5];
-    }
-  });
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    tau[m] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
-    tau[3 + m] = F[m];
+    for (int m = 0; m < 6; ++m) fx[m][lane] = Fb[m];
   }
+  __syncthreads();
+  if (leg == 0 && live) {
+    float F[6];
 #pragma unroll
-  for (int m = 0; m < NV; ++m) a.tau[m * N + e] = tau[m];
+    for (int m = 0; m < 6; ++m) F[m] = Fb[m] + fx[m][lane];
+    // base: world force R0 F.l, body torque F.a (convention of g1_c_fd's tau)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      a.tau[m * N + e] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
+      a.tau[(3 + m) * N + e] = F[m];
+    }
+  }
 }
 
-// The envs whose qacc ABA computes this step: every period-th env starting at
-// `phase` (the rotating judge) plus every env in fallback.
+// The envs whose qacc ABA computes this step: the envs of every block of 64
+// with block % period == phase (the rotating judge, judged_env) plus every
+// env in fallback.
 // One atomic per workgroup: each wave counts its selected envs (ballot), the
 // workgroup takes one block of slots, and every env's slot is its rank in
 // the workgroup, so the list is sorted within each workgroup's run and the
@@ -1174,7 +1238,7 @@ __global__ __launch_bounds__(256) void judge_list_kernel(int nenv, int phase, in
   __shared__ int block_base;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const bool sel = e < nenv && ((e % period) == phase || (fallback && fallback[e]));
+  const bool sel = e < nenv && (judged_env(e, phase, period) || (fallback && fallback[e]));
   const unsigned long long m = __ballot(sel);
   if (lane == 0) wave_count[w] = __popcll(m);
   __syncthreads();
@@ -1341,7 +1405,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
     return -1;
   }
   if (nenv == 0 || nsub == 0) return 0;
-  g1k::StepArgs a;
+  g1k::StepArgs a{};
   a.nenv = nenv; a.nsub = nsub;
   a.cfg = g1k::default_cfg(dt);
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl;
@@ -1370,7 +1434,8 @@ int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, floa
   a.nenv = nenv; a.cfg = g1k::default_cfg(dt);
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor; a.ctrl = ctrl; a.mu = mu; a.tau = tau_out;
   a.contact_forces = contact_forces; a.foot_pos = foot_pos; a.foot_vel = foot_vel;
-  hipLaunchKernelGGL(g1k::forces_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(g1k::forces_kernel, dim3((nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG), dim3(128), 0,
+                     (hipStream_t)stream, a);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) { set_err("forces launch failed: %s", hipGetErrorString(err)); return -2; }
   return 0;
@@ -1405,7 +1470,7 @@ int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, 
     return -1;
   }
   if (nenv == 0) return 0;
-  g1k::StepArgs a;
+  g1k::StepArgs a{};
   a.nenv = nenv; a.nsub = 1;
   a.cfg = g1k::default_cfg(dt);
   a.qpos = const_cast<float*>(qpos); a.qvel = const_cast<float*>(qvel); a.anchor = const_cast<float*>(anchor);
@@ -1414,6 +1479,32 @@ int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, 
   a.qacc = qacc_out; a.tau = nullptr; a.mu = mu; a.base64 = nullptr;
   a.f64_solve = 0;
   a.env_list = list; a.list_count = count; a.stride = nenv;
+  return launch_step(a, flags, (hipStream_t)stream);
+}
+
+int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float* qvel, const float* anchor,
+                             const float* ctrl, const float* mu, const uint8_t* fallback, int phase, int period,
+                             float* qacc_out, unsigned flags, void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (period <= 0 || phase < 0 || phase >= period || !qacc_out) {
+    set_err("step_judge_blocks: bad phase/period or no qacc");
+    return -1;
+  }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("step_judge_blocks: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv; a.nsub = 1;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = const_cast<float*>(qpos); a.qvel = const_cast<float*>(qvel); a.anchor = const_cast<float*>(anchor);
+  a.ctrl = ctrl;
+  a.qacc = qacc_out; a.mu = mu;
+  a.stride = nenv;
+  a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
   return launch_step(a, flags, (hipStream_t)stream);
 }
 

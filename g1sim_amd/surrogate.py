@@ -367,8 +367,6 @@ class SurrogateStepper:
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.tau = torch.zeros((NV, nenv), **f32)
         self.qacc_aba = torch.zeros((NV, nenv), **f32)
-        self.list = torch.zeros(nenv, dtype=torch.int32, device=self.dev)
-        self.count = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.fallback = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
         self.stats = torch.zeros(4, dtype=torch.int32, device=self.dev)
         self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
@@ -384,11 +382,10 @@ class SurrogateStepper:
         lib, s, n = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
         phase = self.t % self.period
         ck = _lib.check
-        ck(lib.g1_hip_judge_list(n, phase, self.period, self.fallback.data_ptr(), self.list.data_ptr(),
-                                 self.count.data_ptr(), s), "judge_list")
-        ck(lib.g1_hip_step_judge(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
-                                 None if mu is None else mu.data_ptr(), self.list.data_ptr(), self.count.data_ptr(),
-                                 self.qacc_aba.data_ptr(), self.judge_flags, s), "step_judge")
+        ck(lib.g1_hip_step_judge_blocks(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                        ctrl.data_ptr(), None if mu is None else mu.data_ptr(),
+                                        self.fallback.data_ptr(), phase, self.period, self.qacc_aba.data_ptr(),
+                                        self.judge_flags, s), "step_judge_blocks")
         ck(lib.g1_hip_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
                              None if mu is None else mu.data_ptr(), self.tau.data_ptr(),
                              self.contact_forces.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(), s),

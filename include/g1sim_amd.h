@@ -139,9 +139,12 @@ int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, c
 int g1_hip_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor,
                   const float* ctrl, const float* mu, float* tau_out, float* contact_forces,
                   float* foot_pos, float* foot_vel, void* stream);
-/* Device list of the envs that ABA steps this step: e % period == phase (the
- * rotating judge) or fallback[e] != 0 (nullable).  list: int [nenv]; *count
- * is reset and filled on the device. */
+/* The rotating judge works on blocks of 64 consecutive envs: env e is judged
+ * at the steps where (e / 64) % period == phase, so each env once per period
+ * steps.
+ * Device list of the envs that ABA steps this step: the judged envs and those
+ * with fallback[e] != 0 (nullable).  list: int [nenv]; *count is reset and
+ * filled on the device. */
 int g1_hip_judge_list(int nenv, int phase, int period, const uint8_t* fallback, int* list, int* count,
                       void* stream);
 /* ABA qacc (one substep at dt, state read-only) of the listed envs of a batch
@@ -151,10 +154,16 @@ int g1_hip_judge_list(int nenv, int phase, int period, const uint8_t* fallback, 
 int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, const float* anchor,
                       const float* ctrl, const float* mu, const int* list, const int* count, float* qacc_out,
                       unsigned flags, void* stream);
+/* g1_hip_judge_list followed by g1_hip_step_judge in one kernel and without
+ * the list: the workgroup of each block of 64 envs steps the block's judged
+ * and fallback envs (fallback nullable), or exits when there are none. */
+int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float* qvel, const float* anchor,
+                             const float* ctrl, const float* mu, const uint8_t* fallback, int phase, int period,
+                             float* qacc_out, unsigned flags, void* stream);
 /* Applies qacc and integrates one step (semi-implicit Euler as the fused
  * step).  pred: the network's z-scored output, row e of [>= nenv][ldp];
- * qacc = pred * ystd + ymean, except for envs judged this step
- * (e % period == phase) or in fallback, which take qacc_aba.  A judged env
+ * qacc = pred * ystd + ymean, except for envs judged this step (the block
+ * rotation above) or in fallback, which take qacc_aba.  A judged env
  * whose surrogate qacc differs from ABA's by more than tol (max over dofs of
  * |a - b| / max(|b|, 1)) enters fallback (ABA every step) until a later judge
  * accepts it.  stats (unsigned [4], accumulated): judged, rejected, max
@@ -181,7 +190,9 @@ int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, con
                              const float* const* b, float* pred, int ldp, void* stream);
 /* Surrogate step: g1_hip_surrogate_forward followed by the commit of
  * g1_hip_surrogate_integrate (same judge, fallback, statistics and
- * integration), in one kernel; qpos / qvel are updated in place. */
+ * integration), in one kernel; qpos / qvel are updated in place.  A block of
+ * 64 envs that is not judged this step and whose envs are all in fallback
+ * skips the network (its qacc is ABA's either way). */
 int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
                           const float* tau, const float* xmean, const float* xinv, const void* const* W,
                           const float* const* b, const float* ymean, const float* ystd,

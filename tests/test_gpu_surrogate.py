@@ -22,6 +22,20 @@ def _state(device, n, seed, steps=40):
     return st
 
 
+def judged_envs(n, phase, period=16):
+    """The rotating judge's envs: blocks of 64 with block % period == phase."""
+    e = np.arange(n)
+    return e[(e // 64) % period == phase]
+
+
+def _judge_blocks(st, fallback, phase, qacc, device, flags=0):
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    _lib.check(lib.g1_hip_step_judge_blocks(st.n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(),
+                                            st.anchor.data_ptr(), st.ctrl.data_ptr(), None, fallback.data_ptr(),
+                                            phase, 16, qacc.data_ptr(), flags, s))
+
+
 def test_forces_kernel_matches_step(device):
     """g1_hip_forces gives the step's tau_total, anchors, normal forces and
     foot kinematics."""
@@ -71,14 +85,20 @@ def test_judge_matches_step_qacc(device):
         assert np.array_equal(before[k], after[k]), k
     c = int(cnt.item())
     e = np.sort(lst[:c].cpu().numpy())
-    expect = np.sort(np.union1d(np.arange(3, st.n, 16), np.nonzero(fallback.cpu().numpy())[0]))
+    expect = np.union1d(judged_envs(st.n, 3), np.nonzero(fallback.cpu().numpy())[0])
     assert np.array_equal(e, expect)
+    # the list-free form steps the same envs to the same bits
+    qacc_b = torch.full((35, st.n), 1e9, device=device)
+    _judge_blocks(st, fallback, 3, qacc_b, device)
+    torch.cuda.synchronize()
+    assert np.array_equal(before["qpos"], st.host()["qpos"])
     st.step(1, 2e-3)
     ref = st.host()["qacc"]
     q = qacc.cpu().numpy()
     assert np.array_equal(q[:, e], ref[:, e])
     others = np.setdiff1d(np.arange(st.n), e)
     assert (q[:, others] == 1e9).all()
+    assert np.array_equal(qacc_b.cpu().numpy(), q)
 
 
 def _stepper_with_oracle_pred(device, n, seed, corrupt=None):
@@ -96,11 +116,7 @@ def _stepper_with_oracle_pred(device, n, seed, corrupt=None):
     stepper.pred[:n, :NOUT] = ((qa - mlp.ymean[:, None]) * mlp.yinv[:, None]).T
     lib = _lib.load()
     s = torch.cuda.current_stream(device).cuda_stream
-    _lib.check(lib.g1_hip_judge_list(n, 0, 16, stepper.fallback.data_ptr(), stepper.list.data_ptr(),
-                                     stepper.count.data_ptr(), s))
-    _lib.check(lib.g1_hip_step_judge(n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
-                                     st.ctrl.data_ptr(), None, stepper.list.data_ptr(), stepper.count.data_ptr(),
-                                     stepper.qacc_aba.data_ptr(), 0, s))
+    _judge_blocks(st, stepper.fallback, 0, stepper.qacc_aba, device)
     _lib.check(lib.g1_hip_surrogate_integrate(n, 2e-3, 0, 16, 1e-3, st.qpos.data_ptr(), st.qvel.data_ptr(),
                                               stepper.pred.data_ptr(), POUT, mlp.ymean.data_ptr(),
                                               stepper.ystd.data_ptr(), stepper.qacc_aba.data_ptr(),
@@ -124,13 +140,13 @@ def test_judge_rejects_and_falls_back(device):
     st, ref, stepper = _stepper_with_oracle_pred(device, 4096, 24, corrupt=bad)
     a, b = st.host(), ref.host()
     fb = stepper.fallback.cpu().numpy().astype(bool)
-    judged = np.arange(0, 4096, 16)
-    assert fb[judged].all()                          # judged envs 0, 16, ... are all in `bad`
-    assert fb.sum() == judged.size
+    judged = judged_envs(4096, 0)
+    rejected = np.intersect1d(judged, bad)
+    assert np.array_equal(np.nonzero(fb)[0], rejected)   # exactly the judged envs predicted wrong
     # judged envs used ABA's qacc: their state matches the fused step
     assert np.abs(a["qpos"][:, judged] - b["qpos"][:, judged]).max() < 1e-5
     sm = stepper.stats.cpu().numpy()
-    assert sm[1] == judged.size
+    assert sm[0] == judged.size and sm[1] == rejected.size
 
 
 def test_trained_network_gain(device):
@@ -255,7 +271,46 @@ def test_judge_two_wave_split_matches_step(device, n):
     torch.cuda.synchronize()
     c = int(cnt.item())
     e = lst[:c].cpu().numpy()
-    assert np.array_equal(np.sort(e), np.union1d(np.arange(7, n, 16), np.arange(0, n, 5)))
+    assert np.array_equal(np.sort(e), np.union1d(judged_envs(n, 7), np.arange(0, n, 5)))
+    qacc_b = torch.full((35, n), 1e9, device=device)
+    _judge_blocks(st, fallback, 7, qacc_b, device)
     st.step(1, 2e-3)
     ref = st.host()["qacc"]
     assert np.array_equal(qacc.cpu().numpy()[:, e], ref[:, e])
+    assert np.array_equal(qacc_b.cpu().numpy(), qacc.cpu().numpy())
+
+
+def test_fallback_tiles_skip_the_network(device):
+    """Blocks of 64 envs all in fallback and not judged this step skip the
+    network in the fused step; the result is the same as through features +
+    GEMMs + integrate, which run the network everywhere."""
+    mlp = _trained_small(device, seed=8)
+    n = 64 * 40 + 17
+    qp, qv, an, cb = W.randomized_actuation(n, seed=28, nblocks=1)
+    fb0 = np.zeros(n, np.uint8)
+    for blk in (1, 2, 5, 17, 33):                 # whole blocks, none judged at phase 0
+        fb0[64 * blk:64 * blk + 64] = 1
+    fb0[64 * 3:64 * 3 + 63] = 1                   # one env short of a whole block
+    fb0[64 * 16:64 * 16 + 64] = 1                 # whole block, judged at phase 0
+    fb0[64 * 40:] = 1                             # the partial tail block
+    runs = []
+    for fused in (True, False):
+        st = GpuState(qp, qv, an, cb[0], device)
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e30, fused=fused)
+        stepper.fallback.copy_(torch.as_tensor(fb0, device=device))
+        stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
+        torch.cuda.synchronize()
+        runs.append((st.host(), stepper.fallback.cpu().numpy(), stepper.stats.cpu().numpy()))
+    (ha, fa, sa), (hb, fb, sb) = runs
+    skipped = np.zeros(n, bool)
+    for blk in (1, 2, 5, 17, 33, 40):
+        skipped[64 * blk:64 * blk + 64] = True
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(ha[k][:, skipped], hb[k][:, skipped]), k
+        d = np.abs(ha[k] - hb[k])
+        assert d.max() <= 1e-4 * max(1.0, np.abs(hb[k]).max()), k
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(sa[[0, 1, 3]], sb[[0, 1, 3]])
+    # tol 1e30 accepts every judgement: block 16 (judged) leaves fallback
+    assert not fa[64 * 16:64 * 17].any() and fa[64 * 3:64 * 3 + 63].all()
+    assert sa[3] == int(fb0.sum()) - 64           # fallback env-steps: all but the judged block
