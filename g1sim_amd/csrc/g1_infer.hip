@@ -34,6 +34,7 @@
 #include "g1_fastmath.h"
 #include "g1_commit.h"
 #include "g1_act.h"
+#include "g1_err.h"
 
 namespace g1inf {
 
@@ -637,7 +638,6 @@ __global__ __launch_bounds__(256) void rollout_init_kernel(int nenv, float lin, 
 // ================================================================== C-ABI
 namespace {
 using namespace g1inf;
-int launch_status() { return hipGetLastError() == hipSuccess ? 0 : -2; }
 
 bool fill_surrogate_weights(SurrogateArgs& a, const void* const* W, const float* const* b) {
   if (!W || !b) return false;
@@ -655,43 +655,49 @@ extern "C" {
 int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, const float* tau, const float* xmean,
                              const float* xinv, const void* const* W, const float* const* b, float* pred, int ldp,
                              void* stream) {
-  if (nenv < 0 || !qpos || !qvel || !tau || !xmean || !xinv || !pred || ldp < SUR_POUT) return -1;
+  g1err::clear();
+  if (nenv < 0 || !qpos || !qvel || !tau || !xmean || !xinv || !pred || ldp < SUR_POUT)
+    return g1err::fail("g1_hip_surrogate_forward: invalid arguments");
   SurrogateArgs a{};
-  if (!fill_surrogate_weights(a, W, b)) return -1;
+  if (!fill_surrogate_weights(a, W, b)) return g1err::fail("g1_hip_surrogate_forward: invalid arguments");
   if (nenv == 0) return 0;
   a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
   a.pred = pred; a.ldp = ldp;
   hipLaunchKernelGGL(surrogate_kernel<0>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
-  return launch_status();
+  return g1err::launch("g1_hip_surrogate_forward");
 }
 
 int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
                           const float* tau, const float* xmean, const float* xinv, const void* const* W,
                           const float* const* b, const float* ymean, const float* ystd, const float* qacc_aba,
                           uint8_t* fallback, unsigned* stats, float* qacc_out, void* stream) {
-  if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !(dt > 0.f)) return -1;
-  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba) return -1;
+  g1err::clear();
+  if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !(dt > 0.f))
+    return g1err::fail("g1_hip_surrogate_step: invalid arguments");
+  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba)
+    return g1err::fail("g1_hip_surrogate_step: invalid arguments");
   SurrogateArgs a{};
-  if (!fill_surrogate_weights(a, W, b)) return -1;
+  if (!fill_surrogate_weights(a, W, b)) return g1err::fail("g1_hip_surrogate_step: invalid arguments");
   if (nenv == 0) return 0;
   a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
   a.phase = phase; a.period = period; a.tol = tol; a.dt = dt; a.ymean = ymean; a.ystd = ystd;
   a.qacc_aba = qacc_aba; a.fallback = fallback; a.stats = stats; a.qacc_out = qacc_out;
   a.qpos_rw = qpos; a.qvel_rw = qvel;
   hipLaunchKernelGGL(surrogate_kernel<1>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
-  return launch_status();
+  return g1err::launch("g1_hip_surrogate_step");
 }
 
 int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const float* commands,
                   const float* last_ctrl, const float* obs_mean, const float* obs_inv, float clip,
                   const void* const* W, const float* const* b, const float* log_std, unsigned seed, int t,
                   float* ctrl, float* actions, float* obs_out, int* push_interval, void* stream) {
+  g1err::clear();
   if (nenv < 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !obs_mean ||
       !obs_inv || !ctrl || !W || !b || !(clip > 0.f))
-    return -1;
+    return g1err::fail("g1_policy_act: invalid arguments");
   PolicyArgs a{};
   for (int l = 0; l < 4; ++l) {
-    if (!W[l] || !b[l]) return -1;
+    if (!W[l] || !b[l]) return g1err::fail("g1_policy_act: invalid arguments");
     a.W[l] = (const bf16*)W[l];
     a.b[l] = b[l];
   }
@@ -701,18 +707,20 @@ int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const fl
   a.obs_mean = obs_mean; a.obs_inv = obs_inv; a.clip = clip; a.log_std = log_std; a.seed = seed; a.t = t;
   a.ctrl = ctrl; a.actions = actions; a.obs_out = obs_out; a.push_interval = push_interval;
   hipLaunchKernelGGL(policy_kernel, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
-  return launch_status();
+  return g1err::launch("g1_policy_act");
 }
 
 int g1_hip_get_obs(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
                    const float* last_ctrl, float* obs, void* stream) {
-  if (nenv < 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !obs) return -1;
+  g1err::clear();
+  if (nenv < 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !obs)
+    return g1err::fail("g1_hip_get_obs: invalid arguments");
   if (nenv == 0) return 0;
   const int nobs = layout == 0 ? 99 : 96;
   const size_t n = (size_t)nenv * nobs;
   hipLaunchKernelGGL(obs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nenv, layout,
                      nobs, qpos, qvel, commands, last_ctrl, obs);
-  return launch_status();
+  return g1err::launch("g1_hip_get_obs");
 }
 
 int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float* foot_pos, const float* foot_vel,
@@ -720,32 +728,36 @@ int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float*
                     int* command_timeout, int* episode_length, int* push_interval, int max_episode_length,
                     float cmd_lin, float cmd_yaw, unsigned seed, int t, float* reward, uint8_t* done,
                     uint8_t* reset, void* stream) {
+  g1err::clear();
   if (nenv < 0 || max_episode_length <= 0 || !qpos || !qvel || !foot_pos || !foot_vel || !contact_forces ||
       !actions || !commands || !prev_actions || !command_timeout || !episode_length || !reward || !done || !reset)
-    return -1;
+    return g1err::fail("g1_rollout_post: invalid arguments");
   if (nenv == 0) return 0;
   PostArgs a{nenv, qpos, qvel, foot_pos, foot_vel, contact_forces, actions, commands, prev_actions,
              command_timeout, episode_length, push_interval, max_episode_length, cmd_lin, cmd_yaw, seed, t,
              reward, done, reset};
   hipLaunchKernelGGL(post_step_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
-  return launch_status();
+  return g1err::launch("g1_rollout_post");
 }
 
 int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float* commands, int* command_timeout,
                     int* episode_length, int* push_interval, void* stream) {
-  if (nenv < 0 || !commands || !command_timeout || !episode_length) return -1;
+  g1err::clear();
+  if (nenv < 0 || !commands || !command_timeout || !episode_length)
+    return g1err::fail("g1_rollout_init: invalid arguments");
   if (nenv == 0) return 0;
   hipLaunchKernelGGL(rollout_init_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv,
                      cmd_lin, cmd_yaw, seed, commands, command_timeout, episode_length, push_interval);
-  return launch_status();
+  return g1err::launch("g1_rollout_init");
 }
 
 int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
                               const float* last_ctrl, float* mean, float* var, float* count, float* inv,
                               double* scratch, void* stream) {
+  g1err::clear();
   if (nenv <= 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !mean || !var ||
       !count || !inv || !scratch)
-    return -1;
+    return g1err::fail("g1_policy_obs_norm_update: invalid arguments");
   const int nobs = layout == 0 ? 99 : 96;
   const int chunk = (nenv + NORM_CHUNKS - 1) / NORM_CHUNKS;
   const int nchunks = (nenv + chunk - 1) / chunk;
@@ -753,7 +765,7 @@ int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const flo
                      chunk, qpos, qvel, commands, last_ctrl, mean, scratch);
   hipLaunchKernelGGL(obs_norm_merge_kernel, dim3(1), dim3(128), 0, (hipStream_t)stream, nenv, nobs, nchunks, scratch,
                      mean, var, count, inv);
-  return launch_status();
+  return g1err::launch("g1_policy_obs_norm_update");
 }
 
 }  // extern "C"

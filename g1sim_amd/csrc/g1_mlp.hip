@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "g1_act.h"
+#include "g1_err.h"
 
 namespace g1mlp {
 
@@ -310,21 +311,26 @@ template <bool AT, bool BT, int E>
 int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
   dim3 grid(a.I / BI, a.J / BJ, split);
   hipLaunchKernelGGL((gemm_kernel<AT, BT, E>), grid, dim3(256), 0, s, a);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_gemm");
 }
 }  // namespace
 
 extern "C" {
 
-// C-ABI of the MLP kernels; see include/g1sim_amd.h for the contract.
+// C-ABI of the MLP kernels; the contract of each entry point is documented with its declaration.
 int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
                 int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, void* stream) {
-  if (I <= 0 || J <= 0 || R <= 0 || split <= 0 || I % BI || J % BJ || R % (BR * split)) return -1;
-  if (!A || !B) return -1;
+  g1err::clear();
+  if (I <= 0 || J <= 0 || R <= 0 || split <= 0 || I % BI || J % BJ || R % (BR * split))
+    return g1err::fail("g1_mlp_gemm: shape %d x %d x %d (split %d) is not a multiple of the %d x %d x %d tile",
+                       I, J, R, split, BI, BJ, BR);
+  if (!A || !B) return g1err::fail("g1_mlp_gemm: A and B are required");
   if ((epi != EPI_BIAS_SILU_H && !C) || ((epi == EPI_BIAS_SILU || epi == EPI_BIAS_SILU_H || epi == EPI_SILU_BWD) && !H))
-    return -1;
-  if ((epi >= EPI_BIAS_SILU && epi <= EPI_BIAS_F32 && !bias) || (epi == EPI_SILU_BWD && !Zin)) return -1;
-  if (split > 1 && epi != EPI_F32_ATOMIC && epi != EPI_F32_SPLIT) return -1;
+    return g1err::fail("g1_mlp_gemm: epilogue %d needs its C / H outputs", epi);
+  if ((epi >= EPI_BIAS_SILU && epi <= EPI_BIAS_F32 && !bias) || (epi == EPI_SILU_BWD && !Zin))
+    return g1err::fail("g1_mlp_gemm: epilogue %d needs bias / Zin", epi);
+  if (split > 1 && epi != EPI_F32_ATOMIC && epi != EPI_F32_SPLIT)
+    return g1err::fail("g1_mlp_gemm: split-R needs the F32_ATOMIC or F32_SPLIT epilogue");
   GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin};
   hipStream_t s = (hipStream_t)stream;
   const int key = (a_trans ? 1 : 0) | (b_trans ? 2 : 0);
@@ -338,75 +344,85 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
     case 3 * 8 + EPI_F32_ATOMIC: return launch_gemm<true, true, EPI_F32_ATOMIC>(g, split, s);
     case 3 * 8 + EPI_F32_SPLIT: return launch_gemm<true, true, EPI_F32_SPLIT>(g, split, s);
     case 3 * 8 + EPI_F32: return launch_gemm<true, true, EPI_F32>(g, split, s);
-    default: return -1;
+    default: return g1err::fail("g1_mlp_gemm: invalid arguments");
   }
 }
 
 int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16, float* dY32,
                     float* sse, void* stream) {
-  if (B <= 0 || ld <= 0 || n_out > ld) return -1;
+  g1err::clear();
+  if (B <= 0 || ld <= 0 || n_out > ld) return g1err::fail("g1_mlp_mse_grad: invalid arguments");
   const int n = B * ld;
   hipLaunchKernelGGL(mse_grad_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, pred, target, B,
                      ld, n_out, (bf16*)dY16, dY32, sse);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_mse_grad");
 }
 
 int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, void* stream) {
-  if (split <= 0 || n < 0 || !parts || !out) return -1;
+  g1err::clear();
+  if (split <= 0 || n < 0 || !parts || !out) return g1err::fail("g1_mlp_sum_splits: invalid arguments");
   if (n == 0) return 0;
   const size_t n4 = ((size_t)n + 3) / 4;
   hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, parts,
                      split, (size_t)n, out);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_sum_splits");
 }
 
 int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream) {
-  if (B <= 0 || ld <= 0) return -1;
+  g1err::clear();
+  if (B <= 0 || ld <= 0) return g1err::fail("g1_mlp_colsum: invalid arguments");
   const int rpb = 64;
   hipLaunchKernelGGL(colsum_kernel, dim3((ld + 63) / 64, (B + rpb - 1) / rpb), dim3(256), 0, (hipStream_t)stream,
                      dZ, B, ld, rpb, db);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_colsum");
 }
 
 int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask, int n, float lr,
                 float b1, float b2, float eps, int t, float wd, void* stream) {
-  if (n <= 0 || t <= 0) return -1;
+  g1err::clear();
+  if (n <= 0 || t <= 0) return g1err::fail("g1_mlp_adam: invalid arguments");
   const float bc1 = 1.f - powf(b1, (float)t), bc2 = 1.f - powf(b2, (float)t);
   hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, m, v, grad,
                      (bf16*)p16, mask, n, lr, b1, b2, eps, bc1, bc2, wd);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_adam");
 }
 
 int g1_mlp_fill(float* x, float v, long long n, void* stream) {
+  g1err::clear();
   if (n <= 0) return n == 0 ? 0 : -1;
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, v,
                      (size_t)n);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_fill");
 }
 
 int g1_mlp_to_bf16(const float* x, void* y, long long n, void* stream) {
+  g1err::clear();
   if (n <= 0) return n == 0 ? 0 : -1;
   hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x,
                      (bf16*)y, (size_t)n);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_to_bf16");
 }
 
 int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfeat, int nout, int nenv,
                   const float* xmean, const float* xinv, const float* ymean, const float* yinv, void* Xn, int ldx,
                   float* Yn, int ldy, void* stream) {
-  if (B <= 0 || nfeat > ldx || (Yn && nout > ldy) || (!idx && B > nenv)) return -1;
+  g1err::clear();
+  if (B <= 0 || nfeat > ldx || (Yn && nout > ldy) || (!idx && B > nenv))
+    return g1err::fail("g1_mlp_gather: invalid arguments");
   hipLaunchKernelGGL(gather_kernel, dim3(B), dim3(128), 0, (hipStream_t)stream, X, Y, idx, B, nfeat, nout, nenv,
                      xmean, xinv, ymean, yinv, (bf16*)Xn, ldx, Yn, ldy);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_gather");
 }
 
 int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, const float* tau, const float* mean,
                     const float* inv, void* Xn, int ldx, void* stream) {
-  if (nenv < 0 || nrows < nenv || ldx < 106 || ldx % 8 || !qpos || !qvel || !tau || !mean || !inv || !Xn) return -1;
+  g1err::clear();
+  if (nenv < 0 || nrows < nenv || ldx < 106 || ldx % 8 || !qpos || !qvel || !tau || !mean || !inv || !Xn)
+    return g1err::fail("g1_mlp_features: invalid arguments");
   if (nrows == 0) return 0;
   hipLaunchKernelGGL(features_kernel, dim3((nrows + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv, nrows,
                      qpos, qvel, tau, mean, inv, (bf16*)Xn, ldx);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_mlp_features");
 }
 
 }  // extern "C"

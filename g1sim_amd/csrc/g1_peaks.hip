@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "g1_err.h"
+
 namespace g1peak {
 
 typedef __bf16 bf16;
@@ -66,18 +68,20 @@ extern "C" {
 // FP32: blocks x 256 threads x iters x 8 FMAs (2 FLOP each).  out: float
 // [blocks * 256].
 int g1_hip_peak_fp32(int blocks, int iters, float* out, void* stream) {
-  if (blocks <= 0 || iters <= 0 || !out) return -1;
+  g1err::clear();
+  if (blocks <= 0 || iters <= 0 || !out) return g1err::fail("g1_hip_peak_fp32: invalid arguments");
   hipLaunchKernelGGL(g1peak::fma_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, iters, 0.999f, 1e-4f,
                      out);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_hip_peak_fp32");
 }
 
 // BF16 MFMA: blocks x 4 waves x iters x 16 MFMAs of 16 x 16 x 32 (16,384
 // FLOP each).  out: float [blocks * 256].
 int g1_hip_peak_bf16(int blocks, int iters, float* out, void* stream) {
-  if (blocks <= 0 || iters <= 0 || !out) return -1;
+  g1err::clear();
+  if (blocks <= 0 || iters <= 0 || !out) return g1err::fail("g1_hip_peak_bf16: invalid arguments");
   hipLaunchKernelGGL(g1peak::mfma_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, iters, out);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return g1err::launch("g1_hip_peak_bf16");
 }
 
 }  // extern "C"

@@ -40,6 +40,7 @@
 #include "g1_model_kernel.h"
 #include "g1_fastmath.h"
 #include "g1_commit.h"
+#include "g1_err.h"
 #include "../../include/g1sim_amd.h"
 
 namespace g1k {
@@ -1342,6 +1343,26 @@ void set_err(const char* fmt, ...) {
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
 }
+}  // namespace
+
+namespace g1err {
+void clear() { g_err[0] = 0; }
+int fail(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return -1;
+}
+int launch(const char* what) {
+  const hipError_t err = hipGetLastError();
+  if (err == hipSuccess) return 0;
+  snprintf(g_err, sizeof(g_err), "%s launch failed: %s", what, hipGetErrorString(err));
+  return -2;
+}
+}  // namespace g1err
+
+namespace {
 
 using KernelFn = void (*)(g1k::StepArgs);
 

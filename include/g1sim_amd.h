@@ -262,7 +262,9 @@ int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float
 int g1_hip_peak_fp32(int blocks, int iters, float* out, void* stream);
 int g1_hip_peak_bf16(int blocks, int iters, float* out, void* stream);
 
-/* Message of the last failure on the calling thread ("" if none). */
+/* Message of the last failure on the calling thread: every int-returning
+ * entry point clears it on entry and names itself and the reason when it
+ * returns non-zero ("" after a successful call). */
 const char* g1_hip_last_error(void);
 
 /* Library version (major * 10000 + minor * 100 + patch). */

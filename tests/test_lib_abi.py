@@ -103,3 +103,21 @@ def test_fused_inference_entry_points_validate_arguments():
     assert lib.g1_rollout_post(8, d, d, d, d, d, d, d, d, d, d, None, 0, 1.0, 0.5, 0, 0, d, d, d, None) == -1
     assert lib.g1_rollout_post(8, d, d, d, d, d, None, d, d, d, d, None, 10, 1.0, 0.5, 0, 0, d, d, d, None) == -1
     assert lib.g1_rollout_init(8, 1.0, 0.5, 0, None, d, d, None, None) == -1
+
+
+def test_every_failure_names_its_entry_point():
+    """A rejected call leaves its reason in g1_hip_last_error, and the next
+    successful call clears it."""
+    lib = _lib.load()
+    d = C.c_void_p(16)
+    ws = _lib.ptr_array([16] * 5)
+    assert lib.g1_hip_surrogate_forward(8, d, d, d, d, d, ws, ws, d, 35, None) == -1
+    assert "g1_hip_surrogate_forward" in _lib.last_error()
+    assert lib.g1_hip_get_obs(0, 0, d, d, d, d, d, None) == 0
+    assert _lib.last_error() == ""
+    assert lib.g1_mlp_gemm(0, 0, 0, d, 64, d, 64, 100, 64, 64, 1, d, None, 64, d, None, None) == -1
+    assert "g1_mlp_gemm" in _lib.last_error() and "tile" in _lib.last_error()
+    assert lib.g1_hip_step_judge_blocks(8, 2e-3, d, d, d, d, None, None, 16, 16, d, 0, None) == -1
+    assert "phase" in _lib.last_error()
+    assert lib.g1_hip_peak_fp32(0, 1, d, None) == -1
+    assert "g1_hip_peak_fp32" in _lib.last_error()
