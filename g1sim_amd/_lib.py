@@ -65,6 +65,8 @@ def load():
     lib.g1_hip_step_judge.argtypes = [i, f, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_step_judge_blocks.restype = i
     lib.g1_hip_step_judge_blocks.argtypes = [i, f, vp, vp, vp, vp, vp, vp, i, i, vp, u, vp]
+    lib.g1_hip_judge_forces.restype = i
+    lib.g1_hip_judge_forces.argtypes = [i, f, vp, vp, vp, vp, vp, vp, i, i, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_surrogate_integrate.restype = i
     lib.g1_hip_surrogate_integrate.argtypes = [i, f, i, i, f, vp, vp, vp, i, vp, vp, vp, vp, vp, vp, vp]
     pa = C.POINTER(C.c_void_p)           # host array of device pointers
@@ -97,7 +99,7 @@ def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
             "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_sum_splits", "g1_mlp_adam",
             "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
-            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_step_judge_blocks", "g1_hip_surrogate_integrate",
+            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_step_judge_blocks", "g1_hip_judge_forces", "g1_hip_surrogate_integrate",
             "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs", "g1_policy_act",
             "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32", "g1_hip_peak_bf16"]
 

@@ -598,7 +598,10 @@ struct StepArgs {
   const int* list_count;
   int stride;
   // block-select judge mode (sel_period > 0): workgroup b steps the envs of
-  // block b that judged_env() or sel_fallback marks, as the list would hold
+  // block b that judged_env() or sel_fallback marks, as the list would hold.
+  // In the judge + forces kernel (step_kernel<..., JF = true>) a block with
+  // any such env is stepped whole (state read-only; qacc, tau, anchors,
+  // contact forces and feet stored) and any other block gets the forces only.
   const uint8_t* sel_fallback;
   int sel_phase, sel_period;
 };
@@ -718,18 +721,25 @@ G1K_DEV void dispatch_post(int chain, const float (&a0)[6], const float (&a15)[6
 // each role keeps two chains in registers); ROLE < 0 dispatches the chain
 // code on the wave index at run time and keeps a single copy of the shared
 // pelvis/waist code (the 4-wave split).
-template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE>
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE, bool JF>
 G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PER_WG],
                        float (&fcx)[2][2][6][ENVS_PER_WG]) {
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   const int lane = threadIdx.x & 63;
   const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool select = args.sel_period > 0;      // judge by blocks, no list
-  const bool judge = select || args.env_list != nullptr;
+  const bool select = !JF && args.sel_period > 0;   // judge by blocks, no list
+  const bool judge = select || (!JF && args.env_list != nullptr);
   bool valid;
   int e;
   size_t N;
-  if (select) {
+  if (JF) {
+    // the whole block, in env order (the kernel has checked it holds a judged
+    // or fallback env)
+    const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
+    valid = e_raw < args.nenv;
+    e = valid ? e_raw : args.nenv - 1;
+    N = (size_t)args.nenv;
+  } else if (select) {
     // this workgroup's block of 64 envs: all of them on the block's judge
     // step, else its fallback envs, packed into the low lanes in env order.
     // Every wave reads the same 64 flags, so the mask and the early exit are
@@ -1022,22 +1032,24 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     return;
   }
   if (wave == 0) {
+    if constexpr (!JF) {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) args.qpos[m * N + e] = (float)bp[m];
+      for (int m = 0; m < 3; ++m) args.qpos[m * N + e] = (float)bp[m];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) args.qpos[(3 + m) * N + e] = (float)bq[m];
-    if constexpr (F64_BASE) {
+      for (int m = 0; m < 4; ++m) args.qpos[(3 + m) * N + e] = (float)bq[m];
+      if constexpr (F64_BASE) {
 #pragma unroll
-      for (int m = 0; m < 3; ++m) args.base64[m * N + e] = (double)bp[m];
+        for (int m = 0; m < 3; ++m) args.base64[m * N + e] = (double)bp[m];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) args.base64[(3 + m) * N + e] = (double)bq[m];
-    }
+        for (int m = 0; m < 4; ++m) args.base64[(3 + m) * N + e] = (double)bq[m];
+      }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      args.qvel[m * N + e] = vw[m];
-      args.qvel[(3 + m) * N + e] = om[m];
-      args.qpos[(7 + 12 + m) * N + e] = wq[m];
-      args.qvel[(6 + 12 + m) * N + e] = wqd[m];
+      for (int m = 0; m < 3; ++m) {
+        args.qvel[m * N + e] = vw[m];
+        args.qvel[(3 + m) * N + e] = om[m];
+        args.qpos[(7 + 12 + m) * N + e] = wq[m];
+        args.qvel[(6 + 12 + m) * N + e] = wqd[m];
+      }
     }
     if (want_tau) {
 #pragma unroll
@@ -1058,8 +1070,10 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 #pragma unroll
     for (int m = 0; m < MAXJ; ++m) {
       if (m < nj) {
-        args.qpos[(7 + h0 + m) * N + e] = cq[c][m];
-        args.qvel[(6 + h0 + m) * N + e] = cqd[c][m];
+        if constexpr (!JF) {
+          args.qpos[(7 + h0 + m) * N + e] = cq[c][m];
+          args.qvel[(6 + h0 + m) * N + e] = cqd[c][m];
+        }
         if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
         if (want_tau) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
       }
@@ -1081,18 +1095,6 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
       }
     }
   });
-}
-
-template <bool F64_BASE, bool F64_CONTACT, int SPLIT>
-__global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
-  __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
-  __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
-  if constexpr (SPLIT == 4) {
-    step_body<F64_BASE, F64_CONTACT, SPLIT, -1>(args, xch, fcx);
-  } else {
-    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) step_body<F64_BASE, F64_CONTACT, SPLIT, 0>(args, xch, fcx);
-    else step_body<F64_BASE, F64_CONTACT, SPLIT, 1>(args, xch, fcx);
-  }
 }
 
 // ------------------------------------------------------- surrogate support
@@ -1175,44 +1177,46 @@ G1K_DEV void leg_forces(const ForcesArgs& a, size_t N, int e, bool live, const f
   }
 }
 
-__global__ __launch_bounds__(128) void forces_kernel(ForcesArgs a) {
-  __shared__ float fx[6][ENVS_PER_WG];
+// One 64-env block of the forces: waves 0 and 1 run a leg each (with half of
+// the upper-body servos); further waves of the workgroup have nothing to do.
+// Every thread of the workgroup must call it (one barrier).
+G1K_DEV void forces_block(const ForcesArgs& a, int wave, float (&fx)[6][ENVS_PER_WG]) {
   const int lane = threadIdx.x & 63;
-  const int leg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
   const bool live = e_raw < a.nenv;
   const int e = live ? e_raw : a.nenv - 1;        // tail lanes mirror the last env, never store
   const size_t N = (size_t)a.nenv;
-  float bq[4], p0[3], vw[3], om[3];
+  float R0[9], p0[3], v0[6], Fb[6];
+  if (wave < 2) {
+    float bq[4], vw[3], om[3];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    p0[m] = a.qpos[m * N + e];
-    vw[m] = a.qvel[m * N + e];
-    om[m] = a.qvel[(3 + m) * N + e];
-  }
+    for (int m = 0; m < 3; ++m) {
+      p0[m] = a.qpos[m * N + e];
+      vw[m] = a.qvel[m * N + e];
+      om[m] = a.qvel[(3 + m) * N + e];
+    }
 #pragma unroll
-  for (int m = 0; m < 4; ++m) bq[m] = a.qpos[(3 + m) * N + e];
-  const float inv = frsq(bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2] + bq[3] * bq[3]);
-  const float w = bq[0] * inv, x = bq[1] * inv, y = bq[2] * inv, z = bq[3] * inv;
-  float R0[9], v0[6];
-  R0[0] = 1 - 2 * (y * y + z * z); R0[1] = 2 * (x * y - w * z); R0[2] = 2 * (x * z + w * y);
-  R0[3] = 2 * (x * y + w * z); R0[4] = 1 - 2 * (x * x + z * z); R0[5] = 2 * (y * z - w * x);
-  R0[6] = 2 * (x * z - w * y); R0[7] = 2 * (y * z + w * x); R0[8] = 1 - 2 * (x * x + y * y);
-  v0[0] = om[0]; v0[1] = om[1]; v0[2] = om[2];
+    for (int m = 0; m < 4; ++m) bq[m] = a.qpos[(3 + m) * N + e];
+    const float inv = frsq(bq[0] * bq[0] + bq[1] * bq[1] + bq[2] * bq[2] + bq[3] * bq[3]);
+    const float w = bq[0] * inv, x = bq[1] * inv, y = bq[2] * inv, z = bq[3] * inv;
+    R0[0] = 1 - 2 * (y * y + z * z); R0[1] = 2 * (x * y - w * z); R0[2] = 2 * (x * z + w * y);
+    R0[3] = 2 * (x * y + w * z); R0[4] = 1 - 2 * (x * x + z * z); R0[5] = 2 * (y * z - w * x);
+    R0[6] = 2 * (x * z - w * y); R0[7] = 2 * (y * z + w * x); R0[8] = 1 - 2 * (x * x + y * y);
+    v0[0] = om[0]; v0[1] = om[1]; v0[2] = om[2];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
-  float Fb[6];
-  if (leg == 0) {
-    leg_forces<0>(a, N, e, live, R0, p0, v0, Fb);
-    servo_store<12, 9>(a, N, e, live);            // waist and left arm
-  } else {
-    leg_forces<1>(a, N, e, live, R0, p0, v0, Fb);
-    servo_store<21, 8>(a, N, e, live);            // right arm
+    for (int m = 0; m < 3; ++m) v0[3 + m] = R0[m] * vw[0] + R0[3 + m] * vw[1] + R0[6 + m] * vw[2];
+    if (wave == 0) {
+      leg_forces<0>(a, N, e, live, R0, p0, v0, Fb);
+      servo_store<12, 9>(a, N, e, live);            // waist and left arm
+    } else {
+      leg_forces<1>(a, N, e, live, R0, p0, v0, Fb);
+      servo_store<21, 8>(a, N, e, live);            // right arm
 #pragma unroll
-    for (int m = 0; m < 6; ++m) fx[m][lane] = Fb[m];
+      for (int m = 0; m < 6; ++m) fx[m][lane] = Fb[m];
+    }
   }
   __syncthreads();
-  if (leg == 0 && live) {
+  if (wave == 0 && live) {
     float F[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) F[m] = Fb[m] + fx[m][lane];
@@ -1222,6 +1226,42 @@ __global__ __launch_bounds__(128) void forces_kernel(ForcesArgs a) {
       a.tau[m * N + e] = R0[3 * m] * F[3] + R0[3 * m + 1] * F[4] + R0[3 * m + 2] * F[5];
       a.tau[(3 + m) * N + e] = F[m];
     }
+  }
+}
+
+__global__ __launch_bounds__(128) void forces_kernel(ForcesArgs a) {
+  __shared__ float fx[6][ENVS_PER_WG];
+  forces_block(a, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), fx);
+}
+
+G1K_DEV ForcesArgs forces_args(const StepArgs& s) {
+  ForcesArgs a;
+  a.nenv = s.nenv; a.cfg = s.cfg;
+  a.qpos = s.qpos; a.qvel = s.qvel; a.anchor = s.anchor; a.ctrl = s.ctrl; a.mu = s.mu; a.tau = s.tau;
+  a.contact_forces = s.contact_forces; a.foot_pos = s.foot_pos; a.foot_vel = s.foot_vel;
+  return a;
+}
+
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT, bool JF>
+__global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
+  __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
+  __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
+  if constexpr (JF) {
+    const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
+    const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
+    const bool in = e_raw < args.nenv && (blk || (args.sel_fallback && args.sel_fallback[e_raw]));
+    if (__ballot(in) == 0ull) {        // uniform: every wave reads the same 64 flags
+      forces_block(forces_args(args), __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), fcx[0][0]);
+      return;
+    }
+  }
+  if constexpr (SPLIT == 4) {
+    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, JF>(args, xch, fcx);
+  } else {
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, JF>(args, xch, fcx);
+    else
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, JF>(args, xch, fcx);
   }
 }
 
@@ -1373,18 +1413,19 @@ using KernelFn = void (*)(g1k::StepArgs);
 constexpr int kSplit2MinEnvs = 32768;
 
 template <int SPLIT>
-KernelFn pick(bool f64b, bool f64c) {
-  return f64b ? (f64c ? g1k::step_kernel<true, true, SPLIT> : g1k::step_kernel<true, false, SPLIT>)
-              : (f64c ? g1k::step_kernel<false, true, SPLIT> : g1k::step_kernel<false, false, SPLIT>);
+KernelFn pick(bool f64b, bool f64c, bool jf) {
+  if (jf) return f64c ? g1k::step_kernel<false, true, SPLIT, true> : g1k::step_kernel<false, false, SPLIT, true>;
+  return f64b ? (f64c ? g1k::step_kernel<true, true, SPLIT, false> : g1k::step_kernel<true, false, SPLIT, false>)
+              : (f64c ? g1k::step_kernel<false, true, SPLIT, false> : g1k::step_kernel<false, false, SPLIT, false>);
 }
 
-int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream) {
+int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream, bool jf = false) {
   const bool f64b = flags & G1_FLAG_F64_BASE, f64c = flags & G1_FLAG_F64_CONTACT;
   a.f64_solve = (flags & G1_FLAG_F64_SOLVE) ? 1 : 0;
   int split = a.nenv >= kSplit2MinEnvs ? 2 : 4;
   if (flags & G1_FLAG_WAVES4) split = 4;
   if (flags & G1_FLAG_WAVES2) split = 2;
-  KernelFn k = split == 2 ? pick<2>(f64b, f64c) : pick<4>(f64b, f64c);
+  KernelFn k = split == 2 ? pick<2>(f64b, f64c, jf) : pick<4>(f64b, f64c, jf);
   const int nblocks = (a.nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG;
   hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * split), 0, stream, a);
   hipError_t err = hipGetLastError();
@@ -1527,6 +1568,34 @@ int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float*
   a.stride = nenv;
   a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
   return launch_step(a, flags, (hipStream_t)stream);
+}
+
+int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor, const float* ctrl,
+                        const float* mu, const uint8_t* fallback, int phase, int period, float* qacc_out,
+                        float* tau_out, float* contact_forces, float* foot_pos, float* foot_vel, unsigned flags,
+                        void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (period <= 0 || phase < 0 || phase >= period || !qacc_out || !tau_out) {
+    set_err("judge_forces: bad phase/period, or no qacc / tau");
+    return -1;
+  }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("judge_forces: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv; a.nsub = 1;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = const_cast<float*>(qpos); a.qvel = const_cast<float*>(qvel); a.anchor = anchor;
+  a.ctrl = ctrl;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
+  a.stride = nenv;
+  a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
+  return launch_step(a, flags, (hipStream_t)stream, true);
 }
 
 int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,

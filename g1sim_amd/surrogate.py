@@ -382,14 +382,13 @@ class SurrogateStepper:
         lib, s, n = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
         phase = self.t % self.period
         ck = _lib.check
-        ck(lib.g1_hip_step_judge_blocks(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
-                                        ctrl.data_ptr(), None if mu is None else mu.data_ptr(),
-                                        self.fallback.data_ptr(), phase, self.period, self.qacc_aba.data_ptr(),
-                                        self.judge_flags, s), "step_judge_blocks")
-        ck(lib.g1_hip_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
-                             None if mu is None else mu.data_ptr(), self.tau.data_ptr(),
-                             self.contact_forces.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(), s),
-           "forces")
+        # ABA qacc of the judged and fallback envs, and everyone's tau_total,
+        # contact update and feet, in one kernel
+        ck(lib.g1_hip_judge_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
+                                   None if mu is None else mu.data_ptr(), self.fallback.data_ptr(), phase,
+                                   self.period, self.qacc_aba.data_ptr(), self.tau.data_ptr(),
+                                   self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
+                                   self.foot_vel.data_ptr(), self.judge_flags, s), "judge_forces")
         m = self.mlp
         if self.fused:
             W, b = m._wb()

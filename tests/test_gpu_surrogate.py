@@ -314,3 +314,46 @@ def test_fallback_tiles_skip_the_network(device):
     # tol 1e30 accepts every judgement: block 16 (judged) leaves fallback
     assert not fa[64 * 16:64 * 17].any() and fa[64 * 3:64 * 3 + 63].all()
     assert sa[3] == int(fb0.sum()) - 64           # fallback env-steps: all but the judged block
+
+
+@pytest.mark.parametrize("n", [5000, 40000])
+def test_judge_forces_matches_its_parts(device, n):
+    """The judge + forces kernel: a block of 64 envs with a judged or fallback
+    env carries the step's own qacc, tau, anchors, normal forces and feet;
+    every other block carries g1_hip_forces' outputs; the state is not
+    written.  Bit for bit, at both work splits."""
+    st = _state(device, n, 29, steps=5)
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    fallback = torch.zeros(n, dtype=torch.uint8, device=device)
+    fallback[::97] = 1
+    phase = 5
+    z = lambda d: torch.zeros((d, n), device=device)  # noqa: E731
+    before = st.host()
+    qacc, tau, fn, fp, fv = torch.full((35, n), 1e9, device=device), z(35), z(8), z(6), z(6)
+    anc = st.anchor.clone()
+    _lib.check(lib.g1_hip_judge_forces(n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), anc.data_ptr(),
+                                       st.ctrl.data_ptr(), None, fallback.data_ptr(), phase, 16, qacc.data_ptr(),
+                                       tau.data_ptr(), fn.data_ptr(), fp.data_ptr(), fv.data_ptr(), 0, s),
+               "judge_forces")
+    tau_f, fn_f, fp_f, fv_f = z(35), z(8), z(6), z(6)
+    anc_f = st.anchor.clone()
+    _lib.check(lib.g1_hip_forces(n, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), anc_f.data_ptr(),
+                                 st.ctrl.data_ptr(), None, tau_f.data_ptr(), fn_f.data_ptr(), fp_f.data_ptr(),
+                                 fv_f.data_ptr(), s), "forces")
+    torch.cuda.synchronize()
+    after = st.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(before[k], after[k]), k
+    st.step(1, 2e-3, tau=True)
+    ref = st.host()
+    blk = np.arange(n) // 64
+    sel = ((blk % 16) == phase) | (fallback.cpu().numpy() != 0)
+    stepped = np.isin(blk, np.unique(blk[sel]))
+    assert 0 < stepped.sum() < n
+    h = {k: v.cpu().numpy() for k, v in dict(qacc=qacc, tau=tau, fn=fn, fp=fp, fv=fv, anc=anc).items()}
+    f = {k: v.cpu().numpy() for k, v in dict(tau=tau_f, fn=fn_f, fp=fp_f, fv=fv_f, anc=anc_f).items()}
+    assert np.array_equal(h["qacc"][:, sel], ref["qacc"][:, sel])
+    for k, rk in (("tau", "tau"), ("fn", "fn"), ("fp", "foot_pos"), ("fv", "foot_vel"), ("anc", "anchor")):
+        assert np.array_equal(h[k][:, stepped], ref[rk][:, stepped]), k
+        assert np.array_equal(h[k][:, ~stepped], f[k][:, ~stepped]), k
