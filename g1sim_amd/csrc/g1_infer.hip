@@ -216,14 +216,25 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
     }
   }
   {
-    // network input: z-scored (qpos 36, qvel 35, tau 35), zero padding
-    const int r = tid & 63, e = e0 + r;
-    for (int f = w; f < SUR_PIN; f += NWAVE) {
+    // network input: z-scored (qpos 36, qvel 35, tau 35), zero padding.
+    // Wave w owns features w, w + 8, ...; all its loads are issued before
+    // the first is used (one memory latency, not 16).
+    constexpr int FPW = SUR_PIN / NWAVE;
+    static_assert(SUR_PIN % NWAVE == 0, "features split evenly over the waves");
+    const int r = tid & 63, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
+    float raw[FPW];
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) {
+      const int f = w + NWAVE * i;
+      const float* src = f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N
+                                                                  : a.tau + (size_t)(f - 71) * N;
+      raw[i] = f < SUR_NIN ? src[ec] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) {
+      const int f = w + NWAVE * i;
       float x = 0.f;
-      if (e < a.nenv && f < SUR_NIN) {
-        const float raw = f < 36 ? a.qpos[f * N + e] : f < 71 ? a.qvel[(f - 36) * N + e] : a.tau[(f - 71) * N + e];
-        x = (raw - a.xmean[f]) * a.xinv[f];
-      }
+      if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
       buf[0][r * LDA + f] = (bf16)x;
     }
   }
@@ -297,13 +308,30 @@ __device__ __forceinline__ void quat_rotate_inverse(float qw, float qx, float qy
   o[2] = v[2] + 2.0f * (qw * a2 + b2);
 }
 
-// Feature f of env e of the observation.  layout 0: the 99-dim G1Sim.get_obs
-// order (commands, projected gravity, base linear velocity in the base frame,
-// base angular velocity, q - default, qd, last ctrl); layout 1: the 96-dim
-// order (base angular velocity, projected gravity, commands, q - default, qd,
-// last ctrl).
-__device__ float obs_feature(int layout, int f, size_t N, int e, const float* qpos, const float* qvel,
-                             const float* commands, const float* last_ctrl) {
+// The raw per-env inputs of the observation, as rows: quaternion (4), hinge
+// q (29), base linear velocity in the world frame (3), base angular velocity
+// (3), hinge qd (29), commands (3), last ctrl (29).
+constexpr int RAW_Q = 0, RAW_JP = 4, RAW_LV = 33, RAW_AV = 36, RAW_JV = 39, RAW_CMD = 68, RAW_LAST = 71,
+              RAW_N = 100;
+
+// Row `row` of the raw inputs in the SoA state ([dim][N]).
+__device__ __forceinline__ const float* raw_row(int row, size_t N, const float* qpos, const float* qvel,
+                                                const float* commands, const float* last_ctrl) {
+  if (row < RAW_JP) return qpos + (size_t)(3 + row - RAW_Q) * N;
+  if (row < RAW_LV) return qpos + (size_t)(7 + row - RAW_JP) * N;
+  if (row < RAW_AV) return qvel + (size_t)(row - RAW_LV) * N;
+  if (row < RAW_JV) return qvel + (size_t)(3 + row - RAW_AV) * N;
+  if (row < RAW_CMD) return qvel + (size_t)(6 + row - RAW_JV) * N;
+  if (row < RAW_LAST) return commands + (size_t)(row - RAW_CMD) * N;
+  return last_ctrl + (size_t)(row - RAW_LAST) * N;
+}
+
+// Feature f of one env's observation from its raw inputs (raw(row) returns
+// raw row `row`).  layout 0: the 99-dim G1Sim.get_obs order (commands,
+// projected gravity, base linear velocity in the base frame, base angular
+// velocity, q - default, qd, last ctrl); layout 1: the 96-dim order (base
+// angular velocity, projected gravity, commands, q - default, qd, last ctrl).
+template <typename Raw> __device__ __forceinline__ float obs_eval(int layout, int f, Raw raw) {
   int kind, i;   // 0 cmd, 1 gravity, 2 lin vel, 3 ang vel, 4 joint pos, 5 joint vel, 6 last ctrl
   if (layout == 0) {
     if (f < 12) { kind = f / 3 == 0 ? 0 : f / 3 == 1 ? 1 : f / 3 == 2 ? 2 : 3; i = f % 3; }
@@ -313,19 +341,24 @@ __device__ float obs_feature(int layout, int f, size_t N, int e, const float* qp
     else { kind = 4 + (f - 9) / NU; i = (f - 9) % NU; }
   }
   switch (kind) {
-    case 0: return commands[i * N + e];
-    case 3: return qvel[(3 + i) * N + e];
-    case 4: return qpos[(7 + i) * N + e] - (float)g1m_qpos_stand[7 + i];
-    case 5: return qvel[(6 + i) * N + e];
-    case 6: return last_ctrl[i * N + e];
+    case 0: return raw(RAW_CMD + i);
+    case 3: return raw(RAW_AV + i);
+    case 4: return raw(RAW_JP + i) - (float)g1m_qpos_stand[7 + i];
+    case 5: return raw(RAW_JV + i);
+    case 6: return raw(RAW_LAST + i);
     default: break;
   }
-  const float qw = qpos[3 * N + e], qx = qpos[4 * N + e], qy = qpos[5 * N + e], qz = qpos[6 * N + e];
   float v[3], o[3];
   if (kind == 1) { v[0] = 0.f; v[1] = 0.f; v[2] = -1.f; }
-  else { v[0] = qvel[e]; v[1] = qvel[N + e]; v[2] = qvel[2 * N + e]; }
-  quat_rotate_inverse(qw, qx, qy, qz, v, o);
+  else { v[0] = raw(RAW_LV); v[1] = raw(RAW_LV + 1); v[2] = raw(RAW_LV + 2); }
+  quat_rotate_inverse(raw(RAW_Q), raw(RAW_Q + 1), raw(RAW_Q + 2), raw(RAW_Q + 3), v, o);
   return o[i];
+}
+
+// Feature f of env e read straight from the state.
+__device__ float obs_feature(int layout, int f, size_t N, int e, const float* qpos, const float* qvel,
+                             const float* commands, const float* last_ctrl) {
+  return obs_eval(layout, f, [&](int row) { return raw_row(row, N, qpos, qvel, commands, last_ctrl)[e]; });
 }
 
 struct PolicyArgs {
@@ -356,11 +389,29 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
   const int e0 = blockIdx.x * ROWS;
   const size_t N = (size_t)a.nenv;
   {
-    const int r = tid & 63, e = e0 + r;
+    // stage the block's raw inputs in LDS (buf[1] is free until the first
+    // layer writes it): every load of a thread is issued before the first
+    // is used, so the stage costs one memory latency
+    float* raw = reinterpret_cast<float*>(buf[1]);
+    static_assert(RAW_N * ROWS * sizeof(float) <= sizeof(buf[1]), "raw stage exceeds its buffer");
+    const int r = tid & 63, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
+    constexpr int RPW = (RAW_N + NWAVE - 1) / NWAVE;   // rows per wave
+    float v[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int row = w + NWAVE * i;
+      v[i] = row < RAW_N ? raw_row(row, N, a.qpos, a.qvel, a.commands, a.last_ctrl)[ec] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int row = w + NWAVE * i;
+      if (row < RAW_N) raw[row * ROWS + r] = v[i];
+    }
+    __syncthreads();
     for (int f = w; f < POL_PIN; f += NWAVE) {
       float x = 0.f;
       if (e < a.nenv && f < a.nobs) {
-        const float o = obs_feature(a.layout, f, N, e, a.qpos, a.qvel, a.commands, a.last_ctrl);
+        const float o = obs_eval(a.layout, f, [&](int row) { return raw[row * ROWS + r]; });
         if (a.obs_out) a.obs_out[(size_t)e * a.nobs + f] = o;
         x = fminf(fmaxf((o - a.obs_mean[f]) * a.obs_inv[f], -a.clip), a.clip);
       }
