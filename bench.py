@@ -255,13 +255,28 @@ def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
 
 
 def kernel_only(kind, nenv, steps, warmup):
-    """One fused inference kernel alone (random weights, randomized-actuation
-    state), timed with HIP events: for profiling runs."""
+    """One fused inference kernel alone, or one surrogate stepper step
+    (judge + forces, network + commit; random weights, every judgement
+    accepted), on the randomized-actuation state, timed with HIP events: for
+    profiling runs."""
     import torch
     from g1sim_amd import workloads as W
     dev = torch.device("cuda:0")
-    qp, qv, _, _ = W.randomized_actuation(nenv, seed=1, nblocks=1)
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=1)
     qpos, qvel = torch.as_tensor(qp, device=dev), torch.as_tensor(qv, device=dev)
+    if kind == "stepper":
+        from g1sim_amd.surrogate import MLP, SurrogateStepper
+        mlp = MLP(dev, seed=0)
+        anchor, ctrl = torch.as_tensor(an, device=dev), torch.as_tensor(cb[0], device=dev)
+        st = SurrogateStepper(mlp, nenv, dev, period=16, tol=1e30)
+        fn = lambda: st.step(qpos, qvel, anchor, ctrl)  # noqa: E731
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ms = _event_avg_ms(fn, steps, torch.cuda.current_stream(dev))
+        print(json.dumps({"kernel": "surrogate stepper step", "nenv": nenv, "steps": steps, "avg_step_ms": ms,
+                          "env_steps_per_s": nenv / (ms * 1e-3)}), flush=True)
+        return
     if kind == "surrogate":
         from g1sim_amd.surrogate import MLP
         mlp = MLP(dev, seed=0)
@@ -333,8 +348,9 @@ def main():
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
     ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
-    ap.add_argument("--kernel", choices=("surrogate", "policy"),
-                    help="time one fused inference kernel alone at --nenv (for profiling) and exit")
+    ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper"),
+                    help="time one fused inference kernel, or one surrogate stepper step, alone at --nenv "
+                         "(for profiling) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")

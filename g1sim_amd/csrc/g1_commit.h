@@ -20,12 +20,15 @@ G1K_DEV bool judged_env(int e, int phase, int period) { return ((e / JUDGE_BLOCK
 
 // `live` lanes are envs < nenv; every lane of the wave must call this (the
 // statistics are reduced across the wave).  sur[35] is the network's qacc in
-// physical units.  stats: [0] judged, [1] rejected, [2] max judged rel. error
-// (float bits, non-negative, so integer max orders them), [3] fallback
-// env-steps.
+// physical units.  The env's inputs come through accessors: aba(j) its ABA
+// qacc j (read only for judged and fallback envs), qp(m) / qv(m) its qpos /
+// qvel row m; the results go to qpos / qvel ([dim][N]).  stats: [0] judged,
+// [1] rejected, [2] max judged rel. error (float bits, non-negative, so
+// integer max orders them), [3] fallback env-steps.
+template <typename ABA, typename QP, typename QV>
 G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period, float tol, float dt,
-                              const float* sur, const float* qacc_aba, uint8_t* fallback, unsigned* stats,
-                              float* qacc_out, float* qpos, float* qvel) {
+                              const float* sur, ABA aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                              QP qp, QV qv, float* qpos, float* qvel) {
   constexpr int NV = 35, NU = 29;
   const bool judged = live && judged_env(e, phase, period);
   const bool fb = live && fallback && fallback[e];
@@ -35,7 +38,7 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
   for (int j = 0; j < NV; ++j) {
     float v = live ? sur[j] : 0.f;
     if (judged || fb) {
-      const float ref = qacc_aba[j * N + e];
+      const float ref = aba(j);
       err = fmaxf(err, fabsf(v - ref) * frcp(fmaxf(fabsf(ref), 1.0f)));
       v = ref;
     }
@@ -67,12 +70,12 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
   float vw[3], om[3], bp[3], bq[4];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    vw[m] = qvel[m * N + e] + dt * qa[m];
-    om[m] = qvel[(3 + m) * N + e] + dt * qa[3 + m];
-    bp[m] = qpos[m * N + e] + dt * vw[m];
+    vw[m] = qv(m) + dt * qa[m];
+    om[m] = qv(3 + m) + dt * qa[3 + m];
+    bp[m] = qp(m) + dt * vw[m];
   }
 #pragma unroll
-  for (int m = 0; m < 4; ++m) bq[m] = qpos[(3 + m) * N + e];
+  for (int m = 0; m < 4; ++m) bq[m] = qp(3 + m);
   const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
   const float angle = wn * dt;
   if (angle >= 1e-12f) {
@@ -97,10 +100,19 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
   for (int m = 0; m < 4; ++m) qpos[(3 + m) * N + e] = bq[m];
 #pragma unroll
   for (int h = 0; h < NU; ++h) {
-    const float v = qvel[(6 + h) * N + e] + dt * qa[6 + h];
+    const float v = qv(6 + h) + dt * qa[6 + h];
     qvel[(6 + h) * N + e] = v;
-    qpos[(7 + h) * N + e] = qpos[(7 + h) * N + e] + dt * v;
+    qpos[(7 + h) * N + e] = qp(7 + h) + dt * v;
   }
+}
+
+// The same with every input read from the SoA state in memory.
+G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period, float tol, float dt,
+                              const float* sur, const float* qacc_aba, uint8_t* fallback, unsigned* stats,
+                              float* qacc_out, float* qpos, float* qvel) {
+  surrogate_commit(
+      live, e, N, phase, period, tol, dt, sur, [&](int j) { return qacc_aba[j * N + e]; }, fallback, stats,
+      qacc_out, [&](int m) { return qpos[m * N + e]; }, [&](int m) { return qvel[m * N + e]; }, qpos, qvel);
 }
 
 }  // namespace g1k

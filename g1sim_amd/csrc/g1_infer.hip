@@ -190,9 +190,14 @@ struct SurrogateArgs {
 constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
 static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 
+// LDS stash of the commit's inputs (MODE 1): rows 0-70 qpos and qvel, rows
+// 71-105 ABA's qacc, each [64 envs].
+constexpr int STASH_ABA = 71, STASH_ROWS = STASH_ABA + NV;
+
 template <int MODE>
 __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
   __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
+  __shared__ float stash[MODE == 1 ? STASH_ROWS * ROWS : 1];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * ROWS;
   const size_t N = (size_t)a.nenv;
@@ -230,12 +235,25 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
                                                                   : a.tau + (size_t)(f - 71) * N;
       raw[i] = f < SUR_NIN ? src[ec] : 0.f;
     }
+    if constexpr (MODE == 1) {
+      // the commit's inputs, read now while the network runs: ABA's qacc of
+      // judged and fallback envs
+      const bool aba = g1k::judged_env(e0, a.phase, a.period) || (a.fallback && a.fallback[ec]);
+#pragma unroll
+      for (int i = 0; i < (NV + NWAVE - 1) / NWAVE; ++i) {
+        const int j = w + NWAVE * i;
+        if (j < NV) stash[(STASH_ABA + j) * ROWS + r] = aba ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < FPW; ++i) {
       const int f = w + NWAVE * i;
       float x = 0.f;
       if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
       buf[0][r * LDA + f] = (bf16)x;
+      if constexpr (MODE == 1) {
+        if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+      }
     }
   }
   __syncthreads();
@@ -263,8 +281,11 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
       float sur[NV];
 #pragma unroll
       for (int j = 0; j < NV; ++j) sur[j] = out[lane * LDO + j] * a.ystd[j] + a.ymean[j];
-      g1k::surrogate_commit(live, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback, a.stats,
-                            a.qacc_out, a.qpos_rw, a.qvel_rw);
+      const float* st = stash + lane;
+      g1k::surrogate_commit(
+          live, e, N, a.phase, a.period, a.tol, a.dt, sur, [&](int j) { return st[(STASH_ABA + j) * ROWS]; },
+          a.fallback, a.stats, a.qacc_out, [&](int m) { return st[m * ROWS]; },
+          [&](int m) { return st[(36 + m) * ROWS]; }, a.qpos_rw, a.qvel_rw);
     }
   }
 }
