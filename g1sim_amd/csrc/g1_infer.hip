@@ -42,6 +42,12 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// k-steps of weight fragments loaded ahead of their MFMAs (a build-time
+// tuning knob; the ring holds this many + 1 steps in registers)
+#ifndef G1INF_WEIGHT_PREFETCH
+#define G1INF_WEIGHT_PREFETCH 3
+#endif
+
 constexpr int ROWS = 64;                 // envs per workgroup
 constexpr int NWAVE = 8;
 constexpr int NTHR = 64 * NWAVE;
@@ -67,7 +73,13 @@ template <int NOUT> struct Tile {
 // contiguous block, so the fragments of a k-step (16 rows x 64 B per load
 // instruction) are contiguous 1 KB runs (row-major [NOUT][NIN] puts them
 // NIN * 2 bytes apart and measured at 0.6x the rate on MI355X).
-template <int NIN, int NOUT>
+//
+// TR = false: acc[r][c][q] is out[row (r0 + r) * 16 + (lane / 16) * 4 + q]
+// [col (c0 + c) * 16 + lane % 16]; TR = true (weights as the first MFMA
+// operand, so the tile comes out transposed): acc[r][c][q] is
+// out[row (r0 + r) * 16 + lane % 16][col (c0 + c) * 16 + (lane / 16) * 4 + q],
+// four adjacent columns of one row per lane.
+template <int NIN, int NOUT, bool TR = false>
 __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf16* __restrict__ W, int w,
                                           int lane, f32x4 (&acc)[Tile<NOUT>::RT][Tile<NOUT>::CT]) {
   using T = Tile<NOUT>;
@@ -85,7 +97,7 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
   // of PF + 1, activation fragments (LDS) one step ahead; a scheduling
   // barrier per k-step keeps the compiler from sinking the prefetches next
   // to their uses.
-  constexpr int PF = KS >= 3 ? 2 : 1, NB = PF + 1;
+  constexpr int PF = KS > G1INF_WEIGHT_PREFETCH ? G1INF_WEIGHT_PREFETCH : (KS >= 3 ? 2 : 1), NB = PF + 1;
   bf16x8 fb[NB][T::CT], fa[2][T::RT];
 #pragma unroll
   for (int k = 0; k < PF && k < KS; ++k)
@@ -110,7 +122,8 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], fb[k % NB][c], acc[r][c], 0, 0, 0);
+        acc[r][c] = TR ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[k % NB][c], fa[k & 1][r], acc[r][c], 0, 0, 0)
+                       : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], fb[k % NB][c], acc[r][c], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -121,23 +134,26 @@ __device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __rest
                                       const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane) {
   using T = Tile<NOUT>;
   f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT>(sin, W, w, lane, acc);
+  mma_layer<NIN, NOUT, true>(sin, W, w, lane, acc);
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
+  typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
-    const int col = (c0 + c) * 16 + lr;
-    const float bc = b[col];
+    // four adjacent columns of one row per lane: one 8-byte LDS store
+    const int col = (c0 + c) * 16 + (lane >> 4) * 4;
+    const float bc[4] = {b[col], b[col + 1], b[col + 2], b[col + 3]};
 #pragma unroll
-    for (int r = 0; r < T::RT; ++r)
+    for (int r = 0; r < T::RT; ++r) {
+      const int row = (r0 + r) * 16 + lr;
+      bf16x4 h;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = (r0 + r) * 16 + (lane >> 4) * 4 + q;
-        const float z = acc[r][c][q] + bc;
-        float h;
-        if constexpr (ACT == ACT_SILU) h = g1act::silu(z);
-        else h = g1act::elu(z);
-        sout[row * LDA + col] = (bf16)h;
+        const float z = acc[r][c][q] + bc[q];
+        if constexpr (ACT == ACT_SILU) h[q] = (bf16)g1act::silu(z);
+        else h[q] = (bf16)g1act::elu(z);
       }
+      *reinterpret_cast<bf16x4*>(sout + row * LDA + col) = h;
+    }
   }
 }
 
