@@ -33,6 +33,7 @@ def load():
         pass
     lib = C.CDLL(LIB_PATH)
     vp, i, u, f = C.c_void_p, C.c_int, C.c_uint, C.c_float
+    pa = C.POINTER(C.c_void_p)           # host array of device pointers
     lib.g1_cuda_step.restype = None
     lib.g1_cuda_step.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_cuda_reset_done.restype = None
@@ -47,6 +48,15 @@ def load():
     lib.g1_mlp_sum_splits.argtypes = [vp, i, C.c_longlong, vp, vp]
     lib.g1_mlp_colsum.restype = i
     lib.g1_mlp_colsum.argtypes = [vp, i, i, vp, vp]
+    lib.g1_mlp_colsum_det.restype = i
+    lib.g1_mlp_colsum_det.argtypes = [vp, i, i, vp, vp, vp]
+    lib.g1_mlp_train_schedule.restype = i
+    lib.g1_mlp_train_schedule.argtypes = [vp, vp, i, f, f, f, f, vp]
+    lib.g1_mlp_sample.restype = i
+    lib.g1_mlp_sample.argtypes = [i, i, i, i, u, vp, vp, vp]
+    lib.g1_mlp_adam_multi.restype = i
+    lib.g1_mlp_adam_multi.argtypes = [i, pa, pa, pa, pa, pa, pa, C.POINTER(C.c_int), C.POINTER(C.c_float), vp, f, f,
+                                      f, vp]
     lib.g1_mlp_adam.restype = i
     lib.g1_mlp_adam.argtypes = [vp, vp, vp, vp, vp, vp, i, f, f, f, f, i, f, vp]
     lib.g1_mlp_fill.restype = i
@@ -69,7 +79,6 @@ def load():
     lib.g1_hip_judge_forces.argtypes = [i, f, vp, vp, vp, vp, vp, vp, i, i, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_surrogate_integrate.restype = i
     lib.g1_hip_surrogate_integrate.argtypes = [i, f, i, i, f, vp, vp, vp, i, vp, vp, vp, vp, vp, vp, vp]
-    pa = C.POINTER(C.c_void_p)           # host array of device pointers
     lib.g1_hip_surrogate_forward.restype = i
     lib.g1_hip_surrogate_forward.argtypes = [i, vp, vp, vp, vp, vp, pa, pa, vp, i, vp]
     lib.g1_hip_surrogate_step.restype = i
@@ -99,15 +108,16 @@ def exported_symbols():
     return ["g1_cuda_step", "g1_cuda_reset_done", "g1_hip_step_ex", "g1_hip_last_error",
             "g1_hip_version", "g1_mlp_gemm", "g1_mlp_mse_grad", "g1_mlp_colsum", "g1_mlp_sum_splits", "g1_mlp_adam",
             "g1_mlp_fill", "g1_mlp_to_bf16", "g1_mlp_gather", "g1_mlp_features", "g1_hip_forces",
-            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_step_judge_blocks", "g1_hip_judge_forces", "g1_hip_surrogate_integrate",
-            "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs", "g1_policy_act",
-            "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32", "g1_hip_peak_bf16"]
+            "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_step_judge_blocks", "g1_hip_judge_forces",
+            "g1_hip_surrogate_integrate", "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs",
+            "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
+            "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi"]
 
 
 def ptr_array(tensors_or_ptrs):
     """Host array of device pointers (the W / b arguments of the fused
     inference entry points)."""
-    vals = [t if isinstance(t, int) else t.data_ptr() for t in tensors_or_ptrs]
+    vals = [None if t is None else t if isinstance(t, int) else t.data_ptr() for t in tensors_or_ptrs]
     return (C.c_void_p * len(vals))(*vals)
 
 

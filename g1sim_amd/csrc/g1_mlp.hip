@@ -240,6 +240,114 @@ __global__ void adam_kernel(float* p, float* m, float* v, const float* grad, bf1
   if (p16) p16[i] = (bf16)pi;
 }
 
+// ---- device-resident training loop: every per-iteration scalar (learning
+// rate, Adam bias corrections, the sampler's iteration) lives in device
+// memory, so one iteration is a fixed kernel sequence that a HIP graph can
+// replay.
+
+// state: int [3] = {next iteration of the run, Adam step t, this iteration};
+// sched: float [3] = {lr, 1 - b1^t, 1 - b2^t}.  Cosine decay from lr to
+// lr_end over `iters` iterations.
+__global__ void train_schedule_kernel(int* state, float* sched, int iters, float lr, float lr_end, float b1,
+                                      float b2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int it = state[0], t = state[1] + 1;
+  state[0] = it + 1;
+  state[1] = t;
+  state[2] = it;
+  const float frac = (float)it / (float)(iters > 1 ? iters - 1 : 1);
+  sched[0] = lr_end + 0.5f * (lr - lr_end) * (1.f + cosf(3.14159265358979f * frac));
+  sched[1] = 1.f - powf(b1, (float)t);
+  sched[2] = 1.f - powf(b2, (float)t);
+}
+
+__device__ __forceinline__ unsigned mix32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352du;
+  x ^= x >> 15; x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Minibatch indices of the iteration in state[2]: sample b is (step, env) =
+// (idx[2b], idx[2b+1]), step uniform in [lo, hi), env in [0, nenv), from a
+// counter-based hash (multiply-shift to the range).
+__global__ void sample_kernel(int B, int lo, int hi, int nenv, unsigned seed, const int* state, int* idx) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  unsigned h = mix32(seed ^ 0x2545f491u);
+  h = mix32(h + (unsigned)state[2] * 0x9E3779B9u);
+  h = mix32(h ^ ((unsigned)b * 0x85EBCA6Bu));
+  const unsigned hs = mix32(h + 0xC2B2AE35u), he = mix32(h + 0x27D4EB2Fu);
+  idx[2 * b] = lo + (int)(((unsigned long long)hs * (unsigned)(hi - lo)) >> 32);
+  idx[2 * b + 1] = (int)(((unsigned long long)he * (unsigned)nenv) >> 32);
+}
+
+// Column sums without atomics: pass 1 writes one partial per 64-row block,
+// part[blk][j]; pass 2 adds the partials of column j in block order, so the
+// result does not depend on scheduling.
+__global__ __launch_bounds__(256) void colsum_part_kernel(const float* dZ, int B, int ld, float* part) {
+  __shared__ float ps[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  const int r0 = blockIdx.y * 64, r1 = min(B, r0 + 64);
+  float s = 0.f;
+  if (j < ld) {
+#pragma unroll 4
+    for (int i = r0 + w; i < r1; i += 4) s += dZ[(size_t)i * ld + j];
+  }
+  ps[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && j < ld) part[(size_t)blockIdx.y * ld + j] = (ps[0][lane] + ps[1][lane]) + (ps[2][lane] + ps[3][lane]);
+}
+
+__global__ void colsum_final_kernel(const float* part, int nblk, int ld, float* db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ld) return;
+  float s = 0.f;
+  for (int k = 0; k < nblk; ++k) s += part[(size_t)k * ld + j];
+  db[j] = s;
+}
+
+// Adam over several tensors in one launch (the update of adam_kernel), with
+// the learning rate and bias corrections read from sched.
+constexpr int ADAM_MAX = 16;
+struct AdamSet {
+  int ntensor;
+  int offset[ADAM_MAX + 1];      // element offset of each tensor in the flat index space
+  float* p[ADAM_MAX];
+  float* m[ADAM_MAX];
+  float* v[ADAM_MAX];
+  const float* g[ADAM_MAX];
+  bf16* p16[ADAM_MAX];
+  const uint8_t* mask[ADAM_MAX];
+  float wd[ADAM_MAX];
+};
+
+__global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float* sched, float b1, float b2,
+                                                         float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.offset[s.ntensor]) return;
+  int k = 0;
+  while (k + 1 < s.ntensor && i >= s.offset[k + 1]) ++k;
+  const int j = i - s.offset[k];
+  bf16* p16 = s.p16[k];
+  if (s.mask[k] && !s.mask[k][j]) {
+    if (p16) p16[j] = (bf16)0.f;
+    return;
+  }
+  const float lr = sched[0], bc1 = sched[1], bc2 = sched[2];
+  float* p = s.p[k];
+  const float gi = s.g[k][j] + s.wd[k] * p[j];
+  const float mi = b1 * s.m[k][j] + (1.f - b1) * gi;
+  const float vi = b2 * s.v[k][j] + (1.f - b2) * gi * gi;
+  s.m[k][j] = mi;
+  s.v[k][j] = vi;
+  const float upd = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  const float pi = p[j] - upd;
+  p[j] = pi;
+  if (p16) p16[j] = (bf16)pi;
+}
+
 __global__ void fill_kernel(float* x, float v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
@@ -375,6 +483,61 @@ int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream) {
   hipLaunchKernelGGL(colsum_kernel, dim3((ld + 63) / 64, (B + rpb - 1) / rpb), dim3(256), 0, (hipStream_t)stream,
                      dZ, B, ld, rpb, db);
   return g1err::launch("g1_mlp_colsum");
+}
+
+int g1_mlp_colsum_det(const float* dZ, int B, int ld, float* part, float* db, void* stream) {
+  g1err::clear();
+  if (B <= 0 || ld <= 0 || !dZ || !part || !db) return g1err::fail("g1_mlp_colsum_det: invalid arguments");
+  const int nblk = (B + 63) / 64;
+  hipLaunchKernelGGL(colsum_part_kernel, dim3((ld + 63) / 64, nblk), dim3(256), 0, (hipStream_t)stream, dZ, B, ld,
+                     part);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((ld + 255) / 256), dim3(256), 0, (hipStream_t)stream, part, nblk, ld,
+                     db);
+  return g1err::launch("g1_mlp_colsum_det");
+}
+
+int g1_mlp_train_schedule(int* state, float* sched, int iters, float lr, float lr_end, float b1, float b2,
+                          void* stream) {
+  g1err::clear();
+  if (!state || !sched || iters <= 0) return g1err::fail("g1_mlp_train_schedule: invalid arguments");
+  hipLaunchKernelGGL(train_schedule_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, state, sched, iters, lr,
+                     lr_end, b1, b2);
+  return g1err::launch("g1_mlp_train_schedule");
+}
+
+int g1_mlp_sample(int B, int lo, int hi, int nenv, unsigned seed, const int* state, int* idx, void* stream) {
+  g1err::clear();
+  if (B <= 0 || lo < 0 || hi <= lo || nenv <= 0 || !state || !idx)
+    return g1err::fail("g1_mlp_sample: invalid arguments");
+  hipLaunchKernelGGL(sample_kernel, dim3((B + 255) / 256), dim3(256), 0, (hipStream_t)stream, B, lo, hi, nenv, seed,
+                     state, idx);
+  return g1err::launch("g1_mlp_sample");
+}
+
+int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
+                      void* const* p16, const uint8_t* const* mask, const int* n, const float* wd,
+                      const float* sched, float b1, float b2, float eps, void* stream) {
+  g1err::clear();
+  if (ntensor <= 0 || ntensor > ADAM_MAX || !p || !m || !v || !grad || !n || !sched)
+    return g1err::fail("g1_mlp_adam_multi: invalid arguments (1 to %d tensors)", ADAM_MAX);
+  AdamSet s{};
+  s.ntensor = ntensor;
+  long long total = 0;
+  for (int k = 0; k < ntensor; ++k) {
+    if (!p[k] || !m[k] || !v[k] || !grad[k] || n[k] <= 0)
+      return g1err::fail("g1_mlp_adam_multi: tensor %d is missing or empty", k);
+    s.offset[k] = (int)total;
+    total += n[k];
+    s.p[k] = p[k]; s.m[k] = m[k]; s.v[k] = v[k]; s.g[k] = grad[k];
+    s.p16[k] = p16 ? (bf16*)p16[k] : nullptr;
+    s.mask[k] = mask ? mask[k] : nullptr;
+    s.wd[k] = wd ? wd[k] : 0.f;
+  }
+  if (total > 0x7fffffffLL) return g1err::fail("g1_mlp_adam_multi: more than 2^31 elements");
+  s.offset[ntensor] = (int)total;
+  hipLaunchKernelGGL(adam_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, s,
+                     sched, b1, b2, eps);
+  return g1err::launch("g1_mlp_adam_multi");
 }
 
 int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask, int n, float lr,

@@ -12,6 +12,7 @@ a multiple of 128.
 Training data come from the ABA step kernel itself (g1_hip_step_ex with the
 qacc and tau outputs) on randomized-actuation rollouts.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -64,8 +65,13 @@ class MLP:
             self.mb.append(torch.zeros_like(self.b[-1])); self.vb.append(torch.zeros_like(self.b[-1]))
             self.maskW.append(torch.as_tensor(mask, device=self.dev))
             self.maskb.append(torch.as_tensor(mb, device=self.dev))
-        self.t = 0
+        self.t = 0                # optimizer steps taken (host mirror of train_state[1])
         self._ws = {}
+        # device-resident loop scalars: state int [3] = {next iteration of the
+        # run, Adam step, this iteration}; sched float [3] = {lr, 1 - b1^t, 1 - b2^t}
+        self.train_state = torch.zeros(4, dtype=torch.int32, device=self.dev)
+        self.sched = torch.zeros(4, **f32)
+        self._adam_args = None
         # z-score statistics (set by fit_normalization)
         self.xmean = torch.zeros(NIN, **f32); self.xinv = torch.ones(NIN, **f32)
         self.ymean = torch.zeros(NOUT, **f32); self.yinv = torch.ones(NOUT, **f32)
@@ -144,8 +150,6 @@ class MLP:
         """Gradients of the loss into ws['gW'], ws['gb'] given dL/dpred (bf16)."""
         ws = self.workspace(B)
         L = len(self.dims)
-        for g in ws["gb"]:
-            self._check(self.lib.g1_mlp_fill(g.data_ptr(), 0.0, g.numel(), self.stream), "fill")
         split = self._split(B)
         d16, dfp = dY16, None
         for l in range(L - 1, -1, -1):
@@ -157,8 +161,9 @@ class MLP:
             self._check(self.lib.g1_mlp_sum_splits(ws["splits"].data_ptr(), split, rows * cols,
                                                    ws["gW"][l].data_ptr(), self.stream), "sum_splits")
             dsrc = ws["dY32"] if l == L - 1 else dfp
-            self._check(self.lib.g1_mlp_colsum(dsrc.data_ptr(), B, rows, ws["gb"][l].data_ptr(), self.stream),
-                        "colsum")
+            # bias gradient: column sums (the split buffer is free again after sum_splits)
+            self._check(self.lib.g1_mlp_colsum_det(dsrc.data_ptr(), B, rows, ws["splits"].data_ptr(),
+                                                   ws["gb"][l].data_ptr(), self.stream), "colsum")
             if l > 0:
                 prow = self.dims[l - 1][0]
                 # dH_{l-1} = d W_l, through SiLU'(Z_{l-1})
@@ -166,18 +171,21 @@ class MLP:
                           C=ws["dZ"][l - 1], H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1])
                 d16, dfp = ws["dZ16"][l - 1], ws["dZ"][l - 1]
 
-    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, B=None):
+    def adam(self, B, b1=0.9, b2=0.999, eps=1e-8):
+        """One Adam step on every weight and bias (one launch), with the
+        learning rate and bias corrections of self.sched (g1_mlp_train_schedule)."""
         ws = self.workspace(B)
-        self.t += 1
-        for l in range(len(self.dims)):
-            self._check(self.lib.g1_mlp_adam(self.W[l].data_ptr(), self.mW[l].data_ptr(), self.vW[l].data_ptr(),
-                                             ws["gW"][l].data_ptr(), self.W16[l].data_ptr(),
-                                             self.maskW[l].data_ptr(), self.W[l].numel(), lr, b1, b2, eps,
-                                             self.t, wd, self.stream), "adam W")
-            self._check(self.lib.g1_mlp_adam(self.b[l].data_ptr(), self.mb[l].data_ptr(), self.vb[l].data_ptr(),
-                                             ws["gb"][l].data_ptr(), None, self.maskb[l].data_ptr(),
-                                             self.b[l].numel(), lr, b1, b2, eps, self.t, 0.0, self.stream),
-                        "adam b")
+        if self._adam_args is None or self._adam_args[0] is not ws:
+            L = len(self.dims)
+            ps, ms, vs = self.W + self.b, self.mW + self.mb, self.vW + self.vb
+            gs, p16 = ws["gW"] + ws["gb"], self.W16 + [None] * L
+            masks = self.maskW + self.maskb
+            n = (C.c_int * (2 * L))(*[t.numel() for t in ps])
+            wd = (C.c_float * (2 * L))(*([0.0] * (2 * L)))
+            self._adam_args = (ws, [_lib.ptr_array(x) for x in (ps, ms, vs, gs, p16, masks)], n, wd)
+        _, arrs, n, wd = self._adam_args
+        self._check(self.lib.g1_mlp_adam_multi(len(n), *arrs, n, wd, self.sched.data_ptr(), b1, b2, eps,
+                                               self.stream), "adam")
 
     def loss_grad(self, pred, target, B, want_sse=True):
         """MSE on the 35 meaningful outputs: dY as bf16 (GEMM operand) and fp32
@@ -191,14 +199,22 @@ class MLP:
                                              ws["sse"].data_ptr() if want_sse else None, self.stream), "mse")
         return ws["sse"] if want_sse else None
 
-    def train_step(self, X16, T, B, lr, want_loss=True):
-        """One minibatch: forward, MSE gradient, backward, Adam.  Returns the
-        device SSE (sum of squared z-scored errors), or None without want_loss."""
-        pred = self.forward(X16, B, train=True)
-        sse = self.loss_grad(pred, T, B, want_sse=want_loss)
-        self.backward(X16, B, self.workspace(B)["dY16"])
-        self.adam(lr, B=B)
-        return sse
+    def iteration(self, ds, idx, B, lo, hi, seed, iters, lr, lr_end):
+        """One training iteration as a fixed launch sequence (every scalar that
+        changes between iterations is read on the device): schedule, batch
+        sampling, gather, forward, MSE gradient (and its SSE into
+        workspace['sse']), backward, Adam."""
+        ws = self.workspace(B)
+        s = self.stream
+        self._check(self.lib.g1_mlp_train_schedule(self.train_state.data_ptr(), self.sched.data_ptr(), iters, lr,
+                                                   lr_end, 0.9, 0.999, s), "schedule")
+        self._check(self.lib.g1_mlp_sample(B, lo, hi, ds.nenv, seed & 0xFFFFFFFF, self.train_state.data_ptr(),
+                                           idx.data_ptr(), s), "sample")
+        self.gather(ds.X, ds.Y, idx, B, ds.nenv, ws["X16"], ws["T"])
+        pred = self.forward(ws["X16"], B, train=True)
+        self.loss_grad(pred, ws["T"], B, want_sse=True)
+        self.backward(ws["X16"], B, ws["dY16"])
+        self.adam(B)
 
     # ------------------------------------------------------ fused inference
     @property
@@ -304,28 +320,42 @@ def fit_normalization(mlp, ds, sample=2_000_000, seed=0):
     mlp.set_normalization(xs.mean(0), xs.std(0), ys.mean(0), ys.std(0))
 
 
-def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=None, seed=0, log=None):
-    """Minibatch Adam on the steps [0, steps - holdout); returns per-iteration
-    mean z-scored MSE every 100 iterations."""
+def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=None, seed=0, log=None,
+          graph=True):
+    """Minibatch Adam on the steps [0, steps - holdout), cosine learning-rate
+    decay from lr to lr_end; returns (iteration, mean z-scored MSE) every 100
+    iterations.  With graph=True the first iteration runs eagerly and the
+    rest replay it as one HIP graph (the iteration reads all its changing
+    scalars on the device, so the replays are the same computation, bit for
+    bit, as eager iterations)."""
     dev = ds.X.device
     holdout = holdout_steps if holdout_steps is not None else max(1, ds.steps // 20)
     train_steps = ds.steps - holdout
-    g = torch.Generator(device=dev).manual_seed(seed)
     ws = mlp.workspace(batch)
-    idx = torch.empty((batch, 2), dtype=torch.int32, device=dev)
+    idx = torch.empty(2 * batch, dtype=torch.int32, device=dev)
+    mlp.train_state[0].zero_()
+    run = lambda: mlp.iteration(ds, idx, batch, 0, train_steps, seed, iters, lr, lr_end)  # noqa: E731
     hist = []
-    for it in range(iters):
-        idx[:, 0] = torch.randint(0, train_steps, (batch,), device=dev, generator=g, dtype=torch.int32)
-        idx[:, 1] = torch.randint(0, ds.nenv, (batch,), device=dev, generator=g, dtype=torch.int32)
-        mlp.gather(ds.X, ds.Y, idx, batch, ds.nenv, ws["X16"], ws["T"])
-        a = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * it / max(1, iters - 1)))
-        log_now = it % 100 == 0 or it == iters - 1
-        sse = mlp.train_step(ws["X16"], ws["T"], batch, a, want_loss=log_now)
-        if log_now:
-            v = float(sse.item()) / (batch * NOUT)
+
+    def logged(it):
+        if it % 100 == 0 or it == iters - 1:
+            v = float(ws["sse"].item()) / (batch * NOUT)
             hist.append((it, v))
             if log:
-                log(f"iter {it} lr {a:.2e} mse {v:.4e}")
+                log(f"iter {it} mse {v:.4e}")
+
+    g = None
+    for it in range(iters):
+        if g is None:
+            run()
+            if graph and it == 0 and iters > 1:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    run()
+        else:
+            g.replay()
+        mlp.t += 1
+        logged(it)
     return hist
 
 

@@ -109,6 +109,26 @@ int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, vo
  * (nullable) marks trainable entries; masked-out entries stay zero. */
 int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask,
                 int n, float lr, float b1, float b2, float eps, int t, float wd, void* stream);
+/* db[j] = sum_i dZ[i][j] over fp32 dZ [B][ld], without atomics (the same
+ * bits on every run): part is fp32 scratch [ceil(B / 64)][ld]. */
+int g1_mlp_colsum_det(const float* dZ, int B, int ld, float* part, float* db, void* stream);
+/* Device-resident training loop: every per-iteration scalar is in device
+ * memory, so an iteration is a fixed launch sequence (graph-capturable).
+ * g1_mlp_train_schedule advances state (int [3]: next iteration of the run,
+ * Adam step t, this iteration) and writes sched (float [3]: cosine-decayed
+ * learning rate from lr to lr_end over iters, 1 - b1^t, 1 - b2^t).
+ * g1_mlp_sample draws B (step, env) pairs into idx [2B] (the layout of
+ * g1_mlp_gather) for the iteration in state[2]: step in [lo, hi), env in
+ * [0, nenv), a counter-based hash of (seed, iteration, b).
+ * g1_mlp_adam_multi is g1_mlp_adam over ntensor (<= 16) tensors in one
+ * launch (host arrays of device pointers; p16, mask, wd and their entries
+ * nullable), with lr and the bias corrections read from sched. */
+int g1_mlp_train_schedule(int* state, float* sched, int iters, float lr, float lr_end, float b1, float b2,
+                          void* stream);
+int g1_mlp_sample(int B, int lo, int hi, int nenv, unsigned seed, const int* state, int* idx, void* stream);
+int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
+                      void* const* p16, const uint8_t* const* mask, const int* n, const float* wd,
+                      const float* sched, float b1, float b2, float eps, void* stream);
 /* x[0..n) = v (a fill kernel, usable where memset nodes are not). */
 int g1_mlp_fill(float* x, float v, long long n, void* stream);
 /* y (bf16) = x (fp32), n elements. */

@@ -7,6 +7,8 @@ batch against a float64 CPU reference that applies the same bf16 roundings
 (activations and gradients are rounded to bf16 where the kernels store them
 as GEMM operands), and one Adam step against its closed form.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -226,3 +228,131 @@ def test_gemm_split_slices_sum_to_product(device):
     q = R // split
     p1 = bf16(Aop[:, q:2 * q]) @ bf16(Bop[:, q:2 * q]).T
     assert np.allclose(parts[1].cpu().numpy(), p1, rtol=1e-5, atol=1e-4)
+
+
+def _stream(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def test_colsum_det_is_exact_and_repeatable(device):
+    lib = _lib.load()
+    rng = np.random.default_rng(5)
+    B, ld = 1000, 200
+    dZ = rng.normal(size=(B, ld)).astype(np.float32)
+    tz = torch.as_tensor(dZ, device=device)
+    part = torch.zeros(((B + 63) // 64) * ld, device=device)
+    outs = []
+    for _ in range(2):
+        db = torch.full((ld,), 7.0, device=device)           # overwritten, not accumulated
+        assert lib.g1_mlp_colsum_det(tz.data_ptr(), B, ld, part.data_ptr(), db.data_ptr(), _stream(device)) == 0
+        outs.append(db.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    np.testing.assert_allclose(outs[0], dZ.astype(np.float64).sum(axis=0), rtol=1e-5, atol=1e-4)
+
+
+def test_train_schedule_and_sampler(device):
+    lib = _lib.load()
+    st = torch.zeros(4, dtype=torch.int32, device=device)
+    st[1] = 10                                                # Adam steps taken before this run
+    sched = torch.zeros(4, device=device)
+    B, lo, hi, nenv, iters = 4096, 3, 40, 1000, 7
+    idx = torch.zeros(2 * B, dtype=torch.int32, device=device)
+    draws = []
+    for k in range(iters):
+        assert lib.g1_mlp_train_schedule(st.data_ptr(), sched.data_ptr(), iters, 1e-3, 1e-5, 0.9, 0.999,
+                                         _stream(device)) == 0
+        assert lib.g1_mlp_sample(B, lo, hi, nenv, 99, st.data_ptr(), idx.data_ptr(), _stream(device)) == 0
+        s = st.cpu().numpy()
+        assert list(s[:3]) == [k + 1, 11 + k, k]
+        lr, bc1, bc2 = sched.cpu().numpy()[:3]
+        t = 11 + k
+        np.testing.assert_allclose(lr, 1e-5 + 0.5 * (1e-3 - 1e-5) * (1 + np.cos(np.pi * k / (iters - 1))), rtol=1e-5)
+        # b1, b2 as the float32 arguments they are (1 - b^t cancels: float32(0.999) != 0.999)
+        b1, b2 = (float(np.float32(b)) for b in (0.9, 0.999))
+        np.testing.assert_allclose([bc1, bc2], [1 - b1 ** t, 1 - b2 ** t], rtol=1e-5)
+        d = idx.cpu().numpy().reshape(B, 2)
+        assert d[:, 0].min() >= lo and d[:, 0].max() < hi and d[:, 1].min() >= 0 and d[:, 1].max() < nenv
+        draws.append(d)
+    # uniform over the step range (loose chi-square), different every iteration
+    counts = np.bincount(draws[0][:, 0] - lo, minlength=hi - lo)
+    expect = B / (hi - lo)
+    assert ((counts - expect) ** 2 / expect).sum() < 3 * (hi - lo)
+    assert not np.array_equal(draws[0], draws[1])
+    # the same (seed, iteration) draws the same batch
+    st2 = torch.tensor([0, 0, 0, 0], dtype=torch.int32, device=device)
+    idx2 = torch.zeros_like(idx)
+    assert lib.g1_mlp_sample(B, lo, hi, nenv, 99, st2.data_ptr(), idx2.data_ptr(), _stream(device)) == 0
+    assert np.array_equal(idx2.cpu().numpy().reshape(B, 2), draws[0])
+
+
+def test_adam_multi_matches_single_tensor_adam(device):
+    """The multi-tensor step with sched = (lr, 1 - b1^t, 1 - b2^t) gives the
+    per-tensor kernel's bits, masks and bf16 copies included."""
+    lib = _lib.load()
+    rng = np.random.default_rng(6)
+    sizes = [1000, 37, 4096]
+    mk = lambda a: torch.as_tensor(a, device=device)  # noqa: E731
+    ps = [rng.normal(size=n).astype(np.float32) for n in sizes]
+    gs = [mk(rng.normal(size=n).astype(np.float32)) for n in sizes]
+    masks = [mk((rng.random(n) < 0.8).astype(np.uint8)) for n in sizes]
+    A = [mk(p.copy()) for p in ps]; Am = [torch.zeros(n, device=device) for n in sizes]
+    Av = [torch.zeros(n, device=device) for n in sizes]; A16 = [torch.zeros(n, dtype=torch.bfloat16, device=device)
+                                                              for n in sizes]
+    Bp = [mk(p.copy()) for p in ps]; Bm = [torch.zeros(n, device=device) for n in sizes]
+    Bv = [torch.zeros(n, device=device) for n in sizes]; B16 = [torch.zeros(n, dtype=torch.bfloat16, device=device)
+                                                              for n in sizes]
+    sched = torch.zeros(4, device=device)
+    n_arr = (C.c_int * 3)(*sizes)
+    for step in (1, 2, 3):
+        for k in range(3):
+            assert lib.g1_mlp_adam(A[k].data_ptr(), Am[k].data_ptr(), Av[k].data_ptr(), gs[k].data_ptr(),
+                                   A16[k].data_ptr(), masks[k].data_ptr(), sizes[k], 1e-3, 0.9, 0.999, 1e-8, step,
+                                   0.0, _stream(device)) == 0
+        # the bias corrections g1_mlp_adam computes on the host (to float rounding)
+        bc = torch.tensor([1e-3, 1 - np.float32(0.9) ** step, 1 - np.float32(0.999) ** step, 0],
+                          dtype=torch.float32)
+        sched.copy_(bc)
+        rc = lib.g1_mlp_adam_multi(3, _lib.ptr_array(Bp), _lib.ptr_array(Bm), _lib.ptr_array(Bv), _lib.ptr_array(gs),
+                                   _lib.ptr_array(B16), _lib.ptr_array(masks), n_arr, None, sched.data_ptr(), 0.9,
+                                   0.999, 1e-8, _stream(device))
+        assert rc == 0
+    torch.cuda.synchronize()
+    for k in range(3):
+        np.testing.assert_allclose(Bp[k].cpu().numpy(), A[k].cpu().numpy(), rtol=2e-6, atol=1e-9)
+        assert np.array_equal(Bp[k].cpu().numpy()[masks[k].cpu().numpy() == 0], ps[k][masks[k].cpu().numpy() == 0])
+        assert (B16[k].float().cpu().numpy()[masks[k].cpu().numpy() == 0] == 0).all()
+        assert np.array_equal(B16[k].float().cpu().numpy(), Bp[k].to(torch.bfloat16).float().cpu().numpy()
+                              * (masks[k].cpu().numpy() != 0))
+
+
+def test_graph_replays_equal_eager_iterations(device):
+    """One training iteration captured as a HIP graph and replayed, with eager
+    work and synchronisations between the replays, gives after every replay
+    the same weights, bit for bit, as eager iterations from the same start."""
+    from g1sim_amd.surrogate import fit_normalization, generate_dataset
+    ds = generate_dataset(device, nenv=1024, steps=12, seed=41)
+    B, iters = 512, 6
+    nets = [MLP(device, seed=3) for _ in range(2)]
+    for m in nets:
+        fit_normalization(m, ds)
+    idx = [torch.zeros(2 * B, dtype=torch.int32, device=device) for _ in range(2)]
+    run = lambda m, ix: m.iteration(ds, ix, B, 0, 10, 5, iters, 1e-3, 1e-5)  # noqa: E731
+    eager = []
+    for _ in range(iters):
+        run(nets[0], idx[0])
+        torch.cuda.synchronize()
+        eager.append([w.cpu().numpy().copy() for w in nets[0].W])
+    run(nets[1], idx[1])                            # iteration 0 eagerly, then capture
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run(nets[1], idx[1])
+    assert all(np.array_equal(a, w.cpu().numpy()) for a, w in zip(eager[0], nets[1].W))
+    for k in range(1, iters):
+        g.replay()
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device=device).sum()   # eager work between replays
+        assert np.isfinite(junk.item())
+        got = [w.cpu().numpy() for w in nets[1].W]
+        for l, (a, b) in enumerate(zip(eager[k], got)):
+            assert np.array_equal(a, b), (k, l)
+    assert int(nets[1].train_state[1].item()) == iters
