@@ -30,7 +30,16 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BI = 128, BJ = 128, BR = 32, PADR = 8, LDR = BR + PADR;
+// Problem shapes are multiples of BI x BJ (x BR per r-slice); a launch
+// tiles them with T x T output tiles, T = 128, or 64 when 128-tiles would
+// leave the chip under-filled (a workgroup's few r-steps cannot hide the
+// global latency by themselves: four 64-tile workgroups per CU hide it for
+// each other).
+constexpr int BI = 128, BJ = 128, BR = 64, PADR = 8, LDR = BR + PADR;
+template <int T> struct Tiling {
+  static constexpr int CHUNKS = T * BR / 8 / 256;   // 16-byte chunks of one operand tile per thread
+  static constexpr int F = T / 32;                  // 16 x 16 MFMA tiles per wave and dimension
+};
 
 enum Epi : int {
   EPI_F32 = 0,         // C (fp32) = acc
@@ -55,77 +64,107 @@ struct GemmArgs {
   const float* Zin;      // pre-activation for SILU_BWD, leading dimension ldc
 };
 
-template <bool TRANS>
-__device__ __forceinline__ void stage(bf16 (*s)[LDR], const bf16* g, int ld, int row0, int r0, int tid) {
-  if constexpr (!TRANS) {
-    // rows row0.., r contiguous: 128 x 32 = 512 chunks of 8
+// One operand tile (128 rows x BR) of a step goes global -> registers (load,
+// issued before the step's MFMAs) -> LDS (put, after them), so the global
+// latency hides behind the MFMAs.  The LDS tile is [row][r] either way.
+template <int T, bool TRANS>
+__device__ __forceinline__ void load(bf16x8 (&v)[Tiling<T>::CHUNKS], const bf16* g, int ld, int row0, int r0,
+                                     int tid) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int c = tid + 256 * t;
-      const int row = c >> 2, rc = (c & 3) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(g + (size_t)(row0 + row) * ld + r0 + rc);
-      *reinterpret_cast<bf16x8*>(&s[row][rc]) = v;
-    }
-  } else {
-    // stored [r][row], row contiguous: 32 x 128 = 512 chunks of 8 rows
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int c = tid + 256 * t;
-      const int rr = c >> 4, rowc = (c & 15) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(g + (size_t)(r0 + rr) * ld + row0 + rowc);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) s[rowc + q][rr] = v[q];
+  for (int t = 0; t < Tiling<T>::CHUNKS; ++t) {
+    const int c = tid + 256 * t;
+    if constexpr (!TRANS) {
+      // stored [row][r], r contiguous: BR / 8 chunks per row
+      const int row = c / (BR / 8), rc = (c % (BR / 8)) * 8;
+      v[t] = *reinterpret_cast<const bf16x8*>(g + (size_t)(row0 + row) * ld + r0 + rc);
+    } else {
+      // stored [r][row], row contiguous: T / 8 chunks of 8 rows per r
+      const int rr = c / (T / 8), rowc = (c % (T / 8)) * 8;
+      v[t] = *reinterpret_cast<const bf16x8*>(g + (size_t)(r0 + rr) * ld + row0 + rowc);
     }
   }
 }
 
-template <bool A_T, bool B_T, int EPI>
+template <int T, bool TRANS>
+__device__ __forceinline__ void put(bf16 (*s)[LDR], const bf16x8 (&v)[Tiling<T>::CHUNKS], int tid) {
+#pragma unroll
+  for (int t = 0; t < Tiling<T>::CHUNKS; ++t) {
+    const int c = tid + 256 * t;
+    if constexpr (!TRANS) {
+      const int row = c / (BR / 8), rc = (c % (BR / 8)) * 8;
+      *reinterpret_cast<bf16x8*>(&s[row][rc]) = v[t];
+    } else {
+      const int rr = c / (T / 8), rowc = (c % (T / 8)) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[rowc + q][rr] = v[t][q];
+    }
+  }
+}
+
+template <bool A_T, bool B_T, int EPI, int T>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 sA[2][BI][LDR];
-  __shared__ __attribute__((aligned(16))) bf16 sB[2][BJ][LDR];
+  using TL = Tiling<T>;
+  constexpr int F = TL::F;
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][T][LDR];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][T][LDR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave >> 1, wj = wave & 1;
-  const int i0 = blockIdx.x * BI, j0 = blockIdx.y * BJ;
+  const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
   const int rb = blockIdx.z * g.r_per_split;
   const int nsteps = g.r_per_split / BR;
 
-  f32x4 acc[4][4];
+  f32x4 acc[F][F];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < F; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < F; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  stage<A_T>(sA[0], g.A, g.lda, i0, rb, tid);
-  stage<B_T>(sB[0], g.B, g.ldb, j0, rb, tid);
+  bf16x8 va[TL::CHUNKS], vb[TL::CHUNKS];
+  load<T, A_T>(va, g.A, g.lda, i0, rb, tid);
+  load<T, B_T>(vb, g.B, g.ldb, j0, rb, tid);
+  put<T, A_T>(sA[0], va, tid);
+  put<T, B_T>(sB[0], vb, tid);
   __syncthreads();
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   for (int st = 0; st < nsteps; ++st) {
     const int cur = st & 1;
-    if (st + 1 < nsteps) {
-      stage<A_T>(sA[cur ^ 1], g.A, g.lda, i0, rb + (st + 1) * BR, tid);
-      stage<B_T>(sB[cur ^ 1], g.B, g.ldb, j0, rb + (st + 1) * BR, tid);
+    const bool next = st + 1 < nsteps;
+    if (next) {
+      load<T, A_T>(va, g.A, g.lda, i0, rb + (st + 1) * BR, tid);
+      load<T, B_T>(vb, g.B, g.ldb, j0, rb + (st + 1) * BR, tid);
     }
-    bf16x8 fa[4], fb[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) fa[a] = *reinterpret_cast<const bf16x8*>(&sA[cur][wi * 64 + a * 16 + lr][lk]);
+    for (int kk = 0; kk < BR / 32; ++kk) {
+      bf16x8 fa[F], fb[F];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) fb[b] = *reinterpret_cast<const bf16x8*>(&sB[cur][wj * 64 + b * 16 + lr][lk]);
+      for (int a = 0; a < F; ++a)
+        fa[a] = *reinterpret_cast<const bf16x8*>(&sA[cur][wi * (T / 2) + a * 16 + lr][kk * 32 + lk]);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < F; ++b)
+        fb[b] = *reinterpret_cast<const bf16x8*>(&sB[cur][wj * (T / 2) + b * 16 + lr][kk * 32 + lk]);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+      for (int a = 0; a < F; ++a)
+#pragma unroll
+        for (int b = 0; b < F; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    // the other buffer was last read in step st - 1, before that step's barrier
+    if (next) {
+      put<T, A_T>(sA[cur ^ 1], va, tid);
+      put<T, B_T>(sB[cur ^ 1], vb, tid);
+    }
     __syncthreads();
   }
 
   // D fragment: lane holds rows 4*(lane/16)+q, column lane%16 of each 16x16 tile
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < F; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < F; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int i = i0 + wi * 64 + a * 16 + (lane >> 4) * 4 + q;
-        const int j = j0 + wj * 64 + b * 16 + lr;
+        const int i = i0 + wi * (T / 2) + a * 16 + (lane >> 4) * 4 + q;
+        const int j = j0 + wj * (T / 2) + b * 16 + lr;
         const size_t o = (size_t)i * g.ldc + j;
         const float v = acc[a][b][q];
         if constexpr (EPI == EPI_F32) {
@@ -153,25 +192,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 
 // ---------------------------------------------------------------- training
 // dY = 2 (pred - target) / (B * n_out) on the first n_out columns, 0 on padding;
-// also accumulates the summed squared error into *sse (fp32 atomics).
+// also accumulates the summed squared error into *sse (fp32 atomics).  A
+// grid-stride loop over at most MSE_BLOCKS workgroups keeps the atomics on
+// the one address few (each costs ~13 ns of serialisation there).
+constexpr int MSE_BLOCKS = 256;
 __global__ __launch_bounds__(256) void mse_grad_kernel(const float* pred, const float* target, int B, int ld,
                                                        int n_out, bf16* dY16, float* dY32, float* sse) {
   __shared__ float wsum[4];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = B * ld;
+  const float scale = 2.0f / (float)(B * n_out);
   float e2 = 0.f;
-  if (idx < B * ld) {
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
     const int j = idx % ld;
     float d = 0.f;
     if (j < n_out) {
       const float e = pred[idx] - target[idx];
-      d = 2.0f * e / (float)(B * n_out);
-      e2 = e * e;
+      d = scale * e;
+      e2 += e * e;
     }
     dY16[idx] = (bf16)d;
     if (dY32) dY32[idx] = d;
   }
-  if (!sse) return;                         // uniform: the loss is only wanted now and then
-  // one atomic per workgroup (per-element atomics on one address serialise)
+  if (!sse) return;                         // uniform
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
@@ -282,17 +324,18 @@ __global__ void sample_kernel(int B, int lo, int hi, int nenv, unsigned seed, co
   idx[2 * b + 1] = (int)(((unsigned long long)he * (unsigned)nenv) >> 32);
 }
 
-// Column sums without atomics: pass 1 writes one partial per 64-row block,
-// part[blk][j]; pass 2 adds the partials of column j in block order, so the
-// result does not depend on scheduling.
+// Column sums without atomics: pass 1 writes one partial per block of
+// COLSUM_ROWS rows, part[blk][j]; pass 2 adds the partials of column j in
+// block order, so the result does not depend on scheduling.
+constexpr int COLSUM_ROWS = 256;
 __global__ __launch_bounds__(256) void colsum_part_kernel(const float* dZ, int B, int ld, float* part) {
   __shared__ float ps[4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + lane;
-  const int r0 = blockIdx.y * 64, r1 = min(B, r0 + 64);
+  const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(B, r0 + COLSUM_ROWS);
   float s = 0.f;
   if (j < ld) {
-#pragma unroll 4
+#pragma unroll 8
     for (int i = r0 + w; i < r1; i += 4) s += dZ[(size_t)i * ld + j];
   }
   ps[w][lane] = s;
@@ -304,6 +347,7 @@ __global__ void colsum_final_kernel(const float* part, int nblk, int ld, float* 
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ld) return;
   float s = 0.f;
+#pragma unroll 8
   for (int k = 0; k < nblk; ++k) s += part[(size_t)k * ld + j];
   db[j] = s;
 }
@@ -417,8 +461,12 @@ namespace {
 using namespace g1mlp;
 template <bool AT, bool BT, int E>
 int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
-  dim3 grid(a.I / BI, a.J / BJ, split);
-  hipLaunchKernelGGL((gemm_kernel<AT, BT, E>), grid, dim3(256), 0, s, a);
+  // 128-tiles when they give at least two workgroups per CU of a 256-CU chip
+  if ((a.I / 128) * (a.J / 128) * split >= 512) {
+    hipLaunchKernelGGL((gemm_kernel<AT, BT, E, 128>), dim3(a.I / 128, a.J / 128, split), dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((gemm_kernel<AT, BT, E, 64>), dim3(a.I / 64, a.J / 64, split), dim3(256), 0, s, a);
+  }
   return g1err::launch("g1_mlp_gemm");
 }
 }  // namespace
@@ -461,8 +509,9 @@ int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n
   g1err::clear();
   if (B <= 0 || ld <= 0 || n_out > ld) return g1err::fail("g1_mlp_mse_grad: invalid arguments");
   const int n = B * ld;
-  hipLaunchKernelGGL(mse_grad_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, pred, target, B,
-                     ld, n_out, (bf16*)dY16, dY32, sse);
+  const int blocks = (n + 255) / 256 < MSE_BLOCKS ? (n + 255) / 256 : MSE_BLOCKS;
+  hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, pred, target, B, ld, n_out,
+                     (bf16*)dY16, dY32, sse);
   return g1err::launch("g1_mlp_mse_grad");
 }
 
@@ -488,7 +537,7 @@ int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream) {
 int g1_mlp_colsum_det(const float* dZ, int B, int ld, float* part, float* db, void* stream) {
   g1err::clear();
   if (B <= 0 || ld <= 0 || !dZ || !part || !db) return g1err::fail("g1_mlp_colsum_det: invalid arguments");
-  const int nblk = (B + 63) / 64;
+  const int nblk = (B + COLSUM_ROWS - 1) / COLSUM_ROWS;
   hipLaunchKernelGGL(colsum_part_kernel, dim3((ld + 63) / 64, nblk), dim3(256), 0, (hipStream_t)stream, dZ, B, ld,
                      part);
   hipLaunchKernelGGL(colsum_final_kernel, dim3((ld + 255) / 256), dim3(256), 0, (hipStream_t)stream, part, nblk, ld,

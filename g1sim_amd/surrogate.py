@@ -142,7 +142,7 @@ class MLP:
     @staticmethod
     def _split(B):
         split = max(1, min(16, B // 512))
-        while B % (32 * split):
+        while B % (64 * split):
             split //= 2
         return split
 

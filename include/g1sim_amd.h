@@ -84,7 +84,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
 
 /* C[I][J] = sum_r A(i, r) B(j, r) with bf16 A, B and fp32 accumulation on
  * gfx950 MFMA.  A(i, r) = a_trans ? A[r*lda + i] : A[i*lda + r], likewise B.
- * I and J must be multiples of 128 and R a multiple of 32*split; split > 1
+ * I and J must be multiples of 128 and R a multiple of 64*split; split > 1
  * (gridDim.z partial sums) only with G1_MLP_EPI_F32_ATOMIC or
  * G1_MLP_EPI_F32_SPLIT (C then holds split slices of I x ldc).  C and H use
  * the leading dimension ldc.  Supported (a_trans, b_trans, epi): (0,0,
@@ -110,7 +110,7 @@ int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, vo
 int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask,
                 int n, float lr, float b1, float b2, float eps, int t, float wd, void* stream);
 /* db[j] = sum_i dZ[i][j] over fp32 dZ [B][ld], without atomics (the same
- * bits on every run): part is fp32 scratch [ceil(B / 64)][ld]. */
+ * bits on every run): part is fp32 scratch [ceil(B / 256)][ld]. */
 int g1_mlp_colsum_det(const float* dZ, int B, int ld, float* part, float* db, void* stream);
 /* Device-resident training loop: every per-iteration scalar is in device
  * memory, so an iteration is a fixed launch sequence (graph-capturable).

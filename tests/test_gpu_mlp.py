@@ -240,7 +240,7 @@ def test_colsum_det_is_exact_and_repeatable(device):
     B, ld = 1000, 200
     dZ = rng.normal(size=(B, ld)).astype(np.float32)
     tz = torch.as_tensor(dZ, device=device)
-    part = torch.zeros(((B + 63) // 64) * ld, device=device)
+    part = torch.zeros(((B + 255) // 256) * ld, device=device)
     outs = []
     for _ in range(2):
         db = torch.full((ld,), 7.0, device=device)           # overwritten, not accumulated
@@ -356,3 +356,27 @@ def test_graph_replays_equal_eager_iterations(device):
         for l, (a, b) in enumerate(zip(eager[k], got)):
             assert np.array_equal(a, b), (k, l)
     assert int(nets[1].train_state[1].item()) == iters
+
+
+@pytest.mark.parametrize("a_t,b_t,epi", [(0, 0, "BIAS_SILU"), (0, 1, "SILU_BWD"), (1, 1, "F32")])
+def test_gemm_128_tiles(device, a_t, b_t, epi):
+    """A grid large enough for the 128 x 128 tiles (the small shapes above
+    run the 64 x 64 tiles)."""
+    rng = np.random.default_rng(7 + EPI[epi])
+    I, J, R = 8192, 1024, 64                    # (I / 128) (J / 128) = 512 workgroups
+    Aop = rng.normal(size=(I, R))
+    Bop = rng.normal(size=(J, R))
+    A = Aop.T.copy() if a_t else Aop
+    B = Bop.T.copy() if b_t else Bop
+    bias = rng.normal(size=J)
+    Z = rng.normal(size=(I, J))
+    C, H = run_gemm(device, a_t, b_t, epi, A, B, I, J, R, bias=bias, Z=Z)
+    acc = bf16(Aop) @ bf16(Bop).T
+    tol = 1e-5 * np.abs(bf16(Aop)) @ np.abs(bf16(Bop)).T + 1e-6
+    if epi == "F32":
+        assert (np.abs(C - acc) <= tol).all()
+    elif epi == "BIAS_SILU":
+        assert (np.abs(C - (acc + bias.astype(np.float32))) <= tol + 1e-6).all()
+    else:
+        d = acc * dsilu(Z.astype(np.float32))
+        assert (np.abs(C - d) <= tol * 1.2 + 1e-6).all()
