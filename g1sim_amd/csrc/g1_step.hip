@@ -597,14 +597,17 @@ struct StepArgs {
   const int* env_list;
   const int* list_count;
   int stride;
-  // block-select judge mode (sel_period > 0): workgroup b steps the envs of
-  // block b that judged_env() or sel_fallback marks, as the list would hold.
-  // In the judge + forces kernel (step_kernel<..., JF = true>) a block with
-  // any such env is stepped whole (state read-only; qacc, tau, anchors,
-  // contact forces and feet stored) and any other block gets the forces only.
+  // block-select judge (step_kernel<..., MODE_JUDGE_BLOCKS>): workgroup b
+  // steps the envs of block b that judged_env() or sel_fallback marks, as the
+  // list would hold.  Judge + forces (MODE_JUDGE_FORCES): a block with any
+  // such env is stepped whole (state read-only; qacc, tau, anchors, contact
+  // forces and feet stored) and any other block gets the forces only.
   const uint8_t* sel_fallback;
   int sel_phase, sel_period;
 };
+// Kernel variants (separate template instances, so the plain step's code is
+// not touched by the judge modes).
+constexpr int MODE_STEP = 0, MODE_JUDGE_BLOCKS = 1, MODE_JUDGE_FORCES = 2;
 static_assert(ENVS_PER_WG == JUDGE_BLOCK, "judge blocks are the step's workgroups");
 
 // Position of the k-th set bit (k from 0, k < popcount(m)) of m.
@@ -721,25 +724,25 @@ G1K_DEV void dispatch_post(int chain, const float (&a0)[6], const float (&a15)[6
 // each role keeps two chains in registers); ROLE < 0 dispatches the chain
 // code on the wave index at run time and keeps a single copy of the shared
 // pelvis/waist code (the 4-wave split).
-template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE, bool JF>
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE, int MODE>
 G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PER_WG],
                        float (&fcx)[2][2][6][ENVS_PER_WG]) {
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
+  constexpr bool JF = MODE == MODE_JUDGE_FORCES;
   const int lane = threadIdx.x & 63;
   const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool select = !JF && args.sel_period > 0;   // judge by blocks, no list
-  const bool judge = select || (!JF && args.env_list != nullptr);
+  const bool judge = MODE == MODE_JUDGE_BLOCKS || (MODE == MODE_STEP && args.env_list != nullptr);
   bool valid;
   int e;
   size_t N;
-  if (JF) {
+  if constexpr (JF) {
     // the whole block, in env order (the kernel has checked it holds a judged
     // or fallback env)
     const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
     valid = e_raw < args.nenv;
     e = valid ? e_raw : args.nenv - 1;
     N = (size_t)args.nenv;
-  } else if (select) {
+  } else if constexpr (MODE == MODE_JUDGE_BLOCKS) {
     // this workgroup's block of 64 envs: all of them on the block's judge
     // step, else its fallback envs, packed into the low lanes in env order.
     // Every wave reads the same 64 flags, so the mask and the early exit are
@@ -1242,11 +1245,11 @@ G1K_DEV ForcesArgs forces_args(const StepArgs& s) {
   return a;
 }
 
-template <bool F64_BASE, bool F64_CONTACT, int SPLIT, bool JF>
+template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int MODE>
 __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
   __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
   __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
-  if constexpr (JF) {
+  if constexpr (MODE == MODE_JUDGE_FORCES) {
     const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
     const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
     const bool in = e_raw < args.nenv && (blk || (args.sel_fallback && args.sel_fallback[e_raw]));
@@ -1256,12 +1259,12 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
     }
   }
   if constexpr (SPLIT == 4) {
-    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, JF>(args, xch, fcx);
+    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx);
   } else {
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, JF>(args, xch, fcx);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, MODE>(args, xch, fcx);
     else
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, JF>(args, xch, fcx);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, MODE>(args, xch, fcx);
   }
 }
 
@@ -1413,19 +1416,25 @@ using KernelFn = void (*)(g1k::StepArgs);
 constexpr int kSplit2MinEnvs = 32768;
 
 template <int SPLIT>
-KernelFn pick(bool f64b, bool f64c, bool jf) {
-  if (jf) return f64c ? g1k::step_kernel<false, true, SPLIT, true> : g1k::step_kernel<false, false, SPLIT, true>;
-  return f64b ? (f64c ? g1k::step_kernel<true, true, SPLIT, false> : g1k::step_kernel<true, false, SPLIT, false>)
-              : (f64c ? g1k::step_kernel<false, true, SPLIT, false> : g1k::step_kernel<false, false, SPLIT, false>);
+KernelFn pick(bool f64b, bool f64c, int mode) {
+  using namespace g1k;
+  if (mode == MODE_JUDGE_FORCES)
+    return f64c ? step_kernel<false, true, SPLIT, MODE_JUDGE_FORCES>
+                : step_kernel<false, false, SPLIT, MODE_JUDGE_FORCES>;
+  if (mode == MODE_JUDGE_BLOCKS)
+    return f64c ? step_kernel<false, true, SPLIT, MODE_JUDGE_BLOCKS>
+                : step_kernel<false, false, SPLIT, MODE_JUDGE_BLOCKS>;
+  return f64b ? (f64c ? step_kernel<true, true, SPLIT, MODE_STEP> : step_kernel<true, false, SPLIT, MODE_STEP>)
+              : (f64c ? step_kernel<false, true, SPLIT, MODE_STEP> : step_kernel<false, false, SPLIT, MODE_STEP>);
 }
 
-int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream, bool jf = false) {
+int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream, int mode = g1k::MODE_STEP) {
   const bool f64b = flags & G1_FLAG_F64_BASE, f64c = flags & G1_FLAG_F64_CONTACT;
   a.f64_solve = (flags & G1_FLAG_F64_SOLVE) ? 1 : 0;
   int split = a.nenv >= kSplit2MinEnvs ? 2 : 4;
   if (flags & G1_FLAG_WAVES4) split = 4;
   if (flags & G1_FLAG_WAVES2) split = 2;
-  KernelFn k = split == 2 ? pick<2>(f64b, f64c, jf) : pick<4>(f64b, f64c, jf);
+  KernelFn k = split == 2 ? pick<2>(f64b, f64c, mode) : pick<4>(f64b, f64c, mode);
   const int nblocks = (a.nenv + g1k::ENVS_PER_WG - 1) / g1k::ENVS_PER_WG;
   hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * split), 0, stream, a);
   hipError_t err = hipGetLastError();
@@ -1567,7 +1576,7 @@ int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float*
   a.qacc = qacc_out; a.mu = mu;
   a.stride = nenv;
   a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
-  return launch_step(a, flags, (hipStream_t)stream);
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_BLOCKS);
 }
 
 int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor, const float* ctrl,
@@ -1595,7 +1604,7 @@ int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel
   a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
   a.stride = nenv;
   a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
-  return launch_step(a, flags, (hipStream_t)stream, true);
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
 }
 
 int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,

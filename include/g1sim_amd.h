@@ -184,9 +184,9 @@ int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float*
  * envs holding a judged or fallback env is stepped whole by ABA (qacc_out,
  * tau_out, the contact update and the feet stored for all its envs; qpos and
  * qvel are not written), every other block gets g1_hip_forces' outputs only.
- * The values are those of the two calls up to float rounding (the stepped
- * blocks' tau_out is the step's own, bit for bit).  contact_forces, foot_pos,
- * foot_vel, mu and fallback are nullable. */
+ * The values are those of the two calls up to float rounding (a separate
+ * kernel instance may contract multiply-adds differently).  contact_forces,
+ * foot_pos, foot_vel, mu and fallback are nullable. */
 int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel, float* anchor,
                         const float* ctrl, const float* mu, const uint8_t* fallback, int phase, int period,
                         float* qacc_out, float* tau_out, float* contact_forces, float* foot_pos,
