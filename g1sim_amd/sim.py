@@ -34,14 +34,33 @@ def _stream_ptr(device):
     return torch.cuda.current_stream(device).cuda_stream
 
 
+def norm_device(device):
+    """torch.device with its index ('cuda' -> 'cuda:<current>'), so that it
+    compares equal to the device of the tensors allocated on it."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
+def check_soa(name, t, rows, nenv, device):
+    """A state tensor as the kernels index it: float32 [rows][nenv],
+    contiguous, on the simulator's device (a reassigned attribute of another
+    shape would otherwise be read out of bounds on the GPU)."""
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or tuple(t.shape) != (rows, nenv) \
+            or t.device != device or not t.is_contiguous():
+        got = (tuple(t.shape), t.dtype, str(t.device), t.is_contiguous()) if isinstance(t, torch.Tensor) else type(t)
+        raise ValueError(f"{name} must be a contiguous float32 ({rows}, {nenv}) tensor on {device}; got {got}")
+
+
 class G1Sim:
     def __init__(self, nenv: int, device: torch.device, *, dt: float = REFERENCE_DT,
                  mu=None, track_qacc: bool = False):
         self.lib = _lib.load()
         self.nenv = nenv
-        self.device = torch.device(device)
-        if self.device.type != "cuda":
+        if torch.device(device).type != "cuda":
             raise ValueError("G1Sim runs on a HIP device (torch 'cuda' device on ROCm)")
+        self.device = norm_device(device)
         self.NQ, self.NV, self.NU, self.NC = NQ, NV, NU, NC
         f32 = dict(dtype=torch.float32, device=self.device)
         # SoA layout: (dim, nenv)  (src/g1sim.py:47-53)
@@ -77,6 +96,8 @@ class G1Sim:
         assert done.dtype == torch.uint8
         assert done.device == self.device
         assert done.is_contiguous()
+        for name, rows in (("qpos", NQ), ("qvel", NV), ("anchor", 2 * NC)):
+            check_soa(name, getattr(self, name), rows, self.nenv, self.device)
         self.lib.g1_cuda_reset_done(self.nenv, self.qpos.data_ptr(), self.qvel.data_ptr(),
                                     self.anchor.data_ptr(), done.data_ptr(), ctypes.c_uint(seed & 0xFFFFFFFF),
                                     noise, drop, _stream_ptr(self.device))
@@ -86,10 +107,24 @@ class G1Sim:
         self.done.fill_(1)
         self.reset_done(self.done, seed, noise, drop)
 
+    def _check_state(self):
+        n, d = self.nenv, self.device
+        for name, rows in (("qpos", NQ), ("qvel", NV), ("anchor", 2 * NC), ("ctrl", NU), ("foot_pos", 6),
+                           ("foot_vel", 6), ("contact_forces", 8), ("commands", 3), ("last_actions", NU)):
+            check_soa(name, getattr(self, name), rows, n, d)
+        if self.qacc is not None:
+            check_soa("qacc", self.qacc, NV, n, d)
+        if self.mu is not None:
+            mu = self.mu
+            if not isinstance(mu, torch.Tensor) or mu.dtype != torch.float32 or tuple(mu.shape) != (n,) \
+                    or mu.device != d or not mu.is_contiguous():
+                raise ValueError(f"mu must be a contiguous float32 ({n},) tensor on {d}")
+
     def step(self, nsub: int = 1):
         """Steps all environments forward by nsub substeps (src/g1sim.py:110-130)."""
         assert self.ctrl.is_contiguous()
         assert self.qpos.is_contiguous()
+        self._check_state()
         self.last_actions.copy_(self.ctrl)
         stream = _stream_ptr(self.device)
         if self.dt == REFERENCE_DT and self.mu is None and self.qacc is None:
@@ -110,6 +145,7 @@ class G1Sim:
         angular velocity (3), joint position error (29), joint velocities (29),
         previous actions (29).  Returns (nenv, 99), built by one HIP kernel
         (g1_hip_get_obs) on the current stream."""
+        self._check_state()
         obs = torch.empty((self.nenv, 99), dtype=torch.float32, device=self.device)
         rc = self.lib.g1_hip_get_obs(self.nenv, 0, self.qpos.data_ptr(), self.qvel.data_ptr(),
                                      self.commands.data_ptr(), self.last_actions.data_ptr(), obs.data_ptr(),

@@ -20,6 +20,7 @@ import torch
 
 from . import _lib
 from .model import NQ, NV
+from .sim import check_soa, norm_device
 
 NIN, NOUT, HID, NHID = NQ + NV + NV, NV, 512, 4
 PIN, POUT = 128, 128
@@ -392,7 +393,7 @@ class SurrogateStepper:
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
         self.fused = fused and mlp.fusable
         self.lib = _lib.load()
-        self.dev = torch.device(device)
+        self.dev = norm_device(device)
         self.rows = (nenv + 127) // 128 * 128
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.tau = torch.zeros((NV, nenv), **f32)
@@ -410,6 +411,10 @@ class SurrogateStepper:
 
     def step(self, qpos, qvel, anchor, ctrl, mu=None):
         lib, s, n = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
+        for name, t, rows in (("qpos", qpos, NQ), ("qvel", qvel, NV), ("anchor", anchor, 16), ("ctrl", ctrl, NV - 6)):
+            check_soa(name, t, rows, n, self.dev)
+        if mu is not None:
+            check_soa("mu", mu.view(1, -1), 1, n, self.dev)
         phase = self.t % self.period
         ck = _lib.check
         # ABA qacc of the judged and fallback envs, and everyone's tau_total,

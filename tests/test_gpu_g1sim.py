@@ -109,3 +109,27 @@ def test_g1sim_additive_options(device):
     env.step(nsub=5)
     torch.cuda.synchronize()
     assert torch.isfinite(env.qacc).all() and env.qacc.abs().sum() > 0
+
+
+def test_g1sim_rejects_reassigned_state_before_launch(device):
+    """A state attribute replaced by a tensor of another shape is refused on
+    the host (the kernels would otherwise index it out of bounds)."""
+    sim = G1Sim(128, device)
+    sim.reset_all()
+    sim.ctrl = torch.zeros((28, 128), device=device)
+    with pytest.raises(ValueError):
+        sim.step()
+    sim.ctrl = torch.zeros((29, 128), device=device, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        sim.get_obs()
+    sim.ctrl = torch.zeros((29, 128), device=device)
+    sim.step()
+    torch.cuda.synchronize()
+
+
+def test_g1sim_on_an_unindexed_device(device):
+    """G1Sim(n, 'cuda') compares its tensors against the indexed device."""
+    sim = G1Sim(64, "cuda")
+    sim.reset_all()
+    sim.step()
+    assert sim.get_obs().shape == (64, 99)
