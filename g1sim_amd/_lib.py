@@ -55,8 +55,8 @@ def load():
     lib.g1_mlp_sample.restype = i
     lib.g1_mlp_sample.argtypes = [i, i, i, i, u, vp, vp, vp]
     lib.g1_mlp_adam_multi.restype = i
-    lib.g1_mlp_adam_multi.argtypes = [i, pa, pa, pa, pa, pa, pa, C.POINTER(C.c_int), C.POINTER(C.c_float), vp, f, f,
-                                      f, vp]
+    lib.g1_mlp_adam_multi.argtypes = [i, pa, pa, pa, pa, pa, pa, C.POINTER(C.c_int), pa, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_float), vp, f, f, f, vp]
     lib.g1_mlp_adam.restype = i
     lib.g1_mlp_adam.argtypes = [vp, vp, vp, vp, vp, vp, i, f, f, f, f, i, f, vp]
     lib.g1_mlp_fill.restype = i

@@ -363,6 +363,8 @@ struct AdamSet {
   float* v[ADAM_MAX];
   const float* g[ADAM_MAX];
   bf16* p16[ADAM_MAX];
+  bf16* p16t[ADAM_MAX];          // transposed bf16 copy [cols][rows] of a [rows][cols] tensor, or null
+  int cols[ADAM_MAX];
   const uint8_t* mask[ADAM_MAX];
   float wd[ADAM_MAX];
 };
@@ -375,8 +377,11 @@ __global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float*
   while (k + 1 < s.ntensor && i >= s.offset[k + 1]) ++k;
   const int j = i - s.offset[k];
   bf16* p16 = s.p16[k];
+  bf16* p16t = s.p16t[k];
+  const int tj = p16t ? (j % s.cols[k]) * ((s.offset[k + 1] - s.offset[k]) / s.cols[k]) + j / s.cols[k] : 0;
   if (s.mask[k] && !s.mask[k][j]) {
     if (p16) p16[j] = (bf16)0.f;
+    if (p16t) p16t[tj] = (bf16)0.f;
     return;
   }
   const float lr = sched[0], bc1 = sched[1], bc2 = sched[2];
@@ -390,6 +395,7 @@ __global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float*
   const float pi = p[j] - upd;
   p[j] = pi;
   if (p16) p16[j] = (bf16)pi;
+  if (p16t) p16t[tj] = (bf16)pi;
 }
 
 __global__ void fill_kernel(float* x, float v, size_t n) {
@@ -495,6 +501,7 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
     case 0 * 8 + EPI_BIAS_SILU_H: return launch_gemm<false, false, EPI_BIAS_SILU_H>(g, split, s);
     case 0 * 8 + EPI_BIAS_F32: return launch_gemm<false, false, EPI_BIAS_F32>(g, split, s);
     case 0 * 8 + EPI_F32: return launch_gemm<false, false, EPI_F32>(g, split, s);
+    case 0 * 8 + EPI_SILU_BWD: return launch_gemm<false, false, EPI_SILU_BWD>(g, split, s);
     case 2 * 8 + EPI_SILU_BWD: return launch_gemm<false, true, EPI_SILU_BWD>(g, split, s);
     case 2 * 8 + EPI_F32: return launch_gemm<false, true, EPI_F32>(g, split, s);
     case 3 * 8 + EPI_F32_ATOMIC: return launch_gemm<true, true, EPI_F32_ATOMIC>(g, split, s);
@@ -564,8 +571,9 @@ int g1_mlp_sample(int B, int lo, int hi, int nenv, unsigned seed, const int* sta
 }
 
 int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
-                      void* const* p16, const uint8_t* const* mask, const int* n, const float* wd,
-                      const float* sched, float b1, float b2, float eps, void* stream) {
+                      void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
+                      const int* n, const float* wd, const float* sched, float b1, float b2, float eps,
+                      void* stream) {
   g1err::clear();
   if (ntensor <= 0 || ntensor > ADAM_MAX || !p || !m || !v || !grad || !n || !sched)
     return g1err::fail("g1_mlp_adam_multi: invalid arguments (1 to %d tensors)", ADAM_MAX);
@@ -579,6 +587,10 @@ int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* cons
     total += n[k];
     s.p[k] = p[k]; s.m[k] = m[k]; s.v[k] = v[k]; s.g[k] = grad[k];
     s.p16[k] = p16 ? (bf16*)p16[k] : nullptr;
+    s.p16t[k] = p16t ? (bf16*)p16t[k] : nullptr;
+    s.cols[k] = cols ? cols[k] : 1;
+    if (s.p16t[k] && (s.cols[k] <= 0 || n[k] % s.cols[k]))
+      return g1err::fail("g1_mlp_adam_multi: tensor %d: %d elements are not rows x %d columns", k, n[k], s.cols[k]);
     s.mask[k] = mask ? mask[k] : nullptr;
     s.wd[k] = wd ? wd[k] : 0.f;
   }

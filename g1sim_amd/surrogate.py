@@ -50,6 +50,7 @@ class MLP:
         rng = np.random.default_rng(seed)
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.W, self.b, self.W16, self.mW, self.vW, self.mb, self.vb, self.maskW, self.maskb = ([] for _ in range(9))
+        self.W16T = []
         for li, (rows, cols, fan_in) in enumerate(self.dims):
             out_true = NOUT if li == len(self.dims) - 1 else rows
             bound = math.sqrt(6.0 / fan_in) if li < len(self.dims) - 1 else math.sqrt(1.0 / fan_in)
@@ -62,6 +63,7 @@ class MLP:
             self.W.append(torch.as_tensor(w, device=self.dev))
             self.b.append(torch.zeros(rows, **f32))
             self.W16.append(self.W[-1].to(torch.bfloat16))
+            self.W16T.append(self.W16[-1].t().contiguous())      # [in][out]: the data-gradient GEMM's operand
             self.mW.append(torch.zeros_like(self.W[-1])); self.vW.append(torch.zeros_like(self.W[-1]))
             self.mb.append(torch.zeros_like(self.b[-1])); self.vb.append(torch.zeros_like(self.b[-1]))
             self.maskW.append(torch.as_tensor(mask, device=self.dev))
@@ -168,7 +170,7 @@ class MLP:
             if l > 0:
                 prow = self.dims[l - 1][0]
                 # dH_{l-1} = d W_l, through SiLU'(Z_{l-1})
-                self.gemm(0, 1, "SILU_BWD", d16, rows, self.W16[l], cols, B, cols, rows,
+                self.gemm(0, 0, "SILU_BWD", d16, rows, self.W16T[l], rows, B, cols, rows,
                           C=ws["dZ"][l - 1], H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1])
                 d16, dfp = ws["dZ16"][l - 1], ws["dZ"][l - 1]
 
@@ -179,14 +181,16 @@ class MLP:
         if self._adam_args is None or self._adam_args[0] is not ws:
             L = len(self.dims)
             ps, ms, vs = self.W + self.b, self.mW + self.mb, self.vW + self.vb
-            gs, p16 = ws["gW"] + ws["gb"], self.W16 + [None] * L
+            gs, p16, p16t = ws["gW"] + ws["gb"], self.W16 + [None] * L, self.W16T + [None] * L
             masks = self.maskW + self.maskb
             n = (C.c_int * (2 * L))(*[t.numel() for t in ps])
+            cols = (C.c_int * (2 * L))(*([w.shape[1] for w in self.W] + [1] * L))
             wd = (C.c_float * (2 * L))(*([0.0] * (2 * L)))
-            self._adam_args = (ws, [_lib.ptr_array(x) for x in (ps, ms, vs, gs, p16, masks)], n, wd)
-        _, arrs, n, wd = self._adam_args
-        self._check(self.lib.g1_mlp_adam_multi(len(n), *arrs, n, wd, self.sched.data_ptr(), b1, b2, eps,
-                                               self.stream), "adam")
+            self._adam_args = (ws, [_lib.ptr_array(x) for x in (ps, ms, vs, gs, p16)], _lib.ptr_array(p16t), cols,
+                               _lib.ptr_array(masks), n, wd)
+        _, arrs, p16t, cols, masks, n, wd = self._adam_args
+        self._check(self.lib.g1_mlp_adam_multi(len(n), *arrs, p16t, cols, masks, n, wd, self.sched.data_ptr(), b1,
+                                               b2, eps, self.stream), "adam")
 
     def loss_grad(self, pred, target, B, want_sse=True):
         """MSE on the 35 meaningful outputs: dY as bf16 (GEMM operand) and fp32

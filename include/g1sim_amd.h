@@ -88,7 +88,7 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
  * (gridDim.z partial sums) only with G1_MLP_EPI_F32_ATOMIC or
  * G1_MLP_EPI_F32_SPLIT (C then holds split slices of I x ldc).  C and H use
  * the leading dimension ldc.  Supported (a_trans, b_trans, epi): (0,0,
- * BIAS_SILU | BIAS_SILU_H | BIAS_F32 | F32), (0,1, SILU_BWD | F32), (1,1,
+ * BIAS_SILU | BIAS_SILU_H | BIAS_F32 | F32 | SILU_BWD), (0,1, SILU_BWD | F32), (1,1,
  * F32_ATOMIC | F32_SPLIT | F32).  Returns 0, -1 (bad arguments) or -2 (launch
  * failure). */
 int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
@@ -121,14 +121,17 @@ int g1_mlp_colsum_det(const float* dZ, int B, int ld, float* part, float* db, vo
  * g1_mlp_gather) for the iteration in state[2]: step in [lo, hi), env in
  * [0, nenv), a counter-based hash of (seed, iteration, b).
  * g1_mlp_adam_multi is g1_mlp_adam over ntensor (<= 16) tensors in one
- * launch (host arrays of device pointers; p16, mask, wd and their entries
- * nullable), with lr and the bias corrections read from sched. */
+ * launch (host arrays of device pointers; p16, p16t, cols, mask, wd and their
+ * entries nullable), with lr and the bias corrections read from sched; p16t[k]
+ * (with cols[k]) also receives the transposed bf16 copy, [cols][rows], of a
+ * [rows][cols] tensor. */
 int g1_mlp_train_schedule(int* state, float* sched, int iters, float lr, float lr_end, float b1, float b2,
                           void* stream);
 int g1_mlp_sample(int B, int lo, int hi, int nenv, unsigned seed, const int* state, int* idx, void* stream);
 int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
-                      void* const* p16, const uint8_t* const* mask, const int* n, const float* wd,
-                      const float* sched, float b1, float b2, float eps, void* stream);
+                      void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
+                      const int* n, const float* wd, const float* sched, float b1, float b2, float eps,
+                      void* stream);
 /* x[0..n) = v (a fill kernel, usable where memset nodes are not). */
 int g1_mlp_fill(float* x, float v, long long n, void* stream);
 /* y (bf16) = x (fp32), n elements. */

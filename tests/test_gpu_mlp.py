@@ -53,7 +53,7 @@ def run_gemm(device, a_t, b_t, epi, A, B, I, J, R, split=1, bias=None, Z=None, C
 
 @pytest.mark.parametrize("a_t,b_t,epi,split", [
     (0, 0, "F32", 1), (0, 0, "BIAS_F32", 1), (0, 0, "BIAS_SILU", 1), (0, 0, "BIAS_SILU_H", 1),
-    (0, 1, "F32", 1), (0, 1, "SILU_BWD", 1), (1, 1, "F32", 1), (1, 1, "F32_ATOMIC", 4)])
+    (0, 0, "SILU_BWD", 1), (0, 1, "F32", 1), (0, 1, "SILU_BWD", 1), (1, 1, "F32", 1), (1, 1, "F32_ATOMIC", 4)])
 def test_gemm_variants(device, a_t, b_t, epi, split):
     rng = np.random.default_rng(100 * a_t + 10 * b_t + EPI[epi])
     I, J, R = 256, 384, 256
@@ -303,6 +303,10 @@ def test_adam_multi_matches_single_tensor_adam(device):
                                                               for n in sizes]
     sched = torch.zeros(4, device=device)
     n_arr = (C.c_int * 3)(*sizes)
+    # transposed copies: the 4096 tensor as 64 x 64, the 1000 one as 8 x 125; none for the 37
+    cols = (C.c_int * 3)(125, 1, 64)
+    B16T = [torch.zeros(n, dtype=torch.bfloat16, device=device) for n in sizes]
+    B16T[1] = None
     for step in (1, 2, 3):
         for k in range(3):
             assert lib.g1_mlp_adam(A[k].data_ptr(), Am[k].data_ptr(), Av[k].data_ptr(), gs[k].data_ptr(),
@@ -313,8 +317,8 @@ def test_adam_multi_matches_single_tensor_adam(device):
                           dtype=torch.float32)
         sched.copy_(bc)
         rc = lib.g1_mlp_adam_multi(3, _lib.ptr_array(Bp), _lib.ptr_array(Bm), _lib.ptr_array(Bv), _lib.ptr_array(gs),
-                                   _lib.ptr_array(B16), _lib.ptr_array(masks), n_arr, None, sched.data_ptr(), 0.9,
-                                   0.999, 1e-8, _stream(device))
+                                   _lib.ptr_array(B16), _lib.ptr_array(B16T), cols, _lib.ptr_array(masks), n_arr,
+                                   None, sched.data_ptr(), 0.9, 0.999, 1e-8, _stream(device))
         assert rc == 0
     torch.cuda.synchronize()
     for k in range(3):
@@ -323,6 +327,10 @@ def test_adam_multi_matches_single_tensor_adam(device):
         assert (B16[k].float().cpu().numpy()[masks[k].cpu().numpy() == 0] == 0).all()
         assert np.array_equal(B16[k].float().cpu().numpy(), Bp[k].to(torch.bfloat16).float().cpu().numpy()
                               * (masks[k].cpu().numpy() != 0))
+        if B16T[k] is not None:
+            c = cols[k]
+            assert np.array_equal(B16T[k].float().cpu().numpy().reshape(c, -1),
+                                  B16[k].float().cpu().numpy().reshape(-1, c).T)
 
 
 def test_graph_replays_equal_eager_iterations(device):
