@@ -38,6 +38,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int BI = 128, BJ = 128, BR = 64, PADR = 8, LDR = BR + PADR;
 template <int T> struct Tiling {
   static constexpr int CHUNKS = T * BR / 8 / 256;   // 16-byte chunks of one operand tile per thread
+  static_assert(CHUNKS % 2 == 0, "transposed staging pairs the chunks");
   static constexpr int F = T / 32;                  // 16 x 16 MFMA tiles per wave and dimension
 };
 
@@ -78,8 +79,11 @@ __device__ __forceinline__ void load(bf16x8 (&v)[Tiling<T>::CHUNKS], const bf16*
       const int row = c / (BR / 8), rc = (c % (BR / 8)) * 8;
       v[t] = *reinterpret_cast<const bf16x8*>(g + (size_t)(row0 + row) * ld + r0 + rc);
     } else {
-      // stored [r][row], row contiguous: T / 8 chunks of 8 rows per r
-      const int rr = c / (T / 8), rowc = (c % (T / 8)) * 8;
+      // stored [r][row], row contiguous: T / 8 chunks of 8 rows per r; a
+      // thread takes the same chunk at two consecutive r (t even, t + 1) so
+      // that put() can write (r, r + 1) pairs as 4-byte words
+      const int cp = tid + 256 * (t / 2);
+      const int rr = 2 * (cp / (T / 8)) + (t & 1), rowc = (cp % (T / 8)) * 8;
       v[t] = *reinterpret_cast<const bf16x8*>(g + (size_t)(r0 + rr) * ld + row0 + rowc);
     }
   }
@@ -93,10 +97,17 @@ __device__ __forceinline__ void put(bf16 (*s)[LDR], const bf16x8 (&v)[Tiling<T>:
     if constexpr (!TRANS) {
       const int row = c / (BR / 8), rc = (c % (BR / 8)) * 8;
       *reinterpret_cast<bf16x8*>(&s[row][rc]) = v[t];
-    } else {
-      const int rr = c / (T / 8), rowc = (c % (T / 8)) * 8;
+    } else if ((t & 1) == 0) {
+      const int cp = tid + 256 * (t / 2);
+      const int rr = 2 * (cp / (T / 8)), rowc = (cp % (T / 8)) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) s[rowc + q][rr] = v[t][q];
+      for (int q = 0; q < 8; ++q) {
+        const bf16 pair[2] = {v[t][q], v[t + 1][q]};
+        // one 4-byte LDS store (aligned: rr is even); memcpy, not a store
+        // through an unsigned*, which the compiler may assume does not alias
+        // the bf16 fragment reads of the tile (that form gave wrong tiles)
+        __builtin_memcpy(&s[rowc + q][rr], pair, 4);
+      }
     }
   }
 }
