@@ -88,7 +88,8 @@ def load():
     lib.g1_policy_act.restype = i
     lib.g1_policy_act.argtypes = [i, i, vp, vp, vp, vp, vp, vp, f, pa, pa, vp, u, i, vp, vp, vp, vp, vp]
     lib.g1_rollout_post.restype = i
-    lib.g1_rollout_post.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, u, i, vp, vp, vp, vp]
+    lib.g1_rollout_post.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, u, i, vp, vp, vp, vp, vp,
+                                    vp]
     lib.g1_policy_obs_norm_update.restype = i
     lib.g1_policy_obs_norm_update.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_rollout_init.restype = i

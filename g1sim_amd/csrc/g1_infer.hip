@@ -621,15 +621,40 @@ struct PostArgs {
   float* reward;
   uint8_t* done;
   uint8_t* reset;
+  double* reward_sum;    // += sum of the rewards (nullable)
+  double* done_count;    // += number of terminated envs (nullable)
 };
 
 // Reward and termination of examples/train.py:143-225 on the post-step state,
 // then the per-step bookkeeping of examples/train.py:319-351: previous
 // actions, command timer, episode length, timeouts, the reset mask and, for
 // reset envs, new commands / timers / push interval.
+__device__ void post_step(const PostArgs& a, int e, float& rew, bool& died);
+
 __global__ __launch_bounds__(256) void post_step_kernel(PostArgs a) {
+  __shared__ float part_r[4];
+  __shared__ int part_d[4];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.nenv) return;
+  float rew = 0.f;
+  bool died = false;
+  if (e < a.nenv) post_step(a, e, rew, died);
+  if (!a.reward_sum && !a.done_count) return;              // uniform
+  // the rollout's per-step sums: one double atomic per workgroup and counter
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) rew += __shfl_xor(rew, off, 64);
+  const int nd = __popcll(__ballot(died));
+  if ((threadIdx.x & 63) == 0) {
+    part_r[threadIdx.x >> 6] = rew;
+    part_d[threadIdx.x >> 6] = nd;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (a.reward_sum) atomicAdd(a.reward_sum, (double)((part_r[0] + part_r[1]) + (part_r[2] + part_r[3])));
+    if (a.done_count) atomicAdd(a.done_count, (double)(part_d[0] + part_d[1] + part_d[2] + part_d[3]));
+  }
+}
+
+__device__ void post_step(const PostArgs& a, int e, float& rew, bool& died) {
   const size_t N = (size_t)a.nenv;
   const float c0 = a.commands[e], c1 = a.commands[N + e], c2 = a.commands[2 * N + e];
   const float qw = a.qpos[3 * N + e], qx = a.qpos[4 * N + e], qy = a.qpos[5 * N + e], qz = a.qpos[6 * N + e];
@@ -681,12 +706,14 @@ __global__ __launch_bounds__(256) void post_step_kernel(PostArgs a) {
   const float rf = a.contact_forces[4 * N + e] + a.contact_forces[5 * N + e] + a.contact_forces[6 * N + e] +
                    a.contact_forces[7 * N + e];
   const float slip = (lv2 * (lf > 1.f ? 1.f : 0.f) + rv2 * (rf > 1.f ? 1.f : 0.f)) * active;
-  a.reward[e] = 2.0f * rew_lin + 2.0f * rew_ang + rew_orient + rew_pose - dof - 2.0f * clearance - 0.5f * slip -
-                0.05f * rate - 0.05f * body_ang;
+  rew = 2.0f * rew_lin + 2.0f * rew_ang + rew_orient + rew_pose - dof - 2.0f * clearance - 0.5f * slip -
+        0.05f * rate - 0.05f * body_ang;
+  a.reward[e] = rew;
   // termination: tilt beyond 70 degrees (the reference's 3.14159)
   const float tilt = fabsf(acosf(fminf(fmaxf(-g[2], -1.f), 1.f)));
   const bool done = tilt > (float)(70.0 * 3.14159 / 180.0);
   a.done[e] = done ? 1 : 0;
+  died = done;
   // bookkeeping
   int ct = a.command_timeout[e] - 1;
   int el = a.episode_length[e] + 1;
@@ -815,7 +842,7 @@ int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float*
                     const float* contact_forces, const float* actions, float* commands, float* prev_actions,
                     int* command_timeout, int* episode_length, int* push_interval, int max_episode_length,
                     float cmd_lin, float cmd_yaw, unsigned seed, int t, float* reward, uint8_t* done,
-                    uint8_t* reset, void* stream) {
+                    uint8_t* reset, double* reward_sum, double* done_count, void* stream) {
   g1err::clear();
   if (nenv < 0 || max_episode_length <= 0 || !qpos || !qvel || !foot_pos || !foot_vel || !contact_forces ||
       !actions || !commands || !prev_actions || !command_timeout || !episode_length || !reward || !done || !reset)
@@ -823,7 +850,7 @@ int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float*
   if (nenv == 0) return 0;
   PostArgs a{nenv, qpos, qvel, foot_pos, foot_vel, contact_forces, actions, commands, prev_actions,
              command_timeout, episode_length, push_interval, max_episode_length, cmd_lin, cmd_yaw, seed, t,
-             reward, done, reset};
+             reward, done, reset, reward_sum, done_count};
   hipLaunchKernelGGL(post_step_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
   return g1err::launch("g1_rollout_post");
 }

@@ -27,6 +27,7 @@ import torch
 
 from . import _lib
 from .model import NC, NQ, NU, NV
+from .sim import norm_device
 
 OBS_DIM = {0: 99, 1: 96}
 LAYERS = [(512, 128), (256, 512), (128, 256), (32, 128)]     # padded (out, in) of the fused kernel
@@ -39,7 +40,7 @@ class Policy:
 
     def __init__(self, device, layout=1, seed=0, log_std=0.0, clip=5.0):
         self.lib = _lib.load()
-        self.dev = torch.device(device)
+        self.dev = norm_device(device)
         self.layout = layout
         self.nobs = OBS_DIM[layout]
         rng = np.random.default_rng(seed)
@@ -108,7 +109,7 @@ class PolicyRollout:
                  max_episode_length=1000, cmd_lin=1.0, cmd_yaw=1.0, sample=True, pushes=True, period=16,
                  tol=1e-3, update_norm=True):
         self.lib = _lib.load()
-        self.dev = torch.device(device)
+        self.dev = norm_device(device)
         self.nenv, self.nsub, self.dt, self.seed = nenv, nsub, float(dt), seed
         self.policy = policy or Policy(device, layout=1, seed=seed)
         self.physics = physics
@@ -160,9 +161,11 @@ class PolicyRollout:
     def _seed(self, t):
         return (self.seed * 0x9E3779B1 + (t + 2) * 0x85EBCA77) & 0xFFFFFFFF
 
-    def step(self):
+    def step(self, reward_sum=None, done_count=None):
         """One policy step: act, nsub physics substeps, reward/termination,
-        bookkeeping and resets."""
+        bookkeeping and resets.  reward_sum / done_count: optional float64
+        device tensors (one element) that the post-step kernel increments by
+        this step's summed reward and number of terminations."""
         lib, s, n, t = self.lib, self.stream, self.nenv, self.t
         if self.update_norm:                      # act(obs) updates the normaliser first (examples/train.py:86-88)
             self.policy.update_obs_norm(n, self.qpos, self.qvel, self.commands, self.ctrl)
@@ -184,7 +187,9 @@ class PolicyRollout:
                                  self.command_timeout.data_ptr(), self.episode_length.data_ptr(),
                                  self.push_interval.data_ptr() if self.pushes else None, self.max_episode_length,
                                  self.cmd_lin, self.cmd_yaw, self._seed(t) ^ 0x5bd1e995, t,
-                                 self.reward.data_ptr(), self.done.data_ptr(), self.reset.data_ptr(), s)
+                                 self.reward.data_ptr(), self.done.data_ptr(), self.reset.data_ptr(),
+                                 None if reward_sum is None else reward_sum.data_ptr(),
+                                 None if done_count is None else done_count.data_ptr(), s)
         _lib.check(rc, "g1_rollout_post")
         lib.g1_cuda_reset_done(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
                                self.reset.data_ptr(), self._seed(t), 0.1, 0.02, s)
@@ -192,13 +197,16 @@ class PolicyRollout:
 
     def rollout(self, steps=24, reward_sums=None, done_counts=None):
         """examples/train.py:284-355 without the buffer writes: prev_actions
-        start at zero; per-step reward sums / done counts (device tensors of
-        length >= steps, optional) are accumulated on the stream."""
+        start at zero; per-step reward sums / done counts (contiguous float64
+        device tensors of length >= steps, optional) are accumulated by the
+        post-step kernel."""
         _lib.check(self.lib.g1_mlp_fill(self.prev_actions.data_ptr(), 0.0, self.prev_actions.numel(), self.stream),
                    "fill")
+        for acc in (reward_sums, done_counts):
+            if acc is not None and (acc.dtype != torch.float64 or acc.device != self.dev or not acc.is_contiguous()
+                                    or acc.numel() < steps):
+                raise ValueError(f"reward_sums / done_counts must be contiguous float64 tensors of >= {steps} "
+                                 f"elements on {self.dev}")
         for k in range(steps):
-            self.step()
-            if reward_sums is not None:
-                reward_sums[k] += self.reward.sum()
-            if done_counts is not None:
-                done_counts[k] += self.done.sum()
+            self.step(None if reward_sums is None else reward_sums[k:k + 1],
+                      None if done_counts is None else done_counts[k:k + 1])

@@ -278,12 +278,16 @@ int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const flo
  * episode_length 0 and a push interval in [50, 150) (push_interval
  * nullable); envs whose timer ran out get new commands and timer.  reward
  * fp32 [nenv], done and reset uint8 [nenv]; the caller resets the envs
- * flagged in `reset` (g1_cuda_reset_done). */
+ * flagged in `reset` (g1_cuda_reset_done).  reward_sum / done_count (one
+ * double each, nullable) are incremented by this step's summed reward and
+ * number of terminated envs (device atomics: the summation order, and so the
+ * last bits of reward_sum, vary from run to run). */
 int g1_rollout_post(int nenv, const float* qpos, const float* qvel, const float* foot_pos,
                     const float* foot_vel, const float* contact_forces, const float* actions, float* commands,
                     float* prev_actions, int* command_timeout, int* episode_length, int* push_interval,
                     int max_episode_length, float cmd_lin, float cmd_yaw, unsigned seed, int t,
-                    float* reward, uint8_t* done, uint8_t* reset, void* stream);
+                    float* reward, uint8_t* done, uint8_t* reset, double* reward_sum,
+                    double* done_count, void* stream);
 /* Rollout start: commands for every env, timers 0, push intervals in
  * [50, 150) (push_interval nullable). */
 int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float* commands,

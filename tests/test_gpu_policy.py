@@ -157,12 +157,14 @@ def test_post_step_matches_oracle(device):
     reward = torch.zeros(n, device=device)
     done = torch.zeros(n, dtype=torch.uint8, device=device)
     reset = torch.zeros(n, dtype=torch.uint8, device=device)
+    sums = torch.tensor([5.0, 2.0], dtype=torch.float64, device=device)    # accumulated into
     lib = _lib.load()
     _lib.check(lib.g1_rollout_post(n, t["qp"].data_ptr(), t["qv"].data_ptr(), t["fp"].data_ptr(), t["fv"].data_ptr(),
                                    t["cf"].data_ptr(), t["a"].data_ptr(), t["cmd"].data_ptr(), t["pa"].data_ptr(),
                                    t["ct"].data_ptr(), t["el"].data_ptr(), t["pi"].data_ptr(), 1000, 1.0, 0.5, 99, 7,
-                                   reward.data_ptr(), done.data_ptr(), reset.data_ptr(),
-                                   torch.cuda.current_stream(device).cuda_stream), "rollout_post")
+                                   reward.data_ptr(), done.data_ptr(), reset.data_ptr(), sums.data_ptr(),
+                                   sums[1:].data_ptr(), torch.cuda.current_stream(device).cuda_stream),
+               "rollout_post")
     torch.cuda.synchronize()
     f64 = lambda a: a.astype(np.float64)  # noqa: E731
     obs = R.get_obs(f64(qp), f64(qv), f64(cmd), np.zeros((29, n)), f64(DEFAULT32))
@@ -173,6 +175,9 @@ def test_post_step_matches_oracle(device):
     np.testing.assert_allclose(r, r_ref, rtol=2e-5, atol=2e-4)
     d = done.cpu().numpy().astype(bool)
     assert np.array_equal(d, d_ref) and d[tilt].all()
+    sm = sums.cpu().numpy()
+    np.testing.assert_allclose(sm[0], 5.0 + r.astype(np.float64).sum(), rtol=1e-6, atol=1e-3)
+    assert sm[1] == 2.0 + d.sum()
     rs_ref, ct_ref, el_ref, resample = R.bookkeeping(d_ref, ct, el, 1000)
     rs = reset.cpu().numpy().astype(bool)
     assert np.array_equal(rs, rs_ref)

@@ -100,8 +100,10 @@ def test_fused_inference_entry_points_validate_arguments():
     assert lib.g1_policy_act(8, 1, d, d, d, d, d, d, 5.0, _lib.ptr_array([16, 0, 16, 16]), w4, None, 0, 0, d,
                              None, None, None, None) == -1
     # rollout bookkeeping needs every buffer except push_interval, and a positive episode length
-    assert lib.g1_rollout_post(8, d, d, d, d, d, d, d, d, d, d, None, 0, 1.0, 0.5, 0, 0, d, d, d, None) == -1
-    assert lib.g1_rollout_post(8, d, d, d, d, d, None, d, d, d, d, None, 10, 1.0, 0.5, 0, 0, d, d, d, None) == -1
+    assert lib.g1_rollout_post(8, d, d, d, d, d, d, d, d, d, d, None, 0, 1.0, 0.5, 0, 0, d, d, d, None, None,
+                               None) == -1
+    assert lib.g1_rollout_post(8, d, d, d, d, d, None, d, d, d, d, None, 10, 1.0, 0.5, 0, 0, d, d, d, None, None,
+                               None) == -1
     assert lib.g1_rollout_init(8, 1.0, 0.5, 0, None, d, d, None, None) == -1
 
 
