@@ -523,9 +523,11 @@ constexpr int NORM_CHUNKS = 64;
 __global__ __launch_bounds__(256) void obs_batch_stats_kernel(int nenv, int layout, int chunk, const float* qpos,
                                                               const float* qvel, const float* commands,
                                                               const float* last_ctrl, const float* run_mean,
-                                                              double* part) {
+                                                              const float* count, double* part) {
   __shared__ double s1[256], s2[256];
   const int f = blockIdx.x, ch = blockIdx.y;
+  // the running count as it was before this update, for every merge block
+  if (f == 0 && ch == 0 && threadIdx.x == 0) part[2 * NORM_CHUNKS * gridDim.x] = (double)count[0];
   const double c = run_mean[f];
   const int e0 = ch * chunk, e1 = min(nenv, e0 + chunk);
   double a = 0.0, b = 0.0;
@@ -551,17 +553,22 @@ __global__ __launch_bounds__(256) void obs_batch_stats_kernel(int nenv, int layo
 }
 
 // mean, var: [nobs] running statistics, count: [1] running count (all fp32,
-// updated in place); inv = 1 / sqrt(var + 1e-8) for the policy kernel.
-__global__ void obs_norm_merge_kernel(int nenv, int nobs, int nchunks, const double* part, float* mean, float* var,
-                                      float* count, float* inv) {
-  const int f = threadIdx.x;
-  const double n = (double)nenv, cnt = (double)count[0], tot = cnt + n;
-  if (f < nobs) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int ch = 0; ch < nchunks; ++ch) {
-      s1 += part[(size_t)(f * NORM_CHUNKS + ch) * 2];
-      s2 += part[(size_t)(f * NORM_CHUNKS + ch) * 2 + 1];
-    }
+// updated in place); inv = 1 / sqrt(var + 1e-8) for the policy kernel.  One
+// wave per feature: lane c holds chunk c's sums, a butterfly adds them (a
+// fixed order), lane 0 merges.
+static_assert(NORM_CHUNKS == 64, "one chunk per lane");
+__global__ __launch_bounds__(64) void obs_norm_merge_kernel(int nenv, int nobs, int nchunks, const double* part,
+                                                            float* mean, float* var, float* count, float* inv) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  const double n = (double)nenv, cnt = part[2 * NORM_CHUNKS * nobs], tot = cnt + n;
+  double s1 = lane < nchunks ? part[(size_t)(f * NORM_CHUNKS + lane) * 2] : 0.0;
+  double s2 = lane < nchunks ? part[(size_t)(f * NORM_CHUNKS + lane) * 2 + 1] : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  if (lane == 0) {
     const double m = mean[f], v = var[f];
     const double d = s1 / n;                                   // batch mean - running mean
     const double bvar = fmax(s2 / n - d * d, 0.0);             // biased batch variance
@@ -571,9 +578,8 @@ __global__ void obs_norm_merge_kernel(int nenv, int nobs, int nchunks, const dou
     mean[f] = (float)nm;
     var[f] = (float)nv;
     inv[f] = (float)(1.0 / sqrt((double)(float)nv + 1e-8));
+    if (f == 0) count[0] = (float)tot;
   }
-  __syncthreads();
-  if (f == 0) count[0] = (float)tot;
 }
 
 // ---------------------------------------------------------------- rollout
@@ -877,8 +883,8 @@ int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const flo
   const int chunk = (nenv + NORM_CHUNKS - 1) / NORM_CHUNKS;
   const int nchunks = (nenv + chunk - 1) / chunk;
   hipLaunchKernelGGL(obs_batch_stats_kernel, dim3(nobs, nchunks), dim3(256), 0, (hipStream_t)stream, nenv, layout,
-                     chunk, qpos, qvel, commands, last_ctrl, mean, scratch);
-  hipLaunchKernelGGL(obs_norm_merge_kernel, dim3(1), dim3(128), 0, (hipStream_t)stream, nenv, nobs, nchunks, scratch,
+                     chunk, qpos, qvel, commands, last_ctrl, mean, count, scratch);
+  hipLaunchKernelGGL(obs_norm_merge_kernel, dim3(nobs), dim3(64), 0, (hipStream_t)stream, nenv, nobs, nchunks, scratch,
                      mean, var, count, inv);
   return g1err::launch("g1_policy_obs_norm_update");
 }

@@ -60,7 +60,7 @@ class Policy:
         self.obs_var = torch.ones(128, dtype=torch.float32, device=self.dev)
         self.obs_inv = torch.ones(128, dtype=torch.float32, device=self.dev)
         self.obs_count = torch.full((1,), 1e-4, dtype=torch.float32, device=self.dev)   # examples/train.py:46
-        self._norm_scratch = torch.zeros(2 * 64 * 128, dtype=torch.float64, device=self.dev)
+        self._norm_scratch = torch.zeros(2 * 64 * 128 + 1, dtype=torch.float64, device=self.dev)
         self.clip = float(clip)
         self.log_std = None if log_std is None else torch.full((NU,), float(log_std), dtype=torch.float32,
                                                                 device=self.dev)

@@ -265,7 +265,7 @@ int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const fl
  * of the observation (layout as g1_hip_get_obs) over the nenv envs, merged
  * into mean / var ([nobs] fp32) and count ([1] fp32) in place by Chan's
  * formula; inv ([nobs]) receives 1 / sqrt(var + 1e-8) for g1_policy_act.
- * scratch: double [2 * 64 * nobs] device workspace. */
+ * scratch: double [2 * 64 * nobs + 1] device workspace. */
 int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
                               const float* last_ctrl, float* mean, float* var, float* count, float* inv,
                               double* scratch, void* stream);
