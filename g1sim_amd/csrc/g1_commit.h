@@ -18,38 +18,31 @@ namespace g1k {
 constexpr int JUDGE_BLOCK = 64;
 G1K_DEV bool judged_env(int e, int phase, int period) { return ((e / JUDGE_BLOCK) % period) == phase; }
 
-// `live` lanes are envs < nenv; every lane of the wave must call this (the
-// statistics are reduced across the wave).  sur[35] is the network's qacc in
-// physical units.  The env's inputs come through accessors: aba(j) its ABA
-// qacc j (read only for judged and fallback envs), qp(m) / qv(m) its qpos /
-// qvel row m; the results go to qpos / qvel ([dim][N]).  stats: [0] judged,
-// [1] rejected, [2] max judged rel. error (float bits, non-negative, so
-// integer max orders them), [3] fallback env-steps.
-template <typename ABA, typename QP, typename QV>
-G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period, float tol, float dt,
-                              const float* sur, ABA aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
-                              QP qp, QV qv, float* qpos, float* qvel) {
-  constexpr int NV = 35, NU = 29;
-  const bool judged = live && judged_env(e, phase, period);
-  const bool fb = live && fallback && fallback[e];
-  float qa[NV];
+// Judge part of the commit for one env per lane: the network's qacc sur(j)
+// against ABA's aba(j) over all dofs (read only for judged and fallback
+// envs), the env's new fallback flag, and the wave's statistics.  `live`
+// lanes are envs < nenv; every lane of the wave must call this (the
+// statistics are reduced across the wave).  stats: [0] judged, [1] rejected,
+// [2] max judged rel. error (float bits, non-negative, so integer max orders
+// them), [3] fallback env-steps.
+template <typename SUR, typename ABA>
+G1K_DEV void surrogate_judge(bool live, bool judged, bool fb, int e, float tol, SUR sur, ABA aba,
+                             uint8_t* fallback, unsigned* stats) {
+  constexpr int NV = 35;
   float err = 0.f;
+  if (judged || fb) {
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    float v = live ? sur[j] : 0.f;
-    if (judged || fb) {
+    for (int j = 0; j < NV; ++j) {
       const float ref = aba(j);
-      err = fmaxf(err, fabsf(v - ref) * frcp(fmaxf(fabsf(ref), 1.0f)));
-      v = ref;
+      err = fmaxf(err, fabsf(sur(j) - ref) * frcp(fmaxf(fabsf(ref), 1.0f)));
     }
-    qa[j] = v;
   }
   const bool reject = judged && !(err <= tol);
   if (judged && fallback) fallback[e] = reject ? 1 : 0;
   // wave-level statistics: one atomic per counter per wave
   const unsigned n_judged = (unsigned)__popcll(__ballot(judged));
   const unsigned n_reject = (unsigned)__popcll(__ballot(reject));
-  const unsigned n_fb = (unsigned)__popcll(__ballot(fb && !judged));
+  const unsigned n_fb = (unsigned)__popcll(__ballot(live && fb && !judged));
   unsigned ebits = judged ? __float_as_uint(err) : 0u;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -62,16 +55,19 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
     if (n_judged) atomicMax(&stats[2], ebits);
     if (n_fb) atomicAdd(&stats[3], n_fb);
   }
-  if (!live) return;
-  if (qacc_out) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) qacc_out[j * N + e] = qa[j];
-  }
+}
+
+// Semi-implicit Euler of the floating base (src/g1_core.h:212-222 with the
+// quaternion exponential of src/spatial.h:73-79): qa(j) is qacc j (0-5),
+// qp(m) / qv(m) the env's qpos / qvel row m; rows 0-6 of qpos and 0-5 of
+// qvel ([dim][N]) are written.
+template <typename QA, typename QP, typename QV>
+G1K_DEV void integrate_base(int e, size_t N, float dt, QA qa, QP qp, QV qv, float* qpos, float* qvel) {
   float vw[3], om[3], bp[3], bq[4];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    vw[m] = qv(m) + dt * qa[m];
-    om[m] = qv(3 + m) + dt * qa[3 + m];
+    vw[m] = qv(m) + dt * qa(m);
+    om[m] = qv(3 + m) + dt * qa(3 + m);
     bp[m] = qp(m) + dt * vw[m];
   }
 #pragma unroll
@@ -98,12 +94,42 @@ G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period,
   }
 #pragma unroll
   for (int m = 0; m < 4; ++m) qpos[(3 + m) * N + e] = bq[m];
+}
+
+// Semi-implicit Euler of hinge h (0-28): qvel row 6 + h, qpos row 7 + h.
+G1K_DEV void integrate_hinge(int h, int e, size_t N, float dt, float qa, float qp, float qv, float* qpos,
+                             float* qvel) {
+  const float v = qv + dt * qa;
+  qvel[(6 + h) * N + e] = v;
+  qpos[(7 + h) * N + e] = qp + dt * v;
+}
+
+// The whole commit of one env per lane: qacc = ABA's for judged and fallback
+// envs, else the network's sur[35] (physical units); judge and count; store
+// qacc_out (nullable); integrate.  aba(j) is the env's ABA qacc j (read only
+// for judged and fallback envs), qp(m) / qv(m) its qpos / qvel row m; the
+// results go to qpos / qvel ([dim][N]).  Every lane of the wave must call
+// this.
+template <typename ABA, typename QP, typename QV>
+G1K_DEV void surrogate_commit(bool live, int e, size_t N, int phase, int period, float tol, float dt,
+                              const float* sur, ABA aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                              QP qp, QV qv, float* qpos, float* qvel) {
+  constexpr int NV = 35, NU = 29;
+  const bool judged = live && judged_env(e, phase, period);
+  const bool fb = live && fallback && fallback[e];
+  surrogate_judge(live, judged, fb, e, tol, [&](int j) { return live ? sur[j] : 0.f; }, aba, fallback, stats);
+  if (!live) return;
+  const bool use_aba = judged || fb;
+  float qa[NV];
 #pragma unroll
-  for (int h = 0; h < NU; ++h) {
-    const float v = qv(6 + h) + dt * qa[6 + h];
-    qvel[(6 + h) * N + e] = v;
-    qpos[(7 + h) * N + e] = qp(7 + h) + dt * v;
+  for (int j = 0; j < NV; ++j) qa[j] = use_aba ? aba(j) : sur[j];
+  if (qacc_out) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) qacc_out[j * N + e] = qa[j];
   }
+  integrate_base(e, N, dt, [&](int j) { return qa[j]; }, qp, qv, qpos, qvel);
+#pragma unroll
+  for (int h = 0; h < NU; ++h) integrate_hinge(h, e, N, dt, qa[6 + h], qp(7 + h), qv(6 + h), qpos, qvel);
 }
 
 // The same with every input read from the SoA state in memory.

@@ -6,8 +6,9 @@
 // activations held in LDS (two 64 x 520 bf16 buffers, 133 KB), and finishes
 // with a fused epilogue, so no activation ever goes to HBM.  Weights are
 // read from L2 (1.7 MB for the surrogate, 0.43 MB for the policy), each
-// lane loading 16 contiguous bytes of a weight row as its B fragment, two
-// k-steps ahead of its use.
+// lane loading 16 contiguous bytes of a weight row as its B fragment, several
+// k-steps ahead of its use; a layer's first k-steps are loaded before the
+// previous layer's epilogue, so no layer starts on a cold weight fetch.
 //
 // * Surrogate step: z-scored (q, qd, tau_total) -> 512 x4 SiLU -> 35, then
 //   the per-env commit (judge / fallback / semi-implicit Euler) of
@@ -42,12 +43,6 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// k-steps of weight fragments loaded ahead of their MFMAs (a build-time
-// tuning knob; the ring holds this many + 1 steps in registers)
-#ifndef G1INF_WEIGHT_PREFETCH
-#define G1INF_WEIGHT_PREFETCH 3
-#endif
-
 constexpr int ROWS = 64;                 // envs per workgroup
 constexpr int NWAVE = 8;
 constexpr int NTHR = 64 * NWAVE;
@@ -68,11 +63,44 @@ template <int NOUT> struct Tile {
   __device__ static int c0(int w) { return WIDE ? w * CT : w % NC; }
 };
 
-// acc = in[64 x NIN] (LDS) . W^T for this wave's tiles.  W is k-tiled,
-// [NIN / 32][NOUT][32] bf16: the 32-wide k-slice of every output row is one
-// contiguous block, so the fragments of a k-step (16 rows x 64 B per load
-// instruction) are contiguous 1 KB runs (row-major [NOUT][NIN] puts them
-// NIN * 2 bytes apart and measured at 0.6x the rate on MI355X).
+// Weight fragments of one layer in flight: a ring of NB k-steps of the
+// wave's CT column tiles, 16 fragments (64 VGPRs) whatever CT is, so a
+// layer with fewer column tiles per wave runs further ahead.  The first PF
+// k-steps are loaded by wprefetch() before the layer (the next layer's ring
+// is filled before this layer's epilogue and barrier: weights do not depend
+// on activations); the k-loop loads the rest PF steps ahead.
+template <int NIN, int NOUT> struct Ring {
+  using T = Tile<NOUT>;
+  static constexpr int KS = NIN / 32;
+  static constexpr int NB = 16 / T::CT < KS ? 16 / T::CT : KS;
+  static constexpr int PF = NB - 1;
+  static_assert(NIN % 32 == 0 && KS >= 2 && PF >= 1, "input width must be a multiple of 32, at least 64");
+  bf16x8 f[NB][T::CT];
+};
+
+// W is k-tiled, [NIN / 32][NOUT][32] bf16: the 32-wide k-slice of every
+// output row is one contiguous block, so the fragments of a k-step (16 rows
+// x 64 B per load instruction) are contiguous 1 KB runs (row-major
+// [NOUT][NIN] puts them NIN * 2 bytes apart and measured at 0.6x the rate on
+// MI355X).  Lane l reads row c * 16 + l % 16, k-elements (l / 16) * 8 + 0..7.
+template <int NOUT>
+__device__ __forceinline__ const bf16* wfrag(const bf16* W, int w, int lane, int k, int c) {
+  using T = Tile<NOUT>;
+  return W + ((size_t)k * NOUT + (size_t)(T::c0(w) + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8;
+}
+
+template <int NIN, int NOUT>
+__device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int lane, Ring<NIN, NOUT>& rg) {
+  using R = Ring<NIN, NOUT>;
+#pragma unroll
+  for (int k = 0; k < R::PF; ++k)
+#pragma unroll
+    for (int c = 0; c < Tile<NOUT>::CT; ++c)
+      rg.f[k][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT>(W, w, lane, k, c));
+}
+
+// acc = in[64 x NIN] (LDS) . W^T for this wave's tiles, with the first PF
+// k-steps of weights already in `rg`.
 //
 // TR = false: acc[r][c][q] is out[row (r0 + r) * 16 + (lane / 16) * 4 + q]
 // [col (c0 + c) * 16 + lane % 16]; TR = true (weights as the first MFMA
@@ -81,29 +109,22 @@ template <int NOUT> struct Tile {
 // four adjacent columns of one row per lane.
 template <int NIN, int NOUT, bool TR = false>
 __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf16* __restrict__ W, int w,
-                                          int lane, f32x4 (&acc)[Tile<NOUT>::RT][Tile<NOUT>::CT]) {
+                                          int lane, Ring<NIN, NOUT>& rg,
+                                          f32x4 (&acc)[Tile<NOUT>::RT][Tile<NOUT>::CT]) {
   using T = Tile<NOUT>;
-  constexpr int KS = NIN / 32;
-  static_assert(NIN % 32 == 0, "input width must be a multiple of 32");
+  using R = Ring<NIN, NOUT>;
+  constexpr int KS = R::KS, PF = R::PF, NB = R::NB;
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  const int r0 = T::r0(w), c0 = T::c0(w);
+  const int r0 = T::r0(w);
 #pragma unroll
   for (int r = 0; r < T::RT; ++r)
 #pragma unroll
     for (int c = 0; c < T::CT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16* wp = W + (size_t)(c0 * 16 + lr) * 32 + lk;   // + k * NOUT * 32 + c * 16 * 32
   const bf16* ap = sin + (r0 * 16 + lr) * LDA + lk;
-  // Software pipeline: weight fragments (L2) run PF k-steps ahead in a ring
-  // of PF + 1, activation fragments (LDS) one step ahead; a scheduling
-  // barrier per k-step keeps the compiler from sinking the prefetches next
-  // to their uses.
-  constexpr int PF = KS > G1INF_WEIGHT_PREFETCH ? G1INF_WEIGHT_PREFETCH : (KS >= 3 ? 2 : 1), NB = PF + 1;
-  bf16x8 fb[NB][T::CT], fa[2][T::RT];
-#pragma unroll
-  for (int k = 0; k < PF && k < KS; ++k)
-#pragma unroll
-    for (int c = 0; c < T::CT; ++c)
-      fb[k][c] = *reinterpret_cast<const bf16x8*>(wp + ((size_t)k * NOUT + c * 16) * 32);
+  // Software pipeline: weight fragments (L2) PF k-steps ahead in the ring,
+  // activation fragments (LDS) one step ahead; a scheduling barrier per
+  // k-step keeps the compiler from sinking the prefetches next to their uses.
+  bf16x8 fa[2][T::RT];
 #pragma unroll
   for (int r = 0; r < T::RT; ++r) fa[0][r] = *reinterpret_cast<const bf16x8*>(ap + r * 16 * LDA);
 #pragma unroll
@@ -111,7 +132,7 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
     if (k + PF < KS) {
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        fb[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wp + ((size_t)(k + PF) * NOUT + c * 16) * 32);
+        rg.f[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT>(W, w, lane, k + PF, c));
     }
     if (k + 1 < KS) {
 #pragma unroll
@@ -122,33 +143,44 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        acc[r][c] = TR ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[k % NB][c], fa[k & 1][r], acc[r][c], 0, 0, 0)
-                       : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], fb[k % NB][c], acc[r][c], 0, 0, 0);
+        acc[r][c] = TR ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(rg.f[k % NB][c], fa[k & 1][r], acc[r][c], 0, 0, 0)
+                       : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], rg.f[k % NB][c], acc[r][c], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// hidden layer: out = act(in W^T + b) as bf16 into LDS
-template <int NIN, int NOUT, int ACT>
+// hidden layer: out = act(in W^T + b) as bf16 into LDS; the next layer's
+// first weight k-steps (Wn) go into `next` before the epilogue
+template <int NIN, int NOUT, int ACT, int NIN2, int NOUT2>
 __device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __restrict__ sout,
-                                      const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane) {
+                                      const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane,
+                                      Ring<NIN, NOUT>& rg, const bf16* __restrict__ Wn, Ring<NIN2, NOUT2>& next) {
   using T = Tile<NOUT>;
-  f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT, true>(sin, W, w, lane, acc);
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
   typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  // bias of the lane's four columns per column tile, loaded before the k-loop
+  float bc[T::CT][4];
+#pragma unroll
+  for (int c = 0; c < T::CT; ++c) {
+    const int col = (c0 + c) * 16 + (lane >> 4) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
+  }
+  f32x4 acc[T::RT][T::CT];
+  mma_layer<NIN, NOUT, true>(sin, W, w, lane, rg, acc);
+  wprefetch<NIN2, NOUT2>(Wn, w, lane, next);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     // four adjacent columns of one row per lane: one 8-byte LDS store
     const int col = (c0 + c) * 16 + (lane >> 4) * 4;
-    const float bc[4] = {b[col], b[col + 1], b[col + 2], b[col + 3]};
 #pragma unroll
     for (int r = 0; r < T::RT; ++r) {
       const int row = (r0 + r) * 16 + lr;
       bf16x4 h;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float z = acc[r][c][q] + bc[q];
+        const float z = acc[r][c][q] + bc[c][q];
         if constexpr (ACT == ACT_SILU) h[q] = (bf16)g1act::silu(z);
         else h[q] = (bf16)g1act::elu(z);
       }
@@ -160,20 +192,23 @@ __device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __rest
 // output layer: out = in W^T + b as fp32 into LDS [64][LDO]
 template <int NIN, int NOUT>
 __device__ __forceinline__ void dense_out(const bf16* __restrict__ sin, float* __restrict__ sout,
-                                          const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane) {
+                                          const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane,
+                                          Ring<NIN, NOUT>& rg) {
   using T = Tile<NOUT>;
   static_assert(NOUT <= LDO, "output tile wider than the LDS row");
-  f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT>(sin, W, w, lane, acc);
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
+  float bc[T::CT];
+#pragma unroll
+  for (int c = 0; c < T::CT; ++c) bc[c] = b[(c0 + c) * 16 + lr];
+  f32x4 acc[T::RT][T::CT];
+  mma_layer<NIN, NOUT>(sin, W, w, lane, rg, acc);
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     const int col = (c0 + c) * 16 + lr;
-    const float bc = b[col];
 #pragma unroll
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sout[((r0 + r) * 16 + (lane >> 4) * 4 + q) * LDO + col] = acc[r][c][q] + bc;
+      for (int q = 0; q < 4; ++q) sout[((r0 + r) * 16 + (lane >> 4) * 4 + q) * LDO + col] = acc[r][c][q] + bc[c];
   }
 }
 
@@ -206,24 +241,59 @@ struct SurrogateArgs {
 constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
 static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 
-// LDS stash of the commit's inputs (MODE 1): rows 0-70 qpos and qvel, rows
-// 71-105 ABA's qacc, each [64 envs].
-constexpr int STASH_ABA = 71, STASH_ROWS = STASH_ABA + NV;
+// LDS stash of the commit's inputs (MODE 1), each row [64 envs]: rows 0-70
+// qpos and qvel as the feature stage read them, rows 71-105 ABA's qacc (of
+// judged and fallback envs), row 106 the env's fallback flag on entry.
+constexpr int STASH_ABA = 71, STASH_FB = STASH_ABA + NV, STASH_ROWS = STASH_FB + 1;
 
 template <int MODE>
 __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
   __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
   __shared__ float stash[MODE == 1 ? STASH_ROWS * ROWS : 1];
+  __shared__ float ysc[MODE == 1 ? 2 * NV : 1];          // ystd, ymean
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * ROWS;
   const size_t N = (size_t)a.nenv;
+  const int r = lane, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
+  // The tile's rows are exactly one judge block, so `judged` is uniform.
+  const bool judged_blk = MODE == 1 && g1k::judged_env(e0, a.phase, a.period);
+  // One batch of loads: the env's fallback flag, the network input (qpos 36,
+  // qvel 35, tau 35; wave w owns features w, w + 8, ...) and, in a judged
+  // block, ABA's qacc of every env; then the first layer's weights.  All are
+  // issued before any but the flag is used, so the stage costs about one
+  // memory latency.
+  constexpr int FPW = SUR_PIN / NWAVE;
+  constexpr int APW = (NV + NWAVE - 1) / NWAVE;
+  static_assert(SUR_PIN % NWAVE == 0, "features split evenly over the waves");
+  // the flag is loaded without a branch (from a valid dummy byte when there
+  // are no flags) and tested after the other loads are issued
+  const bool has_fb = MODE == 1 && a.fallback != nullptr;
+  const uint8_t fbyte = *(has_fb ? a.fallback + ec : reinterpret_cast<const uint8_t*>(a.qpos));
+  float raw[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int f = w + NWAVE * i;
+    const float* src = f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N
+                                                                : a.tau + (size_t)(f - 71) * N;
+    raw[i] = f < SUR_NIN ? src[ec] : 0.f;
+  }
+  float abav[APW] = {};
+  if constexpr (MODE == 1) {
+    if (judged_blk) {
+#pragma unroll
+      for (int i = 0; i < APW; ++i) {
+        const int j = w + NWAVE * i;
+        abav[i] = j < NV ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const bool fb = has_fb && e < a.nenv && fbyte != 0;
   if constexpr (MODE == 1) {
     // A tile that is not judged this step and whose live envs are all in
     // fallback takes ABA's qacc everywhere: commit without the network.
-    // (The tile's rows are exactly one judge block, so `judged` is uniform.)
-    if (a.fallback && !g1k::judged_env(e0, a.phase, a.period)) {
-      const int e = e0 + (tid & 63);
-      const int own = tid < 64 && e < a.nenv && !a.fallback[e];
+    if (a.fallback && !judged_blk) {
+      const int own = e < a.nenv && !fb;
       if (!__syncthreads_or(own)) {
         if (w == 0) {
           float sur[NV];
@@ -234,74 +304,108 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
         }
         return;
       }
-    }
-  }
-  {
-    // network input: z-scored (qpos 36, qvel 35, tau 35), zero padding.
-    // Wave w owns features w, w + 8, ...; all its loads are issued before
-    // the first is used (one memory latency, not 16).
-    constexpr int FPW = SUR_PIN / NWAVE;
-    static_assert(SUR_PIN % NWAVE == 0, "features split evenly over the waves");
-    const int r = tid & 63, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
-    float raw[FPW];
+      // fallback envs of a block that runs the network: their ABA qacc
 #pragma unroll
-    for (int i = 0; i < FPW; ++i) {
-      const int f = w + NWAVE * i;
-      const float* src = f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N
-                                                                  : a.tau + (size_t)(f - 71) * N;
-      raw[i] = f < SUR_NIN ? src[ec] : 0.f;
-    }
-    if constexpr (MODE == 1) {
-      // the commit's inputs, read now while the network runs: ABA's qacc of
-      // judged and fallback envs
-      const bool aba = g1k::judged_env(e0, a.phase, a.period) || (a.fallback && a.fallback[ec]);
-#pragma unroll
-      for (int i = 0; i < (NV + NWAVE - 1) / NWAVE; ++i) {
+      for (int i = 0; i < APW; ++i) {
         const int j = w + NWAVE * i;
-        if (j < NV) stash[(STASH_ABA + j) * ROWS + r] = aba ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < FPW; ++i) {
-      const int f = w + NWAVE * i;
-      float x = 0.f;
-      if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
-      buf[0][r * LDA + f] = (bf16)x;
-      if constexpr (MODE == 1) {
-        if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+        abav[i] = fb && j < NV ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
       }
     }
   }
+  // the input scalings, then the first layer's weights: issued after every
+  // load this stage waits for (the wait counter is in order)
+  float xm[FPW], xi[FPW];
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int f = w + NWAVE * i;
+    xm[i] = f < SUR_NIN ? a.xmean[f] : 0.f;
+    xi[i] = f < SUR_NIN ? a.xinv[f] : 0.f;
+  }
+  float ys = 0.f;
+  if constexpr (MODE == 1) {
+    if (tid < 2 * NV) ys = tid < NV ? a.ystd[tid] : a.ymean[tid - NV];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  Ring<SUR_PIN, SUR_H> ring0;
+  wprefetch<SUR_PIN, SUR_H>(a.W[0], w, lane, ring0);
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (MODE == 1) {
+    // stash the commit's inputs
+#pragma unroll
+    for (int i = 0; i < APW; ++i) {
+      const int j = w + NWAVE * i;
+      if (j < NV) stash[(STASH_ABA + j) * ROWS + r] = (judged_blk || fb) ? abav[i] : 0.f;
+    }
+    if (w == 0) stash[STASH_FB * ROWS + r] = fb ? 1.f : 0.f;
+    if (tid < 2 * NV) ysc[tid] = ys;
+  }
+#pragma unroll
+  for (int i = 0; i < FPW; ++i) {
+    const int f = w + NWAVE * i;
+    float x = 0.f;
+    if (e < a.nenv && f < SUR_NIN) x = (raw[i] - xm[i]) * xi[i];
+    buf[0][r * LDA + f] = (bf16)x;
+    if constexpr (MODE == 1) {
+      if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+    }
+  }
   __syncthreads();
-  dense<SUR_PIN, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[0], a.b[0], w, lane);
+  Ring<SUR_H, SUR_H> ring1, ring2, ring3;
+  Ring<SUR_H, SUR_POUT> ring4;
+  dense<SUR_PIN, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[1], a.b[1], w, lane);
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[2], a.b[2], w, lane);
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[3], a.b[3], w, lane);
+  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[3], a.b[3], w, lane, ring3, a.W[4], ring4);
   __syncthreads();
   float* out = reinterpret_cast<float*>(buf[1]);
-  dense_out<SUR_H, SUR_POUT>(buf[0], out, a.W[4], a.b[4], w, lane);
+  dense_out<SUR_H, SUR_POUT>(buf[0], out, a.W[4], a.b[4], w, lane, ring4);
   __syncthreads();
   if constexpr (MODE == 0) {
     // [64][64] fp32 tile to pred rows; 512 threads, 8 elements each
     for (int i = tid; i < ROWS * SUR_POUT; i += NTHR) {
-      const int r = i / SUR_POUT, c = i % SUR_POUT, e = e0 + r;
-      if (e < a.nenv) a.pred[(size_t)e * a.ldp + c] = out[r * LDO + c];
+      const int rr = i / SUR_POUT, c = i % SUR_POUT, er = e0 + rr;
+      if (er < a.nenv) a.pred[(size_t)er * a.ldp + c] = out[rr * LDO + c];
     }
   } else {
+    // The commit (g1_commit.h), spread over the waves: wave 0 judges and
+    // counts (all dofs) and integrates the base, waves 1-7 the hinges.
+    // Every piece evaluates the same expressions as surrogate_commit().
+    const bool live = e < a.nenv;
+    const bool judged = live && judged_blk;
+    const float* st = stash + lane;
+    const bool fbs = live && st[STASH_FB * ROWS] != 0.f;
+    const bool use_aba = judged || fbs;
+    const float* orow = out + lane * LDO;
+    auto sur = [&](int j) { return orow[j] * ysc[j] + ysc[NV + j]; };
+    auto aba = [&](int j) { return st[(STASH_ABA + j) * ROWS]; };
+    auto qa = [&](int j) { return use_aba ? aba(j) : sur(j); };
     if (w == 0) {
-      const int e = e0 + lane;
-      const bool live = e < a.nenv;
-      float sur[NV];
+      g1k::surrogate_judge(live, judged, fbs, e, a.tol, sur, aba, a.fallback, a.stats);
+      if (live) {
+        float q6[6];
 #pragma unroll
-      for (int j = 0; j < NV; ++j) sur[j] = out[lane * LDO + j] * a.ystd[j] + a.ymean[j];
-      const float* st = stash + lane;
-      g1k::surrogate_commit(
-          live, e, N, a.phase, a.period, a.tol, a.dt, sur, [&](int j) { return st[(STASH_ABA + j) * ROWS]; },
-          a.fallback, a.stats, a.qacc_out, [&](int m) { return st[m * ROWS]; },
-          [&](int m) { return st[(36 + m) * ROWS]; }, a.qpos_rw, a.qvel_rw);
+        for (int j = 0; j < 6; ++j) q6[j] = qa(j);
+        if (a.qacc_out) {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) a.qacc_out[j * N + e] = q6[j];
+        }
+        g1k::integrate_base(
+            e, N, a.dt, [&](int j) { return q6[j]; }, [&](int m) { return st[m * ROWS]; },
+            [&](int m) { return st[(36 + m) * ROWS]; }, a.qpos_rw, a.qvel_rw);
+      }
+    } else if (live) {
+#pragma unroll
+      for (int i = 0; i < (NU + NWAVE - 2) / (NWAVE - 1); ++i) {
+        const int h = (w - 1) + (NWAVE - 1) * i;
+        if (h < NU) {
+          const float q = qa(6 + h);
+          if (a.qacc_out) a.qacc_out[(6 + h) * N + e] = q;
+          g1k::integrate_hinge(h, e, N, a.dt, q, st[(7 + h) * ROWS], st[(36 + 6 + h) * ROWS], a.qpos_rw, a.qvel_rw);
+        }
+      }
     }
   }
 }
@@ -425,6 +529,7 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * ROWS;
   const size_t N = (size_t)a.nenv;
+  Ring<POL_PIN, 512> ring0;
   {
     // stage the block's raw inputs in LDS (buf[1] is free until the first
     // layer writes it): every load of a thread is issued before the first
@@ -433,24 +538,37 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
     static_assert(RAW_N * ROWS * sizeof(float) <= sizeof(buf[1]), "raw stage exceeds its buffer");
     const int r = tid & 63, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
     constexpr int RPW = (RAW_N + NWAVE - 1) / NWAVE;   // rows per wave
-    float v[RPW];
+    constexpr int FPW = POL_PIN / NWAVE;               // features per wave
+    float v[RPW], om[FPW], oi[FPW];
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const int row = w + NWAVE * i;
       v[i] = row < RAW_N ? raw_row(row, N, a.qpos, a.qvel, a.commands, a.last_ctrl)[ec] : 0.f;
     }
 #pragma unroll
+    for (int i = 0; i < FPW; ++i) {
+      const int f = w + NWAVE * i;
+      om[i] = f < a.nobs ? a.obs_mean[f] : 0.f;
+      oi[i] = f < a.nobs ? a.obs_inv[f] : 0.f;
+    }
+    // the first layer's weights go last: the wait counter is in order
+    __builtin_amdgcn_sched_barrier(0);
+    wprefetch<POL_PIN, 512>(a.W[0], w, lane, ring0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
     for (int i = 0; i < RPW; ++i) {
       const int row = w + NWAVE * i;
       if (row < RAW_N) raw[row * ROWS + r] = v[i];
     }
     __syncthreads();
-    for (int f = w; f < POL_PIN; f += NWAVE) {
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) {
+      const int f = w + NWAVE * i;
       float x = 0.f;
       if (e < a.nenv && f < a.nobs) {
         const float o = obs_eval(a.layout, f, [&](int row) { return raw[row * ROWS + r]; });
         if (a.obs_out) a.obs_out[(size_t)e * a.nobs + f] = o;
-        x = fminf(fmaxf((o - a.obs_mean[f]) * a.obs_inv[f], -a.clip), a.clip);
+        x = fminf(fmaxf((o - om[i]) * oi[i], -a.clip), a.clip);
       }
       buf[0][r * LDA + f] = (bf16)x;
     }
@@ -474,14 +592,17 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
       a.push_interval[e] = pi;
     }
   }
-  dense<POL_PIN, 512, ACT_ELU>(buf[0], buf[1], a.W[0], a.b[0], w, lane);
+  Ring<512, 256> ring1;
+  Ring<256, 128> ring2;
+  Ring<128, 32> ring3;
+  dense<POL_PIN, 512, ACT_ELU>(buf[0], buf[1], a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
   __syncthreads();
-  dense<512, 256, ACT_ELU>(buf[1], buf[0], a.W[1], a.b[1], w, lane);
+  dense<512, 256, ACT_ELU>(buf[1], buf[0], a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
   __syncthreads();
-  dense<256, 128, ACT_ELU>(buf[0], buf[1], a.W[2], a.b[2], w, lane);
+  dense<256, 128, ACT_ELU>(buf[0], buf[1], a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
   __syncthreads();
   float* out = reinterpret_cast<float*>(buf[0]);
-  dense_out<128, 32>(buf[1], out, a.W[3], a.b[3], w, lane);
+  dense_out<128, 32>(buf[1], out, a.W[3], a.b[3], w, lane, ring3);
   __syncthreads();
   // epilogue: thread (r, group) -> env r, actions j = group, group + 8, ...
   const int r = tid & 63, e = e0 + r;
