@@ -922,15 +922,22 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 #pragma unroll
     for (int w = 0; w < SPLIT; ++w) {
       const bool to_pelvis = w * CPW < 2;
+      // The 27 reads of one contribution are issued back to back and waited
+      // for once. Left to itself the scheduler (at the register limit) pairs
+      // each read with its add, one LDS round trip per pair of floats.
+      float t[XCH];
+#pragma unroll
+      for (int m = 0; m < XCH; ++m) t[m] = xch[buf][w][m][lane];
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int m = 0; m < 21; ++m) {
-        if (to_pelvis) A0[m] += xch[buf][w][m][lane];
-        else A15[m] += xch[buf][w][m][lane];
+        if (to_pelvis) A0[m] += t[m];
+        else A15[m] += t[m];
       }
 #pragma unroll
       for (int m = 0; m < 6; ++m) {
-        if (to_pelvis) pA0[m] += xch[buf][w][21 + m][lane];
-        else p15[m] += xch[buf][w][21 + m][lane];
+        if (to_pelvis) pA0[m] += t[21 + m];
+        else p15[m] += t[21 + m];
       }
     }
     Waist::backward(cf, A15, p15, wq, wqd, wct, wj, A0, pA0);
