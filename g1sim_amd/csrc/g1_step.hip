@@ -623,6 +623,11 @@ G1K_DEV int nth_set_bit(unsigned long long m, int k) {
 
 // Per-lane chain slots (legs use 6 of 7).
 constexpr int MAXJ = 7;
+// The 2-wave split keeps 12-14 joints per wave between the backward and the
+// forward pass; with every joint's U/D (6 floats) in registers it spills
+// ~400 B per lane to scratch.  It parks U/D in LDS instead (43 KB per
+// workgroup; two workgroups per CU fit in 160 KB).
+template <int SPLIT> constexpr bool W_IN_LDS = SPLIT == 2;
 
 template <int W> struct WaveChain;
 template <> struct WaveChain<0> { static constexpr int first = 1, n = 6, root = 0, foot = 6, k0 = 0; };
@@ -726,7 +731,7 @@ G1K_DEV void dispatch_post(int chain, const float (&a0)[6], const float (&a15)[6
 // pelvis/waist code (the 4-wave split).
 template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE, int MODE>
 G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PER_WG],
-                       float (&fcx)[2][2][6][ENVS_PER_WG]) {
+                       float (&fcx)[2][2][6][ENVS_PER_WG], float* wsh) {
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   constexpr bool JF = MODE == MODE_JUDGE_FORCES;
   const int lane = threadIdx.x & 63;
@@ -900,6 +905,20 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 #pragma unroll
       for (int m = 0; m < 6; ++m) { pw[m] += pc[m]; Fw[m] += Fpel[m]; }
     });
+    if constexpr (W_IN_LDS<SPLIT>) {
+      // U/D of every joint parked in this lane's own LDS slots until the
+      // forward pass (no other lane or wave reads them)
+      static_for<CPW>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const int nj = chain_n(wave * CPW + c);
+#pragma unroll
+        for (int m = 0; m < MAXJ; ++m)
+          if (m < nj) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wsh[(((wave * CPW + c) * MAXJ + m) * 6 + k) * ENVS_PER_WG + lane] = cj[c][m].W[k];
+          }
+      });
+    }
 #pragma unroll
     for (int m = 0; m < 21; ++m) xch[buf][wave][m][lane] = Pw[m];
 #pragma unroll
@@ -947,6 +966,18 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     if (f64_solve) base_solve<double>(A0, pA0, a0);
     else base_solve<float>(A0, pA0, a0);
     float a15[6];
+    if constexpr (W_IN_LDS<SPLIT>) {
+      static_for<CPW>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const int nj = chain_n(wave * CPW + c);
+#pragma unroll
+        for (int m = 0; m < MAXJ; ++m)
+          if (m < nj) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cj[c][m].W[k] = wsh[(((wave * CPW + c) * MAXJ + m) * 6 + k) * ENVS_PER_WG + lane];
+          }
+      });
+    }
     Waist::forward(a0, wj, qdd_w, a15);
     static_for<CPW>([&](auto C) {
       constexpr int c = decltype(C)::value;
@@ -1256,6 +1287,7 @@ template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int MODE>
 __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
   __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
   __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
+  __shared__ float wsh[W_IN_LDS<SPLIT> ? 4 * MAXJ * 6 * ENVS_PER_WG : 1];
   if constexpr (MODE == MODE_JUDGE_FORCES) {
     const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
     const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
@@ -1266,12 +1298,12 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
     }
   }
   if constexpr (SPLIT == 4) {
-    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx);
+    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx, wsh);
   } else {
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, MODE>(args, xch, fcx);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, MODE>(args, xch, fcx, wsh);
     else
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, MODE>(args, xch, fcx);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, MODE>(args, xch, fcx, wsh);
   }
 }
 
