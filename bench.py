@@ -479,7 +479,7 @@ def main():
                    "flags": args.flags},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
-                     "kernel": "g1k::step_kernel<%s, %s, %d, false>" % (
+                     "kernel": "g1k::step_kernel<%s, %s, %d, 0>" % (
                          "true" if args.flags & 2 else "false", "true" if args.flags & 4 else "false",
                          4 if args.flags & 8 else 2 if (args.flags & 16 or nenv >= 32768) else 4),
                      "flop_per_launch": flop_per_launch, "avg_launch_ms": avg_ms,
