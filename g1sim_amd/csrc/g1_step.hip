@@ -545,8 +545,9 @@ template <typename T> G1K_DEV void base_solve(const float (&A)[21], const float 
   for (int j = 0; j < 6; ++j) {
 #pragma unroll
     for (int k = 0; k < j; ++k) L[pk(j, j)] -= L[pk(j, k)] * L[pk(j, k)];
-    const T d = fsqrt(L[pk(j, j)]);
-    const T inv = frcp(d);
+    // 1 / sqrt(pivot) in one refined reciprocal square root (the pivots of
+    // the positive definite base inertia are positive); in float64, 1 / sqrt
+    const T inv = frsq(L[pk(j, j)]);
     L[pk(j, j)] = inv;                         // the diagonal is kept inverted
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
