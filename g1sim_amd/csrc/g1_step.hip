@@ -72,6 +72,15 @@ constexpr float ANCHOR_FREE = 1e30f;     // src/g1_core.h:108
 // unit axis).
 template <int K> struct Plane { static constexpr int I = (K + 1) % 3, J = (K + 2) % 3; };
 
+// c * x for a model constant c (a compile-time constant once the templates
+// are expanded).  Without fast-math flags the compiler must keep 0 * x (x
+// could be inf or NaN): the zero entries of the tree rotations, offsets,
+// skew matrices and rigid-body inertias cost one multiply or FMA each.  A
+// zero coefficient gives an exact -0 instead, which every sum absorbs, so no
+// instruction is issued and finite results are unchanged.  A difference
+// a - c x is written a + cm(-c, x) (bit-identical: negation is exact).
+template <typename T_> G1K_DEV T_ cm(T_ c, T_ x) { return c == T_(0) ? T_(-0.0) : c * x; }
+
 template <int K, typename T_ = float> G1K_DEV void rot(T_* v, T_ c, T_ s) {          // v <- R_K v
   constexpr int I = Plane<K>::I, J = Plane<K>::J;
   const T_ a = v[I], b = v[J];
@@ -88,16 +97,16 @@ template <int K, typename T_ = float> G1K_DEV void rotT(T_* v, T_ c, T_ s) {    
 template <int B> G1K_DEV void rt_mul(float* v) {                          // v <- RT v
   constexpr const double* R = kM.b[B].RT;
   const float x = v[0], y = v[1], z = v[2];
-  v[0] = (float)R[0] * x + (float)R[1] * y + (float)R[2] * z;
-  v[1] = (float)R[3] * x + (float)R[4] * y + (float)R[5] * z;
-  v[2] = (float)R[6] * x + (float)R[7] * y + (float)R[8] * z;
+  v[0] = cm((float)R[0], x) + cm((float)R[1], y) + cm((float)R[2], z);
+  v[1] = cm((float)R[3], x) + cm((float)R[4], y) + cm((float)R[5], z);
+  v[2] = cm((float)R[6], x) + cm((float)R[7], y) + cm((float)R[8], z);
 }
 template <int B, typename T_ = float> G1K_DEV void rt_mulT(T_* v) {      // v <- RT^T v
   constexpr const double* R = kM.b[B].RT;
   const T_ x = v[0], y = v[1], z = v[2];
-  v[0] = (T_)R[0] * x + (T_)R[3] * y + (T_)R[6] * z;
-  v[1] = (T_)R[1] * x + (T_)R[4] * y + (T_)R[7] * z;
-  v[2] = (T_)R[2] * x + (T_)R[5] * y + (T_)R[8] * z;
+  v[0] = cm((T_)R[0], x) + cm((T_)R[3], y) + cm((T_)R[6], z);
+  v[1] = cm((T_)R[1], x) + cm((T_)R[4], y) + cm((T_)R[7], z);
+  v[2] = cm((T_)R[2], x) + cm((T_)R[5], y) + cm((T_)R[8], z);
 }
 
 // ------------------------------------------------------- spatial transforms
@@ -107,8 +116,8 @@ template <int B, typename T_ = float> G1K_DEV void motion_to_child(const T_* m, 
   constexpr int K = kM.b[B].axis;
   constexpr T_ t0 = (T_)kM.b[B].t[0], t1 = (T_)kM.b[B].t[1], t2 = (T_)kM.b[B].t[2];
   T_ a[3] = {m[0], m[1], m[2]};
-  T_ l[3] = {m[3] - (t1 * m[2] - t2 * m[1]), m[4] - (t2 * m[0] - t0 * m[2]),
-             m[5] - (t0 * m[1] - t1 * m[0])};
+  T_ l[3] = {m[3] - (cm(t1, m[2]) + cm(-t2, m[1])), m[4] - (cm(t2, m[0]) + cm(-t0, m[2])),
+             m[5] - (cm(t0, m[1]) + cm(-t1, m[0]))};
   if constexpr (!kM.b[B].rt_ident) { rt_mulT<B, T_>(a); rt_mulT<B, T_>(l); }
   rotT<K, T_>(a, c, s);
   rotT<K, T_>(l, c, s);
@@ -123,9 +132,9 @@ template <int B> G1K_DEV void force_to_parent_add(const float* f, float c, float
   rot<K>(a, c, s);
   rot<K>(l, c, s);
   if constexpr (!kM.b[B].rt_ident) { rt_mul<B>(a); rt_mul<B>(l); }
-  acc[0] += a[0] + (t1 * l[2] - t2 * l[1]);
-  acc[1] += a[1] + (t2 * l[0] - t0 * l[2]);
-  acc[2] += a[2] + (t0 * l[1] - t1 * l[0]);
+  acc[0] += a[0] + (cm(t1, l[2]) + cm(-t2, l[1]));
+  acc[1] += a[1] + (cm(t2, l[0]) + cm(-t0, l[2]));
+  acc[2] += a[2] + (cm(t0, l[1]) + cm(-t1, l[0]));
   acc[3] += l[0]; acc[4] += l[1]; acc[5] += l[2];
 }
 
@@ -174,13 +183,13 @@ template <int B, bool SYM> G1K_DEV void rt_sim(float (&A)[3][3]) {
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      T[r][c] = (float)R[3 * r] * A[0][c] + (float)R[3 * r + 1] * A[1][c] + (float)R[3 * r + 2] * A[2][c];
+      T[r][c] = cm((float)R[3 * r], A[0][c]) + cm((float)R[3 * r + 1], A[1][c]) + cm((float)R[3 * r + 2], A[2][c]);
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (SYM && c > r) continue;
-      A[r][c] = T[r][0] * (float)R[3 * c] + T[r][1] * (float)R[3 * c + 1] + T[r][2] * (float)R[3 * c + 2];
+      A[r][c] = cm((float)R[3 * c], T[r][0]) + cm((float)R[3 * c + 1], T[r][1]) + cm((float)R[3 * c + 2], T[r][2]);
     }
   if (SYM) {
     A[0][1] = A[1][0]; A[0][2] = A[2][0]; A[1][2] = A[2][1];
@@ -218,13 +227,13 @@ G1K_DEV void inertia_to_parent_add(const float (&A)[21], float c, float s, float
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int q = 0; q < 3; ++q)
-      HA[r][q] = H[r][q] + (tx[r][0] * M[0][q] + tx[r][1] * M[1][q] + tx[r][2] * M[2][q]);
+      HA[r][q] = H[r][q] + (cm(tx[r][0], M[0][q]) + cm(tx[r][1], M[1][q]) + cm(tx[r][2], M[2][q]));
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int q = 0; q <= r; ++q) {
-      const float tht = tx[r][0] * H[q][0] + tx[r][1] * H[q][1] + tx[r][2] * H[q][2];
-      const float hat = HA[r][0] * tx[0][q] + HA[r][1] * tx[1][q] + HA[r][2] * tx[2][q];
+      const float tht = cm(tx[r][0], H[q][0]) + cm(tx[r][1], H[q][1]) + cm(tx[r][2], H[q][2]);
+      const float hat = cm(tx[0][q], HA[r][0]) + cm(tx[1][q], HA[r][1]) + cm(tx[2][q], HA[r][2]);
       P[pk(r, q)] += Jb[r][q] + tht - hat;
       P[pk(3 + r, 3 + q)] += M[r][q];
     }
@@ -240,9 +249,9 @@ template <int B> G1K_DEV void bias_force(const float* v, float* f) {
   float h[6];
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
-    float acc = 0.f;
+    float acc = -0.f;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) acc += (float)kM.b[B].I[pk(r, m)] * v[m];
+    for (int m = 0; m < 6; ++m) acc += cm((float)kM.b[B].I[pk(r, m)], v[m]);
     h[r] = acc;
   }
   // [w x h_a + v_l x h_l ; w x h_l]   (src/spatial.h:116-118)
@@ -290,7 +299,7 @@ G1K_DEV void world_pose(const T_ (&Rp)[9], const T_ (&pp)[3], T_ c, T_ s, T_ (&R
   constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
   constexpr T_ t0 = (T_)kM.b[B].t[0], t1 = (T_)kM.b[B].t[1], t2 = (T_)kM.b[B].t[2];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) p[r] = Rp[3 * r] * t0 + Rp[3 * r + 1] * t1 + Rp[3 * r + 2] * t2 + pp[r];
+  for (int r = 0; r < 3; ++r) p[r] = cm(t0, Rp[3 * r]) + cm(t1, Rp[3 * r + 1]) + cm(t2, Rp[3 * r + 2]) + pp[r];
   T_ T[9];
   if constexpr (kM.b[B].rt_ident) {
 #pragma unroll
@@ -301,12 +310,12 @@ G1K_DEV void world_pose(const T_ (&Rp)[9], const T_ (&pp)[3], T_ c, T_ s, T_ (&R
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        T[3 * r + q] = Rp[3 * r] * (T_)Q[q] + Rp[3 * r + 1] * (T_)Q[3 + q] + Rp[3 * r + 2] * (T_)Q[6 + q];
+        T[3 * r + q] = cm((T_)Q[q], Rp[3 * r]) + cm((T_)Q[3 + q], Rp[3 * r + 1]) + cm((T_)Q[6 + q], Rp[3 * r + 2]);
   }
   // columns I, J of T R_K
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const float a = T[3 * r + I], b = T[3 * r + J];
+    const T_ a = T[3 * r + I], b = T[3 * r + J];
     R[3 * r + K] = T[3 * r + K];
     R[3 * r + I] = c * a + s * b;
     R[3 * r + J] = c * b - s * a;
@@ -488,15 +497,15 @@ G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ 
   for (int k = 0; k < 4; ++k) {
     const T_ r0 = (T_)g1m_contact_pos[3 * (K0 + k)], r1 = (T_)g1m_contact_pos[3 * (K0 + k) + 1],
              r2 = (T_)g1m_contact_pos[3 * (K0 + k) + 2];
-    const T_ rwd[3] = {Rd[0] * r0 + Rd[1] * r1 + Rd[2] * r2, Rd[3] * r0 + Rd[4] * r1 + Rd[5] * r2,
-                       Rd[6] * r0 + Rd[7] * r1 + Rd[8] * r2};
+    const T_ rwd[3] = {cm(r0, Rd[0]) + cm(r1, Rd[1]) + cm(r2, Rd[2]), cm(r0, Rd[3]) + cm(r1, Rd[4]) + cm(r2, Rd[5]),
+                       cm(r0, Rd[6]) + cm(r1, Rd[7]) + cm(r2, Rd[8])};
     const T_ pwd[3] = {rwd[0] + p[0], rwd[1] + p[1], rwd[2] + p[2]};
     const float rw[3] = {(float)rwd[0], (float)rwd[1], (float)rwd[2]};
     const float pw[3] = {(float)pwd[0], (float)pwd[1], (float)pwd[2]};
     const float depth = (float)(radius - pwd[2]);
     // point velocity in the body frame, then world: R (v_l + w x r)
-    const T_ vb[3] = {v[3] + (v[1] * r2 - v[2] * r1), v[4] + (v[2] * r0 - v[0] * r2),
-                      v[5] + (v[0] * r1 - v[1] * r0)};
+    const T_ vb[3] = {v[3] + (cm(r2, v[1]) + cm(-r1, v[2])), v[4] + (cm(r0, v[2]) + cm(-r2, v[0])),
+                      v[5] + (cm(r1, v[0]) + cm(-r0, v[1]))};
     const float vp[3] = {(float)(Rd[0] * vb[0] + Rd[1] * vb[1] + Rd[2] * vb[2]),
                          (float)(Rd[3] * vb[0] + Rd[4] * vb[1] + Rd[5] * vb[2]),
                          (float)(Rd[6] * vb[0] + Rd[7] * vb[1] + Rd[8] * vb[2])};
