@@ -139,7 +139,7 @@ template <int B> G1K_DEV void force_to_parent_add(const float* f, float c, float
 }
 
 template <int B> G1K_DEV void force_to_parent(float (&f)[6], float c, float s) {
-  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float acc[6] = {-0.f, -0.f, -0.f, -0.f, -0.f, -0.f};   // sums start at -0: -0 + x is x (no add issued)
   force_to_parent_add<B>(f, c, s, acc);
 #pragma unroll
   for (int m = 0; m < 6; ++m) f[m] = acc[m];
@@ -263,9 +263,11 @@ template <int B> G1K_DEV void bias_force(const float* v, float* f) {
   f[5] = v[0] * h[4] - v[1] * h[3];
 }
 
+// Zero entries start as -0: contributions added to them later need no add
+// (-0 + x is x; +0 + x is not, for x = -0).
 template <int B> G1K_DEV void init_inertia(float (&A)[21]) {
 #pragma unroll
-  for (int m = 0; m < 21; ++m) A[m] = (float)kM.b[B].I[m];
+  for (int m = 0; m < 21; ++m) A[m] = kM.b[B].I[m] == 0.0 ? -0.f : (float)kM.b[B].I[m];
 }
 
 // ------------------------------------------------------------ per-joint data
@@ -383,7 +385,7 @@ template <int B> G1K_DEV float forward_joint(const float* apar, const Joint& jt,
   a[J] += jt.cor[1];
   a[3 + I] += jt.cor[2];
   a[3 + J] += jt.cor[3];
-  float dot = 0.f;
+  float dot = -0.f;
 #pragma unroll
   for (int m = 0; m < 6; ++m) dot += jt.W[m] * a[m];
   const float qdd = jt.uD - dot;
@@ -488,7 +490,7 @@ template <int FIRST, int N> struct ChainOps {
 template <int K0, typename T_>
 G1K_DEV void foot_contacts(const Cfg& cf, float mu, const T_ (&Rd)[9], const T_ (&p)[3], const T_* v,
                            float* anchor, float* fn_out, float (&Fb)[6]) {
-  float Fw[3] = {0.f, 0.f, 0.f}, Tw[3] = {0.f, 0.f, 0.f};
+  float Fw[3] = {-0.f, -0.f, -0.f}, Tw[3] = {-0.f, -0.f, -0.f};
   float R[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) R[m] = (float)Rd[m];
@@ -896,18 +898,18 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     Joint cj[CPW][MAXJ];
     float Pw[21], pw[6], tc[CPW][MAXJ], Fw[6];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) Pw[m] = 0.f;
+    for (int m = 0; m < 21; ++m) Pw[m] = -0.f;   // sums start at -0, which -0 + x drops
 #pragma unroll
-    for (int m = 0; m < 6; ++m) { pw[m] = 0.f; Fw[m] = 0.f; }
+    for (int m = 0; m < 6; ++m) { pw[m] = -0.f; Fw[m] = -0.f; }
     static_for<CPW>([&](auto C) {
       constexpr int c = decltype(C)::value;
       float Pc[21], pc[6], Fpel[6];
 #pragma unroll
       for (int m = 0; m < MAXJ; ++m) tc[c][m] = 0.f;
 #pragma unroll
-      for (int m = 0; m < 21; ++m) Pc[m] = 0.f;
+      for (int m = 0; m < 21; ++m) Pc[m] = -0.f;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) { pc[m] = 0.f; Fpel[m] = 0.f; }
+      for (int m = 0; m < 6; ++m) { pc[m] = -0.f; Fpel[m] = -0.f; }
       dispatch_pre<GeoT>(wave * CPW + c, cf, cq[c], cqd[c], cct[c], mu, v0, v15, R0g, p0g, v0g, cj[c], anc[c],
                          fn[c], footp[c], footv[c], Pc, pc, want_tau, tc[c], Fpel);
 #pragma unroll
