@@ -116,8 +116,9 @@ template <int B, typename T_ = float> G1K_DEV void motion_to_child(const T_* m, 
   constexpr int K = kM.b[B].axis;
   constexpr T_ t0 = (T_)kM.b[B].t[0], t1 = (T_)kM.b[B].t[1], t2 = (T_)kM.b[B].t[2];
   T_ a[3] = {m[0], m[1], m[2]};
-  T_ l[3] = {m[3] - (cm(t1, m[2]) + cm(-t2, m[1])), m[4] - (cm(t2, m[0]) + cm(-t0, m[2])),
-             m[5] - (cm(t0, m[1]) + cm(-t1, m[0]))};
+  // l = m_l - t x m_a, written m_l + (m_a x t) so that a zero offset leaves no add
+  T_ l[3] = {m[3] + (cm(-t1, m[2]) + cm(t2, m[1])), m[4] + (cm(-t2, m[0]) + cm(t0, m[2])),
+             m[5] + (cm(-t0, m[1]) + cm(t1, m[0]))};
   if constexpr (!kM.b[B].rt_ident) { rt_mulT<B, T_>(a); rt_mulT<B, T_>(l); }
   rotT<K, T_>(a, c, s);
   rotT<K, T_>(l, c, s);
