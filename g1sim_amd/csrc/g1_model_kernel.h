@@ -23,6 +23,9 @@ struct BodyK {
   int parent;
   int axis;        // hinge axis index (0 x, 1 y, 2 z); -1 for the free base
   bool rt_ident;   // tree rotation is the identity
+  int rt_axis;     // the tree rotation is a planar rotation about unit axis rt_axis (-1: general)
+  double rt_c, rt_s;  // its cosine and sine (R[I][I], R[J][I] with I, J = axis + 1, + 2)
+  bool rt_fold;    // planar about the hinge's own axis: R_tree R_K(q) = R_K(theta + q)
   double RT[9];    // tree rotation (pose of the joint frame in the parent), row-major
   double t[3];     // tree position
   double I[21];    // spatial inertia about the body origin, packed lower triangle
@@ -54,6 +57,21 @@ constexpr BodyK make_body(int i) {
   B.RT[0] = 1 - 2 * (y * y + z * z); B.RT[1] = 2 * (x * y - w * z); B.RT[2] = 2 * (x * z + w * y);
   B.RT[3] = 2 * (x * y + w * z); B.RT[4] = 1 - 2 * (x * x + z * z); B.RT[5] = 2 * (y * z - w * x);
   B.RT[6] = 2 * (x * z - w * y); B.RT[7] = 2 * (y * z + w * x); B.RT[8] = 1 - 2 * (x * x + y * y);
+  B.rt_axis = -1;
+  B.rt_c = 1.0;
+  B.rt_s = 0.0;
+  for (int k = 0; k < 3 && !B.rt_ident; ++k) {
+    const int I = (k + 1) % 3, J = (k + 2) % 3;
+    const bool planar = B.RT[3 * k + k] == 1.0 && B.RT[3 * k + I] == 0.0 && B.RT[3 * k + J] == 0.0 &&
+                        B.RT[3 * I + k] == 0.0 && B.RT[3 * J + k] == 0.0 && B.RT[3 * I + I] == B.RT[3 * J + J] &&
+                        B.RT[3 * I + J] == -B.RT[3 * J + I];
+    if (planar) {
+      B.rt_axis = k;
+      B.rt_c = B.RT[3 * I + I];
+      B.rt_s = B.RT[3 * J + I];
+    }
+  }
+  B.rt_fold = B.rt_axis >= 0 && B.rt_axis == B.axis;
   for (int k = 0; k < 3; ++k) B.t[k] = g1m_tree_pos[3 * i + k];
   const double m = g1m_mass[i];
   const double cx = g1m_com[3 * i], cy = g1m_com[3 * i + 1], cz = g1m_com[3 * i + 2];

@@ -93,8 +93,17 @@ template <int K, typename T_ = float> G1K_DEV void rotT(T_* v, T_ c, T_ s) {    
   v[I] = c * a + s * b;
   v[J] = c * b - s * a;
 }
-// constant rotation of body B's tree frame
+// Tree rotations folded into the joint angle: planar about the hinge's own
+// axis, so RT R_K(q) = R_K(theta + q) and fk_joint composes theta into the
+// joint's cos/sin; such bodies, like those without a tree rotation, skip it.
+template <int B> constexpr bool rt_none = kM.b[B].rt_ident || kM.b[B].rt_fold;
+
+// constant rotation of body B's tree frame (a planar rotation where it is one)
 template <int B> G1K_DEV void rt_mul(float* v) {                          // v <- RT v
+  if constexpr (kM.b[B].rt_axis >= 0) {
+    rot<kM.b[B].rt_axis>(v, (float)kM.b[B].rt_c, (float)kM.b[B].rt_s);
+    return;
+  }
   constexpr const double* R = kM.b[B].RT;
   const float x = v[0], y = v[1], z = v[2];
   v[0] = cm((float)R[0], x) + cm((float)R[1], y) + cm((float)R[2], z);
@@ -102,6 +111,10 @@ template <int B> G1K_DEV void rt_mul(float* v) {                          // v <
   v[2] = cm((float)R[6], x) + cm((float)R[7], y) + cm((float)R[8], z);
 }
 template <int B, typename T_ = float> G1K_DEV void rt_mulT(T_* v) {      // v <- RT^T v
+  if constexpr (kM.b[B].rt_axis >= 0) {
+    rotT<kM.b[B].rt_axis, T_>(v, (T_)kM.b[B].rt_c, (T_)kM.b[B].rt_s);
+    return;
+  }
   constexpr const double* R = kM.b[B].RT;
   const T_ x = v[0], y = v[1], z = v[2];
   v[0] = cm((T_)R[0], x) + cm((T_)R[3], y) + cm((T_)R[6], z);
@@ -119,7 +132,7 @@ template <int B, typename T_ = float> G1K_DEV void motion_to_child(const T_* m, 
   // l = m_l - t x m_a, written m_l + (m_a x t) so that a zero offset leaves no add
   T_ l[3] = {m[3] + (cm(-t1, m[2]) + cm(t2, m[1])), m[4] + (cm(-t2, m[0]) + cm(t0, m[2])),
              m[5] + (cm(-t0, m[1]) + cm(t1, m[0]))};
-  if constexpr (!kM.b[B].rt_ident) { rt_mulT<B, T_>(a); rt_mulT<B, T_>(l); }
+  if constexpr (!rt_none<B>) { rt_mulT<B, T_>(a); rt_mulT<B, T_>(l); }
   rotT<K, T_>(a, c, s);
   rotT<K, T_>(l, c, s);
   out[0] = a[0]; out[1] = a[1]; out[2] = a[2];
@@ -132,7 +145,7 @@ template <int B> G1K_DEV void force_to_parent_add(const float* f, float c, float
   float a[3] = {f[0], f[1], f[2]}, l[3] = {f[3], f[4], f[5]};
   rot<K>(a, c, s);
   rot<K>(l, c, s);
-  if constexpr (!kM.b[B].rt_ident) { rt_mul<B>(a); rt_mul<B>(l); }
+  if constexpr (!rt_none<B>) { rt_mul<B>(a); rt_mul<B>(l); }
   acc[0] += a[0] + (cm(t1, l[2]) + cm(-t2, l[1]));
   acc[1] += a[1] + (cm(t2, l[0]) + cm(-t0, l[2]));
   acc[2] += a[2] + (cm(t0, l[1]) + cm(-t1, l[0]));
@@ -178,6 +191,12 @@ template <int K> G1K_DEV void rot_gen(float (&H)[3][3], float c, float s) {
 }
 // A <- RT A RT^T for a constant tree rotation (sym: only the lower triangle is used).
 template <int B, bool SYM> G1K_DEV void rt_sim(float (&A)[3][3]) {
+  if constexpr (kM.b[B].rt_axis >= 0) {
+    constexpr double c = kM.b[B].rt_c, s = kM.b[B].rt_s;
+    if constexpr (SYM) rot_sym<kM.b[B].rt_axis>(A, (float)c, (float)s, (float)(c * c - s * s), (float)(2.0 * c * s));
+    else rot_gen<kM.b[B].rt_axis>(A, (float)c, (float)s);
+    return;
+  }
   constexpr const double* R = kM.b[B].RT;
   float T[3][3];
 #pragma unroll
@@ -217,7 +236,7 @@ G1K_DEV void inertia_to_parent_add(const float (&A)[21], float c, float s, float
   rot_sym<K>(Jb, c, s, c2, s2);
   rot_sym<K>(M, c, s, c2, s2);
   rot_gen<K>(H, c, s);
-  if constexpr (!kM.b[B].rt_ident) {
+  if constexpr (!rt_none<B>) {
     rt_sim<B, true>(Jb);
     rt_sim<B, true>(M);
     rt_sim<B, false>(H);
@@ -282,10 +301,22 @@ struct Joint {
   float tau;      // servo + joint-limit torque (src/g1_core.h:166-174)
 };
 
+// cos/sin of the hinge angle -> of theta + q where the tree rotation of body
+// B is R_K(theta) (rt_fold): the composite R_K(theta) R_K(q).
+template <int B, typename T_ = float> G1K_DEV void fold_tree(T_& c, T_& s) {
+  if constexpr (kM.b[B].rt_fold) {
+    constexpr T_ ct = (T_)kM.b[B].rt_c, st = (T_)kM.b[B].rt_s;
+    const T_ c0 = c, s0 = s;
+    c = c0 * ct - s0 * st;
+    s = s0 * ct + c0 * st;
+  }
+}
+
 // Kinematics of hinge body B: twist from the parent's (src/g1_core.h:39-53).
 template <int B> G1K_DEV void fk_joint(float q, float qd, const float* vpar, float* v, Joint& jt) {
   constexpr int K = kM.b[B].axis, I = Plane<K>::I, J = Plane<K>::J;
   fsincos(q, &jt.s, &jt.c);
+  fold_tree<B>(jt.c, jt.s);
   motion_to_child<B>(vpar, jt.c, jt.s, v);
   v[K] += qd;
   // cross_motion(v, qd S) with S = [e_K; 0] (src/g1_core.h:180)
@@ -304,7 +335,7 @@ G1K_DEV void world_pose(const T_ (&Rp)[9], const T_ (&pp)[3], T_ c, T_ s, T_ (&R
 #pragma unroll
   for (int r = 0; r < 3; ++r) p[r] = cm(t0, Rp[3 * r]) + cm(t1, Rp[3 * r + 1]) + cm(t2, Rp[3 * r + 2]) + pp[r];
   T_ T[9];
-  if constexpr (kM.b[B].rt_ident) {
+  if constexpr (rt_none<B>) {
 #pragma unroll
     for (int m = 0; m < 9; ++m) T[m] = Rp[m];
   } else {
@@ -421,6 +452,7 @@ template <int FIRST, int N> struct ChainOps {
         c = jt[i].c; s = jt[i].s;
       } else {
         sincos((double)q[i], &s, &c);
+        fold_tree<FIRST + i, T_>(c, s);
         motion_to_child<FIRST + i, T_>(vp, c, s, v);
         v[kM.b[FIRST + i].axis] += (T_)qd[i];
       }
