@@ -240,7 +240,13 @@ def test_settles_like_the_oracle(device):
     assert np.isfinite(h["qpos"]).all()
 
 
-@pytest.mark.parametrize("flags", [_lib.G1_FLAG_F64_SOLVE, _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE])
+@pytest.mark.parametrize("flags", [
+    _lib.G1_FLAG_F64_SOLVE,
+    _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE,
+    # the 2-wave split's float64 instances (chosen by default from 32,768 envs)
+    _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE | _lib.G1_FLAG_WAVES2,
+    _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_CONTACT | _lib.G1_FLAG_WAVES2,
+])
 def test_f64_islands(device, flags):
     qp, qv, an, ct = W.drop_test(1024, seed=6)
     st = GpuState(qp, qv, an, ct, device)
