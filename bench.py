@@ -14,9 +14,10 @@ physics steps after `--warmup` untimed ones.
 Prints one JSON line: throughput in env-steps/s, the roofline of the step
 kernel (algorithmic FP32 FLOP of the reference formulation, 41,491 per
 env-step, over the kernel's average launch time from HIP events on the launch
-stream), and the CPU baseline (the reference physics core compiled from the
-reference sources, float, OpenMP over the host cores) timed on a bounded
-sample of the same workload.
+stream), the same FP32 ABA step at 131,072 envs (`aba_131072`), the
+surrogate and policy-in-the-loop legs, and the CPU baseline (the reference
+physics core compiled from the reference sources, float, OpenMP over the host
+cores) timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -93,6 +94,47 @@ def cpu_baseline(nenv_gpu, acc_sample=None):
                       f"on {cores} threads; 1-thread rate from {n_one} envs x {s_one} steps",
             "value_1core": v_one}
     return base, (accuracy_vs_fp64(acc_sample) if acc_sample is not None else None)
+
+
+def aba_leg(dev, nenv, steps, warmup):
+    """The FP32 ABA step on the same randomized-actuation workload at another
+    batch size (the 131,072-env target: the 2-wave split), timed as the main
+    leg: wall clock between synchronisations for env-steps/s, one HIP event
+    pair around the back-to-back launches for the kernel's average duration."""
+    import torch
+    from g1sim_amd import _lib, workloads as W
+    lib = _lib.load()
+    nblocks = (warmup + steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=3, nblocks=nblocks)
+    qpos, qvel, anchor = (torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    ctrl = torch.as_tensor(cb, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def run(step0, n):
+        for s in range(step0, step0 + n, CTRL_PERIOD):
+            rc = lib.g1_hip_step_ex(nenv, CTRL_PERIOD, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                    ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None, None,
+                                    0, stream.cuda_stream)
+            if rc:
+                raise RuntimeError(_lib.last_error())
+
+    run(0, warmup)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    run(warmup, steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_ms = e0.elapsed_time(e1) / (steps // CTRL_PERIOD)
+    achieved = FLOP_PER_ENV_STEP * nenv * CTRL_PERIOD / (avg_ms * 1e-3)
+    return {"nenv": nenv, "steps": steps, "warmup": warmup, "nsub_per_launch": CTRL_PERIOD,
+            "env_steps_per_s": nenv * steps / wall, "avg_launch_ms": avg_ms,
+            "kernel": "g1k::step_kernel<false, false, 2, 0>",
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32}}
 
 
 def _event_avg_ms(fn, reps, stream):
@@ -347,6 +389,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 131,072-env FP32 ABA leg of the default run")
     ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
     ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper"),
                     help="time one fused inference kernel, or one surrogate stepper step, alone at --nenv "
@@ -489,6 +533,8 @@ def main():
     peaks = measure_peaks()
     out["measured_peaks"] = peaks
     out["roofline"]["frac_of_measured_fp32_fma"] = achieved / (peaks["fp32_fma_tflops"] * 1e12)
+    if not args.no_large and args.flags == 0 and nenv == 16384 and args.nsub == CTRL_PERIOD:
+        out["aba_131072"] = aba_leg(dev, 131072, 200, 20)
     if not args.no_surrogate and args.flags == 0:
         out["surrogate"], mlp = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
         if not args.no_policy:
