@@ -604,20 +604,32 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
   float* out = reinterpret_cast<float*>(buf[0]);
   dense_out<128, 32>(buf[1], out, a.W[3], a.b[3], w, lane, ring3);
   __syncthreads();
-  // epilogue: thread (r, group) -> env r, actions j = group, group + 8, ...
+  // epilogue: thread (r, group) -> env r, actions j = group, group + 8, ...,
+  // taken in pairs (j, j + 8) that share one Box-Muller draw
   const int r = tid & 63, e = e0 + r;
   if (e < a.nenv) {
-    for (int j = w; j < NU; j += NWAVE) {
+    for (int j = w; j < NU; j += 2 * NWAVE) {
+      const int j2 = j + NWAVE;
+      const bool two = j2 < NU;
       float act = out[r * LDO + j];
+      float act2 = two ? out[r * LDO + j2] : 0.f;
       if (a.log_std) {
-        // Box-Muller: u1 in (0, 1], u2 in [0, 1)
+        // Box-Muller: u1 in (0, 1], u2 in [0, 1); the cosine and sine
+        // outputs are independent standard normals for actions j and j + 8
         const float u1 = 1.0f - draw_u01(a.seed, a.t, e, 2 * j);
         const float u2 = draw_u01(a.seed, a.t, e, 2 * j + 1);
-        const float z = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
-        act = fmaf(expf(a.log_std[j]), z, act);
+        const float rad = sqrtf(-2.0f * logf(u1));
+        float sn, cs;
+        sincosf(6.283185307179586f * u2, &sn, &cs);
+        act = fmaf(expf(a.log_std[j]), rad * cs, act);
+        if (two) act2 = fmaf(expf(a.log_std[j2]), rad * sn, act2);
       }
       a.ctrl[j * N + e] = (float)g1m_qpos_stand[7 + j] + act * kActionScale[j];
       if (a.actions) a.actions[j * N + e] = act;
+      if (two) {
+        a.ctrl[j2 * N + e] = (float)g1m_qpos_stand[7 + j2] + act2 * kActionScale[j2];
+        if (a.actions) a.actions[j2 * N + e] = act2;
+      }
     }
   }
 }
