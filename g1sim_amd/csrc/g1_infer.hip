@@ -657,7 +657,7 @@ __global__ __launch_bounds__(256) void obs_batch_stats_kernel(int nenv, int layo
                                                               const float* qvel, const float* commands,
                                                               const float* last_ctrl, const float* run_mean,
                                                               const float* count, double* part) {
-  __shared__ double s1[256], s2[256];
+  __shared__ double s1[4], s2[4];
   const int f = blockIdx.x, ch = blockIdx.y;
   // the running count as it was before this update, for every merge block
   if (f == 0 && ch == 0 && threadIdx.x == 0) part[2 * NORM_CHUNKS * gridDim.x] = (double)count[0];
@@ -669,19 +669,22 @@ __global__ __launch_bounds__(256) void obs_batch_stats_kernel(int nenv, int layo
     a += x;
     b += x * x;
   }
-  s1[threadIdx.x] = a;
-  s2[threadIdx.x] = b;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      s1[threadIdx.x] += s1[threadIdx.x + off];
-      s2[threadIdx.x] += s2[threadIdx.x + off];
-    }
-    __syncthreads();
+  // butterfly within each wave (no barriers), then the 4 wave sums in a
+  // fixed order: the result does not depend on scheduling
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
   }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s1[w] = a;
+    s2[w] = b;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    part[(size_t)(f * NORM_CHUNKS + ch) * 2] = s1[0];
-    part[(size_t)(f * NORM_CHUNKS + ch) * 2 + 1] = s2[0];
+    part[(size_t)(f * NORM_CHUNKS + ch) * 2] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    part[(size_t)(f * NORM_CHUNKS + ch) * 2 + 1] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
   }
 }
 
@@ -795,6 +798,22 @@ __global__ __launch_bounds__(256) void post_step_kernel(PostArgs a) {
 
 __device__ void post_step(const PostArgs& a, int e, float& rew, bool& died) {
   const size_t N = (size_t)a.nenv;
+  // Every input of the env is loaded before the first is used: one memory
+  // latency instead of one per dependent group (at one thread per env,
+  // 65,536 envs are one wave per SIMD, so nothing else hides it).
+  float q[NU], act[NU], prev[NU], fv[6], cf[8];
+#pragma unroll
+  for (int j = 0; j < NU; ++j) {
+    q[j] = a.qpos[(7 + j) * N + e];
+    act[j] = a.actions[j * N + e];
+    prev[j] = a.prev_actions[j * N + e];
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) fv[m] = a.foot_vel[m * N + e];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) cf[m] = a.contact_forces[m * N + e];
+  const float lz = a.foot_pos[2 * N + e], rz = a.foot_pos[5 * N + e];
+  const int ct0 = a.command_timeout[e], el0 = a.episode_length[e];
   const float c0 = a.commands[e], c1 = a.commands[N + e], c2 = a.commands[2 * N + e];
   const float qw = a.qpos[3 * N + e], qx = a.qpos[4 * N + e], qy = a.qpos[5 * N + e], qz = a.qpos[6 * N + e];
   const float down[3] = {0.f, 0.f, -1.f};
@@ -803,6 +822,7 @@ __device__ void post_step(const PostArgs& a, int e, float& rew, bool& died) {
   quat_rotate_inverse(qw, qx, qy, qz, down, g);
   quat_rotate_inverse(qw, qx, qy, qz, vw, v);
   const float w0 = a.qvel[3 * N + e], w1 = a.qvel[4 * N + e], w2 = a.qvel[5 * N + e];
+  __builtin_amdgcn_sched_barrier(0);
   // 1. velocity tracking
   const float lin_err = (c0 - v[0]) * (c0 - v[0]) + (c1 - v[1]) * (c1 - v[1]) + v[2] * v[2];
   const float rew_lin = expf(-lin_err / 0.25f);
@@ -815,35 +835,31 @@ __device__ void post_step(const PostArgs& a, int e, float& rew, bool& died) {
   float pose = 0.f, dof = 0.f;
 #pragma unroll
   for (int j = 0; j < NU; ++j) {
-    const float q = a.qpos[(7 + j) * N + e];
-    const float err = q - (float)g1m_qpos_stand[7 + j];
+    const float err = q[j] - (float)g1m_qpos_stand[7 + j];
     const float sd = standing ? 0.05f : running ? kStdRunning[j] : kStdWalking[j];
     pose += err * err / (sd * sd);
     const float lo = (float)g1m_jnt_range[2 * j], hi = (float)g1m_jnt_range[2 * j + 1];
     const float range = hi - lo;
     const float slo = lo + 0.05f * range, shi = hi - 0.05f * range;
-    dof += fmaxf(q - shi, 0.f) + fmaxf(slo - q, 0.f);
+    dof += fmaxf(q[j] - shi, 0.f) + fmaxf(slo - q[j], 0.f);
   }
   const float rew_pose = expf(-(pose / (float)NU));
   // 3. penalties
   float rate = 0.f;
 #pragma unroll
   for (int j = 0; j < NU; ++j) {
-    const float d = a.actions[j * N + e] - a.prev_actions[j * N + e];
+    const float d = act[j] - prev[j];
     rate += d * d;
   }
   const float body_ang = w0 * w0 + w1 * w1;
   // 4. feet
-  const float lz = a.foot_pos[2 * N + e], rz = a.foot_pos[5 * N + e];
-  const float lv2 = a.foot_vel[e] * a.foot_vel[e] + a.foot_vel[N + e] * a.foot_vel[N + e];
-  const float rv2 = a.foot_vel[3 * N + e] * a.foot_vel[3 * N + e] + a.foot_vel[4 * N + e] * a.foot_vel[4 * N + e];
+  const float lv2 = fv[0] * fv[0] + fv[1] * fv[1];
+  const float rv2 = fv[3] * fv[3] + fv[4] * fv[4];
   const float active = speed > 0.05f ? 1.f : 0.f;
   const float clearance =
       (fabsf(lz - 0.1f) * sqrtf(lv2 + 1e-6f) + fabsf(rz - 0.1f) * sqrtf(rv2 + 1e-6f)) * active;
-  const float lf = a.contact_forces[e] + a.contact_forces[N + e] + a.contact_forces[2 * N + e] +
-                   a.contact_forces[3 * N + e];
-  const float rf = a.contact_forces[4 * N + e] + a.contact_forces[5 * N + e] + a.contact_forces[6 * N + e] +
-                   a.contact_forces[7 * N + e];
+  const float lf = cf[0] + cf[1] + cf[2] + cf[3];
+  const float rf = cf[4] + cf[5] + cf[6] + cf[7];
   const float slip = (lv2 * (lf > 1.f ? 1.f : 0.f) + rv2 * (rf > 1.f ? 1.f : 0.f)) * active;
   rew = 2.0f * rew_lin + 2.0f * rew_ang + rew_orient + rew_pose - dof - 2.0f * clearance - 0.5f * slip -
         0.05f * rate - 0.05f * body_ang;
@@ -854,11 +870,11 @@ __device__ void post_step(const PostArgs& a, int e, float& rew, bool& died) {
   a.done[e] = done ? 1 : 0;
   died = done;
   // bookkeeping
-  int ct = a.command_timeout[e] - 1;
-  int el = a.episode_length[e] + 1;
+  int ct = ct0 - 1;
+  int el = el0 + 1;
   const bool rs = done || el >= a.max_episode_length;
 #pragma unroll
-  for (int j = 0; j < NU; ++j) a.prev_actions[j * N + e] = rs ? 0.f : a.actions[j * N + e];
+  for (int j = 0; j < NU; ++j) a.prev_actions[j * N + e] = rs ? 0.f : act[j];
   if (rs) {
     resample_command(a.commands, N, e, a.seed, a.t, K_RESET, a.cmd_lin, a.cmd_yaw);
     ct = draw_int(a.seed, a.t, e, K_RESET + 3, 150, 100);
