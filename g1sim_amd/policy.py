@@ -191,6 +191,8 @@ class PolicyRollout:
                                  None if reward_sum is None else reward_sum.data_ptr(),
                                  None if done_count is None else done_count.data_ptr(), s)
         _lib.check(rc, "g1_rollout_post")
+        if self.stepper is not None:              # a reset env is stepped by ABA until the judge accepts it
+            self.stepper.mark_reset(self.reset)
         lib.g1_cuda_reset_done(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
                                self.reset.data_ptr(), self._seed(t), 0.1, 0.02, s)
         self.t += 1

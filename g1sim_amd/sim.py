@@ -43,14 +43,15 @@ def norm_device(device):
     return d
 
 
-def check_soa(name, t, rows, nenv, device):
-    """A state tensor as the kernels index it: float32 [rows][nenv],
-    contiguous, on the simulator's device (a reassigned attribute of another
-    shape would otherwise be read out of bounds on the GPU)."""
-    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or tuple(t.shape) != (rows, nenv) \
+def check_soa(name, t, rows, nenv, device, dtype=torch.float32):
+    """A state tensor as the kernels index it: [rows][nenv] of dtype
+    (float32 by default), contiguous, on the simulator's device (a
+    reassigned attribute of another shape would otherwise be read out of
+    bounds on the GPU)."""
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype or tuple(t.shape) != (rows, nenv) \
             or t.device != device or not t.is_contiguous():
         got = (tuple(t.shape), t.dtype, str(t.device), t.is_contiguous()) if isinstance(t, torch.Tensor) else type(t)
-        raise ValueError(f"{name} must be a contiguous float32 ({rows}, {nenv}) tensor on {device}; got {got}")
+        raise ValueError(f"{name} must be a contiguous {dtype} ({rows}, {nenv}) tensor on {device}; got {got}")
 
 
 class G1Sim:

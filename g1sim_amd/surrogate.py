@@ -390,10 +390,17 @@ def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
 # ====================================================================== step
 class SurrogateStepper:
     """qacc from the network for most envs, ABA as judge on a rotating
-    1/period of the envs every step and as per-env fallback after a rejected
-    judgement (g1_hip_surrogate_integrate)."""
+    1/period of the envs every step and as per-env fallback
+    (g1_hip_surrogate_integrate).
 
-    def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True):
+    Fail-safe start: every env begins in fallback and is stepped by ABA
+    until the judge has accepted the network on it; a rejected judgement puts
+    it back.  Envs that the caller resets (mark_reset) start over in
+    fallback.  So the network's qacc is integrated only for envs whose last
+    judgement, at most period - 1 steps earlier, was within the tolerance."""
+
+    def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True,
+                 track_qacc=False):
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
         self.fused = fused and mlp.fusable
         self.lib = _lib.load()
@@ -402,7 +409,7 @@ class SurrogateStepper:
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.tau = torch.zeros((NV, nenv), **f32)
         self.qacc_aba = torch.zeros((NV, nenv), **f32)
-        self.fallback = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
+        self.fallback = torch.ones(nenv, dtype=torch.uint8, device=self.dev)
         self.stats = torch.zeros(4, dtype=torch.int32, device=self.dev)
         self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
         self.pred = torch.zeros((self.rows, POUT), **f32)
@@ -411,9 +418,21 @@ class SurrogateStepper:
         self.foot_pos = torch.zeros((6, nenv), **f32)
         self.foot_vel = torch.zeros((6, nenv), **f32)
         self.contact_forces = torch.zeros((8, nenv), **f32)
+        # the qacc each step integrated (network or ABA), [35][nenv], when tracked
+        self.qacc = torch.zeros((NV, nenv), **f32) if track_qacc else None
         self.t = 0
 
-    def step(self, qpos, qvel, anchor, ctrl, mu=None):
+    def mark_reset(self, reset):
+        """Envs with reset[e] != 0 (uint8 [nenv] on the device) return to
+        fallback until their next accepted judgement."""
+        check_soa("reset", reset.view(1, -1), 1, self.nenv, self.dev, dtype=torch.uint8)
+        self.fallback.bitwise_or_(reset)
+
+    def step(self, qpos, qvel, anchor, ctrl, mu=None, prediction=None):
+        """One step of every env.  prediction (optional): z-scored qacc rows
+        (float32 [>= nenv][128], first 35 columns) used in place of the
+        network's output, through the unfused commit -- for predictors other
+        than this MLP."""
         lib, s, n = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
         for name, t, rows in (("qpos", qpos, NQ), ("qvel", qvel, NV), ("anchor", anchor, 16), ("ctrl", ctrl, NV - 6)):
             check_soa(name, t, rows, n, self.dev)
@@ -429,21 +448,29 @@ class SurrogateStepper:
                                    self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
                                    self.foot_vel.data_ptr(), self.judge_flags, s), "judge_forces")
         m = self.mlp
-        if self.fused:
+        qacc_out = None if self.qacc is None else self.qacc.data_ptr()
+        if prediction is not None and (prediction.dtype != torch.float32 or prediction.device != self.dev
+                                       or not prediction.is_contiguous() or prediction.dim() != 2
+                                       or prediction.shape[0] < n or prediction.shape[1] != POUT):
+            raise ValueError(f"prediction must be a contiguous float32 (>= {n}, {POUT}) tensor on {self.dev}")
+        if self.fused and prediction is None:
             W, b = m._wb()
             ck(lib.g1_hip_surrogate_step(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
                                          self.tau.data_ptr(), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b,
                                          m.ymean.data_ptr(), self.ystd.data_ptr(), self.qacc_aba.data_ptr(),
-                                         self.fallback.data_ptr(), self.stats.data_ptr(), None, s), "surrogate_step")
+                                         self.fallback.data_ptr(), self.stats.data_ptr(), qacc_out, s),
+               "surrogate_step")
         else:
-            rc = lib.g1_mlp_features(n, self.rows, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
-                                     m.xmean.data_ptr(), m.xinv.data_ptr(), self.X16.data_ptr(), PIN, s)
-            ck(rc, "features")
-            m.forward(self.X16, self.rows, train=False, out=self.pred)
+            pred = prediction
+            if pred is None:
+                rc = lib.g1_mlp_features(n, self.rows, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
+                                         m.xmean.data_ptr(), m.xinv.data_ptr(), self.X16.data_ptr(), PIN, s)
+                ck(rc, "features")
+                pred = m.forward(self.X16, self.rows, train=False, out=self.pred)
             ck(lib.g1_hip_surrogate_integrate(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(),
-                                              qvel.data_ptr(), self.pred.data_ptr(), POUT, m.ymean.data_ptr(),
+                                              qvel.data_ptr(), pred.data_ptr(), POUT, m.ymean.data_ptr(),
                                               self.ystd.data_ptr(), self.qacc_aba.data_ptr(),
-                                              self.fallback.data_ptr(), self.stats.data_ptr(), None, s),
+                                              self.fallback.data_ptr(), self.stats.data_ptr(), qacc_out, s),
                "surrogate_integrate")
         self.t += 1
 

@@ -222,6 +222,35 @@ def test_reset_done_bit_exact(device):
                 assert (qp[:, e] == 7).all() and (qv[:, e] == 7).all() and (an[:, e] == 7).all()
 
 
+def test_reset_done_matches_reference_fixture(device, golden):
+    """The GPU reset against the reference kernel's own output
+    (tests/golden/reset.npz: g1_reset_done_kernel, src/g1_kernel.cu:85-103,
+    over a 65,536-env batch), bit for bit, for every fixture seed, noise and
+    drop."""
+    g = golden("reset")
+    n, envs, rows = int(g["nenv"]), g["envs"], g["rows"]
+    lib = _lib.load()
+    done = np.zeros(n, np.uint8)
+    done[envs] = 1
+    d = torch.as_tensor(done, device=device)
+    qpos = torch.empty((36, n), device=device)
+    qvel = torch.empty((35, n), device=device)
+    anc = torch.empty((16, n), device=device)
+    for a, seed in enumerate(g["seeds"]):
+        for b, noise in enumerate(g["noises"]):
+            for c, drop in enumerate(g["drops"]):
+                for t in (qpos, qvel, anc):
+                    t.fill_(7.0)
+                lib.g1_cuda_reset_done(n, qpos.data_ptr(), qvel.data_ptr(), anc.data_ptr(), d.data_ptr(), int(seed),
+                                       float(noise), float(drop), torch.cuda.current_stream(device).cuda_stream)
+                torch.cuda.synchronize()
+                got = qpos[:, torch.as_tensor(envs, device=device)].cpu().numpy()
+                want = g["qpos"][a, b, c]
+                assert np.array_equal(got[rows].view(np.uint32), want.view(np.uint32)), (seed, noise, drop)
+                assert int((qpos != 7).sum()) == 36 * envs.size      # only the done envs are written
+                assert bool((qvel[:, torch.as_tensor(envs, device=device)] == 0).all())
+
+
 def test_settles_like_the_oracle(device):
     """10 s (5,000 steps at dt 2e-3) of the drop test: every env stands with
     the robot's weight on the feet (tests/test_vs_mujoco.py:114-133)."""

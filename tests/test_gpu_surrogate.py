@@ -118,12 +118,18 @@ def test_judge_matches_step_qacc(device):
     same_at_f32(qacc_b.cpu().numpy(), q, "blocks vs list")
 
 
-def _stepper_with_oracle_pred(device, n, seed, corrupt=None):
+def _stepper_with_oracle_pred(device, n, seed, corrupt=None, accepted_before=True):
     """A stepper whose 'network' output is ABA's own qacc (z-scored), or a
-    corrupted copy, to test the integrate/judge/fallback logic alone."""
+    corrupted copy, to test the integrate/judge/fallback logic alone.  With
+    accepted_before, every env starts out of fallback (as after an accepted
+    judgement); otherwise from the stepper's fail-safe start (all in
+    fallback)."""
     st = _state(device, n, seed)
     mlp = MLP(device, seed=0)
     stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e-3)
+    assert bool((stepper.fallback == 1).all())          # fail-safe start
+    if accepted_before:
+        stepper.fallback.zero_()
     ref = GpuState(st.qpos.cpu().numpy(), st.qvel.cpu().numpy(), st.anchor.cpu().numpy(), st.ctrl.cpu().numpy(),
                    device)
     ref.step(1, 2e-3)
@@ -150,6 +156,24 @@ def test_integrate_with_exact_prediction(device):
     sm = stepper.stats.cpu().numpy()
     assert sm[0] == 4096 // 16 and sm[1] == 0
     assert int(stepper.fallback.sum()) == 0
+
+
+def test_fail_safe_start(device):
+    """A new stepper integrates no unjudged network output: every env starts
+    in fallback (ABA), and only the envs of the judged blocks whose
+    prediction the judge accepts leave it."""
+    bad = np.arange(0, 4096, 3)
+    st, ref, stepper = _stepper_with_oracle_pred(device, 4096, 30, corrupt=bad, accepted_before=False)
+    a, b = st.host(), ref.host()
+    fb = stepper.fallback.cpu().numpy().astype(bool)
+    judged = judged_envs(4096, 0)
+    accepted = np.setdiff1d(judged, bad)
+    assert np.array_equal(np.nonzero(~fb)[0], accepted)
+    # every env was stepped with ABA's qacc (judged envs: the judge's; the
+    # others: fallback), so all match the fused step
+    assert np.abs(a["qpos"] - b["qpos"]).max() < 1e-5
+    sm = stepper.stats.cpu().numpy()
+    assert sm[0] == judged.size and sm[1] == np.intersect1d(judged, bad).size and sm[3] == 4096 - judged.size
 
 
 def test_judge_rejects_and_falls_back(device):

@@ -8,7 +8,12 @@
    free-fall closed form, LAPACK-equivalent Cholesky, the quaternion
    exponential, and the contact invariants of the reference's settling test
    (tests/test_vs_mujoco.py:114-133).
+4. The reset hash and draw order: fixtures from the reference's own reset
+   kernel (src/g1_kernel.cu:54-103) built on the CPU through its shim, the
+   live kernel where built, and the published inverse of the hash.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -16,6 +21,7 @@ from oracle import oracle as O
 from g1sim_amd import workloads as W
 from g1sim_amd.model import MODEL
 
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 
 # ------------------------------------------------------------ golden vectors
 
@@ -250,14 +256,83 @@ def test_friction_cone_and_anchor_semantics():
 
 # ------------------------------------------------------------ reset RNG
 
-def test_hash_is_a_bijection_sample():
-    """Each stage of the reset hash (xor-shift, odd multiply) is invertible on
-    32-bit words, so distinct inputs map to distinct outputs."""
+def _lowbias32_inverse(x):
+    """The published inverse of the lowbias32 hash (C. Wellons,
+    "Prospecting for Hash Functions", 2018; hash-prospector lowbias32_r):
+    the xor-shifts undone in reverse order and the multipliers replaced by
+    their inverses mod 2^32 (0x846ca68b^-1 = 0x43021123, 0x7feb352d^-1 =
+    0x1d69e2a5).  numpy uint32, vectorised."""
+    x = np.asarray(x, np.uint32).copy()
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint32(16)
+        x *= np.uint32(0x43021123)
+        x ^= (x >> np.uint32(15)) ^ (x >> np.uint32(30))
+        x *= np.uint32(0x1D69E2A5)
+        x ^= x >> np.uint32(16)
+    return x
+
+
+def test_hash_inverts_by_published_inverse():
+    """The reset hash (src/g1_kernel.cu:54-58) is lowbias32: composed with
+    its published inverse it is the identity on random words (a wrong shift
+    or multiplier fails this), and the inverse multipliers are the forward
+    ones' inverses mod 2^32."""
+    assert (0x846CA68B * 0x43021123) % 2 ** 32 == 1 and (0x7FEB352D * 0x1D69E2A5) % 2 ** 32 == 1
     lib = O.lib64()
-    xs = np.random.default_rng(0).choice(2 ** 32, 200000, replace=False)
-    hs = {lib.g1o_hash(int(x)) for x in xs}
-    assert len(hs) == len(xs)
+    xs = np.random.default_rng(0).integers(0, 2 ** 32, 20000, dtype=np.uint64).astype(np.uint32)
+    hs = np.array([lib.g1o_hash(int(x)) for x in xs], np.uint32)
+    assert np.array_equal(_lowbias32_inverse(hs), xs)
     assert lib.g1o_hash(0) == 0
+
+
+def test_reset_matches_reference_fixture(golden):
+    """Reset parity with the reference kernel itself: qpos rows 2 and 7-35
+    of g1_reset_done_kernel (src/g1_kernel.cu:85-103, built on the CPU from
+    the reference source through its FAKE_CUDA shim with CUDA's default FMA
+    contraction; tests/golden/make_reset_golden.py) for seeds {0, 42, 1234,
+    2^32 - 1} x noise {0, 0.05, 0.3} x drop {0, 0.02, 1.0}, bit for bit;
+    and the hash on fixed words."""
+    g = golden("reset")
+    lib = O.lib64()
+    assert np.array_equal(np.array([lib.g1o_hash(int(w)) for w in g["hash_in"]], np.uint32), g["hash_out"])
+    rows = g["rows"]
+    for a, seed in enumerate(g["seeds"]):
+        for b, noise in enumerate(g["noises"]):
+            for c, drop in enumerate(g["drops"]):
+                for k, e in enumerate(g["envs"]):
+                    qp, qv, an = O.reset_env_f32(int(e), int(seed), float(noise), float(drop))
+                    want = g["qpos"][a, b, c, :, k]
+                    assert np.array_equal(qp[rows].view(np.uint32), want.view(np.uint32)), (seed, noise, drop, e)
+                    assert (qv == 0).all() and (an == np.float32(1e30)).all()
+
+
+def test_reset_matches_reference_live():
+    """Where the reference tree was built (oracle/_ref/libg1ref_reset.so):
+    every env of a 3,000-env batch under a random done mask, all of qpos,
+    qvel and anchors, for several seeds and noise/drop values."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libg1ref_reset.so")
+    if not os.path.exists(path):
+        pytest.skip("reference reset kernel not built (no reference tree)")
+    import ctypes as C
+    lib = C.CDLL(path)
+    lib.g1ref_reset_done.restype = None
+    lib.g1ref_reset_done.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
+                                     C.c_float, C.c_float]
+    n = 3000
+    rng = np.random.default_rng(5)
+    for seed, noise, drop in ((7, 0.1, 0.02), (2 ** 31 + 3, 0.5, 0.3), (99, 0.05, 0.0)):
+        done = (rng.random(n) < 0.5).astype(np.uint8)
+        qp = np.full((36, n), 7.0, np.float32)
+        qv = np.full((35, n), 7.0, np.float32)
+        an = np.full((16, n), 7.0, np.float32)
+        lib.g1ref_reset_done(n, qp.ctypes.data, qv.ctypes.data, an.ctypes.data, done.ctypes.data, seed, noise, drop)
+        for e in range(n):
+            if done[e]:
+                q, v, a = O.reset_env_f32(e, seed, noise, drop)
+                assert np.array_equal(q.view(np.uint32), qp[:, e].view(np.uint32)), (seed, e)
+                assert np.array_equal(v, qv[:, e]) and np.array_equal(a, an[:, e])
+            else:
+                assert (qp[:, e] == 7).all() and (qv[:, e] == 7).all() and (an[:, e] == 7).all()
 
 
 def test_reset_env_structure():
