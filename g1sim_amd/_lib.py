@@ -92,6 +92,8 @@ def load():
                                     vp]
     lib.g1_policy_obs_norm_update.restype = i
     lib.g1_policy_obs_norm_update.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.g1_policy_obs_norm_update_noalloc.restype = i
+    lib.g1_policy_obs_norm_update_noalloc.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_rollout_init.restype = i
     lib.g1_rollout_init.argtypes = [i, f, f, u, vp, vp, vp, vp, vp]
     for name in ("g1_hip_peak_fp32", "g1_hip_peak_bf16"):
@@ -112,7 +114,8 @@ def exported_symbols():
             "g1_hip_judge_list", "g1_hip_step_judge", "g1_hip_step_judge_blocks", "g1_hip_judge_forces",
             "g1_hip_surrogate_integrate", "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs",
             "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
-            "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi"]
+            "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
+            "g1_policy_obs_norm_update_noalloc"]
 
 
 def ptr_array(tensors_or_ptrs):

@@ -31,6 +31,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "g1_model_kernel.h"
 #include "g1_fastmath.h"
 #include "g1_commit.h"
@@ -1021,13 +1023,12 @@ int g1_rollout_init(int nenv, float cmd_lin, float cmd_yaw, unsigned seed, float
   return g1err::launch("g1_rollout_init");
 }
 
-int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
-                              const float* last_ctrl, float* mean, float* var, float* count, float* inv,
-                              double* scratch, void* stream) {
-  g1err::clear();
-  if (nenv <= 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !mean || !var ||
-      !count || !inv || !scratch)
-    return g1err::fail("g1_policy_obs_norm_update: invalid arguments");
+// The two normaliser kernels on a workspace of 2 * NORM_CHUNKS * nobs + 1
+// doubles: every slot the merge reads is written by the batch pass of the
+// same call, so nothing needs clearing beforehand.
+static int obs_norm_launch(const char* who, int nenv, int layout, const float* qpos, const float* qvel,
+                           const float* commands, const float* last_ctrl, float* mean, float* var, float* count,
+                           float* inv, double* scratch, void* stream) {
   const int nobs = layout == 0 ? 99 : 96;
   const int chunk = (nenv + NORM_CHUNKS - 1) / NORM_CHUNKS;
   const int nchunks = (nenv + chunk - 1) / chunk;
@@ -1035,7 +1036,68 @@ int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const flo
                      chunk, qpos, qvel, commands, last_ctrl, mean, count, scratch);
   hipLaunchKernelGGL(obs_norm_merge_kernel, dim3(nobs), dim3(64), 0, (hipStream_t)stream, nenv, nobs, nchunks, scratch,
                      mean, var, count, inv);
-  return g1err::launch("g1_policy_obs_norm_update");
+  return g1err::launch(who);
+}
+
+// Workspace of g1_policy_obs_norm_update_noalloc, one copy per GPU for the
+// life of the process: the per-(feature, chunk) sums of the widest layout
+// (99 features) and the running count, 2 * 64 * 99 + 1 doubles (99 KiB).
+// In host code `scratch` only names the symbol; its device address on the
+// current GPU comes from hipGetSymbolAddress.
+constexpr int NORM_MAX_OBS = 99;
+constexpr size_t NORM_SCRATCH_DOUBLES = 2 * (size_t)NORM_CHUNKS * NORM_MAX_OBS + 1;
+static __device__ __align__(256) double scratch[NORM_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];   // device address of `scratch` on each GPU, set once
+
+// Device address of `scratch` on the current GPU (looked up on the first
+// call on that GPU), or null with the error text set.
+static double* norm_scratch(const char* who) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    g1err::fail("%s: no current HIP device", who);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p) {
+      g1err::fail("%s: hipGetSymbolAddress(scratch) failed on device %d: %s", who, dev, hipGetErrorString(err));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const float* qvel, const float* commands,
+                              const float* last_ctrl, float* mean, float* var, float* count, float* inv,
+                              double* scratch_ws, void* stream) {
+  g1err::clear();
+  if (nenv <= 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !mean || !var ||
+      !count || !inv || !scratch_ws)
+    return g1err::fail("g1_policy_obs_norm_update: invalid arguments");
+  return obs_norm_launch("g1_policy_obs_norm_update", nenv, layout, qpos, qvel, commands, last_ctrl, mean, var,
+                         count, inv, scratch_ws, stream);
+}
+
+int g1_policy_obs_norm_update_noalloc(int nenv, int layout, const float* qpos, const float* qvel,
+                                      const float* commands, const float* last_ctrl, float* mean, float* var,
+                                      float* count, float* inv, void* stream) {
+  static const char* who = "g1_policy_obs_norm_update_noalloc";
+  g1err::clear();
+  if (nenv <= 0 || (layout != 0 && layout != 1) || !qpos || !qvel || !commands || !last_ctrl || !mean || !var ||
+      !count || !inv)
+    return g1err::fail("%s: invalid arguments", who);
+  const int nobs = layout == 0 ? 99 : 96;
+  if (2 * (size_t)NORM_CHUNKS * nobs + 1 > NORM_SCRATCH_DOUBLES)
+    return g1err::fail("%s: %d features exceed the static workspace of %d", who, nobs, NORM_MAX_OBS);
+  double* ws = norm_scratch(who);
+  if (!ws) return -1;
+  return obs_norm_launch(who, nenv, layout, qpos, qvel, commands, last_ctrl, mean, var, count, inv, ws, stream);
 }
 
 }  // extern "C"

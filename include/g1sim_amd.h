@@ -270,6 +270,19 @@ int g1_policy_obs_norm_update(int nenv, int layout, const float* qpos, const flo
                               const float* last_ctrl, float* mean, float* var, float* count, float* inv,
                               double* scratch, void* stream);
 
+/* g1_policy_obs_norm_update without the scratch argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc); it has no counterpart in the reference.
+ * Its workspace is a static device array of 2 * 64 * 99 + 1 doubles (99 KiB,
+ * the widest layout) that each GPU holds until the process ends; its address
+ * on the current device is looked up on the first call there.  It allocates
+ * nothing.  Every call on one GPU shares that workspace: calls that overlap
+ * on one GPU (concurrent streams, or host threads on one device) race.
+ * The current device must be the one the buffers and stream belong to. */
+int g1_policy_obs_norm_update_noalloc(int nenv, int layout, const float* qpos, const float* qvel,
+                                      const float* commands, const float* last_ctrl, float* mean, float* var,
+                                      float* count, float* inv, void* stream);
+
 /* Reward and termination (examples/train.py:143-225) of the post-step state,
  * then the rollout bookkeeping (examples/train.py:319-351): prev_actions <-
  * actions (0 for reset envs), command_timeout - 1, episode_length + 1,
