@@ -166,12 +166,28 @@ class PolicyRollout:
         bookkeeping and resets.  reward_sum / done_count: optional float64
         device tensors (one element) that the post-step kernel increments by
         this step's summed reward and number of terminations."""
-        lib, s, n, t = self.lib, self.stream, self.nenv, self.t
-        if self.update_norm:                      # act(obs) updates the normaliser first (examples/train.py:86-88)
+        self.act_phase()
+        self.physics_phase()
+        self.post_phase(reward_sum, done_count)
+        self.reset_phase()
+        self.t += 1
+
+    # The four launches of a step, in order (step() runs them; they are
+    # separate so a caller can observe the state between them).
+    def act_phase(self):
+        """Normaliser update (act(obs) updates it first, examples/train.py:86-88),
+        then the policy: actions, ctrl and pushes."""
+        n = self.nenv
+        if self.update_norm:
             self.policy.update_obs_norm(n, self.qpos, self.qvel, self.commands, self.ctrl)
         self.policy.act(n, self.qpos, self.qvel, self.commands, self.ctrl, self.ctrl, actions=self.actions,
-                        push_interval=self.push_interval if self.pushes else None, seed=self.seed, t=t,
+                        push_interval=self.push_interval if self.pushes else None, seed=self.seed, t=self.t,
                         sample=self.sample)
+
+    def physics_phase(self):
+        """nsub physics steps: the fused FP32 ABA step, or the surrogate with
+        its judge; the feet and contact forces are left for the reward."""
+        lib, s, n = self.lib, self.stream, self.nenv
         if self.stepper is None:
             rc = lib.g1_hip_step_ex(n, self.nsub, self.dt, self.qpos.data_ptr(), self.qvel.data_ptr(),
                                     self.anchor.data_ptr(), self.ctrl.data_ptr(), self.foot_pos.data_ptr(),
@@ -181,6 +197,10 @@ class PolicyRollout:
         else:
             for _ in range(self.nsub):
                 self.stepper.step(self.qpos, self.qvel, self.anchor, self.ctrl)
+
+    def post_phase(self, reward_sum=None, done_count=None):
+        """Reward, termination, timers, commands and the reset mask."""
+        lib, s, n, t = self.lib, self.stream, self.nenv, self.t
         rc = lib.g1_rollout_post(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.foot_pos.data_ptr(),
                                  self.foot_vel.data_ptr(), self.contact_forces.data_ptr(), self.actions.data_ptr(),
                                  self.commands.data_ptr(), self.prev_actions.data_ptr(),
@@ -191,11 +211,14 @@ class PolicyRollout:
                                  None if reward_sum is None else reward_sum.data_ptr(),
                                  None if done_count is None else done_count.data_ptr(), s)
         _lib.check(rc, "g1_rollout_post")
+
+    def reset_phase(self):
+        """Resets the envs of the device mask (and returns them to the
+        surrogate's fallback)."""
         if self.stepper is not None:              # a reset env is stepped by ABA until the judge accepts it
             self.stepper.mark_reset(self.reset)
-        lib.g1_cuda_reset_done(n, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
-                               self.reset.data_ptr(), self._seed(t), 0.1, 0.02, s)
-        self.t += 1
+        self.lib.g1_cuda_reset_done(self.nenv, self.qpos.data_ptr(), self.qvel.data_ptr(), self.anchor.data_ptr(),
+                                    self.reset.data_ptr(), self._seed(self.t), 0.1, 0.02, self.stream)
 
     def rollout(self, steps=24, reward_sums=None, done_counts=None):
         """examples/train.py:284-355 without the buffer writes: prev_actions

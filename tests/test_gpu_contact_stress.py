@@ -1,8 +1,16 @@
 """Contact-stress rollout (131,072 envs x 1,000 steps at dt 2e-3, targets
 resampled every 10 steps, a horizontal push every 100 steps, per-env friction
 U(0.4, 1.2)): all-FP32 against the FP32 step with the float64 islands, each
-measured for drift against FP64 oracle rollouts of a 256-env sample from the
-same float32 start, and for throughput."""
+
+* checked per step: every 100 steps (after the push) one step of the whole
+  batch with the qacc output, and 4,096 sampled envs of it scored against the
+  FP64 oracle from the same float32 state (teacher forcing) at the float32
+  bar: at most twice the error of the reference algorithm evaluated in
+  float32 on the same states (max, p99, median), envs at a contact
+  threshold counted and excluded;
+* measured for drift against FP64 oracle rollouts of a 256-env sample from
+  the same float32 start (the float64 islands must not make it worse);
+* timed."""
 import numpy as np
 import pytest
 import torch
@@ -10,11 +18,12 @@ import torch
 from g1sim_amd import _lib, workloads as W
 from oracle import oracle as O
 
-from .gpu_helpers import GpuState, write_metrics
+from .gpu_helpers import GpuState, branch_flips, f32_reference_step, rel_err, write_metrics
 
 pytestmark = pytest.mark.gpu
 
-N, STEPS, NSAMPLE = 131072, 1000, 256
+N, STEPS, NSAMPLE, NCHECK = 131072, 1000, 256, 4096
+F32_FACTOR, FLIP_FRAC = 2.0, 0.01
 MODES = {"fp32": 0,
          "fp64_solve+base": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE,
          "fp64_solve+base+contact": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE | _lib.G1_FLAG_F64_CONTACT}
@@ -55,7 +64,35 @@ def test_contact_stress_drift_and_throughput(device):
     torch.cuda.synchronize()
     metrics = {"nenv": N, "steps": STEPS, "sample": int(envs.size)}
     finals = {}
-    def rollout(flags):
+    check_envs = np.arange(0, N, N // NCHECK)
+    check_idx = torch.as_tensor(check_envs, device=device)
+    mu_check = mu[check_envs]
+
+    def check_step(st, ctrl, flags, acc):
+        """One step of the whole batch from the rollout's current state (on
+        copies), scored on the sampled envs against the oracle."""
+        qpos, qvel, anchor = st.qpos.clone(), st.qvel.clone(), st.anchor.clone()
+        z = lambda d: torch.zeros((d, N), device=device)  # noqa: E731
+        qacc, fn = z(35), z(8)
+        before = {"qpos": st.qpos[:, check_idx].cpu().numpy(), "qvel": st.qvel[:, check_idx].cpu().numpy(),
+                  "anchor": st.anchor[:, check_idx].cpu().numpy(), "ctrl": ctrl[:, check_idx].cpu().numpy()}
+        rc = lib.g1_hip_step_ex(N, 1, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
+                                None, None, fn.data_ptr(), qacc.data_ptr(), None, st.mu.data_ptr(), None,
+                                flags & ~_lib.G1_FLAG_F64_BASE, torch.cuda.current_stream(device).cuda_stream)
+        _lib.check(rc)
+        g = {"qacc": qacc[:, check_idx].cpu().numpy(), "fn": fn[:, check_idx].cpu().numpy(),
+             "anchor": anchor[:, check_idx].cpu().numpy()}
+        ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], 2e-3,
+                               mu=mu_check, want=("qacc", "fn", "anchor"))
+        f32 = f32_reference_step(before, 2e-3, mu_check)
+        flip = branch_flips(g["fn"], ref["fn"], g["anchor"], ref["anchor"])
+        flip32 = branch_flips(f32["fn"], ref["fn"], f32["anchor"], ref["anchor"])
+        acc["flips"] += int(flip.sum())
+        acc["checked"] += NCHECK
+        acc["gpu"].append(rel_err(g["qacc"][:, ~flip], ref["qacc"][:, ~flip]).max(axis=0))
+        acc["f32"].append(rel_err(f32["qacc"][:, ~flip32], ref["qacc"][:, ~flip32]).max(axis=0))
+
+    def rollout(flags, acc=None):
         st = GpuState(qp, qv, an, cb[0], device, mu=mu)
         if flags & _lib.G1_FLAG_F64_BASE:
             st.enable_f64_base()
@@ -67,6 +104,8 @@ def test_contact_stress_drift_and_throughput(device):
             s = blk * 10
             if s % 100 == 0:
                 st.qvel[0:2] += push_dev[s // 100]
+                if acc is not None:
+                    check_step(st, ctrl_dev[blk], flags, acc)
             rc = lib.g1_hip_step_ex(N, 10, 2e-3, st.qpos.data_ptr(), st.qvel.data_ptr(), st.anchor.data_ptr(),
                                     ctrl_dev[blk].data_ptr(), None, None, None, None, None, st.mu.data_ptr(),
                                     None if st.base64 is None else st.base64.data_ptr(), flags, stream.cuda_stream)
@@ -75,8 +114,16 @@ def test_contact_stress_drift_and_throughput(device):
         torch.cuda.synchronize()
         return st, e0.elapsed_time(e1)
 
+    per_step = {}
     for name, flags in MODES.items():
-        rollout(flags)                  # the GPU idled through the CPU rollouts: bring its clocks up first
+        acc = {"flips": 0, "checked": 0, "gpu": [], "f32": []}
+        rollout(flags, acc)             # per-step checks (this run also brings the GPU clocks up)
+        e, e32 = np.concatenate(acc["gpu"]), np.concatenate(acc["f32"])
+        per_step[name] = {"qacc_rel_max": float(e.max()), "qacc_rel_p99": float(np.percentile(e, 99)),
+                          "qacc_rel_median": float(np.median(e)), "frac_le_1e-5": float((e <= 1e-5).mean()),
+                          "f32ref_qacc_rel_max": float(e32.max()), "f32ref_qacc_rel_p99": float(np.percentile(e32, 99)),
+                          "f32ref_qacc_rel_median": float(np.median(e32)),
+                          "flip_frac": acc["flips"] / acc["checked"], "checked_env_steps": acc["checked"]}
         st, ms = rollout(flags)
         q = st.qpos.cpu().numpy()
         assert np.isfinite(q).all(), name
@@ -98,7 +145,16 @@ def test_contact_stress_drift_and_throughput(device):
     # amplified by contact chaos over 1,000 steps)
     d_isl = np.linalg.norm(finals["fp32"][0:3] - finals["fp64_solve+base"][0:3], axis=0)
     metrics["fp32_vs_islands_base_pos_m_median"] = float(np.median(d_isl))
+    metrics["per_step_teacher_forced"] = per_step
     write_metrics("contact_stress_131072x1000", metrics)
+    for name, m in per_step.items():
+        assert m["flip_frac"] < FLIP_FRAC, name
+        for q in ("max", "p99", "median"):
+            assert m["qacc_rel_" + q] <= F32_FACTOR * m["f32ref_qacc_rel_" + q], (name, q)
+    # the islands must not make the rollout drift worse (a 256-env sample of
+    # chaotic falls: 1.5x headroom for sampling; measured 0.5x at the median)
+    for q in ("median", "p90"):
+        assert metrics["fp64_solve+base"]["drift_base_pos_m"][q] <= 1.5 * metrics["fp32"]["drift_base_pos_m"][q], q
     # only the foot spheres collide (src/g1_model.h:339-350), so a fallen
     # robot's pelvis may end below the plane; the sample must still track the
     # FP64 rollouts for most envs
