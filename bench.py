@@ -3,21 +3,32 @@
 
     python bench.py [--gpus 1] [--steps 250] [--warmup 50] [--nenv 16384]
 
-Workload (synthetic, as the randomized-actuation configuration): 16,384 envs
-of the 29-DoF G1, dt = 2e-3, initial base tilt U(+-0.05) rad, base velocity
-U(+-0.5) m/s in x/y, joint qd ~ N(0, 0.3), PD targets stand + U(+-0.3) rad
-resampled every 10 steps (all target blocks are generated beforehand and are
-resident in HBM).  One launch of the step kernel advances all envs by the 10
-steps that share a target block (nsub = 10); the timed region is `--steps`
-physics steps after `--warmup` untimed ones.
+Headline workload (synthetic, the randomized-actuation configuration):
+16,384 envs of the 29-DoF G1, dt = 2e-3, initial base tilt U(+-0.05) rad,
+base velocity U(+-0.5) m/s in x/y, joint qd ~ N(0, 0.3), PD targets
+stand + U(+-0.3) rad resampled every 10 steps (all target blocks are
+generated beforehand and are resident in HBM).  The warmup steps use target
+blocks of their own and the timed region starts on a block boundary, so one
+launch of the step kernel advances all envs by the 10 steps of one target
+block (nsub = 10) however many warmup steps there are.
 
 Prints one JSON line: throughput in env-steps/s, the roofline of the step
 kernel (algorithmic FP32 FLOP of the reference formulation, 41,491 per
 env-step, over the kernel's average launch time from HIP events on the launch
-stream), the same FP32 ABA step at 131,072 envs (`aba_131072`), the
-surrogate and policy-in-the-loop legs, and the CPU baseline (the reference
-physics core compiled from the reference sources, float, OpenMP over the host
-cores) timed on a bounded sample of the same workload.
+stream), and these legs:
+* aba_131072 -- the same FP32 ABA step at 131,072 envs;
+* surrogate -- dataset, training (with the training iteration's roofline),
+  held-out accuracy of the bf16 kernels beside an FP32 evaluation of the same
+  weights, and the surrogate + ABA-judge stepper at 16,384 and 131,072 envs
+  (each with the fused network kernel's roofline) and its 65,536 x 1,000
+  evaluation with drift against plain ABA;
+* policy_in_the_loop -- 65,536 envs x 24 steps x 50 iterations;
+* cpu_baseline -- the reference physics core (float, OpenMP) on the host
+  cores, on bounded samples of the headline workload and of the drop test;
+* precision_vs_fp64 -- teacher-forced qacc error against the FP64 oracle for
+  the drop test (1,024 envs), randomized actuation (16,384) and contact
+  stress (131,072, all-FP32 and with the float64 islands, plus rollout
+  drift), scored in the CPU leg.
 """
 import argparse
 import json
@@ -40,101 +51,33 @@ PEAK_BF16 = 2.5e15                 # MI355X dense BF16 MFMA
 SURR_FLOP = 1717248                # 106 -> 512 x4 -> 35 MLP forward, per env-step
 POLICY_FLOP = 433408               # 96 -> 512 -> 256 -> 128 -> 29 actor forward, per env-step
 CTRL_PERIOD = 10
+DT = 2e-3
 
 
-def accuracy_vs_fp64(sample):
-    """qacc of one GPU step against the FP64 oracle from the same float32
-    state (teacher forcing): per env the max over dofs of |a - b| / max(|b|, 1)
-    (tests/test_vs_mujoco.py:74-75).  The reference algorithm evaluated in
-    float32 on the CPU is scored the same way: the yardstick of what float32
-    arithmetic reaches on these states."""
-    from oracle import oracle as O
-    before = sample["before"]
-    ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], 2e-3,
-                           want=("qacc",))["qacc"]
-    n = ref.shape[1]
-    q32 = np.zeros((35, n))
-    args = [np.ascontiguousarray(before[k], np.float32) for k in ("qpos", "qvel", "anchor", "ctrl")]
-    O.lib32().g1o_step_batch_f32(n, 2e-3, *(O.fptr(a) for a in args), None, O.dptr(q32), None, None, None, None,
-                                 None, None, None, 0)
-
-    def score(q):
-        e = (np.abs(q - ref) / np.maximum(np.abs(ref), 1.0)).max(axis=0)
-        return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "median": float(np.median(e))}
-
-    return {"metric": "max over dofs of |qacc - qacc_fp64| / max(|qacc_fp64|, 1), per env",
-            "sample": f"{n} envs after the timed region, one step at dt 2e-3 (teacher forced)",
-            "gpu_fp32": score(sample["qacc"]), "reference_algorithm_in_float32": score(q32)}
+def _blocks(steps):
+    return (steps + CTRL_PERIOD - 1) // CTRL_PERIOD
 
 
-def cpu_baseline(nenv_gpu, acc_sample=None):
-    """Reference physics core (src/g1_core.h, float, OpenMP one env per
-    iteration) on the host cores, on a bounded sample of the workload; with
-    acc_sample, also the FP64 oracle's score of the GPU's qacc on it.
-    Returns (baseline, accuracy or None)."""
-    from g1sim_amd import workloads as W
-    from oracle import oracle as O
-    cores = len(os.sched_getaffinity(0))
-    try:
-        fn = O.ref_bench().g1ref_bench_f32
-        kind = "reference"
-    except (OSError, FileNotFoundError):
-        fn = O.lib32().g1o_bench_f32
-        kind = "port"
-    res = {}
-    for threads, nenv, steps in ((cores, 16384, 50), (1, 2048, 25)):
-        qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=1)
-        ct = np.ascontiguousarray(cb[0])
-        t = fn(nenv, steps, 2e-3, O.fptr(qp), O.fptr(qv), O.fptr(an), O.fptr(ct), threads)
-        res[threads] = (nenv * steps / t, nenv, steps)
-    v_all, n_all, s_all = res[cores]
-    v_one, n_one, s_one = res[1]
-    base = {"value": v_all, "unit": "env-steps/s", "cores": cores, "kind": kind,
-            "sample": f"{n_all} envs x {s_all} steps of the same workload (dt 2e-3, fixed targets) "
-                      f"on {cores} threads; 1-thread rate from {n_one} envs x {s_one} steps",
-            "value_1core": v_one}
-    return base, (accuracy_vs_fp64(acc_sample) if acc_sample is not None else None)
-
-
-def aba_leg(dev, nenv, steps, warmup):
-    """The FP32 ABA step on the same randomized-actuation workload at another
-    batch size (the 131,072-env target: the 2-wave split), timed as the main
-    leg: wall clock between synchronisations for env-steps/s, one HIP event
-    pair around the back-to-back launches for the kernel's average duration."""
-    import torch
-    from g1sim_amd import _lib, workloads as W
-    lib = _lib.load()
-    nblocks = (warmup + steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
-    qp, qv, an, cb = W.randomized_actuation(nenv, seed=3, nblocks=nblocks)
-    qpos, qvel, anchor = (torch.as_tensor(x, device=dev) for x in (qp, qv, an))
-    ctrl = torch.as_tensor(cb, device=dev)
-    stream = torch.cuda.current_stream(dev)
-
-    def run(step0, n):
-        for s in range(step0, step0 + n, CTRL_PERIOD):
-            rc = lib.g1_hip_step_ex(nenv, CTRL_PERIOD, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
-                                    ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None, None,
-                                    0, stream.cuda_stream)
-            if rc:
-                raise RuntimeError(_lib.last_error())
-
-    run(0, warmup)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    run(warmup, steps)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    avg_ms = e0.elapsed_time(e1) / (steps // CTRL_PERIOD)
-    achieved = FLOP_PER_ENV_STEP * nenv * CTRL_PERIOD / (avg_ms * 1e-3)
-    return {"nenv": nenv, "steps": steps, "warmup": warmup, "nsub_per_launch": CTRL_PERIOD,
-            "env_steps_per_s": nenv * steps / wall, "avg_launch_ms": avg_ms,
-            "kernel": "g1k::step_kernel<false, false, 2, 0>",
-            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32}}
+def step_launches(lib, nenv, state, ctrl, block0, nsteps, nsub, flags=0, base64=None, sptr=None, outs=None):
+    """Launches of g1_hip_step_ex for nsteps physics steps starting at the
+    first step of target block block0 (nsub divides the 10-step period, so
+    no launch crosses a block).  Returns the substeps of each launch."""
+    from g1sim_amd import _lib
+    qpos, qvel, anchor = state
+    outs = outs or {}
+    subs = []
+    s = 0
+    while s < nsteps:
+        n = min(nsub - s % nsub, nsteps - s)
+        p = lambda k: outs[k].data_ptr() if outs.get(k) is not None else None  # noqa: E731
+        rc = lib.g1_hip_step_ex(nenv, n, DT, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                ctrl[block0 + s // CTRL_PERIOD].data_ptr(), p("foot_pos"), p("foot_vel"), p("fn"),
+                                p("qacc"), None, p("mu"), None if base64 is None else base64.data_ptr(), flags, sptr)
+        if rc:
+            raise RuntimeError(_lib.last_error())
+        subs.append(n)
+        s += n
+    return subs
 
 
 def _event_avg_ms(fn, reps, stream):
@@ -149,55 +92,251 @@ def _event_avg_ms(fn, reps, stream):
     return float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
 
 
-def surrogate_leg(dev, nenv, steps, warmup, train_iters):
-    """The learned surrogate on the same workload: a 20M-transition dataset
-    from the ABA kernel (16,384 envs x 1,221 steps), training by the
-    project's MFMA kernels, held-out accuracy, then surrogate + rotating 1/16
-    ABA judge + per-env fallback (tolerance 1e-3) stepping `steps` steps.
-    Returns (result, trained network)."""
+# ======================================================================= ABA
+def aba_leg(dev, nenv, steps, warmup):
+    """The FP32 ABA step on the randomized-actuation workload at another batch
+    size (the 131,072-env target: the 2-wave split), timed as the headline:
+    wall clock between synchronisations for env-steps/s, one HIP event pair
+    around the back-to-back launches for the kernel's average duration."""
+    import torch
+    from g1sim_amd import _lib, workloads as W
+    lib = _lib.load()
+    wb = _blocks(warmup)
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=3, nblocks=wb + _blocks(steps))
+    state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    ctrl = torch.as_tensor(cb, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    step_launches(lib, nenv, state, ctrl, 0, warmup, CTRL_PERIOD, sptr=stream.cuda_stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    subs = step_launches(lib, nenv, state, ctrl, wb, steps, CTRL_PERIOD, sptr=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_ms = e0.elapsed_time(e1) / len(subs)
+    achieved = FLOP_PER_ENV_STEP * nenv * (steps / len(subs)) / (avg_ms * 1e-3)
+    return {"nenv": nenv, "steps": steps, "warmup": warmup, "nsub_per_launch": CTRL_PERIOD,
+            "env_steps_per_s": nenv * steps / wall, "avg_launch_ms": avg_ms,
+            "kernel": "g1k::step_kernel<false, false, 2, 0>",
+            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32}}
+
+
+# ============================================================ precision (GPU)
+def precision_samples(dev):
+    """GPU side of the per-configuration precision: teacher-forced single
+    steps (qacc, normal forces, anchors) from states of each configuration's
+    rollout, with the input states, as host arrays.  They are scored against
+    the FP64 oracle in the CPU leg (score_precision)."""
+    import torch
+    from g1sim_amd import _lib, workloads as W
+    lib = _lib.load()
+    sptr = torch.cuda.current_stream(dev).cuda_stream
+
+    def one_step(n, state, ctrl, flags, mu=None, idx=None):
+        """One step with outputs from copies of the state; host copies of the
+        inputs and outputs (columns idx only, when given)."""
+        qpos, qvel, anchor = (t.clone() for t in state)
+        z = lambda d: torch.zeros((d, n), dtype=torch.float32, device=dev)  # noqa: E731
+        outs = {"qacc": z(35), "fn": z(8), "mu": mu}
+        cols = (lambda t: t[:, idx]) if idx is not None else (lambda t: t)
+        before = {k: cols(t).cpu().numpy() for k, t in (("qpos", state[0]), ("qvel", state[1]),
+                                                         ("anchor", state[2]), ("ctrl", ctrl))}
+        rc = lib.g1_hip_step_ex(n, 1, DT, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
+                                None, None, outs["fn"].data_ptr(), outs["qacc"].data_ptr(), None,
+                                None if mu is None else mu.data_ptr(), None, flags & ~_lib.G1_FLAG_F64_BASE, sptr)
+        if rc:
+            raise RuntimeError(_lib.last_error())
+        after = {"qacc": cols(outs["qacc"]).cpu().numpy(), "fn": cols(outs["fn"]).cpu().numpy(),
+                 "anchor": cols(anchor).cpu().numpy()}
+        return before, after
+
+    out = {}
+    # drop test: 1,024 envs x 250 steps, every 5th step, every env
+    qp, qv, an, ct = W.drop_test(1024, seed=0)
+    state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    ctrl = torch.as_tensor(ct, device=dev)[None]
+    samples = []
+    for s in range(250):
+        if s % 5 == 0:
+            samples.append(one_step(1024, state, ctrl[0], 0))
+        step_launches(lib, 1024, state, ctrl, 0, 1, 1, sptr=sptr)
+    out["c1"] = {"fp32": samples}
+    # randomized actuation: 16,384 envs x 250 steps, every 25th step, every
+    # env; the float64 contact-geometry + solve islands on the same states
+    # for the stage decomposition
+    qp, qv, an, cb = W.randomized_actuation(16384, seed=1, nblocks=25)
+    state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    ctrl = torch.as_tensor(cb, device=dev)
+    f32s, f64c = [], []
+    islands = _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_CONTACT
+    for s in range(250):
+        if s % 25 == 0:
+            f32s.append(one_step(16384, state, ctrl[s // 10], 0))
+            f64c.append(one_step(16384, state, ctrl[s // 10], islands))
+        step_launches(lib, 16384, state, ctrl, s // 10, 1, 1, sptr=sptr)
+    out["c2"] = {"fp32": f32s, "fp64_solve+contact": f64c}
+    # contact stress: 131,072 envs x 1,000 steps, pushes every 100 steps,
+    # per-env friction; each mode is its own rollout; every 100 steps (after
+    # the push) 4,096 sampled envs are scored; the final state of 256 envs
+    # for the drift against FP64 rollouts
+    N = 131072
+    qp, qv, an, cb, mu, pushes = W.contact_stress(N, seed=2, nblocks=100, npush=10)
+    ctrl = torch.as_tensor(cb, device=dev)
+    mu_d = torch.as_tensor(mu, device=dev)
+    push_d = torch.as_tensor(pushes, device=dev)
+    check = torch.as_tensor(np.arange(0, N, N // 4096), device=dev)
+    drift_envs = np.arange(0, N, N // 256)
+    modes = {"fp32": 0, "fp64_solve+base": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE,
+             "fp64_solve+base+contact": _lib.G1_FLAG_F64_SOLVE | _lib.G1_FLAG_F64_BASE | _lib.G1_FLAG_F64_CONTACT}
+    c3 = {"modes": {}, "drift_envs": drift_envs}
+    for name, flags in modes.items():
+        state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+        base64 = state[0][:7].double().contiguous() if flags & _lib.G1_FLAG_F64_BASE else None
+        samples = []
+        for blk in range(100):
+            if blk % 10 == 0:
+                state[1][0:2] += push_d[blk // 10]
+                samples.append(one_step(N, state, ctrl[blk], flags, mu=mu_d, idx=check))
+            step_launches(lib, N, state, ctrl, blk, CTRL_PERIOD, CTRL_PERIOD, flags=flags, base64=base64, sptr=sptr,
+                          outs={"mu": mu_d})
+        c3["modes"][name] = {"samples": samples, "final_qpos": state[0][:, torch.as_tensor(drift_envs, device=dev)]
+                             .cpu().numpy().astype(np.float64)}
+    c3["mu_check"] = mu[np.arange(0, N, N // 4096)]
+    out["c3"] = c3
+    torch.cuda.synchronize()
+    return out
+
+
+# ================================================================ surrogate
+def _fp32_heldout(mlp, ds, idx, tol):
+    """The trained network evaluated in float32 (torch GEMMs, no reduced
+    precision) on the same held-out samples as the bf16 kernels: fp32 master
+    weights with fp32 inputs, bf16-rounded weights with fp32 inputs, and fp32
+    weights with bf16-rounded inputs -- where the accuracy floor sits."""
+    import torch
+    from g1sim_amd.surrogate import NIN, NOUT, PIN, rel_err_stats
+    torch.backends.cuda.matmul.allow_tf32 = False
+    st, en = idx[:, 0].long(), idx[:, 1].long()
+    X = ds.X[st, :, en]                                   # (B, 106) physical inputs
+    Y = ds.Y[st, :, en]                                   # (B, 35) ABA qacc
+    xz = torch.zeros((X.shape[0], PIN), dtype=torch.float32, device=X.device)
+    xz[:, :NIN] = (X - mlp.xmean) * mlp.xinv
+    ystd = 1.0 / mlp.yinv
+
+    def run(x, wround):
+        h = x
+        L = len(mlp.W)
+        for li in range(L):
+            W = mlp.W[li].to(torch.bfloat16).float() if wround else mlp.W[li]
+            z = h @ W.t() + mlp.b[li]
+            h = torch.nn.functional.silu(z) if li < L - 1 else z
+        q = h[:, :NOUT] * ystd + mlp.ymean
+        err = ((q - Y).abs() / Y.abs().clamp_min(1.0)).max(dim=1).values
+        return rel_err_stats(err.double().cpu().numpy(), tol)
+
+    return {"fp32_weights_fp32_inputs": run(xz, False), "bf16_weights_fp32_inputs": run(xz, True),
+            "fp32_weights_bf16_inputs": run(xz.to(torch.bfloat16).float(), False)}
+
+
+def _stepper_run(dev, mlp, nenv, steps, warmup, tol, seed):
+    """Surrogate + judge stepper on the randomized-actuation workload:
+    (wall seconds of the timed steps, summary)."""
     import torch
     from g1sim_amd import workloads as W
-    from g1sim_amd.surrogate import MLP, SurrogateStepper, evaluate, fit_normalization, generate_dataset, train
+    from g1sim_amd.surrogate import SurrogateStepper
+    nb = _blocks(warmup) + _blocks(steps)
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nb)
+    ctrl = torch.as_tensor(cb, device=dev)
+    qpos, qvel, anchor = (torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
+    for s in range(warmup):
+        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+    wb = _blocks(warmup)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for s in range(steps):
+        st.step(qpos, qvel, anchor, ctrl[wb + s // CTRL_PERIOD])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    summ = st.summary()
+    del st
+    return wall, summ, (qpos, qvel)
+
+
+def _network_roofline(dev, mlp, nenv, qpos, qvel):
+    """The fused network kernel (features + 5 layers) alone at nenv, HIP
+    events."""
+    import torch
+    tau = torch.randn((35, nenv), device=dev) * 20.0
+    pred = torch.empty((nenv, 64), dtype=torch.float32, device=dev)
+    fwd = lambda: mlp.forward_state(nenv, qpos, qvel, tau, pred)  # noqa: E731
+    fwd()
+    ms = _event_avg_ms(fwd, 50, torch.cuda.current_stream(dev))
+    tf = SURR_FLOP * nenv / (ms * 1e-3)
+    return {"bound": "mfma", "kernel": "g1inf::surrogate_kernel<0> (features + 5 layers)", "nenv": nenv,
+            "achieved": tf / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": tf / PEAK_BF16,
+            "avg_launch_ms": ms, "flop_per_launch": SURR_FLOP * nenv}
+
+
+def surrogate_leg(dev, nenv, steps, warmup, train_iters):
+    """The learned surrogate: a 20M-transition dataset from the ABA kernel
+    (16,384 envs x 1,221 steps), training by the project's MFMA kernels,
+    held-out accuracy, then the surrogate + rotating 1/16 ABA judge +
+    per-env fallback (tolerance 1e-3) at 16,384 and 131,072 envs, and the
+    65,536 x 1,000 evaluation.  Returns (result, trained network)."""
+    import torch
+    from g1sim_amd import _lib, workloads as W
+    from g1sim_amd.surrogate import (MLP, SurrogateStepper, evaluate, fit_normalization, generate_dataset,
+                                     heldout_indices, train)
     t0 = time.perf_counter()
     ds = generate_dataset(dev, nenv=16384, steps=1221, seed=5)
     mlp = MLP(dev, seed=0)
     fit_normalization(mlp, ds)
-    hist = train(mlp, ds, iters=train_iters, batch=8192, lr=1e-3)
+    timing = {}
+    batch = 8192
+    hist = train(mlp, ds, iters=train_iters, batch=batch, lr=1e-3, timing=timing)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     acc = evaluate(mlp, ds, nsamp=65536, tol=1e-3)
+    floor = _fp32_heldout(mlp, ds, heldout_indices(ds, nsamp=65536), 1e-3)
     del ds
     torch.cuda.empty_cache()
-    nblocks = (warmup + steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
-    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
-    ctrl = torch.as_tensor(cb, device=dev)
+    it_ms = timing.get("avg_iteration_ms", float("nan"))
+    train_flop = 3 * SURR_FLOP * batch
+    train_roof = {"bound": "mfma", "what": "one graph-replayed training iteration (sample, gather, forward, "
+                  "MSE gradient, backward, Adam) at 8,192 samples", "avg_iteration_ms": it_ms,
+                  "flop_per_iteration": train_flop, "achieved": train_flop / (it_ms * 1e-3) / 1e12,
+                  "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": train_flop / (it_ms * 1e-3) / PEAK_BF16}
 
-    def run(tol):
-        qpos = torch.as_tensor(qp, device=dev)
-        qvel = torch.as_tensor(qv, device=dev)
-        anchor = torch.as_tensor(an, device=dev)
-        st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
-        for s in range(warmup):
-            st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for s in range(warmup, warmup + steps):
-            st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
-        torch.cuda.synchronize()
-        return time.perf_counter() - t1, st.summary(), (qpos, qvel, st.tau)
-
-    wall, summ, _ = run(1e-3)
-    # the same pipeline with every judgement accepted: the cost of the
-    # surrogate path itself, independent of the network's accuracy
-    wall_acc, _, (qpos, qvel, tau) = run(1e30)
-    # looser acceptance tolerances than the configuration's 1e-3, for context
-    sweep = []
-    for tol in (1e-1, 1.0):
-        w_t, s_t, _ = run(tol)
-        sweep.append({"tol": tol, "env_steps_per_s": nenv * steps / w_t, "aba_frac": s_t["aba_frac"],
-                      "reject_frac": s_t["reject_frac"], "fallback_envs_at_end": s_t["fallback_envs_now"]})
-    # the surrogate's own evaluation configuration: 65,536 envs x 1,000 steps,
-    # 4,096 judged per step, state drift against plain ABA from the same start
+    res = {"judge": "rotating 1/16 of envs per step (blocks of 64), per-env fallback at rel. qacc err > 1e-3; "
+                    "every env starts in fallback until a judge accepts it",
+           "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps", "train_iters": train_iters,
+           "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train, "train_roofline": train_roof,
+           "heldout_bf16_kernels": acc, "heldout_fp32_evaluation": floor}
+    # the stepper at the headline size and at 131,072 envs; tolerance 1e-3,
+    # then every judgement accepted (tolerance 1e30: the cost of the
+    # surrogate path itself, independent of the network's accuracy)
+    for n, key in ((nenv, "stepper_16384"), (131072, "stepper_131072")):
+        wall, summ, (qpos, qvel) = _stepper_run(dev, mlp, n, steps if n == nenv else 100, max(warmup, 20), 1e-3, 1)
+        wall_acc, summ_acc, _ = _stepper_run(dev, mlp, n, steps if n == nenv else 100, max(warmup, 20), 1e30, 1)
+        k = steps if n == nenv else 100
+        res[key] = {"nenv": n, "steps": k, "env_steps_per_s": n * k / wall, "ms_per_step": wall * 1e3 / k,
+                    "aba_frac": summ["aba_frac"], "reject_frac": summ["reject_frac"],
+                    "fallback_envs_at_end": summ["fallback_envs_now"],
+                    "all_accepted": {"env_steps_per_s": n * k / wall_acc, "ms_per_step": wall_acc * 1e3 / k,
+                                     "aba_frac": summ_acc["aba_frac"]},
+                    "network_roofline": _network_roofline(dev, mlp, n, qpos, qvel)}
+        del qpos, qvel
+        torch.cuda.empty_cache()
+    res["value"] = res["stepper_16384"]["env_steps_per_s"]
+    res["unit"] = "env-steps/s"
+    # the evaluation configuration: 65,536 envs x 1,000 steps, 4,096 judged
+    # per step, state drift against plain ABA from the same start
+    lib = _lib.load()
     n4 = 65536
     qp4, qv4, an4, cb4 = W.randomized_actuation(n4, seed=4, nblocks=100)
     ctrl4 = torch.as_tensor(cb4, device=dev)
@@ -210,45 +349,20 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
         st4.step(sq, sv, sa, ctrl4[s // CTRL_PERIOD])
     torch.cuda.synchronize()
     wall4 = time.perf_counter() - t4
-    from g1sim_amd import _lib
-    lib = _lib.load()
-    for blk in range(100):
-        rc = lib.g1_hip_step_ex(n4, CTRL_PERIOD, 2e-3, aq.data_ptr(), av.data_ptr(), aa.data_ptr(),
-                                ctrl4[blk].data_ptr(), None, None, None, None, None, None, None, 0,
-                                torch.cuda.current_stream(dev).cuda_stream)
-        if rc:
-            raise RuntimeError(_lib.last_error())
+    step_launches(lib, n4, (aq, av, aa), ctrl4, 0, 1000, CTRL_PERIOD, sptr=torch.cuda.current_stream(dev).cuda_stream)
     torch.cuda.synchronize()
     s4 = st4.summary()
     dpos = (sq[0:3] - aq[0:3]).norm(dim=0).cpu().numpy()
-    eval4 = {"nenv": n4, "steps": 1000, "judged_per_step": n4 // 16, "env_steps_per_s": n4 * 1000 / wall4,
-             "aba_frac": s4["aba_frac"], "reject_frac": s4["reject_frac"],
-             "fallback_envs_at_end": s4["fallback_envs_now"],
-             "base_pos_drift_vs_aba_m": {"median": float(np.median(dpos)), "max": float(dpos.max())}}
-    del sq, sv, sa, aq, av, aa, ctrl4, st4
-    # the fused network kernel alone (features + 5 layers), HIP events
-    pred = torch.empty((nenv, 64), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    fwd = lambda: mlp.forward_state(nenv, qpos, qvel, tau, pred)  # noqa: E731
-    fwd()
-    k_ms = _event_avg_ms(fwd, 50, stream)
-    k_tflops = SURR_FLOP * nenv / (k_ms * 1e-3) / 1e12
-    return {"value": nenv * steps / wall, "unit": "env-steps/s", "ms_per_step": wall * 1e3 / steps,
-            "judge": "rotating 1/16 of envs per step, per-env fallback at rel. qacc err > 1e-3",
-            "heldout_accept_frac": acc["accept_frac"], "heldout_median_rel_err": acc["median_rel_err"],
-            "heldout_p90_rel_err": acc["p90_rel_err"], "heldout_accept_frac_at": acc["accept_frac_at"],
-            "tolerance_sweep": sweep, "train_iters": train_iters,
-            "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train,
-            "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps",
-            "value_all_accepted": nenv * steps / wall_acc, "ms_per_step_all_accepted": wall_acc * 1e3 / steps,
-            "aba_frac": summ["aba_frac"], "reject_frac": summ["reject_frac"],
-            "fallback_envs_at_end": summ["fallback_envs_now"], "evaluation_65536x1000": eval4,
-            "roofline": {"bound": "mfma", "kernel": "g1inf::surrogate_kernel<0> (features + 5 layers)",
-                         "achieved": k_tflops, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                         "frac": k_tflops * 1e12 / PEAK_BF16, "avg_launch_ms": k_ms,
-                         "flop_per_launch": SURR_FLOP * nenv}}, mlp
+    res["evaluation_65536x1000"] = {
+        "nenv": n4, "steps": 1000, "judged_per_step": n4 // 16, "env_steps_per_s": n4 * 1000 / wall4,
+        "aba_frac": s4["aba_frac"], "reject_frac": s4["reject_frac"], "fallback_envs_at_end": s4["fallback_envs_now"],
+        "max_judged_rel_err": s4["max_judged_rel_err"],
+        "base_pos_drift_vs_aba_m": {"median": float(np.median(dpos)), "p99": float(np.percentile(dpos, 99)),
+                                    "max": float(dpos.max())}}
+    return res, mlp
 
 
+# =================================================================== policy
 def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
     """Policy in the loop: a randomly initialised actor (96 -> 512 -> 256 ->
     128 -> 29, ELU, fused on MFMA) drives nenv envs for iters rollouts of
@@ -258,7 +372,7 @@ def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
     judge."""
     import torch
     from g1sim_amd.policy import PolicyRollout
-    out = {"nenv": nenv, "steps": steps, "iterations": iters, "nsub": nsub, "dt": 2e-3,
+    out = {"nenv": nenv, "steps": steps, "iterations": iters, "nsub": nsub, "dt": DT,
            "note": "rollouts only: the PPO update between iterations is not part of the loop timed here"}
     stream = torch.cuda.current_stream(dev)
     for physics in ("aba", "surrogate"):
@@ -296,23 +410,165 @@ def policy_leg(dev, mlp, nenv=65536, steps=24, iters=50, nsub=10):
     return out
 
 
+# ===================================================================== CPU
+def _branch_flips(gpu_fn, ref_fn, gpu_anchor, ref_anchor, anchor_tol=1e-4):
+    """Envs where a contact decision of a float step differs from the FP64
+    step on the same input: on/off, or stick/slip."""
+    on = (gpu_fn > 0) != (ref_fn > 0)
+    fa, ra = gpu_anchor >= 1e29, ref_anchor >= 1e29
+    an = (fa != ra) | (~fa & ~ra & (np.abs(gpu_anchor - np.where(ra, 0, ref_anchor)) > anchor_tol))
+    return on.any(axis=0) | an.any(axis=0)
+
+
+def score_precision(samples):
+    """Teacher-forced qacc error of the GPU samples against the FP64 oracle
+    from the same float32 states: per env-step the max over dofs of
+    |a - b| / max(|b|, 1) (tests/test_vs_mujoco.py:74-75), over the env-steps
+    whose contact decisions agree with FP64 (the others are counted as
+    branch flips); the reference algorithm evaluated in float32 on the same
+    states beside it."""
+    from oracle import oracle as O
+
+    def ref_steps(before, mu):
+        ref = O.step_batch_f32(before["qpos"], before["qvel"], before["anchor"], before["ctrl"], DT, mu=mu,
+                               want=("qacc", "fn", "anchor"))
+        n = before["qpos"].shape[1]
+        f32 = {"qacc": np.zeros((35, n)), "fn": np.zeros((8, n)), "anchor": np.zeros((16, n))}
+        args = [np.ascontiguousarray(before[k], np.float32) for k in ("qpos", "qvel", "anchor", "ctrl")]
+        m = None if mu is None else np.ascontiguousarray(mu, np.float32)
+        O.lib32().g1o_step_batch_f32(n, DT, *(O.fptr(a) for a in args), O.fptr(m), O.dptr(f32["qacc"]), None, None,
+                                     O.dptr(f32["anchor"]), O.dptr(f32["fn"]), None, None, None, 0)
+        return ref, f32
+
+    def score(pairs, mu=None, refs=None):
+        errs, flips, total = [], 0, 0
+        errs32, flips32 = [], 0
+        refs_out = []
+        for i, (before, after) in enumerate(pairs):
+            ref, f32 = refs[i] if refs is not None else ref_steps(before, mu)
+            refs_out.append((ref, f32))
+            fl = _branch_flips(after["fn"], ref["fn"], after["anchor"], ref["anchor"])
+            fl32 = _branch_flips(f32["fn"], ref["fn"], f32["anchor"], ref["anchor"])
+            e = (np.abs(after["qacc"] - ref["qacc"]) / np.maximum(np.abs(ref["qacc"]), 1.0)).max(axis=0)
+            e32 = (np.abs(f32["qacc"] - ref["qacc"]) / np.maximum(np.abs(ref["qacc"]), 1.0)).max(axis=0)
+            errs.append(e[~fl]); errs32.append(e32[~fl32])
+            flips += int(fl.sum()); flips32 += int(fl32.sum()); total += fl.size
+        e, e32 = np.concatenate(errs), np.concatenate(errs32)
+
+        def st(x, nflip):
+            return {"max": float(x.max()), "p99": float(np.percentile(x, 99)), "median": float(np.median(x)),
+                    "frac_le_1e-5": float((x <= 1e-5).mean()), "branch_flips": nflip, "env_steps": total}
+        return st(e, flips), st(e32, flips32), refs_out
+
+    out = {"metric": "per env-step max over dofs of |qacc - qacc_fp64| / max(|qacc_fp64|, 1), teacher forced "
+                     "(the oracle steps the GPU's float32 input state in float64); env-steps at a contact "
+                     "threshold (on/off or stick/slip differs from FP64) are counted as branch_flips and excluded",
+           "target": 1e-5}
+    g, r32, _ = score(samples["c1"]["fp32"])
+    out["drop_test_1024x250"] = {"sample": "all 1,024 envs at every 5th of 250 steps", "gpu_fp32": g,
+                                 "reference_algorithm_float32": r32}
+    g, r32, refs = score(samples["c2"]["fp32"])
+    g64, _, _ = score(samples["c2"]["fp64_solve+contact"], refs=refs)
+    out["randomized_actuation_16384x250"] = {"sample": "all 16,384 envs at every 25th of 250 steps",
+                                             "gpu_fp32": g, "gpu_fp64_contact_and_solve": g64,
+                                             "reference_algorithm_float32": r32}
+    c3 = samples["c3"]
+    res = {"sample": "4,096 envs (every 32nd) at steps 0, 100, ..., 900 (after the push) of each mode's own "
+                     "131,072-env rollout; drift: 256 envs (every 512th) after 1,000 steps against FP64 rollouts"}
+    for name, m in c3["modes"].items():
+        g, r32, _ = score(m["samples"], mu=c3["mu_check"])
+        res[name] = {"gpu": g, "reference_algorithm_float32": r32}
+    # rollout drift: FP64 oracle rollouts of the drift sample from the same float32 start
+    from g1sim_amd import workloads as W
+    qp, qv, an, cb, mu, pushes = W.contact_stress(131072, seed=2, nblocks=100, npush=10)
+    envs = c3["drift_envs"]
+    ref = np.zeros((36, envs.size))
+    for i, e in enumerate(envs):
+        q, v, a = qp[:, e].astype(np.float64), qv[:, e].astype(np.float64), an[:, e].astype(np.float64)
+        for blk in range(100):
+            if blk % 10 == 0:
+                v[0:2] += pushes[blk // 10][:, e]
+            r = O.step_full(q, v, cb[blk][:, e], a, 10, DT, mu=float(mu[e]))
+            q, v, a = r["qpos"], r["qvel"], r["anchor"]
+        ref[:, i] = q
+    for name, m in c3["modes"].items():
+        d = np.linalg.norm(m["final_qpos"][0:3] - ref[0:3], axis=0)
+        res[name]["drift_base_pos_m"] = {"median": float(np.median(d)), "p90": float(np.percentile(d, 90)),
+                                         "max": float(d.max())}
+    out["contact_stress_131072x1000"] = res
+    return out
+
+
+def cpu_baseline(samples=None):
+    """The reference physics core (src/g1_core.h, float, OpenMP one env per
+    iteration) on the host cores, on bounded samples: the headline workload
+    (16,384 envs x 50 steps, three repetitions, all threads; 2,048 x 25 on
+    one thread) and the drop test (1,024 envs x 250 steps at dt 2e-3, the
+    configuration the reference runs on its CPU shim), all threads and one
+    thread.  With samples, also the precision scores (score_precision).
+    Returns (baseline, precision or None)."""
+    from g1sim_amd import workloads as W
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    try:
+        fn = O.ref_bench().g1ref_bench_f32
+        kind = "reference"
+    except (OSError, FileNotFoundError):
+        fn = O.lib32().g1o_bench_f32
+        kind = "port"
+
+    def rate(state, steps, threads):
+        qp, qv, an, ct = (np.ascontiguousarray(a, np.float32).copy() for a in state)
+        t = fn(qp.shape[1], steps, DT, O.fptr(qp), O.fptr(qv), O.fptr(an), O.fptr(ct), threads)
+        return qp.shape[1] * steps / t
+
+    qp, qv, an, cb = W.randomized_actuation(16384, seed=1, nblocks=1)
+    reps = [rate((qp, qv, an, cb[0]), 50, cores) for _ in range(3)]
+    qp1, qv1, an1, cb1 = W.randomized_actuation(2048, seed=1, nblocks=1)
+    one = rate((qp1, qv1, an1, cb1[0]), 25, 1)
+    drop = W.drop_test(1024, seed=0)
+    drop_all = rate(drop, 250, cores)
+    drop_one = rate(drop, 250, 1)
+    base = {"value": float(np.median(reps)), "unit": "env-steps/s", "cores": cores, "kind": kind,
+            "sample": f"randomized actuation, 16,384 envs x 50 steps (dt 2e-3, fixed targets) on {cores} threads, "
+                      f"median of 3 (range {min(reps):.3g}-{max(reps):.3g}); 1-thread rate from 2,048 envs x 25 steps",
+            "value_1core": one,
+            "drop_test_1024x250": {"env_steps_per_s": drop_all, "cores": cores, "env_steps_per_s_1core": drop_one,
+                                   "sample": "1,024 envs x 250 steps at dt 2e-3, zero policy action"}}
+    return base, (score_precision(samples) if samples is not None else None)
+
+
+# ================================================================ profiling
 def kernel_only(kind, nenv, steps, warmup):
-    """One fused inference kernel alone, or one surrogate stepper step
-    (judge + forces, network + commit; random weights, every judgement
-    accepted), on the randomized-actuation state, timed with HIP events: for
-    profiling runs."""
+    """One fused inference kernel alone, one surrogate stepper step (judge +
+    forces, network + commit; random weights, every judgement accepted after
+    the warmup), or one training iteration (graph replays), timed with HIP
+    events: for profiling runs."""
     import torch
     from g1sim_amd import workloads as W
     dev = torch.device("cuda:0")
     qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=1)
     qpos, qvel = torch.as_tensor(qp, device=dev), torch.as_tensor(qv, device=dev)
+    if kind == "train":
+        from g1sim_amd.surrogate import MLP, fit_normalization, generate_dataset, train
+        ds = generate_dataset(dev, nenv=8192, steps=40, seed=5)
+        mlp = MLP(dev, seed=0)
+        fit_normalization(mlp, ds)
+        timing = {}
+        train(mlp, ds, iters=max(steps, 2), batch=8192, lr=1e-3, timing=timing)
+        it = timing["avg_iteration_ms"]
+        tf = 3 * SURR_FLOP * 8192 / (it * 1e-3) / 1e12
+        print(json.dumps({"kernel": "training iteration (graph replay)", "batch": 8192, "iterations": steps,
+                          "avg_iteration_ms": it, "achieved_tflops": tf, "frac_bf16_peak": tf * 1e12 / PEAK_BF16}),
+              flush=True)
+        return
     if kind == "stepper":
         from g1sim_amd.surrogate import MLP, SurrogateStepper
         mlp = MLP(dev, seed=0)
         anchor, ctrl = torch.as_tensor(an, device=dev), torch.as_tensor(cb[0], device=dev)
         st = SurrogateStepper(mlp, nenv, dev, period=16, tol=1e30)
         fn = lambda: st.step(qpos, qvel, anchor, ctrl)  # noqa: E731
-        for _ in range(warmup):
+        for _ in range(max(warmup, 16)):
             fn()
         torch.cuda.synchronize()
         ms = _event_avg_ms(fn, steps, torch.cuda.current_stream(dev))
@@ -374,13 +630,15 @@ def measure_peaks():
     return res
 
 
+# ===================================================================== main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=250)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--nenv", type=int, default=16384)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU baseline and the precision scoring against the FP64 oracle")
     ap.add_argument("--flags", type=int, default=0,
                     help="g1_hip_step_ex flags: 1 float64 base solve, 2 float64 base pose, 4 float64 "
                          "foot-contact geometry")
@@ -391,10 +649,11 @@ def main():
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 131,072-env FP32 ABA leg of the default run")
+    ap.add_argument("--no-precision", action="store_true", help="skip the per-configuration precision samples")
     ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
-    ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper"),
-                    help="time one fused inference kernel, or one surrogate stepper step, alone at --nenv "
-                         "(for profiling) and exit")
+    ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper", "train"),
+                    help="time one fused inference kernel, one surrogate stepper step or one training "
+                         "iteration alone at --nenv (for profiling) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -423,46 +682,35 @@ def main():
     base64 = None
     if args.flags & _lib.G1_FLAG_F64_BASE:
         base64 = torch.empty((7, nenv), dtype=torch.float64, device=dev)
-    nblocks = (args.warmup + args.steps + CTRL_PERIOD - 1) // CTRL_PERIOD + 1
-    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=nblocks)
-    qpos = torch.as_tensor(qp, device=dev)
-    qvel = torch.as_tensor(qv, device=dev)
-    anchor = torch.as_tensor(an, device=dev)
+    wb = _blocks(args.warmup)                              # the timed region starts at target block wb
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=1, nblocks=wb + _blocks(args.steps))
+    state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
     ctrl = torch.as_tensor(cb, device=dev)                 # all target blocks resident in HBM
     stream = torch.cuda.current_stream(dev)
     sptr = stream.cuda_stream
-
-    def launch(step0, nsteps):
-        """Launches covering physics steps [step0, step0 + nsteps)."""
-        s = step0
-        while s < step0 + nsteps:
-            n = min(args.nsub - s % args.nsub, step0 + nsteps - s)
-            rc = lib.g1_hip_step_ex(nenv, n, 2e-3, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
-                                    ctrl[s // CTRL_PERIOD].data_ptr(), None, None, None, None, None, None,
-                                    None if base64 is None else base64.data_ptr(), args.flags, sptr)
-            if rc:
-                raise RuntimeError(_lib.last_error())
-            yield n
-            s += n
-
     if base64 is not None:
-        base64.copy_(qpos[:7])
-    for _ in launch(0, args.warmup):
-        pass
+        base64.copy_(state[0][:7])
+    default_shape = args.flags == 0 and nenv == 16384 and args.nsub == CTRL_PERIOD
+    # The 131,072-env leg runs first, so the headline's warmup and timed
+    # steps follow GPU work without an idle gap: after ~0.5 s idle the GPU's
+    # clock ramps up again over the first launches (measured on MI355X: a
+    # 20-step region right after a 5-step warmup averaged 124 us per 10-step
+    # launch from idle, 97-99 us after busy work).
+    if not args.no_large and default_shape:
+        aba131 = aba_leg(dev, 131072, 200, 20)
+    step_launches(lib, nenv, state, ctrl, 0, args.warmup, args.nsub, args.flags, base64, sptr)
     torch.cuda.synchronize()
 
     # timed region: wall clock between synchronisations, and one HIP event
     # pair on the launch stream around its back-to-back launches for the
     # kernel's average duration (an event pair around every launch would put
-    # ~3 us of event processing between the kernels: 1.18e9 instead of
-    # 1.28e9 env-steps/s on the same GPU)
-    gen = launch(args.warmup, args.steps)
+    # ~3 us of event processing between the kernels)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    subs = list(gen)
+    subs = step_launches(lib, nenv, state, ctrl, wb, args.steps, args.nsub, args.flags, base64, sptr)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -472,28 +720,12 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-    sub_arr = np.array(subs)
-    # accuracy sample: one more step of the final state with the qacc output
-    acc_sample = None
-    if not args.no_cpu_baseline:
-        blk = ctrl[(args.warmup + args.steps) // CTRL_PERIOD]
-        before = {"qpos": qpos.cpu().numpy(), "qvel": qvel.cpu().numpy(), "anchor": anchor.cpu().numpy(),
-                  "ctrl": blk.cpu().numpy()}
-        qp2, qv2, an2 = qpos.clone(), qvel.clone(), anchor.clone()
-        qacc = torch.zeros((35, nenv), dtype=torch.float32, device=dev)
-        rc = lib.g1_hip_step_ex(nenv, 1, 2e-3, qp2.data_ptr(), qv2.data_ptr(), an2.data_ptr(), blk.data_ptr(),
-                                None, None, None, qacc.data_ptr(), None, None, None,
-                                args.flags & ~_lib.G1_FLAG_F64_BASE, sptr)
-        if rc:
-            raise RuntimeError(_lib.last_error())
-        acc_sample = {"before": before, "qacc": qacc.cpu().numpy().astype(np.float64)}
-    h = qpos[2].cpu().numpy()
-    ok = bool(np.isfinite(qpos.cpu().numpy()).all() and ((h > 0.2) & (h < 1.2)).mean() > 0.99)
+    h = state[0][2].cpu().numpy()
+    ok = bool(np.isfinite(state[0].cpu().numpy()).all() and ((h > 0.2) & (h < 1.2)).mean() > 0.99)
 
-    env_steps = nenv * args.steps
-    value = env_steps / wall
-    avg_ms = region_ms / len(sub_arr)
-    nsub_avg = float(sub_arr.mean())
+    value = nenv * args.steps / wall
+    avg_ms = region_ms / len(subs)
+    nsub_avg = args.steps / len(subs)
     flop_per_launch = FLOP_PER_ENV_STEP * nenv * nsub_avg
     achieved = flop_per_launch / (avg_ms * 1e-3)
     traffic = None
@@ -519,8 +751,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "randomized actuation: 16,384 envs, dt 2e-3, PD targets stand+U(+-0.3) "
                                "resampled every 10 steps, FP32 ABA step",
-                   "nenv": nenv, "dt": 2e-3, "nsub_per_launch": args.nsub, "path": "FP32 ABA",
-                   "flags": args.flags},
+                   "nenv": nenv, "dt": DT, "nsub_per_launch": args.nsub, "launches": len(subs),
+                   "path": "FP32 ABA", "flags": args.flags},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
                      "kernel": "g1k::step_kernel<%s, %s, %d, 0>" % (
@@ -533,14 +765,20 @@ def main():
     peaks = measure_peaks()
     out["measured_peaks"] = peaks
     out["roofline"]["frac_of_measured_fp32_fma"] = achieved / (peaks["fp32_fma_tflops"] * 1e12)
-    if not args.no_large and args.flags == 0 and nenv == 16384 and args.nsub == CTRL_PERIOD:
-        out["aba_131072"] = aba_leg(dev, 131072, 200, 20)
+    if not args.no_large and default_shape:
+        out["aba_131072"] = aba131
+    samples = None
+    if not args.no_precision and not args.no_cpu_baseline and default_shape:
+        samples = precision_samples(dev)
     if not args.no_surrogate and args.flags == 0:
         out["surrogate"], mlp = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
         if not args.no_policy:
             out["policy_in_the_loop"] = policy_leg(dev, mlp)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"], out["max_qacc_rel_err_vs_fp64"] = cpu_baseline(nenv, acc_sample)
+        out["cpu_baseline"], prec = cpu_baseline(samples)
+        if prec is not None:
+            out["precision_vs_fp64"] = prec
+            out["max_qacc_rel_err_vs_fp64"] = prec["randomized_actuation_16384x250"]["gpu_fp32"]
     print(json.dumps(out), flush=True)
 
 

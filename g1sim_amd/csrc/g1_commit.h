@@ -58,34 +58,60 @@ G1K_DEV void surrogate_judge(bool live, bool judged, bool fb, int e, float tol, 
 }
 
 // Semi-implicit Euler of the floating base (src/g1_core.h:212-222 with the
-// quaternion exponential of src/spatial.h:73-79): qa(j) is qacc j (0-5),
-// qp(m) / qv(m) the env's qpos / qvel row m; rows 0-6 of qpos and 0-5 of
-// qvel ([dim][N]) are written.
-template <typename QA, typename QP, typename QV>
-G1K_DEV void integrate_base(int e, size_t N, float dt, QA qa, QP qp, QV qv, float* qpos, float* qvel) {
-  float vw[3], om[3], bp[3], bq[4];
+// quaternion exponential of src/spatial.h:73-79): velocities += dt qacc, then
+// position += dt v and q <- q (x) exp(w dt / 2), normalised.  qa: qacc 0-5
+// (world linear, body angular); vw / om: world linear / body angular
+// velocity; bp / bq: base position and quaternion (float, or double for the
+// float64 base-pose accumulation).  Every multiply-add is an explicit fma in a
+// fixed order, so the fused step and the surrogate's commit, separate kernel
+// instances, produce the same bits from the same inputs.
+template <typename BT>
+G1K_DEV void euler_base(float dt, const float* qa, float (&vw)[3], float (&om)[3], BT (&bp)[3], BT (&bq)[4]) {
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    vw[m] = qv(m) + dt * qa(m);
-    om[m] = qv(3 + m) + dt * qa(3 + m);
-    bp[m] = qp(m) + dt * vw[m];
+    vw[m] = fmaf(dt, qa[m], vw[m]);
+    om[m] = fmaf(dt, qa[3 + m], om[m]);
+    bp[m] = fma((BT)dt, (BT)vw[m], bp[m]);
   }
-#pragma unroll
-  for (int m = 0; m < 4; ++m) bq[m] = qp(3 + m);
-  const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+  const float wn = fsqrt(fmaf(om[2], om[2], fmaf(om[1], om[1], om[0] * om[0])));
   const float angle = wn * dt;
   if (angle >= 1e-12f) {
     float sh, ch;
     fsincos(0.5f * angle, &sh, &ch);
     const float s = sh * frcp(angle / dt);
-    const float dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
-    const float nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
-    const float nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
-    const float ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
-    const float nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
-    const float inv = frsq(nw * nw + nx * nx + ny * ny + nz * nz);
+    const BT dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
+    const BT nw = fma(-bq[3], dz, fma(-bq[2], dy, fma(-bq[1], dx, bq[0] * dw)));
+    const BT nx = fma(-bq[3], dy, fma(bq[2], dz, fma(bq[1], dw, bq[0] * dx)));
+    const BT ny = fma(bq[3], dx, fma(bq[2], dw, fma(-bq[1], dz, bq[0] * dy)));
+    const BT nz = fma(bq[3], dw, fma(-bq[2], dx, fma(bq[1], dy, bq[0] * dz)));
+    const BT inv = frsq(fma(nz, nz, fma(ny, ny, fma(nx, nx, nw * nw))));
     bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
   }
+}
+
+// Semi-implicit Euler of one hinge: qd += dt qdd, then q += dt qd (the same
+// fma order as euler_base's convention, shared by the step and the commit).
+G1K_DEV void euler_hinge(float dt, float qdd, float& qd, float& q) {
+  qd = fmaf(dt, qdd, qd);
+  q = fmaf(dt, qd, q);
+}
+
+// euler_base on an env's state read through qp(m) / qv(m) (its qpos / qvel
+// row m), written to rows 0-6 of qpos and 0-5 of qvel ([dim][N]).
+template <typename QA, typename QP, typename QV>
+G1K_DEV void integrate_base(int e, size_t N, float dt, QA qa, QP qp, QV qv, float* qpos, float* qvel) {
+  float a[6], vw[3], om[3], bp[3], bq[4];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) a[m] = qa(m);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    vw[m] = qv(m);
+    om[m] = qv(3 + m);
+    bp[m] = qp(m);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) bq[m] = qp(3 + m);
+  euler_base(dt, a, vw, om, bp, bq);
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     qvel[m * N + e] = vw[m];
@@ -99,9 +125,10 @@ G1K_DEV void integrate_base(int e, size_t N, float dt, QA qa, QP qp, QV qv, floa
 // Semi-implicit Euler of hinge h (0-28): qvel row 6 + h, qpos row 7 + h.
 G1K_DEV void integrate_hinge(int h, int e, size_t N, float dt, float qa, float qp, float qv, float* qpos,
                              float* qvel) {
-  const float v = qv + dt * qa;
+  float v = qv, q = qp;
+  euler_hinge(dt, qa, v, q);
   qvel[(6 + h) * N + e] = v;
-  qpos[(7 + h) * N + e] = qp + dt * v;
+  qpos[(7 + h) * N + e] = q;
 }
 
 // The whole commit of one env per lane: qacc = ABA's for judged and fallback

@@ -1061,42 +1061,15 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
       qacc_base[3] = a0[0]; qacc_base[4] = a0[1]; qacc_base[5] = a0[2];
     }
 
-    // ---- semi-implicit Euler (src/g1_core.h:212-222)
+    // ---- semi-implicit Euler (src/g1_core.h:212-222; quaternion exponential
+    // src/spatial.h:73-79), the same bits as the surrogate's commit
+    euler_base(dt, qacc_base, vw, om, bp, bq);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      vw[m] += dt * qacc_base[m];
-      om[m] += dt * qacc_base[3 + m];
-      bp[m] += (BaseT)dt * (BaseT)vw[m];
-    }
-    {
-      // q <- q (x) exp(w dt / 2) (src/spatial.h:73-79)
-      const float wn = fsqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
-      const float angle = wn * dt;
-      if (angle >= 1e-12f) {
-        float sh, ch;
-        fsincos(0.5f * angle, &sh, &ch);
-        const float s = sh * frcp(angle / dt);
-        const BaseT dw = ch, dx = s * om[0], dy = s * om[1], dz = s * om[2];
-        const BaseT nw = bq[0] * dw - bq[1] * dx - bq[2] * dy - bq[3] * dz;
-        const BaseT nx = bq[0] * dx + bq[1] * dw + bq[2] * dz - bq[3] * dy;
-        const BaseT ny = bq[0] * dy - bq[1] * dz + bq[2] * dw + bq[3] * dx;
-        const BaseT nz = bq[0] * dz + bq[1] * dy - bq[2] * dx + bq[3] * dw;
-        const BaseT inv = frsq(nw * nw + nx * nx + ny * ny + nz * nz);
-        bq[0] = nw * inv; bq[1] = nx * inv; bq[2] = ny * inv; bq[3] = nz * inv;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      wqd[m] += dt * qdd_w[m];
-      wq[m] += dt * wqd[m];
-    }
+    for (int m = 0; m < 3; ++m) euler_hinge(dt, qdd_w[m], wqd[m], wq[m]);
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
 #pragma unroll
-      for (int m = 0; m < MAXJ; ++m) {
-        cqd[c][m] += dt * qdd_c[c][m];
-        cq[c][m] += dt * cqd[c][m];
-      }
+      for (int m = 0; m < MAXJ; ++m) euler_hinge(dt, qdd_c[c][m], cqd[c][m], cq[c][m]);
   }
 
   // ---- store

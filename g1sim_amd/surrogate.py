@@ -326,13 +326,15 @@ def fit_normalization(mlp, ds, sample=2_000_000, seed=0):
 
 
 def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=None, seed=0, log=None,
-          graph=True):
+          graph=True, timing=None):
     """Minibatch Adam on the steps [0, steps - holdout), cosine learning-rate
     decay from lr to lr_end; returns (iteration, mean z-scored MSE) every 100
     iterations.  With graph=True the first iteration runs eagerly and the
     rest replay it as one HIP graph (the iteration reads all its changing
     scalars on the device, so the replays are the same computation, bit for
-    bit, as eager iterations)."""
+    bit, as eager iterations).  timing (a dict, optional) receives the
+    average duration of a replayed iteration, from HIP events on the stream
+    around iterations 1-99 (no host synchronisation among them)."""
     dev = ds.X.device
     holdout = holdout_steps if holdout_steps is not None else max(1, ds.steps // 20)
     train_steps = ds.steps - holdout
@@ -350,6 +352,8 @@ def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=N
                 log(f"iter {it} mse {v:.4e}")
 
     g = None
+    ev = None
+    last_timed = min(iters - 1, 99)
     for it in range(iters):
         if g is None:
             run()
@@ -358,15 +362,25 @@ def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=N
                 with torch.cuda.graph(g):
                     run()
         else:
+            if timing is not None and it == 1:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
             g.replay()
+            if ev is not None and it == last_timed:
+                ev[1].record()
         mlp.t += 1
         logged(it)
+    if ev is not None:
+        torch.cuda.synchronize()
+        timing["iterations"] = last_timed
+        timing["avg_iteration_ms"] = ev[0].elapsed_time(ev[1]) / last_timed
+        timing["batch"] = batch
     return hist
 
 
-def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
-    """Held-out accuracy in physical units: rel. qacc error |a - b| / max(|b|, 1)
-    (max over dofs) of the surrogate against ABA; returns a dict."""
+def heldout_indices(ds, steps_from=None, nsamp=65536, seed=1):
+    """(step, env) int32 [B][2] of B = nsamp rounded up to 128 samples drawn
+    from the held-out steps [steps_from, steps) (default: the last 5 %)."""
     dev = ds.X.device
     s0 = steps_from if steps_from is not None else ds.steps - max(1, ds.steps // 20)
     B = (nsamp + 127) // 128 * 128
@@ -374,6 +388,23 @@ def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
     idx = torch.empty((B, 2), dtype=torch.int32, device=dev)
     idx[:, 0] = torch.randint(s0, ds.steps, (B,), device=dev, generator=g, dtype=torch.int32)
     idx[:, 1] = torch.randint(0, ds.nenv, (B,), device=dev, generator=g, dtype=torch.int32)
+    return idx
+
+
+def rel_err_stats(err, tol):
+    """Summary of per-sample rel. qacc errors (numpy float64)."""
+    e = np.asarray(err, np.float64)
+    return {"samples": int(e.size), "accept_frac": float((e <= tol).mean()), "median_rel_err": float(np.median(e)),
+            "p90_rel_err": float(np.percentile(e, 90)), "max_rel_err": float(e.max()), "tol": tol,
+            "accept_frac_at": {str(t): float((e <= t).mean()) for t in (1e-3, 1e-2, 1e-1, 1.0)}}
+
+
+def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
+    """Held-out accuracy in physical units: rel. qacc error |a - b| / max(|b|, 1)
+    (max over dofs) of the surrogate (the bf16 MFMA kernels) against ABA;
+    returns a dict."""
+    idx = heldout_indices(ds, steps_from, nsamp, seed)
+    B = idx.shape[0]
     ws = mlp.workspace(B)
     mlp.gather(ds.X, ds.Y, idx, B, ds.nenv, ws["X16"], ws["T"])
     pred = mlp.forward(ws["X16"], B, train=False)[:, :NOUT]
@@ -381,10 +412,7 @@ def evaluate(mlp, ds, steps_from=None, nsamp=65536, tol=1e-3, seed=1):
     qs = pred * ystd + mlp.ymean
     qa = ws["T"][:, :NOUT] * ystd + mlp.ymean
     err = ((qs - qa).abs() / qa.abs().clamp_min(1.0)).max(dim=1).values
-    e = err.double().cpu().numpy()
-    return {"samples": int(B), "accept_frac": float((e <= tol).mean()), "median_rel_err": float(np.median(e)),
-            "p90_rel_err": float(np.percentile(e, 90)), "max_rel_err": float(e.max()), "tol": tol,
-            "accept_frac_at": {str(t): float((e <= t).mean()) for t in (1e-3, 1e-2, 1e-1, 1.0)}}
+    return rel_err_stats(err.double().cpu().numpy(), tol)
 
 
 # ====================================================================== step

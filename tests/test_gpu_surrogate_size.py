@@ -46,7 +46,7 @@ def _rel_stats(a, b, unset=None):
     return float(srt[srt.numel() // 2]), float(srt[int(0.99 * (srt.numel() - 1))]), float(srt[-1])
 
 
-def _lockstep(device, exact):
+def _lockstep(device, exact, N=N):
     lib = _lib.load()
     s = torch.cuda.current_stream(device).cuda_stream
     qp, qv, an, cb = W.randomized_actuation(N, seed=41, nblocks=(STEPS + 9) // 10)
@@ -103,16 +103,23 @@ def _check_worst(worst):
         assert med <= 1e-6 and p99 <= 1e-4 and mx <= 5e-2, (key, med, p99, mx)
 
 
-def test_never_accepted_network_is_aba_at_65536(device):
-    worst, summ, fb16, drift, finite = _lockstep(device, exact=False)
-    write_metrics("surrogate_never_accepted_65536", {
+@pytest.mark.parametrize("n", [N, 16384])
+def test_never_accepted_network_is_aba(device, n):
+    """At 65,536 envs (the 2-wave split) the fallback path is plain ABA bit
+    for bit: the judge's kernel instance gives the step's qacc and the commit
+    integrates with the step's fma sequence (g1_commit.h euler_base), so
+    the free-running rollouts do not drift apart at all."""
+    worst, summ, fb16, drift, finite = _lockstep(device, exact=False, N=n)
+    write_metrics(f"surrogate_never_accepted_{n}", {
         "steps": STEPS, "worst_per_step_rel_diff_vs_aba (median, p99, max)": worst, "summary": summ,
         "free_running_base_drift_vs_aba_m": {"median": float(np.median(drift)), "p90": float(np.percentile(drift, 90)),
                                              "max": float(drift.max())}})
     assert finite
-    assert summ["judged"] == STEPS * N // 16 and summ["rejected"] == summ["judged"]
-    assert summ["aba_frac"] == 1.0 and summ["fallback_envs_now"] == N and fb16 == N
+    assert summ["judged"] == STEPS * n // 16 and summ["rejected"] == summ["judged"]
+    assert summ["aba_frac"] == 1.0 and summ["fallback_envs_now"] == n and fb16 == n
     _check_worst(worst)
+    if n == N:
+        assert all(v == [0.0, 0.0, 0.0] for v in worst.values()) and drift.max() == 0.0
 
 
 def test_exact_predictor_accepted_at_65536(device):
