@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_g1sim_init(device):
+    """Restates tests/test_g1sim.py:13-20 (shapes and dtype)."""
     env = G1Sim(nenv=10, device=device)
     assert env.qpos.shape == (36, 10)
     assert env.qvel.shape == (35, 10)
@@ -23,6 +24,8 @@ def test_g1sim_init(device):
 
 
 def test_g1sim_reset_all(device):
+    """Restates tests/test_g1sim.py:21-36 (reset overwrites the state; the
+    stand keyframe's 0.79 m pelvis height)."""
     env = G1Sim(nenv=5, device=device)
     env.qpos.fill_(999.0)
     env.qvel.fill_(999.0)
@@ -34,6 +37,8 @@ def test_g1sim_reset_all(device):
 
 
 def test_g1sim_step(device):
+    """Restates tests/test_g1sim.py:38-50 (the state moves); last_actions is
+    ctrl after a step (src/g1sim.py:115)."""
     env = G1Sim(nenv=2, device=device)
     env.reset_all(seed=42, noise=0.0, drop=0.0)
     torch.cuda.synchronize()
@@ -45,6 +50,8 @@ def test_g1sim_step(device):
 
 
 def test_g1sim_selective_reset(device):
+    """Restates tests/test_g1sim.py:52-76 (only the done env is reset, to
+    0.79 + 1.0 m with drop 1.0)."""
     env = G1Sim(nenv=2, device=device)
     env.reset_all(seed=42, noise=0.0, drop=0.0)
     env.step(nsub=50)
@@ -60,6 +67,8 @@ def test_g1sim_selective_reset(device):
 
 
 def test_g1sim_get_obs(device):
+    """Restates tests/test_g1sim.py:78-91 (shape, projected gravity at rest,
+    zero joint offsets at the keyframe)."""
     env = G1Sim(nenv=5, device=device)
     env.reset_all(noise=0.0, drop=0.0)
     obs = env.get_obs()
@@ -69,6 +78,8 @@ def test_g1sim_get_obs(device):
 
 
 def test_g1sim_foot_data(device):
+    """Restates tests/test_g1sim.py:93-119 (foot output shapes, foot heights
+    in (-0.05, 0.15) m after a short stance, some contact force)."""
     env = G1Sim(nenv=2, device=device)
     assert env.foot_pos.shape == (6, 2) and env.foot_vel.shape == (6, 2)
     assert env.contact_forces.shape == (8, 2)
@@ -83,7 +94,10 @@ def test_g1sim_foot_data(device):
 
 def test_g1sim_foot_kinematics_vs_oracle(device):
     """Foot outputs are the pre-step kinematics of bodies 6 and 12
-    (src/g1_kernel.cu:128-145): world origin and world linear velocity."""
+    (src/g1_kernel.cu:128-145): world origin and world linear velocity.  The
+    reference's check (tests/test_g1sim.py:121-169) compares them with
+    MuJoCo at atol 1e-5; here the FP64 oracle's kinematics stand in for
+    MuJoCo, with the same tolerance."""
     env = G1Sim(nenv=64, device=device)
     env.reset_all(seed=42, noise=0.1, drop=0.0)
     env.qvel.normal_(0.0, 0.5)
