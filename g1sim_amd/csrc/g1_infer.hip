@@ -1,10 +1,12 @@
 // g1_infer.hip -- fused network inference for the step loop (gfx950).
 //
-// One workgroup of 8 wave64s takes 64 envs through a whole MLP: it builds
-// the network input from the SoA state, runs every dense layer on
+// One workgroup of 8 wave64s takes 64 envs (128 from 32,768 envs on, in the
+// forward-only surrogate and the policy) through a whole MLP: it builds the
+// network input from the SoA state, runs every dense layer on
 // v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32 accumulation) with the
-// activations held in LDS (two 64 x 520 bf16 buffers, 133 KB), and finishes
-// with a fused epilogue, so no activation ever goes to HBM.  Weights are
+// activations held in LDS (two 64 x 520 bf16 buffers, or one 128 x 520
+// buffer updated in place: 133 KB), and finishes with a fused epilogue, so
+// no activation ever goes to HBM.  Weights are
 // read from L2 (1.7 MB for the surrogate, 0.43 MB for the policy), each
 // lane loading 16 contiguous bytes of a weight row as its B fragment, several
 // k-steps ahead of its use; a layer's first k-steps are loaded before the
@@ -45,36 +47,56 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int ROWS = 64;                 // envs per workgroup
-constexpr int NWAVE = 8;
-constexpr int NTHR = 64 * NWAVE;
+constexpr int ROWS = 64;                 // envs per workgroup of the 64-row kernels
 constexpr int LDA = 512 + 8;             // activation row stride (bf16); 16-byte pad
 constexpr int LDO = 65;                  // fp32 output row stride in LDS (odd: no bank conflicts)
 constexpr int NV = 35, NU = 29;
 
 enum Act : int { ACT_SILU = 0, ACT_ELU = 1 };
 
-template <int NOUT> struct Tile {
+// Rows (envs) per workgroup, both with 8 waves (two per SIMD).  64: two
+// activation buffers, layer l reads one and writes the other.  128: one
+// buffer (128 x 520 bf16 = 133 KB) updated in place -- each layer's output
+// stays in the accumulators until every wave has read the layer's input.
+// Every weight fragment a wave loads then serves 8 row tiles instead of 4,
+// which halves the L2 weight traffic per FLOP (measured on MI355X: the
+// surrogate forward at 65,536 envs 156 -> 131 us, at 131,072 264 -> 248 us;
+// the policy unchanged); it fills the 256 CUs from 32,768 envs.  (4 waves
+// of 128 x 128 tiles, one per SIMD with 512 registers, measured slower:
+// 159 and 294 us.)
+template <int R> struct Rows {
+  static_assert(R == 64 || R == 128, "64 or 128 rows per workgroup");
+  static constexpr int RT = R / 16;              // row tiles
+  static constexpr int NBUF = R == 64 ? 2 : 1;
+  static constexpr int NW = 8;                   // waves per workgroup
+  static constexpr int NT = 64 * NW;             // threads per workgroup
+};
+
+template <int NOUT, int RTILES> struct Tile {
+  static constexpr int NWAVE = Rows<RTILES * 16>::NW;
   static constexpr int NC = NOUT / 16;
   static constexpr bool WIDE = NC >= NWAVE;
   static constexpr int CT = WIDE ? NC / NWAVE : 1;
-  static constexpr int RT = WIDE ? 4 : (4 * NC) / NWAVE;
+  static constexpr int RT = WIDE ? RTILES : (RTILES * NC) / NWAVE;
   static_assert(NOUT % 16 == 0, "output width must be a multiple of 16");
-  static_assert(WIDE ? NC % NWAVE == 0 : (NWAVE % NC == 0 && (4 * NC) % NWAVE == 0), "unsupported width");
+  static_assert(WIDE ? NC % NWAVE == 0 : (NWAVE % NC == 0 && (RTILES * NC) % NWAVE == 0), "unsupported width");
   __device__ static int r0(int w) { return WIDE ? 0 : (w / NC) * RT; }
   __device__ static int c0(int w) { return WIDE ? w * CT : w % NC; }
 };
 
 // Weight fragments of one layer in flight: a ring of NB k-steps of the
-// wave's CT column tiles, 16 fragments (64 VGPRs) whatever CT is, so a
-// layer with fewer column tiles per wave runs further ahead.  The first PF
-// k-steps are loaded by wprefetch() before the layer (the next layer's ring
-// is filled before this layer's epilogue and barrier: weights do not depend
-// on activations); the k-loop loads the rest PF steps ahead.
-template <int NIN, int NOUT> struct Ring {
-  using T = Tile<NOUT>;
+// wave's CT column tiles, FR fragments in all (16 = 64 VGPRs with 64 rows, 8
+// with 128 rows, whose accumulators take twice the registers), so a layer
+// with fewer column tiles per wave runs further ahead.  The first PF k-steps are loaded by wprefetch() before the layer
+// (the next layer's ring is filled before this layer's epilogue and
+// barrier: weights do not depend on activations); the k-loop loads the rest
+// PF steps ahead.
+template <int NIN, int NOUT, int RTILES> struct Ring {
+  using T = Tile<NOUT, RTILES>;
   static constexpr int KS = NIN / 32;
-  static constexpr int NB = 16 / T::CT < KS ? 16 / T::CT : KS;
+  static constexpr int FR = RTILES == 4 ? 16 : 8;
+  static constexpr int NB0 = FR / T::CT < 2 ? 2 : FR / T::CT;
+  static constexpr int NB = NB0 < KS ? NB0 : KS;
   static constexpr int PF = NB - 1;
   static_assert(NIN % 32 == 0 && KS >= 2 && PF >= 1, "input width must be a multiple of 32, at least 64");
   bf16x8 f[NB][T::CT];
@@ -85,23 +107,23 @@ template <int NIN, int NOUT> struct Ring {
 // x 64 B per load instruction) are contiguous 1 KB runs (row-major
 // [NOUT][NIN] puts them NIN * 2 bytes apart and measured at 0.6x the rate on
 // MI355X).  Lane l reads row c * 16 + l % 16, k-elements (l / 16) * 8 + 0..7.
-template <int NOUT>
+template <int NOUT, int RTILES>
 __device__ __forceinline__ const bf16* wfrag(const bf16* W, int w, int lane, int k, int c) {
-  using T = Tile<NOUT>;
+  using T = Tile<NOUT, RTILES>;
   return W + ((size_t)k * NOUT + (size_t)(T::c0(w) + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8;
 }
 
-template <int NIN, int NOUT>
-__device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int lane, Ring<NIN, NOUT>& rg) {
-  using R = Ring<NIN, NOUT>;
+template <int NIN, int NOUT, int RTILES>
+__device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int lane, Ring<NIN, NOUT, RTILES>& rg) {
+  using R = Ring<NIN, NOUT, RTILES>;
 #pragma unroll
   for (int k = 0; k < R::PF; ++k)
 #pragma unroll
-    for (int c = 0; c < Tile<NOUT>::CT; ++c)
-      rg.f[k][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT>(W, w, lane, k, c));
+    for (int c = 0; c < Tile<NOUT, RTILES>::CT; ++c)
+      rg.f[k][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT, RTILES>(W, w, lane, k, c));
 }
 
-// acc = in[64 x NIN] (LDS) . W^T for this wave's tiles, with the first PF
+// acc = in[rows x NIN] (LDS) . W^T for this wave's tiles, with the first PF
 // k-steps of weights already in `rg`.
 //
 // TR = false: acc[r][c][q] is out[row (r0 + r) * 16 + (lane / 16) * 4 + q]
@@ -109,12 +131,12 @@ __device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int
 // operand, so the tile comes out transposed): acc[r][c][q] is
 // out[row (r0 + r) * 16 + lane % 16][col (c0 + c) * 16 + (lane / 16) * 4 + q],
 // four adjacent columns of one row per lane.
-template <int NIN, int NOUT, bool TR = false>
+template <int NIN, int NOUT, int RTILES, bool TR = false>
 __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf16* __restrict__ W, int w,
-                                          int lane, Ring<NIN, NOUT>& rg,
-                                          f32x4 (&acc)[Tile<NOUT>::RT][Tile<NOUT>::CT]) {
-  using T = Tile<NOUT>;
-  using R = Ring<NIN, NOUT>;
+                                          int lane, Ring<NIN, NOUT, RTILES>& rg,
+                                          f32x4 (&acc)[Tile<NOUT, RTILES>::RT][Tile<NOUT, RTILES>::CT]) {
+  using T = Tile<NOUT, RTILES>;
+  using R = Ring<NIN, NOUT, RTILES>;
   constexpr int KS = R::KS, PF = R::PF, NB = R::NB;
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   const int r0 = T::r0(w);
@@ -134,7 +156,7 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
     if (k + PF < KS) {
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        rg.f[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT>(W, w, lane, k + PF, c));
+        rg.f[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT, RTILES>(W, w, lane, k + PF, c));
     }
     if (k + 1 < KS) {
 #pragma unroll
@@ -152,26 +174,35 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
 }
 
 // hidden layer: out = act(in W^T + b) as bf16 into LDS; the next layer's
-// first weight k-steps (Wn) go into `next` before the epilogue
-template <int NIN, int NOUT, int ACT, int NIN2, int NOUT2>
-__device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __restrict__ sout,
-                                      const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane,
-                                      Ring<NIN, NOUT>& rg, const bf16* __restrict__ Wn, Ring<NIN2, NOUT2>& next) {
-  using T = Tile<NOUT>;
+// first weight k-steps (Wn) go into `next` before the epilogue.  With one
+// buffer (sout == sin, 128 rows) a barrier separates the last read of the
+// input from the first write of the output.
+template <int NIN, int NOUT, int ACT, int NIN2, int NOUT2, int RTILES>
+__device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* __restrict__ W,
+                                      const float* __restrict__ b, int w, int lane, Ring<NIN, NOUT, RTILES>& rg,
+                                      const bf16* __restrict__ Wn, Ring<NIN2, NOUT2, RTILES>& next) {
+  using T = Tile<NOUT, RTILES>;
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
   typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-  // bias of the lane's four columns per column tile, loaded before the k-loop
+  // bias of the lane's four columns per column tile: loaded before the k-loop
+  // with 64 rows, after it with 128 (its registers are needed in the loop)
   float bc[T::CT][4];
+  auto load_bias = [&]() {
 #pragma unroll
-  for (int c = 0; c < T::CT; ++c) {
-    const int col = (c0 + c) * 16 + (lane >> 4) * 4;
+    for (int c = 0; c < T::CT; ++c) {
+      const int col = (c0 + c) * 16 + (lane >> 4) * 4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
-  }
+      for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
+    }
+  };
+  if constexpr (RTILES == 4) load_bias();
   f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT, true>(sin, W, w, lane, rg, acc);
-  wprefetch<NIN2, NOUT2>(Wn, w, lane, next);
+  mma_layer<NIN, NOUT, RTILES, true>(sin, W, w, lane, rg, acc);
   __builtin_amdgcn_sched_barrier(0);
+  if constexpr (RTILES == 8) load_bias();
+  wprefetch<NIN2, NOUT2, RTILES>(Wn, w, lane, next);
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (RTILES == 8) __syncthreads();      // in place: every wave has read the input
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     // four adjacent columns of one row per lane: one 8-byte LDS store
@@ -191,19 +222,21 @@ __device__ __forceinline__ void dense(const bf16* __restrict__ sin, bf16* __rest
   }
 }
 
-// output layer: out = in W^T + b as fp32 into LDS [64][LDO]
-template <int NIN, int NOUT>
-__device__ __forceinline__ void dense_out(const bf16* __restrict__ sin, float* __restrict__ sout,
-                                          const bf16* __restrict__ W, const float* __restrict__ b, int w, int lane,
-                                          Ring<NIN, NOUT>& rg) {
-  using T = Tile<NOUT>;
+// output layer: out = in W^T + b as fp32 into LDS [rows][LDO] (over the
+// input buffer with 128 rows, after a barrier)
+template <int NIN, int NOUT, int RTILES>
+__device__ __forceinline__ void dense_out(const bf16* sin, float* sout, const bf16* __restrict__ W,
+                                          const float* __restrict__ b, int w, int lane,
+                                          Ring<NIN, NOUT, RTILES>& rg) {
+  using T = Tile<NOUT, RTILES>;
   static_assert(NOUT <= LDO, "output tile wider than the LDS row");
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
   float bc[T::CT];
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) bc[c] = b[(c0 + c) * 16 + lr];
   f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT>(sin, W, w, lane, rg, acc);
+  mma_layer<NIN, NOUT, RTILES>(sin, W, w, lane, rg, acc);
+  if constexpr (RTILES == 8) __syncthreads();      // in place: every wave has read the input
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     const int col = (c0 + c) * 16 + lr;
@@ -248,43 +281,52 @@ static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 // judged and fallback envs), row 106 the env's fallback flag on entry.
 constexpr int STASH_ABA = 71, STASH_FB = STASH_ABA + NV, STASH_ROWS = STASH_FB + 1;
 
-template <int MODE>
-__global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
+template <int MODE, int R>
+__global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a) {
+  constexpr int NTHR = Rows<R>::NT, NWAVE = Rows<R>::NW;
+  static_assert(MODE == 0 || R == ROWS, "the step (commit) kernel works on 64-env judge blocks");
+  constexpr int RT = Rows<R>::RT;
+  __shared__ __attribute__((aligned(16))) bf16 buf[Rows<R>::NBUF][R * LDA];
   __shared__ float stash[MODE == 1 ? STASH_ROWS * ROWS : 1];
   __shared__ float ysc[MODE == 1 ? 2 * NV : 1];          // ystd, ymean
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int e0 = blockIdx.x * ROWS;
+  const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
-  const int r = lane, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
+  // thread -> env r of the block and feature group fg (features fg, fg + G, ...)
+  constexpr int G = NTHR / R;
+  const int r = tid % R, fg = tid / R, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
   // The tile's rows are exactly one judge block, so `judged` is uniform.
   const bool judged_blk = MODE == 1 && g1k::judged_env(e0, a.phase, a.period);
   // One batch of loads: the env's fallback flag, the network input (qpos 36,
-  // qvel 35, tau 35; wave w owns features w, w + 8, ...) and, in a judged
-  // block, ABA's qacc of every env; then the first layer's weights.  All are
-  // issued before any but the flag is used, so the stage costs about one
-  // memory latency.
-  constexpr int FPW = SUR_PIN / NWAVE;
-  constexpr int APW = (NV + NWAVE - 1) / NWAVE;
-  static_assert(SUR_PIN % NWAVE == 0, "features split evenly over the waves");
+  // qvel 35, tau 35) and, in a judged block, ABA's qacc of every env; then
+  // the first layer's weights.  All are issued before any but the flag is
+  // used, so the stage costs about one memory latency.  With 128 rows (two
+  // groups of 64 features per thread) the features go in two passes, so the
+  // first layer's weights and one pass's inputs fit the registers together.
+  constexpr int FPW = SUR_PIN / G;
+  constexpr int NPASS = R == 128 ? 2 : 1;
+  constexpr int FPP = FPW / NPASS;                       // features per pass
+  constexpr int APW = (NV + G - 1) / G;
+  static_assert(SUR_PIN % G == 0 && FPW % NPASS == 0, "features split evenly over the groups");
   // the flag is loaded without a branch (from a valid dummy byte when there
   // are no flags) and tested after the other loads are issued
   const bool has_fb = MODE == 1 && a.fallback != nullptr;
   const uint8_t fbyte = *(has_fb ? a.fallback + ec : reinterpret_cast<const uint8_t*>(a.qpos));
-  float raw[FPW];
+  auto feature_src = [&](int f) -> const float* {
+    return f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N : a.tau + (size_t)(f - 71) * N;
+  };
+  float raw[FPP];
 #pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int f = w + NWAVE * i;
-    const float* src = f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N
-                                                                : a.tau + (size_t)(f - 71) * N;
-    raw[i] = f < SUR_NIN ? src[ec] : 0.f;
+  for (int i = 0; i < FPP; ++i) {
+    const int f = fg + G * i;
+    raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
   }
   float abav[APW] = {};
   if constexpr (MODE == 1) {
     if (judged_blk) {
 #pragma unroll
       for (int i = 0; i < APW; ++i) {
-        const int j = w + NWAVE * i;
+        const int j = fg + G * i;
         abav[i] = j < NV ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
       }
     }
@@ -309,65 +351,71 @@ __global__ __launch_bounds__(NTHR) void surrogate_kernel(SurrogateArgs a) {
       // fallback envs of a block that runs the network: their ABA qacc
 #pragma unroll
       for (int i = 0; i < APW; ++i) {
-        const int j = w + NWAVE * i;
+        const int j = fg + G * i;
         abav[i] = fb && j < NV ? a.qacc_aba[(size_t)j * N + ec] : 0.f;
       }
     }
-  }
-  // the input scalings, then the first layer's weights: issued after every
-  // load this stage waits for (the wait counter is in order)
-  float xm[FPW], xi[FPW];
-#pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int f = w + NWAVE * i;
-    xm[i] = f < SUR_NIN ? a.xmean[f] : 0.f;
-    xi[i] = f < SUR_NIN ? a.xinv[f] : 0.f;
   }
   float ys = 0.f;
   if constexpr (MODE == 1) {
     if (tid < 2 * NV) ys = tid < NV ? a.ystd[tid] : a.ymean[tid - NV];
   }
   __builtin_amdgcn_sched_barrier(0);
-  Ring<SUR_PIN, SUR_H> ring0;
-  wprefetch<SUR_PIN, SUR_H>(a.W[0], w, lane, ring0);
+  Ring<SUR_PIN, SUR_H, RT> ring0;
+  wprefetch<SUR_PIN, SUR_H, RT>(a.W[0], w, lane, ring0);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (MODE == 1) {
     // stash the commit's inputs
 #pragma unroll
     for (int i = 0; i < APW; ++i) {
-      const int j = w + NWAVE * i;
+      const int j = fg + G * i;
       if (j < NV) stash[(STASH_ABA + j) * ROWS + r] = (judged_blk || fb) ? abav[i] : 0.f;
     }
-    if (w == 0) stash[STASH_FB * ROWS + r] = fb ? 1.f : 0.f;
+    if (fg == 0) stash[STASH_FB * ROWS + r] = fb ? 1.f : 0.f;
     if (tid < 2 * NV) ysc[tid] = ys;
   }
 #pragma unroll
-  for (int i = 0; i < FPW; ++i) {
-    const int f = w + NWAVE * i;
-    float x = 0.f;
-    if (e < a.nenv && f < SUR_NIN) x = (raw[i] - xm[i]) * xi[i];
-    buf[0][r * LDA + f] = (bf16)x;
-    if constexpr (MODE == 1) {
-      if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+  for (int p = 0; p < NPASS; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int i = 0; i < FPP; ++i) {
+        const int f = fg + G * (p * FPP + i);
+        raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int i = 0; i < FPP; ++i) {
+      const int f = fg + G * (p * FPP + i);
+      float x = 0.f;
+      if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
+      buf[0][r * LDA + f] = (bf16)x;
+      if constexpr (MODE == 1) {
+        if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();
-  Ring<SUR_H, SUR_H> ring1, ring2, ring3;
-  Ring<SUR_H, SUR_POUT> ring4;
-  dense<SUR_PIN, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
+  // layer l reads b[l % 2] and writes b[(l + 1) % 2]; one buffer with 128 rows
+  bf16* b0 = buf[0];
+  bf16* b1 = buf[Rows<R>::NBUF - 1];
+  Ring<SUR_H, SUR_H, RT> ring1, ring2, ring3;
+  Ring<SUR_H, SUR_POUT, RT> ring4;
+  dense<SUR_PIN, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
+  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b1, b0, a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[0], buf[1], a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
+  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
   __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU>(buf[1], buf[0], a.W[3], a.b[3], w, lane, ring3, a.W[4], ring4);
+  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_POUT, RT>(b1, b0, a.W[3], a.b[3], w, lane, ring3, a.W[4], ring4);
   __syncthreads();
-  float* out = reinterpret_cast<float*>(buf[1]);
-  dense_out<SUR_H, SUR_POUT>(buf[0], out, a.W[4], a.b[4], w, lane, ring4);
+  float* out = reinterpret_cast<float*>(b1);
+  dense_out<SUR_H, SUR_POUT, RT>(b0, out, a.W[4], a.b[4], w, lane, ring4);
   __syncthreads();
   if constexpr (MODE == 0) {
-    // [64][64] fp32 tile to pred rows; 512 threads, 8 elements each
-    for (int i = tid; i < ROWS * SUR_POUT; i += NTHR) {
+    // [R][64] fp32 tile to pred rows; 512 threads
+    for (int i = tid; i < R * SUR_POUT; i += NTHR) {
       const int rr = i / SUR_POUT, c = i % SUR_POUT, er = e0 + rr;
       if (er < a.nenv) a.pred[(size_t)er * a.ldp + c] = out[rr * LDO + c];
     }
@@ -526,49 +574,63 @@ struct PolicyArgs {
 
 constexpr int POL_PIN = 128;
 
-__global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16 buf[2][ROWS * LDA];
+template <int R>
+__global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
+  constexpr int NTHR = Rows<R>::NT;
+  constexpr int RT = Rows<R>::RT;
+  __shared__ __attribute__((aligned(16))) bf16 buf[Rows<R>::NBUF][R * LDA];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int e0 = blockIdx.x * ROWS;
+  const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
-  Ring<POL_PIN, 512> ring0;
+  // thread -> env r of the block and group g of G (raw rows g, g + G, ...;
+  // features likewise)
+  constexpr int G = NTHR / R;
+  const int r = tid % R, g = tid / R;
+  Ring<POL_PIN, 512, RT> ring0;
   {
-    // stage the block's raw inputs in LDS (buf[1] is free until the first
-    // layer writes it): every load of a thread is issued before the first
-    // is used, so the stage costs one memory latency
-    float* raw = reinterpret_cast<float*>(buf[1]);
-    static_assert(RAW_N * ROWS * sizeof(float) <= sizeof(buf[1]), "raw stage exceeds its buffer");
-    const int r = tid & 63, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
-    constexpr int RPW = (RAW_N + NWAVE - 1) / NWAVE;   // rows per wave
-    constexpr int FPW = POL_PIN / NWAVE;               // features per wave
-    float v[RPW], om[FPW], oi[FPW];
+    // stage the block's raw inputs in LDS: with 64 rows in buf[1] (free
+    // until the first layer writes it), with 128 rows in each env row's
+    // columns 128-519 of the single buffer (the features take columns
+    // 0-127).  Every load of a thread is issued before the first is used, so
+    // the stage costs one memory latency.
+    float* rawbase = Rows<R>::NBUF == 2 ? reinterpret_cast<float*>(buf[Rows<R>::NBUF - 1])
+                                        : reinterpret_cast<float*>(buf[0]) + POL_PIN / 2;
+    constexpr int RS = Rows<R>::NBUF == 2 ? 1 : LDA / 2;     // env stride of the stage (floats)
+    constexpr int QS = Rows<R>::NBUF == 2 ? R : 1;           // raw-row stride of the stage (floats)
+    static_assert(Rows<R>::NBUF == 1 || RAW_N * R * sizeof(float) <= R * LDA * sizeof(bf16), "raw stage");
+    static_assert(Rows<R>::NBUF == 2 || (POL_PIN / 2 + RAW_N) * sizeof(float) <= LDA * sizeof(bf16), "raw stage");
+    auto rawat = [&](int row, int env) -> float& { return rawbase[env * RS + row * QS]; };
+    const int e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
+    constexpr int RPG = (RAW_N + G - 1) / G;           // rows per group
+    constexpr int FPG = POL_PIN / G;                   // features per group
+    float v[RPG], om[FPG], oi[FPG];
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const int row = w + NWAVE * i;
+    for (int i = 0; i < RPG; ++i) {
+      const int row = g + G * i;
       v[i] = row < RAW_N ? raw_row(row, N, a.qpos, a.qvel, a.commands, a.last_ctrl)[ec] : 0.f;
     }
 #pragma unroll
-    for (int i = 0; i < FPW; ++i) {
-      const int f = w + NWAVE * i;
+    for (int i = 0; i < FPG; ++i) {
+      const int f = g + G * i;
       om[i] = f < a.nobs ? a.obs_mean[f] : 0.f;
       oi[i] = f < a.nobs ? a.obs_inv[f] : 0.f;
     }
     // the first layer's weights go last: the wait counter is in order
     __builtin_amdgcn_sched_barrier(0);
-    wprefetch<POL_PIN, 512>(a.W[0], w, lane, ring0);
+    wprefetch<POL_PIN, 512, RT>(a.W[0], w, lane, ring0);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) {
-      const int row = w + NWAVE * i;
-      if (row < RAW_N) raw[row * ROWS + r] = v[i];
+    for (int i = 0; i < RPG; ++i) {
+      const int row = g + G * i;
+      if (row < RAW_N) rawat(row, r) = v[i];
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < FPW; ++i) {
-      const int f = w + NWAVE * i;
+    for (int i = 0; i < FPG; ++i) {
+      const int f = g + G * i;
       float x = 0.f;
       if (e < a.nenv && f < a.nobs) {
-        const float o = obs_eval(a.layout, f, [&](int row) { return raw[row * ROWS + r]; });
+        const float o = obs_eval(a.layout, f, [&](int row) { return rawat(row, r); });
         if (a.obs_out) a.obs_out[(size_t)e * a.nobs + f] = o;
         x = fminf(fmaxf((o - om[i]) * oi[i], -a.clip), a.clip);
       }
@@ -576,10 +638,10 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
     }
   }
   __syncthreads();
-  if (a.push_interval && w == 0) {
+  if (a.push_interval && tid < R) {
     // examples/train.py:301-310: countdown; on expiry add a random base
     // velocity kick and draw the next interval from [50, 150)
-    const int e = e0 + lane;
+    const int e = e0 + tid;
     if (e < a.nenv) {
       int pi = a.push_interval[e] - 1;
       if (pi <= 0) {
@@ -594,24 +656,28 @@ __global__ __launch_bounds__(NTHR) void policy_kernel(PolicyArgs a) {
       a.push_interval[e] = pi;
     }
   }
-  Ring<512, 256> ring1;
-  Ring<256, 128> ring2;
-  Ring<128, 32> ring3;
-  dense<POL_PIN, 512, ACT_ELU>(buf[0], buf[1], a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
+  bf16* b0 = buf[0];
+  bf16* b1 = buf[Rows<R>::NBUF - 1];
+  Ring<512, 256, RT> ring1;
+  Ring<256, 128, RT> ring2;
+  Ring<128, 32, RT> ring3;
+  dense<POL_PIN, 512, ACT_ELU, 512, 256, RT>(b0, b1, a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
   __syncthreads();
-  dense<512, 256, ACT_ELU>(buf[1], buf[0], a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
+  dense<512, 256, ACT_ELU, 256, 128, RT>(b1, b0, a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
   __syncthreads();
-  dense<256, 128, ACT_ELU>(buf[0], buf[1], a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
+  dense<256, 128, ACT_ELU, 128, 32, RT>(b0, b1, a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
   __syncthreads();
-  float* out = reinterpret_cast<float*>(buf[0]);
-  dense_out<128, 32>(buf[1], out, a.W[3], a.b[3], w, lane, ring3);
+  float* out = reinterpret_cast<float*>(b0);
+  dense_out<128, 32, RT>(b1, out, a.W[3], a.b[3], w, lane, ring3);
   __syncthreads();
-  // epilogue: thread (r, group) -> env r, actions j = group, group + 8, ...,
-  // taken in pairs (j, j + 8) that share one Box-Muller draw
-  const int r = tid & 63, e = e0 + r;
+  // epilogue: thread (r, g) -> env r; actions in pairs (j, j + 8) that share
+  // one Box-Muller draw, j from {0..7, 16..23}, split over the G groups
+  const int e = e0 + r;
   if (e < a.nenv) {
-    for (int j = w; j < NU; j += 2 * NWAVE) {
-      const int j2 = j + NWAVE;
+    for (int jb = g; jb < 16; jb += G) {
+      const int j = jb < 8 ? jb : jb + 8;
+      if (j >= NU) continue;
+      const int j2 = j + 8;
       const bool two = j2 < NU;
       float act = out[r * LDO + j];
       float act2 = two ? out[r * LDO + j2] : 0.f;
@@ -911,6 +977,10 @@ __global__ __launch_bounds__(256) void rollout_init_kernel(int nenv, float lin, 
 namespace {
 using namespace g1inf;
 
+// From 32,768 envs the forward-only surrogate and the policy kernels take
+// 128 envs per workgroup (256+ workgroups: every CU busy); below, 64.
+constexpr int kRows128MinEnvs = 32768;
+
 bool fill_surrogate_weights(SurrogateArgs& a, const void* const* W, const float* const* b) {
   if (!W || !b) return false;
   for (int l = 0; l < 5; ++l) {
@@ -935,7 +1005,12 @@ int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, con
   if (nenv == 0) return 0;
   a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
   a.pred = pred; a.ldp = ldp;
-  hipLaunchKernelGGL(surrogate_kernel<0>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  if (nenv >= kRows128MinEnvs)
+    hipLaunchKernelGGL((surrogate_kernel<0, 128>), dim3((nenv + 127) / 128), dim3(Rows<128>::NT), 0,
+                       (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL((surrogate_kernel<0, 64>), dim3((nenv + 63) / 64), dim3(Rows<64>::NT), 0, (hipStream_t)stream,
+                       a);
   return g1err::launch("g1_hip_surrogate_forward");
 }
 
@@ -955,7 +1030,8 @@ int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, 
   a.phase = phase; a.period = period; a.tol = tol; a.dt = dt; a.ymean = ymean; a.ystd = ystd;
   a.qacc_aba = qacc_aba; a.fallback = fallback; a.stats = stats; a.qacc_out = qacc_out;
   a.qpos_rw = qpos; a.qvel_rw = qvel;
-  hipLaunchKernelGGL(surrogate_kernel<1>, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL((surrogate_kernel<1, ROWS>), dim3((nenv + ROWS - 1) / ROWS), dim3(Rows<ROWS>::NT), 0,
+                     (hipStream_t)stream, a);
   return g1err::launch("g1_hip_surrogate_step");
 }
 
@@ -978,7 +1054,10 @@ int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const fl
   a.qpos = qpos; a.qvel = qvel; a.commands = commands; a.last_ctrl = last_ctrl;
   a.obs_mean = obs_mean; a.obs_inv = obs_inv; a.clip = clip; a.log_std = log_std; a.seed = seed; a.t = t;
   a.ctrl = ctrl; a.actions = actions; a.obs_out = obs_out; a.push_interval = push_interval;
-  hipLaunchKernelGGL(policy_kernel, dim3((nenv + ROWS - 1) / ROWS), dim3(NTHR), 0, (hipStream_t)stream, a);
+  if (nenv >= kRows128MinEnvs)
+    hipLaunchKernelGGL(policy_kernel<128>, dim3((nenv + 127) / 128), dim3(Rows<128>::NT), 0, (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(policy_kernel<64>, dim3((nenv + 63) / 64), dim3(Rows<64>::NT), 0, (hipStream_t)stream, a);
   return g1err::launch("g1_policy_act");
 }
 

@@ -36,17 +36,67 @@ def test_large_batch_matches_small_batches(device, nenv, split_flag):
             assert np.array_equal(big[k][:, sl], small[k]), (k, lo, hi)
 
 
-def test_fused_surrogate_batch_independence(device):
+@pytest.mark.parametrize("n", [65536, 40017])
+def test_fused_surrogate_batch_independence(device, n):
+    """From 32,768 envs the fused forward runs 128 envs per workgroup (one
+    in-place activation buffer), below 64 (two buffers); the k-order of
+    every accumulation is the same, so every env's output is the same bits
+    whichever kernel and batch computes it.  The big batch is covered
+    completely by slices below 32,768 envs (and a tail slice)."""
     mlp = MLP(device, seed=5)
-    n = 65536
     qp, qv, _, _ = W.randomized_actuation(n, seed=10, nblocks=1)
     tau = np.random.default_rng(11).normal(size=(35, n)).astype(np.float32) * 20
     d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=device)  # noqa: E731
     big = torch.zeros((n, 64), device=device)
     mlp.forward_state(n, d(qp), d(qv), d(tau), big)
-    for lo, hi in ((0, 64), (1000, 1100), (n - 100, n)):
+    cuts = list(range(0, n, 32767)) + [n]
+    slices = [(lo, hi) for lo, hi in zip(cuts[:-1], cuts[1:])] + [(1000, 1100), (n - 100, n)]
+    for lo, hi in slices:
         sl = slice(lo, hi)
         small = torch.zeros((hi - lo, 64), device=device)
         mlp.forward_state(hi - lo, d(qp[:, sl]), d(qv[:, sl]), d(tau[:, sl]), small)
         torch.cuda.synchronize()
         assert torch.equal(big[sl], small), (lo, hi)
+
+
+@pytest.mark.parametrize("sample", [False, True])
+def test_policy_batch_independence(device, sample):
+    """The policy kernel at 40,017 envs (128 envs per workgroup, a partial
+    last workgroup) against 64-env workgroups on smaller batches: the same
+    actions, ctrl, observations and pushes, bit for bit.  With sampling the
+    noise of env e is drawn from (seed, t, e), so the small batches start at
+    env 0 (same indices); without it any slice matches."""
+    from g1sim_amd.policy import Policy
+    from g1sim_amd.model import MODEL
+    n = 40017
+    rng = np.random.default_rng(12)
+    qp, qv = W.random_state(rng, n)
+    qp = np.ascontiguousarray(qp.T, np.float32)
+    qv = np.ascontiguousarray(qv.T, np.float32)
+    cmd = rng.uniform(-1, 1, size=(3, n)).astype(np.float32)
+    last = (MODEL.qpos_stand[7:, None] + rng.normal(size=(29, n)) * 0.2).astype(np.float32)
+    push = rng.integers(-2, 5, n).astype(np.int32)
+    pol = Policy(device, layout=1, seed=13)
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=device)  # noqa: E731
+
+    def act(lo, hi):
+        m = hi - lo
+        qvel = d(qv[:, lo:hi])
+        ctrl = torch.zeros((29, m), device=device)
+        actions = torch.zeros((29, m), device=device)
+        obs = torch.zeros((m, 96), device=device)
+        pi = d(push[lo:hi])
+        pol.act(m, d(qp[:, lo:hi]), qvel, d(cmd[:, lo:hi]), d(last[:, lo:hi]), ctrl, actions=actions, obs_out=obs,
+                push_interval=pi, seed=3, t=5, sample=sample)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in (ctrl, actions, obs, qvel, pi)]
+
+    big = act(0, n)
+    small_slices = [(0, 30000), (30000, n)] if not sample else [(0, 30000), (0, 64)]
+    for lo, hi in small_slices:
+        small = act(lo, hi)
+        for name, b, sm in zip(("ctrl", "actions", "obs", "qvel", "push"), big, small):
+            if lo > 0 and name in ("qvel", "push"):
+                continue                        # the push kicks are drawn from the env's index in its batch
+            bs = b[lo:hi] if name == "obs" else b[..., lo:hi]
+            assert np.array_equal(bs, sm), (name, lo, hi)
