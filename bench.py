@@ -80,6 +80,35 @@ def step_launches(lib, nenv, state, ctrl, block0, nsteps, nsub, flags=0, base64=
     return subs
 
 
+def schedule_subs(step0, nsteps, period=CTRL_PERIOD):
+    """Substeps of each launch g1_hip_step_schedule makes for steps
+    [step0, step0 + nsteps): one launch per target block touched."""
+    subs, s = [], 0
+    while s < nsteps:
+        n = min(period - (step0 + s) % period, nsteps - s)
+        subs.append(n)
+        s += n
+    return subs
+
+
+def schedule_call(lib, nenv, state, ctrl, step0, nsteps, flags=0, base64=None, sptr=None):
+    """The ctypes call stepping [step0, step0 + nsteps) through the target
+    blocks ctrl [nblocks][29][nenv] in one host call (argument list built
+    beforehand, so a timed region pays only the call)."""
+    from g1sim_amd import _lib
+    qpos, qvel, anchor = state
+    args = (nenv, nsteps, CTRL_PERIOD, step0, DT, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+            ctrl.data_ptr(), ctrl.shape[1] * ctrl.shape[2], None, None, None, None,
+            None if base64 is None else base64.data_ptr(), flags, sptr)
+    fn = lib.g1_hip_step_schedule
+
+    def call():
+        rc = fn(*args)
+        if rc:
+            raise RuntimeError(_lib.last_error())
+    return call
+
+
 def _event_avg_ms(fn, reps, stream):
     """Average duration of fn()'s launches from HIP events on `stream`."""
     import torch
@@ -106,13 +135,15 @@ def aba_leg(dev, nenv, steps, warmup):
     state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
     ctrl = torch.as_tensor(cb, device=dev)
     stream = torch.cuda.current_stream(dev)
-    step_launches(lib, nenv, state, ctrl, 0, warmup, CTRL_PERIOD, sptr=stream.cuda_stream)
+    schedule_call(lib, nenv, state, ctrl, 0, warmup, sptr=stream.cuda_stream)()
+    run = schedule_call(lib, nenv, state, ctrl, wb * CTRL_PERIOD, steps, sptr=stream.cuda_stream)
+    subs = schedule_subs(wb * CTRL_PERIOD, steps)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    subs = step_launches(lib, nenv, state, ctrl, wb, steps, CTRL_PERIOD, sptr=stream.cuda_stream)
+    run()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -698,7 +729,17 @@ def main():
     # launch from idle, 97-99 us after busy work).
     if not args.no_large and default_shape:
         aba131 = aba_leg(dev, 131072, 200, 20)
-    step_launches(lib, nenv, state, ctrl, 0, args.warmup, args.nsub, args.flags, base64, sptr)
+    if args.nsub == CTRL_PERIOD:
+        # one host call enqueues every launch of a region (g1_hip_step_schedule)
+        schedule_call(lib, nenv, state, ctrl, 0, args.warmup, args.flags, base64, sptr)()
+        run = schedule_call(lib, nenv, state, ctrl, wb * CTRL_PERIOD, args.steps, args.flags, base64, sptr)
+        subs = schedule_subs(wb * CTRL_PERIOD, args.steps)
+    else:
+        step_launches(lib, nenv, state, ctrl, 0, args.warmup, args.nsub, args.flags, base64, sptr)
+        run = lambda: step_launches(lib, nenv, state, ctrl, wb, args.steps, args.nsub, args.flags,  # noqa: E731
+                                    base64, sptr)
+        subs = [n for s0 in range(0, args.steps, CTRL_PERIOD)
+                for n in schedule_subs(0, min(CTRL_PERIOD, args.steps - s0), args.nsub)]
     torch.cuda.synchronize()
 
     # timed region: wall clock between synchronisations, and one HIP event
@@ -710,7 +751,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    subs = step_launches(lib, nenv, state, ctrl, wb, args.steps, args.nsub, args.flags, base64, sptr)
+    run()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0

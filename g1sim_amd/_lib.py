@@ -40,6 +40,8 @@ def load():
     lib.g1_cuda_reset_done.argtypes = [i, vp, vp, vp, vp, u, f, f, vp]
     lib.g1_hip_step_ex.restype = i
     lib.g1_hip_step_ex.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
+    lib.g1_hip_step_schedule.restype = i
+    lib.g1_hip_step_schedule.argtypes = [i, i, i, i, f, vp, vp, vp, vp, C.c_longlong, vp, vp, vp, vp, vp, u, vp]
     lib.g1_mlp_gemm.restype = i
     lib.g1_mlp_gemm.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp]
     lib.g1_mlp_mse_grad.restype = i
@@ -115,7 +117,7 @@ def exported_symbols():
             "g1_hip_surrogate_integrate", "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs",
             "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
             "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
-            "g1_policy_obs_norm_update_noalloc"]
+            "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule"]
 
 
 def ptr_array(tensors_or_ptrs):

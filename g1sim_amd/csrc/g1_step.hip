@@ -29,6 +29,8 @@
 //   ([dim][nenv], env fastest, as src/g1_kernel.cu:9-10), once per launch;
 //   nsub substeps run from registers.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -1542,6 +1544,47 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   a.f64_solve = 0;
   a.env_list = nullptr; a.list_count = nullptr; a.stride = nenv;
   return launch_step(a, flags, (hipStream_t)stream);
+}
+
+int g1_hip_step_schedule(int nenv, int nsteps, int period, int step0, float dt, float* qpos, float* qvel,
+                         float* anchor, const float* ctrl_blocks, long long block_stride, float* foot_pos,
+                         float* foot_vel, float* contact_forces, const float* mu, double* base_pose64,
+                         unsigned flags, void* stream) {
+  g_err[0] = 0;
+  if (nsteps < 0 || period <= 0 || step0 < 0 || block_stride < 29LL * nenv) {
+    set_err("step_schedule: nsteps (%d) >= 0, period (%d) > 0, step0 (%d) >= 0 and block_stride (%lld) >= 29 nenv "
+            "required", nsteps, period, step0, block_stride);
+    return -1;
+  }
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl_blocks);
+  if (rc) return rc;
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_BASE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2) ||
+      ((flags & G1_FLAG_WAVES4) && (flags & G1_FLAG_WAVES2))) { set_err("unknown flag bits 0x%x", flags); return -1; }
+  if (((flags & G1_FLAG_F64_BASE) != 0) != (base_pose64 != nullptr)) {
+    set_err("base_pose64 must be given exactly when G1_FLAG_F64_BASE is set");
+    return -1;
+  }
+  if (nenv == 0 || nsteps == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.qacc = nullptr; a.tau = nullptr; a.mu = mu; a.base64 = base_pose64;
+  a.env_list = nullptr; a.list_count = nullptr; a.stride = nenv;
+  // one launch per target block touched: steps step0 + s share block
+  // (step0 + s) / period; each launch runs to the end of its block or of the
+  // schedule, so no launch crosses a target change
+  for (int s = 0; s < nsteps;) {
+    const int abs_step = step0 + s;
+    const int n = std::min(period - abs_step % period, nsteps - s);
+    a.nsub = n;
+    a.ctrl = ctrl_blocks + (size_t)(abs_step / period) * (size_t)block_stride;
+    rc = launch_step(a, flags, (hipStream_t)stream);
+    if (rc) return rc;
+    s += n;
+  }
+  return 0;
 }
 
 // src/g1_kernel.cu:154-163: dt = 1e-3 (g1_default_config), model friction.

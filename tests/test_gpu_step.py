@@ -251,6 +251,37 @@ def test_reset_done_matches_reference_fixture(device, golden):
                 assert bool((qvel[:, torch.as_tensor(envs, device=device)] == 0).all())
 
 
+@pytest.mark.parametrize("nenv,step0,nsteps", [(4096, 3, 37), (40000, 20, 25)])
+def test_step_schedule_matches_launches(device, nenv, step0, nsteps):
+    """g1_hip_step_schedule (one host call, one launch per target block)
+    gives the bits of g1_hip_step_ex launched block by block with the same
+    targets, at both work splits, with per-env friction and feet outputs."""
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    nblocks = (step0 + nsteps + 9) // 10
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=14, nblocks=nblocks)
+    mu = np.random.default_rng(15).uniform(0.4, 1.2, nenv).astype(np.float32)
+    ctrl = torch.as_tensor(cb, device=device)
+    a = GpuState(qp, qv, an, cb[0], device, mu=mu)
+    b = GpuState(qp, qv, an, cb[0], device, mu=mu)
+    rc = lib.g1_hip_step_schedule(nenv, nsteps, 10, step0, 2e-3, a.qpos.data_ptr(), a.qvel.data_ptr(),
+                                  a.anchor.data_ptr(), ctrl.data_ptr(), 29 * nenv, a.foot_pos.data_ptr(),
+                                  a.foot_vel.data_ptr(), a.fn.data_ptr(), a.mu.data_ptr(), None, 0, s)
+    _lib.check(rc, "step_schedule")
+    k = step0
+    while k < step0 + nsteps:
+        n = min(10 - k % 10, step0 + nsteps - k)
+        b.ctrl = ctrl[k // 10]
+        b.step(n, 2e-3, qacc=False)
+        k += n
+    ha, hb = a.host(), b.host()
+    for key in ("qpos", "qvel", "anchor", "foot_pos", "foot_vel", "fn"):
+        assert np.array_equal(ha[key], hb[key]), key
+    # argument checks return before any launch
+    assert lib.g1_hip_step_schedule(nenv, 5, 0, 0, 2e-3, a.qpos.data_ptr(), a.qvel.data_ptr(), a.anchor.data_ptr(),
+                                    ctrl.data_ptr(), 29 * nenv, None, None, None, None, None, 0, s) == -1
+
+
 def test_settles_like_the_oracle(device):
     """10 s (5,000 steps at dt 2e-3) of the drop test: every env stands with
     the robot's weight on the feet (tests/test_vs_mujoco.py:114-133)."""

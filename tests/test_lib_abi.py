@@ -123,3 +123,19 @@ def test_every_failure_names_its_entry_point():
     assert "phase" in _lib.last_error()
     assert lib.g1_hip_peak_fp32(0, 1, d, None) == -1
     assert "g1_hip_peak_fp32" in _lib.last_error()
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((4, 5, 0, 0, 1e3), "period"),
+    ((4, -1, 10, 0, 1e3), "nsteps"),
+    ((4, 5, 10, -2, 1e3), "step0"),
+    ((4, 5, 10, 0, 100), "block_stride"),
+])
+def test_step_schedule_rejects_bad_arguments(args, msg):
+    lib = _lib.load()
+    nenv, nsteps, period, step0, stride = args
+    dummy = C.c_void_p(16)
+    rc = lib.g1_hip_step_schedule(nenv, nsteps, period, step0, 2e-3, dummy, dummy, dummy, dummy, int(stride),
+                                  None, None, None, None, None, 0, None)
+    assert rc == -1
+    assert msg in _lib.last_error()
