@@ -745,12 +745,15 @@ def main():
     # timed region: wall clock between synchronisations, and one HIP event
     # pair on the launch stream around its back-to-back launches for the
     # kernel's average duration (an event pair around every launch would put
-    # ~3 us of event processing between the kernels)
+    # ~3 us of event processing between the kernels).  The start event is
+    # enqueued before the clock starts (an idle stream, so it is stamped at
+    # once): the wall-clock region holds the launches and the final
+    # synchronisation only.
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     e0.record(stream)
+    t0 = time.perf_counter()
     run()
     e1.record(stream)
     torch.cuda.synchronize()
