@@ -34,6 +34,8 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <type_traits>
+#include <utility>
 
 #include "g1_model_kernel.h"
 #include "g1_fastmath.h"
@@ -46,6 +48,15 @@ namespace g1inf {
 typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// f(std::integral_constant<int, i>) for i = 0 .. N_ - 1, unrolled at compile time
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N_, typename F> __device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N_>{});
+}
 
 constexpr int ROWS = 64;                 // envs per workgroup of the 64-row kernels
 constexpr int LDA = 512 + 8;             // activation row stride (bf16); 16-byte pad
@@ -292,7 +303,10 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
-  // thread -> env r of the block and feature group fg (features fg, fg + G, ...)
+  // thread -> env r of the block and feature group fg: chunks of 8 features,
+  // chunk c of the group holding features 8 (fg + G c) to 8 (fg + G c) + 7
+  // (the 106 inputs spread evenly over the groups; each chunk goes to LDS in
+  // one 16-byte store)
   constexpr int G = NTHR / R;
   const int r = tid % R, fg = tid / R, e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
   // The tile's rows are exactly one judge block, so `judged` is uniform.
@@ -307,7 +321,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   constexpr int NPASS = R == 128 ? 2 : 1;
   constexpr int FPP = FPW / NPASS;                       // features per pass
   constexpr int APW = (NV + G - 1) / G;
-  static_assert(SUR_PIN % G == 0 && FPW % NPASS == 0, "features split evenly over the groups");
+  static_assert(SUR_PIN % G == 0 && FPW % NPASS == 0 && FPP % 8 == 0, "features split evenly over the groups");
   // the flag is loaded without a branch (from a valid dummy byte when there
   // are no flags) and tested after the other loads are issued
   const bool has_fb = MODE == 1 && a.fallback != nullptr;
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   float raw[FPP];
 #pragma unroll
   for (int i = 0; i < FPP; ++i) {
-    const int f = fg + G * i;
+    const int f = 8 * (fg + G * (i / 8)) + i % 8;
     raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
   }
   float abav[APW] = {};
@@ -379,20 +393,27 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     if (p > 0) {
 #pragma unroll
       for (int i = 0; i < FPP; ++i) {
-        const int f = fg + G * (p * FPP + i);
+        const int ii = p * FPP + i, f = 8 * (fg + G * (ii / 8)) + ii % 8;
         raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
       }
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int i = 0; i < FPP; ++i) {
-      const int f = fg + G * (p * FPP + i);
-      float x = 0.f;
-      if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
-      buf[0][r * LDA + f] = (bf16)x;
-      if constexpr (MODE == 1) {
-        if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+    for (int i0 = 0; i0 < FPP; i0 += 8) {
+      bf16x8 x8;
+#pragma unroll
+      for (int i = i0; i < i0 + 8; ++i) {
+        const int ii = p * FPP + i, f = 8 * (fg + G * (ii / 8)) + ii % 8;
+        float x = 0.f;
+        if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
+        x8[i - i0] = (bf16)x;
+        if constexpr (MODE == 1) {
+          if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
+        }
       }
+      // one 16-byte LDS store (a 2-byte store per feature at the 1,040-byte
+      // row stride put 8 lanes on each bank)
+      *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + 8 * (fg + G * ((p * FPP + i0) / 8))]) = x8;
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -583,26 +604,33 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
   const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
   // thread -> env r of the block and group g of G (raw rows g, g + G, ...;
-  // features likewise)
+  // chunks of 8 features, chunk c of the group holding features 8 (g + G c)
+  // to 8 (g + G c) + 7, so the 96 or 99 features spread evenly over the
+  // groups)
   constexpr int G = NTHR / R;
   const int r = tid % R, g = tid / R;
   Ring<POL_PIN, 512, RT> ring0;
   {
-    // stage the block's raw inputs in LDS: with 64 rows in buf[1] (free
-    // until the first layer writes it), with 128 rows in each env row's
-    // columns 128-519 of the single buffer (the features take columns
-    // 0-127).  Every load of a thread is issued before the first is used, so
-    // the stage costs one memory latency.
+    // stage the block's raw inputs in LDS, raw row q of env r at float
+    // q * QS + r (the lanes of a wave, consecutive envs, hit consecutive
+    // banks): with 64 rows in buf[1] (free until the first layer writes
+    // it), with 128 rows in floats 64-191 of buffer row q, which the
+    // features (floats 0-63 of each row) do not touch.  (An env-major stage,
+    // env stride 260 floats, put 8 lanes on each bank: the stage and the
+    // feature pass took 16k of a workgroup's 61k cycles.)  Every load of a
+    // thread is issued before the first is used, so the stage costs one
+    // memory latency.
     float* rawbase = Rows<R>::NBUF == 2 ? reinterpret_cast<float*>(buf[Rows<R>::NBUF - 1])
                                         : reinterpret_cast<float*>(buf[0]) + POL_PIN / 2;
-    constexpr int RS = Rows<R>::NBUF == 2 ? 1 : LDA / 2;     // env stride of the stage (floats)
-    constexpr int QS = Rows<R>::NBUF == 2 ? R : 1;           // raw-row stride of the stage (floats)
+    constexpr int QS = Rows<R>::NBUF == 2 ? R : LDA / 2;     // raw-row stride of the stage (floats)
     static_assert(Rows<R>::NBUF == 1 || RAW_N * R * sizeof(float) <= R * LDA * sizeof(bf16), "raw stage");
-    static_assert(Rows<R>::NBUF == 2 || (POL_PIN / 2 + RAW_N) * sizeof(float) <= LDA * sizeof(bf16), "raw stage");
-    auto rawat = [&](int row, int env) -> float& { return rawbase[env * RS + row * QS]; };
+    static_assert(Rows<R>::NBUF == 2 || (RAW_N <= R && (POL_PIN / 2 + R) * sizeof(float) <= LDA * sizeof(bf16)),
+                  "raw stage");
+    auto rawat = [&](int row, int env) -> float& { return rawbase[row * QS + env]; };
     const int e = e0 + r, ec = e < a.nenv ? e : a.nenv - 1;
     constexpr int RPG = (RAW_N + G - 1) / G;           // rows per group
     constexpr int FPG = POL_PIN / G;                   // features per group
+    static_assert(FPG % 8 == 0, "features go to LDS 8 at a time");
     float v[RPG], om[FPG], oi[FPG];
 #pragma unroll
     for (int i = 0; i < RPG; ++i) {
@@ -611,7 +639,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
     }
 #pragma unroll
     for (int i = 0; i < FPG; ++i) {
-      const int f = g + G * i;
+      const int f = 8 * (g + G * (i / 8)) + i % 8;
       om[i] = f < a.nobs ? a.obs_mean[f] : 0.f;
       oi[i] = f < a.nobs ? a.obs_inv[f] : 0.f;
     }
@@ -625,17 +653,40 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
       if (row < RAW_N) rawat(row, r) = v[i];
     }
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < FPG; ++i) {
-      const int f = g + G * i;
-      float x = 0.f;
-      if (e < a.nenv && f < a.nobs) {
-        const float o = obs_eval(a.layout, f, [&](int row) { return rawat(row, r); });
-        if (a.obs_out) a.obs_out[(size_t)e * a.nobs + f] = o;
-        x = fminf(fmaxf((o - om[i]) * oi[i], -a.clip), a.clip);
-      }
-      buf[0][r * LDA + f] = (bf16)x;
-    }
+    // The group's features with the layout and the group index as compile-time
+    // constants (one uniform branch picks the instance): every feature's kind,
+    // raw row and offset fold into its code.  (With both known only at run
+    // time, the per-feature selection took 9k of a workgroup's 61k cycles.)
+    auto features = [&](auto L, auto G0, auto OUT) {
+      constexpr int LAY = decltype(L)::value, NOBS = LAY == 0 ? 99 : 96;
+      static_for<FPG / 8>([&](auto C) {
+        constexpr int c = decltype(C)::value, f0 = 8 * (decltype(G0)::value + G * c);
+        bf16x8 x8;
+        static_for<8>([&](auto Q) {
+          constexpr int q = decltype(Q)::value, f = f0 + q;
+          float x = 0.f;
+          if constexpr (f < NOBS) {
+            const float o = obs_eval(LAY, f, [&](int row) { return rawat(row, r); });
+            if constexpr (decltype(OUT)::value) {
+              if (e < a.nenv) a.obs_out[(size_t)e * NOBS + f] = o;
+            }
+            // (lanes past the last env hold a copy of its raw inputs: finite
+            // rows whose outputs are never stored)
+            x = fminf(fmaxf((o - om[8 * c + q]) * oi[8 * c + q], -a.clip), a.clip);
+          }
+          x8[q] = (bf16)x;
+        });
+        *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + f0]) = x8;   // one 16-byte LDS store
+      });
+    };
+    static_for<2>([&](auto L) {
+      static_for<G>([&](auto G0) {
+        if (a.layout == decltype(L)::value && g == decltype(G0)::value) {
+          if (a.obs_out) features(L, G0, std::true_type{});
+          else features(L, G0, std::false_type{});
+        }
+      });
+    });
   }
   __syncthreads();
   if (a.push_interval && tid < R) {
@@ -686,9 +737,11 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
         // outputs are independent standard normals for actions j and j + 8
         const float u1 = 1.0f - draw_u01(a.seed, a.t, e, 2 * j);
         const float u2 = draw_u01(a.seed, a.t, e, 2 * j + 1);
-        const float rad = sqrtf(-2.0f * logf(u1));
-        float sn, cs;
-        sincosf(6.283185307179586f * u2, &sn, &cs);
+        // v_log_f32 is log2 (u1 >= 2^-24: no denormal input); v_sin_f32 and
+        // v_cos_f32 take their argument in revolutions, so sin(2 pi u2) is
+        // one instruction with no range reduction (u2 in [0, 1))
+        const float rad = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));   // -2 ln 2 log2(u1)
+        const float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
         act = fmaf(expf(a.log_std[j]), rad * cs, act);
         if (two) act2 = fmaf(expf(a.log_std[j2]), rad * sn, act2);
       }
