@@ -483,7 +483,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
 
 // ---------------------------------------------------------------- policy
 // src/g1sim.py:76-82
-__constant__ float kActionScale[NU] = {
+constexpr float kActionScale[NU] = {
     0.443200f, 0.274054f, 0.443200f, 0.274054f, 0.365131f, 0.365131f,
     0.443200f, 0.274054f, 0.443200f, 0.274054f, 0.365131f, 0.365131f,
     0.443200f, 0.365131f, 0.365131f,
@@ -609,6 +609,18 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
   // groups)
   constexpr int G = NTHR / R;
   const int r = tid % R, g = tid / R;
+  // log std of the group's action pairs (j, j + 8), jb = g + G p: uniform,
+  // held in scalar registers through the network
+  constexpr int NPAIR = (16 + G - 1) / G;
+  float lsd[2 * NPAIR];
+#pragma unroll
+  for (int p = 0; p < NPAIR; ++p) {
+    const int jb = g + G * p, j = jb < 8 ? jb : jb + 8;
+    const float l0 = a.log_std && jb < 16 && j < NU ? a.log_std[j] : 0.f;
+    const float l1 = a.log_std && jb < 16 && j + 8 < NU ? a.log_std[j + 8] : 0.f;
+    lsd[2 * p] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, l0)));
+    lsd[2 * p + 1] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, l1)));
+  }
   Ring<POL_PIN, 512, RT> ring0;
   {
     // stage the block's raw inputs in LDS, raw row q of env r at float
@@ -657,7 +669,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
     // constants (one uniform branch picks the instance): every feature's kind,
     // raw row and offset fold into its code.  (With both known only at run
     // time, the per-feature selection took 9k of a workgroup's 61k cycles.)
-    auto features = [&](auto L, auto G0, auto OUT) {
+    auto features = [&](auto L, auto G0, auto OUT) __attribute__((always_inline)) {
       constexpr int LAY = decltype(L)::value, NOBS = LAY == 0 ? 99 : 96;
       static_for<FPG / 8>([&](auto C) {
         constexpr int c = decltype(C)::value, f0 = 8 * (decltype(G0)::value + G * c);
@@ -722,36 +734,47 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
   dense_out<128, 32, RT>(b1, out, a.W[3], a.b[3], w, lane, ring3);
   __syncthreads();
   // epilogue: thread (r, g) -> env r; actions in pairs (j, j + 8) that share
-  // one Box-Muller draw, j from {0..7, 16..23}, split over the G groups
+  // one Box-Muller draw, j from {0..7, 16..23}, pair p of group g being
+  // jb = g + G p.  The group index is a compile-time constant (one uniform
+  // branch picks the instance), so the action scales and default angles are
+  // instruction literals; the standard deviations were read with the stage.
+  // (A loop over runtime j loaded them from memory on every iteration: 6.5k
+  // of a workgroup's 55k cycles.)
   const int e = e0 + r;
+  auto pairs = [&](auto G0) __attribute__((always_inline)) {
+    static_for<NPAIR>([&](auto P) {
+      constexpr int p = decltype(P)::value, jb = decltype(G0)::value + G * p;
+      constexpr int j = jb < 8 ? jb : jb + 8, j2 = j + 8;
+      if constexpr (jb < 16 && j < NU) {
+        constexpr bool two = j2 < NU;
+        float act = out[r * LDO + j];
+        float act2 = two ? out[r * LDO + j2] : 0.f;
+        if (a.log_std) {
+          // Box-Muller: u1 in (0, 1], u2 in [0, 1); the cosine and sine
+          // outputs are independent standard normals for actions j and j + 8.
+          // v_log_f32 is log2 (u1 >= 2^-24: no denormal input); v_sin_f32 and
+          // v_cos_f32 take their argument in revolutions, so sin(2 pi u2) is
+          // one instruction with no range reduction (u2 in [0, 1))
+          const float u1 = 1.0f - draw_u01(a.seed, a.t, e, 2 * j);
+          const float u2 = draw_u01(a.seed, a.t, e, 2 * j + 1);
+          const float rad = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));   // -2 ln 2 log2(u1)
+          const float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
+          act = fmaf(expf(lsd[2 * p]), rad * cs, act);
+          if constexpr (two) act2 = fmaf(expf(lsd[2 * p + 1]), rad * sn, act2);
+        }
+        a.ctrl[j * N + e] = (float)g1m_qpos_stand[7 + j] + act * kActionScale[j];
+        if (a.actions) a.actions[j * N + e] = act;
+        if constexpr (two) {
+          a.ctrl[j2 * N + e] = (float)g1m_qpos_stand[7 + j2] + act2 * kActionScale[j2];
+          if (a.actions) a.actions[j2 * N + e] = act2;
+        }
+      }
+    });
+  };
   if (e < a.nenv) {
-    for (int jb = g; jb < 16; jb += G) {
-      const int j = jb < 8 ? jb : jb + 8;
-      if (j >= NU) continue;
-      const int j2 = j + 8;
-      const bool two = j2 < NU;
-      float act = out[r * LDO + j];
-      float act2 = two ? out[r * LDO + j2] : 0.f;
-      if (a.log_std) {
-        // Box-Muller: u1 in (0, 1], u2 in [0, 1); the cosine and sine
-        // outputs are independent standard normals for actions j and j + 8
-        const float u1 = 1.0f - draw_u01(a.seed, a.t, e, 2 * j);
-        const float u2 = draw_u01(a.seed, a.t, e, 2 * j + 1);
-        // v_log_f32 is log2 (u1 >= 2^-24: no denormal input); v_sin_f32 and
-        // v_cos_f32 take their argument in revolutions, so sin(2 pi u2) is
-        // one instruction with no range reduction (u2 in [0, 1))
-        const float rad = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));   // -2 ln 2 log2(u1)
-        const float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
-        act = fmaf(expf(a.log_std[j]), rad * cs, act);
-        if (two) act2 = fmaf(expf(a.log_std[j2]), rad * sn, act2);
-      }
-      a.ctrl[j * N + e] = (float)g1m_qpos_stand[7 + j] + act * kActionScale[j];
-      if (a.actions) a.actions[j * N + e] = act;
-      if (two) {
-        a.ctrl[j2 * N + e] = (float)g1m_qpos_stand[7 + j2] + act2 * kActionScale[j2];
-        if (a.actions) a.actions[j2 * N + e] = act2;
-      }
-    }
+    static_for<G>([&](auto G0) {
+      if (g == decltype(G0)::value) pairs(G0);
+    });
   }
 }
 
