@@ -44,6 +44,10 @@ def load():
     lib.g1_hip_step_schedule.argtypes = [i, i, i, i, f, vp, vp, vp, vp, C.c_longlong, vp, vp, vp, vp, vp, u, vp]
     lib.g1_mlp_gemm.restype = i
     lib.g1_mlp_gemm.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp]
+    lib.g1_mlp_gemm_ex.restype = i
+    lib.g1_mlp_gemm_ex.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp, vp]
+    lib.g1_mlp_mse_grad_part.restype = i
+    lib.g1_mlp_mse_grad_part.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
     lib.g1_mlp_mse_grad.restype = i
     lib.g1_mlp_mse_grad.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
     lib.g1_mlp_sum_splits.restype = i
@@ -59,6 +63,10 @@ def load():
     lib.g1_mlp_adam_multi.restype = i
     lib.g1_mlp_adam_multi.argtypes = [i, pa, pa, pa, pa, pa, pa, C.POINTER(C.c_int), pa, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float), vp, f, f, f, vp]
+    lib.g1_mlp_adam_multi_sum.restype = i
+    lib.g1_mlp_adam_multi_sum.argtypes = [i, pa, pa, pa, pa, pa, pa, C.POINTER(C.c_int), pa, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_float), vp, f,
+                                          f, f, vp]
     lib.g1_mlp_adam.restype = i
     lib.g1_mlp_adam.argtypes = [vp, vp, vp, vp, vp, vp, i, f, f, f, f, i, f, vp]
     lib.g1_mlp_fill.restype = i
@@ -67,6 +75,8 @@ def load():
     lib.g1_mlp_to_bf16.argtypes = [vp, vp, C.c_longlong, vp]
     lib.g1_mlp_gather.restype = i
     lib.g1_mlp_gather.argtypes = [vp, vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
+    lib.g1_mlp_gather_rows.restype = i
+    lib.g1_mlp_gather_rows.argtypes = [vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
     lib.g1_mlp_features.restype = i
     lib.g1_mlp_features.argtypes = [i, i, vp, vp, vp, vp, vp, vp, i, vp]
     lib.g1_hip_forces.restype = i
@@ -117,7 +127,8 @@ def exported_symbols():
             "g1_hip_surrogate_integrate", "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs",
             "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
             "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
-            "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule"]
+            "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_mse_grad_part",
+            "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows"]
 
 
 def ptr_array(tensors_or_ptrs):

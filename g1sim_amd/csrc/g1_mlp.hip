@@ -63,6 +63,7 @@ struct GemmArgs {
   int ldc;               // leading dimension of C and H
   const float* bias;     // [J]
   const float* Zin;      // pre-activation for SILU_BWD, leading dimension ldc
+  float* colpart;        // SILU_BWD: column sums of dZ per 64-row block, [I / 64][ldc] (or null)
 };
 
 // One operand tile (128 rows x BR) of a step goes global -> registers (load,
@@ -167,6 +168,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __syncthreads();
   }
 
+  // column sums of the SILU_BWD output over this lane's rows, per column tile
+  float csum[F];
+#pragma unroll
+  for (int b = 0; b < F; ++b) csum[b] = 0.f;
   // D fragment: lane holds rows 4*(lane/16)+q, column lane%16 of each 16x16 tile
 #pragma unroll
   for (int a = 0; a < F; ++a)
@@ -195,10 +200,42 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
           g.C[o] = v + g.bias[j];
         } else if constexpr (EPI == EPI_SILU_BWD) {
           const float d = v * g1act::silu_grad(g.Zin[o]);
-          g.C[o] = d;
+          if (g.C) g.C[o] = d;
           g.H[o] = (bf16)d;
+          csum[b] += d;
         }
       }
+  if constexpr (EPI == EPI_SILU_BWD) {
+    // The bias gradient's partial sums, in a fixed order: per lane over its
+    // row tiles and rows (a, q ascending), then over the 4 lane groups of a
+    // column (xor 16, then xor 32: every lane ends with the same bits), then
+    // (64-row tiles) the two waves of a column half, wave row 0 first.
+    if (g.colpart) {                          // uniform
+#pragma unroll
+      for (int b = 0; b < F; ++b) {
+        csum[b] += __shfl_xor(csum[b], 16, 64);
+        csum[b] += __shfl_xor(csum[b], 32, 64);
+      }
+      if constexpr (T == 128) {
+        // a wave's 64 rows are one 64-row block
+        if (lane < 16) {
+#pragma unroll
+          for (int b = 0; b < F; ++b)
+            g.colpart[(size_t)(blockIdx.x * 2 + wi) * g.ldc + j0 + wj * (T / 2) + b * 16 + lr] = csum[b];
+        }
+      } else {
+        // the workgroup's 64 rows are one block: the waves of a column half
+        // (wi = 0, 1) meet in LDS (the operand tiles are no longer read)
+        float* red = reinterpret_cast<float*>(&sA[0][0][0]);          // [2][T]
+        if (lane < 16) {
+#pragma unroll
+          for (int b = 0; b < F; ++b) red[wi * T + wj * (T / 2) + b * 16 + lr] = csum[b];
+        }
+        __syncthreads();
+        if (tid < T) g.colpart[(size_t)blockIdx.x * g.ldc + j0 + tid] = red[tid] + red[T + tid];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- training
@@ -230,6 +267,48 @@ __global__ __launch_bounds__(256) void mse_grad_kernel(const float* pred, const 
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = e2;
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(sse, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+}
+
+// The same loss gradient without atomics, for the training iteration: one
+// workgroup per block of 64 rows x 32 columns writes dY as bf16, the block's
+// column sums of dY (the output layer's bias-gradient partials,
+// colpart[(i / 64) * ld + j]) and its summed squared error
+// (sse_part[(i / 64) * (ld / 32) + j / 32]), each in a fixed order: a
+// thread takes rows h, h + 8, ... of its column, the 8 row groups then add
+// in order.
+__global__ __launch_bounds__(256) void mse_grad_part_kernel(const float* pred, const float* target, int ld, int n_out,
+                                                            float scale, bf16* dY16, float* colpart,
+                                                            float* sse_part) {
+  __shared__ float red[8][32];
+  __shared__ float wsum[4];
+  const int tid = threadIdx.x, jl = tid & 31, h = tid >> 5;
+  const int j = blockIdx.y * 32 + jl;
+  const size_t r0 = (size_t)blockIdx.x * 64;
+  float cs = 0.f, e2 = 0.f;
+#pragma unroll
+  for (int r = h; r < 64; r += 8) {
+    const size_t idx = (r0 + r) * ld + j;
+    float d = 0.f;
+    if (j < n_out) {
+      const float e = pred[idx] - target[idx];
+      d = scale * e;
+      e2 += e * e;
+    }
+    dY16[idx] = (bf16)d;
+    cs += d;
+  }
+  red[h][jl] = cs;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+  if ((tid & 63) == 0) wsum[tid >> 6] = e2;
+  __syncthreads();
+  if (tid < 32) {
+    float c = red[0][tid];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) c += red[k][tid];
+    colpart[(size_t)blockIdx.x * ld + j] = c;
+  }
+  if (tid == 0) sse_part[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 // out[i] = sum over z < split of parts[z * n + i] (the r-slices of a split
@@ -366,9 +445,13 @@ __global__ void colsum_final_kernel(const float* part, int nblk, int ld, float* 
 // Adam over several tensors in one launch (the update of adam_kernel), with
 // the learning rate and bias corrections read from sched.
 constexpr int ADAM_MAX = 16;
+constexpr int ADAM_WIDE = 16;    // more gradient slices than this: one wave per element
 struct AdamSet {
   int ntensor;
-  int offset[ADAM_MAX + 1];      // element offset of each tensor in the flat index space
+  int offset[ADAM_MAX + 1];      // first thread of each tensor (a wave-aligned run for a wide tensor)
+  int n[ADAM_MAX];               // elements
+  int nsum[ADAM_MAX];            // gradient = sum of nsum slices g + z * gstride, z ascending
+  long long gstride[ADAM_MAX];
   float* p[ADAM_MAX];
   float* m[ADAM_MAX];
   float* v[ADAM_MAX];
@@ -386,10 +469,32 @@ __global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float*
   if (i >= s.offset[s.ntensor]) return;
   int k = 0;
   while (k + 1 < s.ntensor && i >= s.offset[k + 1]) ++k;
-  const int j = i - s.offset[k];
+  const int nk = s.nsum[k];
+  const float* gk = s.g[k];
+  const long long gs = s.gstride[k];
+  int j;
+  float gsum;
+  if (nk > ADAM_WIDE) {
+    // one wave per element (the run is wave-aligned and whole waves long):
+    // lane l sums slices l, l + 64, ... in order, then a butterfly over the
+    // wave; lane 0 carries on with the element
+    const int t = i - s.offset[k], l = t & 63;
+    j = t >> 6;
+    float acc = 0.f;
+    for (int z = l; z < nk; z += 64) acc += gk[(size_t)z * gs + j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (l != 0) return;
+    gsum = acc;
+  } else {
+    j = i - s.offset[k];
+    if (j >= s.n[k]) return;                   // gap before a wave-aligned run
+    gsum = gk[j];
+    for (int z = 1; z < nk; ++z) gsum += gk[(size_t)z * gs + j];
+  }
   bf16* p16 = s.p16[k];
   bf16* p16t = s.p16t[k];
-  const int tj = p16t ? (j % s.cols[k]) * ((s.offset[k + 1] - s.offset[k]) / s.cols[k]) + j / s.cols[k] : 0;
+  const int tj = p16t ? (j % s.cols[k]) * (s.n[k] / s.cols[k]) + j / s.cols[k] : 0;
   if (s.mask[k] && !s.mask[k][j]) {
     if (p16) p16[j] = (bf16)0.f;
     if (p16t) p16t[tj] = (bf16)0.f;
@@ -397,7 +502,7 @@ __global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float*
   }
   const float lr = sched[0], bc1 = sched[1], bc2 = sched[2];
   float* p = s.p[k];
-  const float gi = s.g[k][j] + s.wd[k] * p[j];
+  const float gi = gsum + s.wd[k] * p[j];
   const float mi = b1 * s.m[k][j] + (1.f - b1) * gi;
   const float vi = b2 * s.v[k][j] + (1.f - b2) * gi * gi;
   s.m[k][j] = mi;
@@ -442,6 +547,29 @@ __global__ void gather_kernel(const float* X, const float* Y, const int* idx, in
       if (f < nout) v = (Y[(step * nout + f) * nenv + env] - ymean[f]) * yinv[f];
       Yn[(size_t)b * ldy + f] = v;
     }
+  }
+}
+
+// The same samples from a sample-major copy of the dataset, rows
+// S[step][env][nfeat + nout] (features, then targets): one wave per sample
+// reads its row with consecutive lanes on consecutive floats; the same
+// arithmetic as gather_kernel, so the same bits.
+__global__ __launch_bounds__(256) void gather_rows_kernel(const float* S, const int* idx, int B, int nfeat, int nout,
+                                                          int nenv, const float* xmean, const float* xinv,
+                                                          const float* ymean, const float* yinv, bf16* Xn, int ldx,
+                                                          float* Yn, int ldy) {
+  const int lane = threadIdx.x & 63, b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= B) return;
+  const float* row = S + ((size_t)idx[2 * b] * nenv + idx[2 * b + 1]) * (size_t)(nfeat + nout);
+  for (int f = lane; f < ldx; f += 64) {
+    float v = 0.f;
+    if (f < nfeat) v = (row[f] - xmean[f]) * xinv[f];
+    Xn[(size_t)b * ldx + f] = (bf16)v;
+  }
+  for (int f = lane; f < ldy; f += 64) {
+    float v = 0.f;
+    if (f < nout) v = (row[nfeat + f] - ymean[f]) * yinv[f];
+    Yn[(size_t)b * ldy + f] = v;
   }
 }
 
@@ -491,20 +619,24 @@ int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
 extern "C" {
 
 // C-ABI of the MLP kernels; the contract of each entry point is documented with its declaration.
-int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
-                int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, void* stream) {
+int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
+                   int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, float* colpart,
+                   void* stream) {
   g1err::clear();
   if (I <= 0 || J <= 0 || R <= 0 || split <= 0 || I % BI || J % BJ || R % (BR * split))
     return g1err::fail("g1_mlp_gemm: shape %d x %d x %d (split %d) is not a multiple of the %d x %d x %d tile",
                        I, J, R, split, BI, BJ, BR);
   if (!A || !B) return g1err::fail("g1_mlp_gemm: A and B are required");
-  if ((epi != EPI_BIAS_SILU_H && !C) || ((epi == EPI_BIAS_SILU || epi == EPI_BIAS_SILU_H || epi == EPI_SILU_BWD) && !H))
+  if (colpart && (epi != EPI_SILU_BWD || ldc < J))
+    return g1err::fail("g1_mlp_gemm: column partial sums are an output of the SILU_BWD epilogue (ldc >= J)");
+  const bool needs_c = epi != EPI_BIAS_SILU_H && !(epi == EPI_SILU_BWD && colpart);
+  if ((needs_c && !C) || ((epi == EPI_BIAS_SILU || epi == EPI_BIAS_SILU_H || epi == EPI_SILU_BWD) && !H))
     return g1err::fail("g1_mlp_gemm: epilogue %d needs its C / H outputs", epi);
   if ((epi >= EPI_BIAS_SILU && epi <= EPI_BIAS_F32 && !bias) || (epi == EPI_SILU_BWD && !Zin))
     return g1err::fail("g1_mlp_gemm: epilogue %d needs bias / Zin", epi);
   if (split > 1 && epi != EPI_F32_ATOMIC && epi != EPI_F32_SPLIT)
     return g1err::fail("g1_mlp_gemm: split-R needs the F32_ATOMIC or F32_SPLIT epilogue");
-  GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin};
+  GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin, colpart};
   hipStream_t s = (hipStream_t)stream;
   const int key = (a_trans ? 1 : 0) | (b_trans ? 2 : 0);
   switch (key * 8 + epi) {
@@ -522,6 +654,11 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
   }
 }
 
+int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
+                int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, void* stream) {
+  return g1_mlp_gemm_ex(a_trans, b_trans, epi, A, lda, B, ldb, I, J, R, split, C, H, ldc, bias, Zin, nullptr, stream);
+}
+
 int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16, float* dY32,
                     float* sse, void* stream) {
   g1err::clear();
@@ -531,6 +668,17 @@ int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n
   hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, pred, target, B, ld, n_out,
                      (bf16*)dY16, dY32, sse);
   return g1err::launch("g1_mlp_mse_grad");
+}
+
+int g1_mlp_mse_grad_part(const float* pred, const float* target, int B, int ld, int n_out, void* dY16,
+                         float* colpart, float* sse_part, void* stream) {
+  g1err::clear();
+  if (B <= 0 || B % 64 || ld <= 0 || ld % 32 || n_out < 0 || n_out > ld || !pred || !target || !dY16 || !colpart ||
+      !sse_part)
+    return g1err::fail("g1_mlp_mse_grad_part: invalid arguments (B a multiple of 64, ld of 32)");
+  hipLaunchKernelGGL(mse_grad_part_kernel, dim3(B / 64, ld / 32), dim3(256), 0, (hipStream_t)stream, pred, target, ld,
+                     n_out, 2.0f / (float)(B * n_out), (bf16*)dY16, colpart, sse_part);
+  return g1err::launch("g1_mlp_mse_grad_part");
 }
 
 int g1_mlp_sum_splits(const float* parts, int split, long long n, float* out, void* stream) {
@@ -581,10 +729,10 @@ int g1_mlp_sample(int B, int lo, int hi, int nenv, unsigned seed, const int* sta
   return g1err::launch("g1_mlp_sample");
 }
 
-int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
-                      void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
-                      const int* n, const float* wd, const float* sched, float b1, float b2, float eps,
-                      void* stream) {
+int g1_mlp_adam_multi_sum(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
+                          void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
+                          const int* n, const int* nsum, const long long* gstride, const float* wd,
+                          const float* sched, float b1, float b2, float eps, void* stream) {
   g1err::clear();
   if (ntensor <= 0 || ntensor > ADAM_MAX || !p || !m || !v || !grad || !n || !sched)
     return g1err::fail("g1_mlp_adam_multi: invalid arguments (1 to %d tensors)", ADAM_MAX);
@@ -594,8 +742,16 @@ int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* cons
   for (int k = 0; k < ntensor; ++k) {
     if (!p[k] || !m[k] || !v[k] || !grad[k] || n[k] <= 0)
       return g1err::fail("g1_mlp_adam_multi: tensor %d is missing or empty", k);
+    s.n[k] = n[k];
+    s.nsum[k] = nsum ? nsum[k] : 1;
+    s.gstride[k] = gstride ? gstride[k] : 0;
+    if (s.nsum[k] <= 0 || (s.nsum[k] > 1 && s.gstride[k] < n[k]))
+      return g1err::fail("g1_mlp_adam_multi: tensor %d: %d gradient slices at stride %lld (>= %d elements)", k,
+                         s.nsum[k], s.gstride[k], n[k]);
+    const bool wide = s.nsum[k] > ADAM_WIDE;
+    if (wide) total = (total + 63) / 64 * 64;       // a whole number of waves per element
     s.offset[k] = (int)total;
-    total += n[k];
+    total += wide ? 64LL * n[k] : (long long)n[k];
     s.p[k] = p[k]; s.m[k] = m[k]; s.v[k] = v[k]; s.g[k] = grad[k];
     s.p16[k] = p16 ? (bf16*)p16[k] : nullptr;
     s.p16t[k] = p16t ? (bf16*)p16t[k] : nullptr;
@@ -610,6 +766,14 @@ int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* cons
   hipLaunchKernelGGL(adam_multi_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, s,
                      sched, b1, b2, eps);
   return g1err::launch("g1_mlp_adam_multi");
+}
+
+int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
+                      void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
+                      const int* n, const float* wd, const float* sched, float b1, float b2, float eps,
+                      void* stream) {
+  return g1_mlp_adam_multi_sum(ntensor, p, m, v, grad, p16, p16t, cols, mask, n, nullptr, nullptr, wd, sched, b1, b2,
+                               eps, stream);
 }
 
 int g1_mlp_adam(float* p, float* m, float* v, const float* grad, void* p16, const uint8_t* mask, int n, float lr,
@@ -647,6 +811,18 @@ int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfe
   hipLaunchKernelGGL(gather_kernel, dim3(B), dim3(128), 0, (hipStream_t)stream, X, Y, idx, B, nfeat, nout, nenv,
                      xmean, xinv, ymean, yinv, (bf16*)Xn, ldx, Yn, ldy);
   return g1err::launch("g1_mlp_gather");
+}
+
+int g1_mlp_gather_rows(const float* S, const int* idx, int B, int nfeat, int nout, int nenv, const float* xmean,
+                       const float* xinv, const float* ymean, const float* yinv, void* Xn, int ldx, float* Yn, int ldy,
+                       void* stream) {
+  g1err::clear();
+  if (B <= 0 || nfeat <= 0 || nout <= 0 || nenv <= 0 || nfeat > ldx || nout > ldy || !S || !idx || !xmean || !xinv ||
+      !ymean || !yinv || !Xn || !Yn)
+    return g1err::fail("g1_mlp_gather_rows: invalid arguments");
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((B + 3) / 4), dim3(256), 0, (hipStream_t)stream, S, idx, B, nfeat, nout,
+                     nenv, xmean, xinv, ymean, yinv, (bf16*)Xn, ldx, Yn, ldy);
+  return g1err::launch("g1_mlp_gather_rows");
 }
 
 int g1_mlp_features(int nenv, int nrows, const float* qpos, const float* qvel, const float* tau, const float* mean,

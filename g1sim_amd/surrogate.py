@@ -87,11 +87,12 @@ class MLP:
         if rc != 0:
             raise RuntimeError(f"{what} failed ({rc}): {_lib.last_error()}")
 
-    def gemm(self, a_t, b_t, epi, A, lda, B, ldb, I, J, R, C=None, H=None, ldc=0, bias=None, Z=None, split=1):
-        rc = self.lib.g1_mlp_gemm(a_t, b_t, EPI[epi], A.data_ptr(), lda, B.data_ptr(), ldb, I, J, R, split,
-                                  None if C is None else C.data_ptr(), None if H is None else H.data_ptr(), ldc,
-                                  None if bias is None else bias.data_ptr(), None if Z is None else Z.data_ptr(),
-                                  self.stream)
+    def gemm(self, a_t, b_t, epi, A, lda, B, ldb, I, J, R, C=None, H=None, ldc=0, bias=None, Z=None, split=1,
+             colpart=None):
+        rc = self.lib.g1_mlp_gemm_ex(a_t, b_t, EPI[epi], A.data_ptr(), lda, B.data_ptr(), ldb, I, J, R, split,
+                                     None if C is None else C.data_ptr(), None if H is None else H.data_ptr(), ldc,
+                                     None if bias is None else bias.data_ptr(), None if Z is None else Z.data_ptr(),
+                                     None if colpart is None else colpart.data_ptr(), self.stream)
         self._check(rc, f"gemm {epi}")
 
     def workspace(self, B):
@@ -100,21 +101,25 @@ class MLP:
             return self._ws[B]
         f32 = dict(dtype=torch.float32, device=self.dev)
         L = len(self.dims)
+        split = self._split(B)
         ws = {
             "X16": torch.zeros((B, PIN), dtype=torch.bfloat16, device=self.dev),
             "T": torch.zeros((B, POUT), **f32),
-            "dY32": torch.zeros((B, POUT), **f32),
             "Z": [torch.zeros((B, self.dims[l][0]), **f32) for l in range(L - 1)],
             "H16": [torch.zeros((B, self.dims[l][0]), dtype=torch.bfloat16, device=self.dev) for l in range(L - 1)],
             "pred": torch.zeros((B, POUT), **f32),
             "dY16": torch.zeros((B, POUT), dtype=torch.bfloat16, device=self.dev),
-            "dZ": [torch.zeros((B, self.dims[l][0]), **f32) for l in range(L - 1)],
             "dZ16": [torch.zeros((B, self.dims[l][0]), dtype=torch.bfloat16, device=self.dev) for l in range(L - 1)],
+            # the gradients as the iteration leaves them for Adam: the r-slices of
+            # each weight-gradient GEMM (split over the batch) and the 64-row
+            # column partials of each bias gradient
+            "gW_split": [torch.zeros((split, r * c), **f32) for r, c, _ in self.dims],
+            "gb_part": [torch.zeros((B // 64, r), **f32) for r, _, _ in self.dims],
+            # summed squared error per block of 64 rows x 32 columns
+            "sse": torch.zeros((B // 64, POUT // 32), **f32),
+            # the summed gradients (gradients(): for inspection; Adam adds the slices itself)
             "gW": [torch.zeros_like(w) for w in self.W],
             "gb": [torch.zeros_like(b) for b in self.b],
-            "sse": torch.zeros(1, **f32),
-            # r-slices of the weight-gradient GEMMs (split over the batch), summed by g1_mlp_sum_splits
-            "splits": torch.zeros(self._split(B) * max(r * c for r, c, _ in self.dims), **f32),
         }
         self._ws[B] = ws
         return ws
@@ -150,64 +155,77 @@ class MLP:
         return split
 
     def backward(self, X16, B, dY16):
-        """Gradients of the loss into ws['gW'], ws['gb'] given dL/dpred (bf16)."""
+        """Gradients of the loss given dL/dpred (bf16; the output layer's bias
+        partials come with it from loss_grad): the weight gradients as the
+        r-slices of split GEMMs in ws['gW_split'], the bias gradients as
+        64-row column partials in ws['gb_part'] (written by the data-gradient
+        GEMMs' epilogue).  Adam adds the slices; gradients() sums them for
+        inspection."""
         ws = self.workspace(B)
         L = len(self.dims)
         split = self._split(B)
-        d16, dfp = dY16, None
+        d16 = dY16
         for l in range(L - 1, -1, -1):
             rows, cols, _ = self.dims[l]
             hin = X16 if l == 0 else ws["H16"][l - 1]
             # dW = d^T h: reduction over the batch, split across gridDim.z into
-            # r-slices (plain stores), then one pass adds the slices
-            self.gemm(1, 1, "F32_SPLIT", d16, rows, hin, cols, rows, cols, B, C=ws["splits"], ldc=cols, split=split)
-            self._check(self.lib.g1_mlp_sum_splits(ws["splits"].data_ptr(), split, rows * cols,
-                                                   ws["gW"][l].data_ptr(), self.stream), "sum_splits")
-            dsrc = ws["dY32"] if l == L - 1 else dfp
-            # bias gradient: column sums (the split buffer is free again after sum_splits)
-            self._check(self.lib.g1_mlp_colsum_det(dsrc.data_ptr(), B, rows, ws["splits"].data_ptr(),
-                                                   ws["gb"][l].data_ptr(), self.stream), "colsum")
+            # r-slices (plain stores)
+            self.gemm(1, 1, "F32_SPLIT", d16, rows, hin, cols, rows, cols, B, C=ws["gW_split"][l], ldc=cols,
+                      split=split)
             if l > 0:
                 prow = self.dims[l - 1][0]
-                # dH_{l-1} = d W_l, through SiLU'(Z_{l-1})
+                # dH_{l-1} = d W_l through SiLU'(Z_{l-1}), as bf16, and the
+                # column partials of the layer l-1 bias gradient
                 self.gemm(0, 0, "SILU_BWD", d16, rows, self.W16T[l], rows, B, cols, rows,
-                          C=ws["dZ"][l - 1], H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1])
-                d16, dfp = ws["dZ16"][l - 1], ws["dZ"][l - 1]
+                          H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1], colpart=ws["gb_part"][l - 1])
+                d16 = ws["dZ16"][l - 1]
+
+    def gradients(self, B):
+        """Sums the slices backward() left into ws['gW'] and ws['gb'] (in the
+        order Adam adds them for the weights; z ascending for the biases)."""
+        ws = self.workspace(B)
+        for l, (rows, cols, _) in enumerate(self.dims):
+            for parts, out, n in ((ws["gW_split"][l], ws["gW"][l], rows * cols), (ws["gb_part"][l], ws["gb"][l], rows)):
+                self._check(self.lib.g1_mlp_sum_splits(parts.data_ptr(), parts.shape[0], n, out.data_ptr(),
+                                                       self.stream), "sum_splits")
+        return ws["gW"], ws["gb"]
 
     def adam(self, B, b1=0.9, b2=0.999, eps=1e-8):
-        """One Adam step on every weight and bias (one launch), with the
-        learning rate and bias corrections of self.sched (g1_mlp_train_schedule)."""
+        """One Adam step on every weight and bias (one launch) from the
+        gradient slices backward() left, with the learning rate and bias
+        corrections of self.sched (g1_mlp_train_schedule)."""
         ws = self.workspace(B)
         if self._adam_args is None or self._adam_args[0] is not ws:
             L = len(self.dims)
             ps, ms, vs = self.W + self.b, self.mW + self.mb, self.vW + self.vb
-            gs, p16, p16t = ws["gW"] + ws["gb"], self.W16 + [None] * L, self.W16T + [None] * L
+            gs, p16, p16t = ws["gW_split"] + ws["gb_part"], self.W16 + [None] * L, self.W16T + [None] * L
             masks = self.maskW + self.maskb
             n = (C.c_int * (2 * L))(*[t.numel() for t in ps])
+            nsum = (C.c_int * (2 * L))(*[g.shape[0] for g in gs])
+            gstride = (C.c_longlong * (2 * L))(*[g.shape[1] for g in gs])
             cols = (C.c_int * (2 * L))(*([w.shape[1] for w in self.W] + [1] * L))
             wd = (C.c_float * (2 * L))(*([0.0] * (2 * L)))
             self._adam_args = (ws, [_lib.ptr_array(x) for x in (ps, ms, vs, gs, p16)], _lib.ptr_array(p16t), cols,
-                               _lib.ptr_array(masks), n, wd)
-        _, arrs, p16t, cols, masks, n, wd = self._adam_args
-        self._check(self.lib.g1_mlp_adam_multi(len(n), *arrs, p16t, cols, masks, n, wd, self.sched.data_ptr(), b1,
-                                               b2, eps, self.stream), "adam")
+                               _lib.ptr_array(masks), n, nsum, gstride, wd)
+        _, arrs, p16t, cols, masks, n, nsum, gstride, wd = self._adam_args
+        self._check(self.lib.g1_mlp_adam_multi_sum(len(n), *arrs, p16t, cols, masks, n, nsum, gstride, wd,
+                                                   self.sched.data_ptr(), b1, b2, eps, self.stream), "adam")
 
-    def loss_grad(self, pred, target, B, want_sse=True):
-        """MSE on the 35 meaningful outputs: dY as bf16 (GEMM operand) and fp32
-        (output bias gradient); returns the device SSE tensor (None when not
-        wanted: the reduction is skipped)."""
+    def loss_grad(self, pred, target, B):
+        """MSE on the 35 meaningful outputs: dY as bf16 (the GEMM operand),
+        the output layer's bias-gradient partials (ws['gb_part'][-1]) and the
+        summed squared error per block of 64 rows x 32 columns, which it
+        returns (a device tensor; its sum is the batch's SSE)."""
         ws = self.workspace(B)
-        if want_sse:
-            self._check(self.lib.g1_mlp_fill(ws["sse"].data_ptr(), 0.0, 1, self.stream), "fill")
-        self._check(self.lib.g1_mlp_mse_grad(pred.data_ptr(), target.data_ptr(), B, POUT, NOUT,
-                                             ws["dY16"].data_ptr(), ws["dY32"].data_ptr(),
-                                             ws["sse"].data_ptr() if want_sse else None, self.stream), "mse")
-        return ws["sse"] if want_sse else None
+        self._check(self.lib.g1_mlp_mse_grad_part(pred.data_ptr(), target.data_ptr(), B, POUT, NOUT,
+                                                  ws["dY16"].data_ptr(), ws["gb_part"][-1].data_ptr(),
+                                                  ws["sse"].data_ptr(), self.stream), "mse")
+        return ws["sse"]
 
     def iteration(self, ds, idx, B, lo, hi, seed, iters, lr, lr_end):
         """One training iteration as a fixed launch sequence (every scalar that
         changes between iterations is read on the device): schedule, batch
-        sampling, gather, forward, MSE gradient (and its SSE into
+        sampling, gather, forward, MSE gradient (its SSE per block into
         workspace['sse']), backward, Adam."""
         ws = self.workspace(B)
         s = self.stream
@@ -215,9 +233,12 @@ class MLP:
                                                    lr_end, 0.9, 0.999, s), "schedule")
         self._check(self.lib.g1_mlp_sample(B, lo, hi, ds.nenv, seed & 0xFFFFFFFF, self.train_state.data_ptr(),
                                            idx.data_ptr(), s), "sample")
-        self.gather(ds.X, ds.Y, idx, B, ds.nenv, ws["X16"], ws["T"])
+        rc = self.lib.g1_mlp_gather_rows(ds.rows().data_ptr(), idx.data_ptr(), B, NIN, NOUT, ds.nenv,
+                                         self.xmean.data_ptr(), self.xinv.data_ptr(), self.ymean.data_ptr(),
+                                         self.yinv.data_ptr(), ws["X16"].data_ptr(), PIN, ws["T"].data_ptr(), POUT, s)
+        self._check(rc, "gather_rows")
         pred = self.forward(ws["X16"], B, train=True)
-        self.loss_grad(pred, ws["T"], B, want_sse=True)
+        self.loss_grad(pred, ws["T"], B)
         self.backward(ws["X16"], B, ws["dY16"])
         self.adam(B)
 
@@ -277,10 +298,25 @@ class Dataset:
     def __init__(self, X, Y):
         self.X, self.Y = X, Y
         self.steps, _, self.nenv = X.shape
+        self._rows = None
 
     @property
     def n(self):
         return self.steps * self.nenv
+
+    def rows(self, chunk=64):
+        """Sample-major copy S [steps][nenv][106 + 35] (features, then targets),
+        built on first use a few steps at a time: the training minibatch reads
+        each sample as one contiguous row (g1_mlp_gather_rows) instead of 141
+        scattered floats."""
+        if self._rows is None:
+            S = torch.empty((self.steps, self.nenv, NIN + NOUT), dtype=torch.float32, device=self.X.device)
+            for s0 in range(0, self.steps, chunk):
+                s1 = min(self.steps, s0 + chunk)
+                S[s0:s1, :, :NIN].copy_(self.X[s0:s1].transpose(1, 2))
+                S[s0:s1, :, NIN:].copy_(self.Y[s0:s1].transpose(1, 2))
+            self._rows = S
+        return self._rows
 
 
 def generate_dataset(device, nenv=16384, steps=1221, dt=2e-3, seed=7, reset_below=0.35):
@@ -346,7 +382,7 @@ def train(mlp, ds, iters=4000, batch=8192, lr=1e-3, lr_end=1e-5, holdout_steps=N
 
     def logged(it):
         if it % 100 == 0 or it == iters - 1:
-            v = float(ws["sse"].item()) / (batch * NOUT)
+            v = float(ws["sse"].sum().item()) / (batch * NOUT)
             hist.append((it, v))
             if log:
                 log(f"iter {it} mse {v:.4e}")

@@ -109,11 +109,24 @@ int g1_hip_step_schedule(int nenv, int nsteps, int period, int step0, float dt, 
 int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
                 int I, int J, int R, int split, float* C, void* H, int ldc, const float* bias,
                 const float* Zin, void* stream);
+/* g1_mlp_gemm with colpart (nullable; G1_MLP_EPI_SILU_BWD only): the column
+ * sums of the epilogue's fp32 output over each block of 64 rows,
+ * colpart[(i / 64) * ldc + j], each summed in a fixed order (a bias
+ * gradient's partials); with colpart given, C is nullable. */
+int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
+                   int I, int J, int R, int split, float* C, void* H, int ldc, const float* bias,
+                   const float* Zin, float* colpart, void* stream);
 /* dY = 2 (pred - target) / (B * n_out) on columns < n_out, 0 beyond, as
  * bf16 (dY16) and fp32 (dY32, nullable); *sse += sum of squared errors when
  * sse is non-null.  pred, target: fp32 [B][ld]. */
 int g1_mlp_mse_grad(const float* pred, const float* target, int B, int ld, int n_out, void* dY16,
                     float* dY32, float* sse, void* stream);
+/* The same dY (bf16 only) without atomics: per block of 64 rows, the column
+ * sums of dY, colpart[(i / 64) * ld + j], and per block of 64 rows x 32
+ * columns the summed squared error, sse_part[(i / 64) * (ld / 32) + j / 32],
+ * each in a fixed order.  B a multiple of 64, ld of 32. */
+int g1_mlp_mse_grad_part(const float* pred, const float* target, int B, int ld, int n_out, void* dY16,
+                         float* colpart, float* sse_part, void* stream);
 /* db[j] += sum_i dZ[i][j] over fp32 dZ [B][ld]. */
 int g1_mlp_colsum(const float* dZ, int B, int ld, float* db, void* stream);
 /* out[i] = sum_{z < split} parts[z * n + i] (the slices of a G1_MLP_EPI_F32_SPLIT
@@ -147,6 +160,17 @@ int g1_mlp_adam_multi(int ntensor, float* const* p, float* const* m, float* cons
                       void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
                       const int* n, const float* wd, const float* sched, float b1, float b2, float eps,
                       void* stream);
+/* g1_mlp_adam_multi with each gradient given as nsum[k] slices (grad[k] +
+ * z * gstride[k], z < nsum[k], gstride[k] >= n[k]) that the step adds in a
+ * fixed order: z ascending for up to 16 slices (the slices of a split
+ * G1_MLP_EPI_F32_SPLIT GEMM, the same bits as g1_mlp_sum_splits), one wave
+ * per element beyond (lane l adds slices l, l + 64, ..., then a butterfly
+ * over the wave: the 64-row partials of g1_mlp_gemm_ex and
+ * g1_mlp_mse_grad_part).  nsum and gstride nullable (one slice each). */
+int g1_mlp_adam_multi_sum(int ntensor, float* const* p, float* const* m, float* const* v, const float* const* grad,
+                          void* const* p16, void* const* p16t, const int* cols, const uint8_t* const* mask,
+                          const int* n, const int* nsum, const long long* gstride, const float* wd,
+                          const float* sched, float b1, float b2, float eps, void* stream);
 /* x[0..n) = v (a fill kernel, usable where memset nodes are not). */
 int g1_mlp_fill(float* x, float v, long long n, void* stream);
 /* y (bf16) = x (fp32), n elements. */
@@ -158,6 +182,13 @@ int g1_mlp_to_bf16(const float* x, void* y, long long n, void* stream);
 int g1_mlp_gather(const float* X, const float* Y, const int* idx, int B, int nfeat, int nout, int nenv,
                   const float* xmean, const float* xinv, const float* ymean, const float* yinv,
                   void* Xn, int ldx, float* Yn, int ldy, void* stream);
+
+/* g1_mlp_gather from a sample-major copy of the dataset, rows
+ * S[step][env][nfeat + nout] (features, then targets); Yn required.  The
+ * same bits as g1_mlp_gather. */
+int g1_mlp_gather_rows(const float* S, const int* idx, int B, int nfeat, int nout, int nenv, const float* xmean,
+                       const float* xinv, const float* ymean, const float* yinv, void* Xn, int ldx, float* Yn,
+                       int ldy, void* stream);
 
 /* Network input of every env: z-scored (qpos, qvel, tau) as bf16 rows of
  * Xn [nrows][ldx] (nrows >= nenv, padding rows and columns >= 106 zero;

@@ -134,6 +134,7 @@ def test_forward_backward_against_cpu(device):
     pred = mlp.forward(ws["X16"], B, train=True)
     mlp.loss_grad(pred, ws["T"], B)
     mlp.backward(ws["X16"], B, ws["dY16"])
+    mlp.gradients(B)
     torch.cuda.synchronize()
     rpred, rgW, rgb = _cpu_reference(mlp, X, T)
     assert np.abs(pred.cpu().numpy()[:, :NOUT] - rpred[:, :NOUT]).max() < 2e-3 * max(1, np.abs(rpred).max())
@@ -202,6 +203,16 @@ def test_gather_normalises(device):
     assert np.allclose(got[:, :NIN], bf16(ex), rtol=0, atol=1e-6) or np.abs(got[:, :NIN] - ex).max() < 2e-2
     assert (got[:, NIN:] == 0).all()
     assert np.allclose(Yn.cpu().numpy()[:, :NOUT], ey, rtol=1e-5, atol=1e-5)
+    # the sample-major rows give the same bits
+    S = d(np.concatenate([X, Y], axis=1).transpose(0, 2, 1))
+    Xn2 = torch.full((B, PIN), 3.0, dtype=torch.bfloat16, device=device)
+    Yn2 = torch.full((B, POUT), 3.0, device=device)
+    rc = lib.g1_mlp_gather_rows(S.data_ptr(), ti.data_ptr(), B, NIN, NOUT, nenv, txm.data_ptr(), txi.data_ptr(),
+                                tym.data_ptr(), tyi.data_ptr(), Xn2.data_ptr(), PIN, Yn2.data_ptr(), POUT,
+                                torch.cuda.current_stream(device).cuda_stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert torch.equal(Xn2, Xn) and torch.equal(Yn2, Yn)
 
 
 def test_gemm_split_slices_sum_to_product(device):
@@ -388,3 +399,126 @@ def test_gemm_128_tiles(device, a_t, b_t, epi):
     else:
         d = acc * dsilu(Z.astype(np.float32))
         assert (np.abs(C - d) <= tol * 1.2 + 1e-6).all()
+
+
+@pytest.mark.parametrize("I,J,R", [(256, 384, 256), (8192, 1024, 64)])
+def test_silu_bwd_column_partials(device, I, J, R):
+    """The SILU_BWD epilogue's bias-gradient partials: column sums of its fp32
+    output over each 64-row block (both tile sizes: 256 x 384 runs 64 x 64
+    tiles, 8192 x 1024 the 128 x 128 ones), and the same bf16 output with
+    the fp32 one left out."""
+    lib = _lib.load()
+    rng = np.random.default_rng(11)
+    Aop, Bop, Z = rng.normal(size=(I, R)), rng.normal(size=(J, R)), rng.normal(size=(I, J))
+    s = _stream(device)
+    At = torch.as_tensor(Aop.astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    Bt = torch.as_tensor(Bop.T.copy().astype(np.float32), device=device).to(torch.bfloat16).contiguous()
+    Zt = torch.as_tensor(Z.astype(np.float32), device=device)
+    Cf = torch.zeros((I, J), device=device)
+    H1 = torch.zeros((I, J), dtype=torch.bfloat16, device=device)
+    H2 = torch.zeros_like(H1)
+    part = torch.full((I // 64, J), 7.0, device=device)
+    assert lib.g1_mlp_gemm(0, 1, EPI["SILU_BWD"], At.data_ptr(), R, Bt.data_ptr(), J, I, J, R, 1, Cf.data_ptr(),
+                           H1.data_ptr(), J, None, Zt.data_ptr(), s) == 0
+    assert lib.g1_mlp_gemm_ex(0, 1, EPI["SILU_BWD"], At.data_ptr(), R, Bt.data_ptr(), J, I, J, R, 1, None,
+                              H2.data_ptr(), J, None, Zt.data_ptr(), part.data_ptr(), s) == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    assert torch.equal(H1, H2)
+    c = Cf.cpu().numpy().astype(np.float64).reshape(I // 64, 64, J)
+    got = part.cpu().numpy()
+    np.testing.assert_allclose(got, c.sum(axis=1), rtol=1e-5, atol=1e-5 * np.abs(c).sum(axis=1).max())
+    # colpart belongs to the SILU_BWD epilogue only
+    assert lib.g1_mlp_gemm_ex(0, 1, EPI["F32"], At.data_ptr(), R, Bt.data_ptr(), J, I, J, R, 1, Cf.data_ptr(),
+                              None, J, None, None, part.data_ptr(), s) == -1
+
+
+def test_mse_grad_partials_match_atomic_version(device):
+    """g1_mlp_mse_grad_part: the same bf16 dY as g1_mlp_mse_grad, 64-row column
+    sums of the fp32 dY and per-block squared errors that add up to its SSE."""
+    lib = _lib.load()
+    rng = np.random.default_rng(12)
+    B, ld, n_out = 1024, POUT, NOUT
+    pred = rng.normal(size=(B, ld)).astype(np.float32)
+    tgt = rng.normal(size=(B, ld)).astype(np.float32)
+    tp, tt = torch.as_tensor(pred, device=device), torch.as_tensor(tgt, device=device)
+    d16a = torch.zeros((B, ld), dtype=torch.bfloat16, device=device)
+    d16b = torch.full((B, ld), 5.0, dtype=torch.bfloat16, device=device)
+    d32 = torch.zeros((B, ld), device=device)
+    sse = torch.zeros(1, device=device)
+    part = torch.full((B // 64, ld), 7.0, device=device)
+    sp = torch.full((B // 64, ld // 32), 7.0, device=device)
+    s = _stream(device)
+    assert lib.g1_mlp_mse_grad(tp.data_ptr(), tt.data_ptr(), B, ld, n_out, d16a.data_ptr(), d32.data_ptr(),
+                               sse.data_ptr(), s) == 0
+    assert lib.g1_mlp_mse_grad_part(tp.data_ptr(), tt.data_ptr(), B, ld, n_out, d16b.data_ptr(), part.data_ptr(),
+                                    sp.data_ptr(), s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(d16a, d16b)
+    d = d32.cpu().numpy().astype(np.float64)
+    assert (d[:, n_out:] == 0).all()
+    np.testing.assert_allclose(part.cpu().numpy(), d.reshape(B // 64, 64, ld).sum(axis=1), rtol=1e-5, atol=1e-9)
+    e = np.zeros((B, ld))
+    e[:, :n_out] = (pred[:, :n_out].astype(np.float64) - tgt[:, :n_out]) ** 2
+    np.testing.assert_allclose(sp.cpu().numpy(), e.reshape(B // 64, 64, ld // 32, 32).sum(axis=(1, 3)), rtol=1e-5,
+                               atol=1e-9)
+    np.testing.assert_allclose(sp.cpu().numpy().sum(), sse.item(), rtol=1e-5)
+    assert lib.g1_mlp_mse_grad_part(tp.data_ptr(), tt.data_ptr(), B + 1, ld, n_out, d16b.data_ptr(),
+                                    part.data_ptr(), sp.data_ptr(), s) == -1        # B not a multiple of 64
+
+
+def test_adam_over_gradient_slices(device):
+    """g1_mlp_adam_multi_sum: a gradient given as up to 16 slices gives the
+    bits of g1_mlp_sum_splits followed by the one-slice step; 128 slices (one
+    wave per element) match the step on the float64 sum of the slices."""
+    lib = _lib.load()
+    rng = np.random.default_rng(13)
+    s = _stream(device)
+    mk = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=device)  # noqa: E731
+    sizes, nslices = [3000, 512, 35], [8, 128, 128]
+    slices = [rng.normal(size=(k, n)).astype(np.float32) for n, k in zip(sizes, nslices)]
+    p0 = [rng.normal(size=n).astype(np.float32) for n in sizes]
+    masks = [mk((rng.random(n) < 0.9).astype(np.uint8)) for n in sizes]
+    sched = mk(np.array([1e-3, 1 - 0.9, 1 - 0.999, 0], np.float32))
+    n_arr = (C.c_int * 3)(*sizes)
+    # reference: summed gradients (sum_splits for the 8-slice tensor, float64 for the wide ones), one-slice step
+    summed = []
+    for k in range(3):
+        if nslices[k] <= 16:
+            out = torch.zeros(sizes[k], device=device)
+            assert lib.g1_mlp_sum_splits(mk(slices[k]).data_ptr(), nslices[k], sizes[k], out.data_ptr(), s) == 0
+            summed.append(out)
+        else:
+            summed.append(mk(slices[k].astype(np.float64).sum(axis=0).astype(np.float32)))
+    runs = []
+    for use_slices in (False, True):
+        P = [mk(p.copy()) for p in p0]
+        M = [torch.zeros(n, device=device) for n in sizes]
+        V = [torch.zeros(n, device=device) for n in sizes]
+        P16 = [torch.zeros(n, dtype=torch.bfloat16, device=device) for n in sizes]
+        if use_slices:
+            G = [mk(x) for x in slices]
+            nsum = (C.c_int * 3)(*nslices)
+            gstride = (C.c_longlong * 3)(*sizes)
+            rc = lib.g1_mlp_adam_multi_sum(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
+                                           _lib.ptr_array(G), _lib.ptr_array(P16), None, None,
+                                           _lib.ptr_array(masks), n_arr, nsum, gstride, None, sched.data_ptr(), 0.9,
+                                           0.999, 1e-8, s)
+        else:
+            rc = lib.g1_mlp_adam_multi(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
+                                       _lib.ptr_array(summed), _lib.ptr_array(P16), None, None,
+                                       _lib.ptr_array(masks), n_arr, None, sched.data_ptr(), 0.9, 0.999, 1e-8, s)
+        assert rc == 0, _lib.last_error()
+        torch.cuda.synchronize()
+        runs.append(([x.cpu().numpy() for x in P], [x.cpu().numpy() for x in M]))
+    (pa, ma), (pb, mb) = runs
+    assert np.array_equal(pa[0], pb[0]) and np.array_equal(ma[0], mb[0])          # up to 16 slices: same bits
+    for k in (1, 2):
+        np.testing.assert_allclose(mb[k], ma[k], rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(pb[k], pa[k], rtol=1e-6, atol=1e-7)
+        mk_ = masks[k].cpu().numpy() == 0
+        assert np.array_equal(pb[k][mk_], p0[k][mk_])
+    # slices closer than the tensor length are refused
+    bad = (C.c_longlong * 3)(sizes[0] - 1, sizes[1], sizes[2])
+    assert lib.g1_mlp_adam_multi_sum(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
+                                     _lib.ptr_array([mk(x) for x in slices]), None, None, None, None, n_arr,
+                                     (C.c_int * 3)(*nslices), bad, None, sched.data_ptr(), 0.9, 0.999, 1e-8, s) == -1
