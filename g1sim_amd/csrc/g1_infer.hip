@@ -329,12 +329,24 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   auto feature_src = [&](int f) -> const float* {
     return f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N : a.tau + (size_t)(f - 71) * N;
   };
+  // The feature group is a compile-time constant in the loads and the
+  // normalisation (one uniform branch picks the instance): each feature's
+  // source row and its statistics' offsets are literals, the 8 statistics of
+  // a chunk one scalar load.  (With fg known only at run time the stage took
+  // 10k of a 128-env workgroup's 117k cycles.)
   float raw[FPP];
-#pragma unroll
-  for (int i = 0; i < FPP; ++i) {
-    const int f = 8 * (fg + G * (i / 8)) + i % 8;
-    raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
-  }
+  auto load_pass = [&](auto P) __attribute__((always_inline)) {
+    static_for<G>([&](auto FG) {
+      if (fg == decltype(FG)::value) {
+        static_for<FPP>([&](auto I) {
+          constexpr int ii = decltype(P)::value * FPP + decltype(I)::value;
+          constexpr int f = 8 * (decltype(FG)::value + G * (ii / 8)) + ii % 8;
+          raw[decltype(I)::value] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
+        });
+      }
+    });
+  };
+  load_pass(std::integral_constant<int, 0>{});
   float abav[APW] = {};
   if constexpr (MODE == 1) {
     if (judged_blk) {
@@ -388,35 +400,36 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     if (fg == 0) stash[STASH_FB * ROWS + r] = fb ? 1.f : 0.f;
     if (tid < 2 * NV) ysc[tid] = ys;
   }
-#pragma unroll
-  for (int p = 0; p < NPASS; ++p) {
-    if (p > 0) {
-#pragma unroll
-      for (int i = 0; i < FPP; ++i) {
-        const int ii = p * FPP + i, f = 8 * (fg + G * (ii / 8)) + ii % 8;
-        raw[i] = f < SUR_NIN ? feature_src(f)[ec] : 0.f;
-      }
+  static_for<NPASS>([&](auto P) {
+    constexpr int p = decltype(P)::value;
+    if constexpr (p > 0) {
+      load_pass(P);
       __builtin_amdgcn_sched_barrier(0);
     }
-#pragma unroll
-    for (int i0 = 0; i0 < FPP; i0 += 8) {
-      bf16x8 x8;
-#pragma unroll
-      for (int i = i0; i < i0 + 8; ++i) {
-        const int ii = p * FPP + i, f = 8 * (fg + G * (ii / 8)) + ii % 8;
-        float x = 0.f;
-        if (e < a.nenv && f < SUR_NIN) x = (raw[i] - a.xmean[f]) * a.xinv[f];
-        x8[i - i0] = (bf16)x;
-        if constexpr (MODE == 1) {
-          if (f < STASH_ABA) stash[f * ROWS + r] = raw[i];     // qpos, qvel as read
-        }
+    static_for<G>([&](auto FG) {
+      if (fg == decltype(FG)::value) {
+        static_for<FPP / 8>([&](auto C8) {
+          constexpr int i0 = 8 * decltype(C8)::value, ii0 = p * FPP + i0;
+          constexpr int f0 = 8 * (decltype(FG)::value + G * (ii0 / 8));
+          bf16x8 x8;
+          static_for<8>([&](auto Q) {
+            constexpr int q = decltype(Q)::value, f = f0 + q;
+            float x = 0.f;
+            if constexpr (f < SUR_NIN) {
+              x = (raw[i0 + q] - a.xmean[f]) * a.xinv[f];
+              if (e >= a.nenv) x = 0.f;
+              if constexpr (MODE == 1 && f < STASH_ABA) stash[f * ROWS + r] = raw[i0 + q];   // qpos, qvel as read
+            }
+            x8[q] = (bf16)x;
+          });
+          // one 16-byte LDS store (a 2-byte store per feature at the 1,040-byte
+          // row stride put 8 lanes on each bank)
+          *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + f0]) = x8;
+        });
       }
-      // one 16-byte LDS store (a 2-byte store per feature at the 1,040-byte
-      // row stride put 8 lanes on each bank)
-      *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + 8 * (fg + G * ((p * FPP + i0) / 8))]) = x8;
-    }
+    });
     __builtin_amdgcn_sched_barrier(0);
-  }
+  });
   __syncthreads();
   // layer l reads b[l % 2] and writes b[(l + 1) % 2]; one buffer with 128 rows
   bf16* b0 = buf[0];
