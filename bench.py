@@ -284,13 +284,17 @@ def _stepper_run(dev, mlp, nenv, steps, warmup, tol, seed):
     ctrl = torch.as_tensor(cb, device=dev)
     qpos, qvel, anchor = (torch.as_tensor(x, device=dev) for x in (qp, qv, an))
     st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
-    for s in range(warmup):
-        st.step(qpos, qvel, anchor, ctrl[s // CTRL_PERIOD])
+
+    def run(first_block, nsteps):
+        # the steps of each target block in one host call
+        for s0 in range(0, nsteps, CTRL_PERIOD):
+            st.steps(qpos, qvel, anchor, ctrl[first_block + s0 // CTRL_PERIOD], min(CTRL_PERIOD, nsteps - s0))
+
+    run(0, warmup)
     wb = _blocks(warmup)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for s in range(steps):
-        st.step(qpos, qvel, anchor, ctrl[wb + s // CTRL_PERIOD])
+    run(wb, steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     summ = st.summary()
@@ -376,8 +380,8 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     st4 = SurrogateStepper(mlp, n4, dev, period=16, tol=1e-3)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
-    for s in range(1000):
-        st4.step(sq, sv, sa, ctrl4[s // CTRL_PERIOD])
+    for blk in range(1000 // CTRL_PERIOD):
+        st4.steps(sq, sv, sa, ctrl4[blk], CTRL_PERIOD)
     torch.cuda.synchronize()
     wall4 = time.perf_counter() - t4
     step_launches(lib, n4, (aq, av, aa), ctrl4, 0, 1000, CTRL_PERIOD, sptr=torch.cuda.current_stream(dev).cuda_stream)

@@ -75,6 +75,9 @@ def load():
     lib.g1_mlp_to_bf16.argtypes = [vp, vp, C.c_longlong, vp]
     lib.g1_mlp_gather.restype = i
     lib.g1_mlp_gather.argtypes = [vp, vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
+    lib.g1_hip_surrogate_steps.restype = i
+    lib.g1_hip_surrogate_steps.argtypes = [i, i, i, f, i, f, vp, vp, vp, vp, vp, vp, vp, pa, pa, vp, vp, vp, vp, vp,
+                                           vp, vp, vp, vp, vp, u, vp]
     lib.g1_mlp_gather_rows.restype = i
     lib.g1_mlp_gather_rows.argtypes = [vp, vp, i, i, i, i, vp, vp, vp, vp, vp, i, vp, i, vp]
     lib.g1_mlp_features.restype = i
@@ -128,7 +131,7 @@ def exported_symbols():
             "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
             "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
             "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_mse_grad_part",
-            "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows"]
+            "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows", "g1_hip_surrogate_steps"]
 
 
 def ptr_array(tensors_or_ptrs):

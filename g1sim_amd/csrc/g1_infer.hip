@@ -42,6 +42,7 @@
 #include "g1_commit.h"
 #include "g1_act.h"
 #include "g1_err.h"
+#include "../../include/g1sim_amd.h"
 
 namespace g1inf {
 
@@ -1122,6 +1123,32 @@ int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, 
   hipLaunchKernelGGL((surrogate_kernel<1, ROWS>), dim3((nenv + ROWS - 1) / ROWS), dim3(Rows<ROWS>::NT), 0,
                      (hipStream_t)stream, a);
   return g1err::launch("g1_hip_surrogate_step");
+}
+
+// nsteps stepper steps in one host call: per step the judge + forces kernel
+// (g1_hip_judge_forces, defined in g1_step.hip) and the network + commit
+// kernel, phase (t0 + s) % period, the same launches in the same order as
+// nsteps pairs of calls from the host.
+int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos, float* qvel,
+                           float* anchor, const float* ctrl, const float* mu, const float* xmean, const float* xinv,
+                           const void* const* W, const float* const* b, const float* ymean, const float* ystd,
+                           float* qacc_aba, float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                           float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
+                           void* stream) {
+  g1err::clear();
+  if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
+      !qacc_aba || !tau || !stats)
+    return g1err::fail("g1_hip_surrogate_steps: invalid arguments");
+  for (int s = 0; s < nsteps; ++s) {
+    const int phase = (int)(((long long)t0 + s) % period);
+    int rc = g1_hip_judge_forces(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_aba, tau,
+                                 contact_forces, foot_pos, foot_vel, judge_flags, stream);
+    if (rc) return rc;
+    rc = g1_hip_surrogate_step(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean, ystd,
+                               qacc_aba, fallback, stats, qacc_out, stream);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const float* commands,

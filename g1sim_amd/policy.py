@@ -195,8 +195,7 @@ class PolicyRollout:
                                     None, 0, s)
             _lib.check(rc)
         else:
-            for _ in range(self.nsub):
-                self.stepper.step(self.qpos, self.qvel, self.anchor, self.ctrl)
+            self.stepper.steps(self.qpos, self.qvel, self.anchor, self.ctrl, self.nsub)
 
     def post_phase(self, reward_sum=None, done_count=None):
         """Reward, termination, timers, commands and the reset mask."""

@@ -538,6 +538,34 @@ class SurrogateStepper:
                "surrogate_integrate")
         self.t += 1
 
+    def steps(self, qpos, qvel, anchor, ctrl, n, mu=None):
+        """n steps with ctrl (and mu) fixed: the launches of n step() calls,
+        in the same order, from one host call (g1_hip_surrogate_steps) -- the
+        per-step host work of step() otherwise bounds the stepper at small
+        env counts.  The unfused configuration loops over step()."""
+        if n < 0:
+            raise ValueError("n must be >= 0")
+        if not self.fused:
+            for _ in range(n):
+                self.step(qpos, qvel, anchor, ctrl, mu)
+            return
+        lib, s, ne = self.lib, torch.cuda.current_stream(self.dev).cuda_stream, self.nenv
+        for name, t, rows in (("qpos", qpos, NQ), ("qvel", qvel, NV), ("anchor", anchor, 16), ("ctrl", ctrl, NV - 6)):
+            check_soa(name, t, rows, ne, self.dev)
+        if mu is not None:
+            check_soa("mu", mu.view(1, -1), 1, ne, self.dev)
+        m = self.mlp
+        W, b = m._wb()
+        ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),
+                                              qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(), ptr(mu),
+                                              m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
+                                              self.ystd.data_ptr(), self.qacc_aba.data_ptr(), self.tau.data_ptr(),
+                                              self.fallback.data_ptr(), self.stats.data_ptr(), ptr(self.qacc),
+                                              self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
+                                              self.foot_vel.data_ptr(), self.judge_flags, s), "surrogate_steps")
+        self.t += n
+
     def summary(self):
         st = self.stats.cpu().numpy().astype(np.uint32)
         judged, rejected, fb = int(st[0]), int(st[1]), int(st[3])

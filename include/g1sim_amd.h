@@ -278,6 +278,18 @@ int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, 
                           const float* const* b, const float* ymean, const float* ystd,
                           const float* qacc_aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
                           void* stream);
+/* nsteps steps of the surrogate stepper in one host call: for s < nsteps,
+ * with phase (t0 + s) % period, g1_hip_judge_forces (ABA qacc of the judged
+ * and fallback envs into qacc_aba, everyone's tau_total into tau, the contact
+ * update) then g1_hip_surrogate_step; the same launches, in the same order,
+ * as 2 * nsteps calls (ctrl and mu fixed over the call).  qacc_out,
+ * contact_forces, foot_pos, foot_vel, mu and fallback nullable. */
+int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
+                           float* qvel, float* anchor, const float* ctrl, const float* mu, const float* xmean,
+                           const float* xinv, const void* const* W, const float* const* b,
+                           const float* ymean, const float* ystd, float* qacc_aba, float* tau,
+                           uint8_t* fallback, unsigned* stats, float* qacc_out, float* contact_forces,
+                           float* foot_pos, float* foot_vel, unsigned judge_flags, void* stream);
 
 /* RL observation of every env (replaces the torch code of
  * src/g1sim.py:132-170).  layout 0: 99 values per env in the reference order

@@ -398,3 +398,42 @@ def test_judge_forces_matches_its_parts(device, n):
     for k, rk in (("tau", "tau"), ("fn", "fn"), ("fp", "foot_pos"), ("fv", "foot_vel"), ("anc", "anchor")):
         same_at_f32(h[k][:, stepped], ref[rk][:, stepped], k + " stepped", unset=1e30)
         same_at_f32(h[k][:, ~stepped], f[k][:, ~stepped], k + " forces", unset=1e30)
+
+
+@pytest.mark.parametrize("tol", [1e-3, 1e30])
+def test_steps_equal_repeated_step(device, tol):
+    """SurrogateStepper.steps (one host call, g1_hip_surrogate_steps) and the
+    same number of step() calls give the same bits: state, feet, contact
+    forces, qacc, fallback flags and statistics; with the judge accepting
+    everything (tol 1e30) the network's qacc is integrated, at 1e-3 every
+    env stays with ABA."""
+    mlp = MLP(device, seed=4)
+    n = 5000
+    qp, qv, an, cb = W.randomized_actuation(n, seed=27, nblocks=1)
+    runs = []
+    for one_call in (True, False):
+        st = GpuState(qp, qv, an, cb[0], device)
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=tol, track_qacc=True)
+        stepper.steps(st.qpos, st.qvel, st.anchor, st.ctrl, 3)          # t = 3: later calls start mid-period
+        if one_call:
+            stepper.steps(st.qpos, st.qvel, st.anchor, st.ctrl, 21)
+        else:
+            for _ in range(21):
+                stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
+        torch.cuda.synchronize()
+        assert stepper.t == 24
+        extra = {k: getattr(stepper, k).cpu().numpy() for k in ("fallback", "stats", "qacc", "foot_pos", "foot_vel",
+                                                                  "contact_forces")}
+        runs.append((st.host(), extra))
+    (ha, ea), (hb, eb) = runs
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(ha[k], hb[k]), k
+    for k in ea:
+        assert np.array_equal(ea[k], eb[k]), k
+    summ = None
+    if tol == 1e30:
+        summ = int(ea["stats"][3])
+        assert int(ea["fallback"].sum()) == 0          # every env accepted once judged
+    else:
+        assert int(ea["fallback"].sum()) == n
+    write_metrics(f"surrogate_steps_one_call_tol{tol:g}", {"stats": ea["stats"].tolist(), "fallback_env_steps": summ})
