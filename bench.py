@@ -45,6 +45,7 @@ METRIC = "env-steps/s @16k envs: FP32 ABA vs surrogate; max qacc rel err vs FP64
 FLOP_PER_ENV_STEP = 41491          # reference formulation, stance (8 contacts)
 BYTES_PER_ENV_LAUNCH = 812         # SoA loads (qpos, qvel, anchor, ctrl) + stores (qpos, qvel, anchor) of
                                    # one launch without the optional foot/contact outputs (nsub independent)
+BYTES_PER_ENV_BLOCK = 116          # ctrl of each further target block a scheduled launch reads
 PEAK_FP32 = 157.3e12               # MI355X FP32 (vector and matrix), dense
 PUBLISHED_16K = 12.8e6             # reference README.md:38-42, RTX 3080 Laptop, 16,384 envs
 PEAK_BF16 = 2.5e15                 # MI355X dense BF16 MFMA
@@ -80,12 +81,16 @@ def step_launches(lib, nenv, state, ctrl, block0, nsteps, nsub, flags=0, base64=
     return subs
 
 
+SCHEDULE_MAX_SUB = 1000      # substeps per launch of g1_hip_step_schedule
+
+
 def schedule_subs(step0, nsteps, period=CTRL_PERIOD):
     """Substeps of each launch g1_hip_step_schedule makes for steps
-    [step0, step0 + nsteps): one launch per target block touched."""
+    [step0, step0 + nsteps): launches of up to 1,000 substeps, each
+    switching target blocks itself."""
     subs, s = [], 0
     while s < nsteps:
-        n = min(period - (step0 + s) % period, nsteps - s)
+        n = min(SCHEDULE_MAX_SUB, nsteps - s)
         subs.append(n)
         s += n
     return subs
@@ -149,7 +154,7 @@ def aba_leg(dev, nenv, steps, warmup):
     wall = time.perf_counter() - t0
     avg_ms = e0.elapsed_time(e1) / len(subs)
     achieved = FLOP_PER_ENV_STEP * nenv * (steps / len(subs)) / (avg_ms * 1e-3)
-    return {"nenv": nenv, "steps": steps, "warmup": warmup, "nsub_per_launch": CTRL_PERIOD,
+    return {"nenv": nenv, "steps": steps, "warmup": warmup, "nsub_per_launch": steps / len(subs),
             "env_steps_per_s": nenv * steps / wall, "avg_launch_ms": avg_ms,
             "kernel": "g1k::step_kernel<false, false, 2, 0>",
             "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
@@ -678,7 +683,11 @@ def main():
                     help="g1_hip_step_ex flags: 1 float64 base solve, 2 float64 base pose, 4 float64 "
                          "foot-contact geometry")
     ap.add_argument("--nsub", type=int, default=CTRL_PERIOD,
-                    help="substeps per launch (must divide the 10-step target period)")
+                    help="substeps per g1_hip_step_ex launch (a divisor of the 10-step target period); "
+                         "with the default 10 each region runs as g1_hip_step_schedule launches of up to "
+                         "1,000 substeps that switch target blocks themselves")
+    ap.add_argument("--per-block", action="store_true",
+                    help="one g1_hip_step_ex launch per target block (10 substeps) instead of the schedule")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--train-iters", type=int, default=12000)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-in-the-loop leg")
@@ -725,7 +734,8 @@ def main():
     sptr = stream.cuda_stream
     if base64 is not None:
         base64.copy_(state[0][:7])
-    default_shape = args.flags == 0 and nenv == 16384 and args.nsub == CTRL_PERIOD
+    use_schedule = args.nsub == CTRL_PERIOD and not args.per_block
+    default_shape = args.flags == 0 and nenv == 16384 and use_schedule
     # The 131,072-env leg runs first, so the headline's warmup and timed
     # steps follow GPU work without an idle gap: after ~0.5 s idle the GPU's
     # clock ramps up again over the first launches (measured on MI355X: a
@@ -733,8 +743,8 @@ def main():
     # launch from idle, 97-99 us after busy work).
     if not args.no_large and default_shape:
         aba131 = aba_leg(dev, 131072, 200, 20)
-    if args.nsub == CTRL_PERIOD:
-        # one host call enqueues every launch of a region (g1_hip_step_schedule)
+    if use_schedule:
+        # one host call enqueues the region's launches (g1_hip_step_schedule)
         schedule_call(lib, nenv, state, ctrl, 0, args.warmup, args.flags, base64, sptr)()
         run = schedule_call(lib, nenv, state, ctrl, wb * CTRL_PERIOD, args.steps, args.flags, base64, sptr)
         subs = schedule_subs(wb * CTRL_PERIOD, args.steps)
@@ -775,14 +785,19 @@ def main():
     avg_ms = region_ms / len(subs)
     nsub_avg = args.steps / len(subs)
     flop_per_launch = FLOP_PER_ENV_STEP * nenv * nsub_avg
+    # target blocks a launch reads beyond its first (the schedule's launches span several)
+    extra_blocks = (len(set((wb * CTRL_PERIOD + k) // CTRL_PERIOD for k in range(args.steps))) - len(subs)
+                    if use_schedule else 0)
+    alg_bytes = (BYTES_PER_ENV_LAUNCH * len(subs) + BYTES_PER_ENV_BLOCK * extra_blocks) * nenv / len(subs)
     achieved = flop_per_launch / (avg_ms * 1e-3)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
-        if t.get("nenv") == nenv and t.get("nsub") == args.nsub and t.get("flags", 0) == args.flags:
-            traffic = t.get("bytes_per_launch")
+        for ent in t.get("entries", []):
+            if ent.get("nenv") == nenv and ent.get("nsub") == nsub_avg and ent.get("flags", 0) == args.flags:
+                traffic = ent.get("bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": value,
@@ -799,7 +814,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "randomized actuation: 16,384 envs, dt 2e-3, PD targets stand+U(+-0.3) "
                                "resampled every 10 steps, FP32 ABA step",
-                   "nenv": nenv, "dt": DT, "nsub_per_launch": args.nsub, "launches": len(subs),
+                   "nenv": nenv, "dt": DT, "target_period": CTRL_PERIOD, "nsub_per_launch": nsub_avg,
+                   "launches": len(subs),
                    "path": "FP32 ABA", "flags": args.flags},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic,
@@ -807,7 +823,7 @@ def main():
                          "true" if args.flags & 2 else "false", "true" if args.flags & 4 else "false",
                          4 if args.flags & 8 else 2 if (args.flags & 16 or nenv >= 32768) else 4),
                      "flop_per_launch": flop_per_launch, "avg_launch_ms": avg_ms,
-                     "algorithmic_bytes_per_launch": BYTES_PER_ENV_LAUNCH * nenv},
+                     "algorithmic_bytes_per_launch": alg_bytes},
         "sanity_standing": ok,
     }
     peaks = measure_peaks()

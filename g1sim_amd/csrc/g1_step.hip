@@ -651,6 +651,12 @@ struct StepArgs {
   // forces and feet stored) and any other block gets the forces only.
   const uint8_t* sel_fallback;
   int sel_phase, sel_period;
+  // PD-target schedule inside one launch (ctrl_stride > 0): the targets of
+  // substep s are block (ctrl_phase + s) / ctrl_period, at ctrl + block *
+  // ctrl_stride (ctrl is the block of substep 0, ctrl_phase < ctrl_period
+  // its substeps done before the launch)
+  long long ctrl_stride;
+  int ctrl_period, ctrl_phase;
 };
 // Kernel variants (separate template instances, so the plain step's code is
 // not touched by the judge modes).
@@ -840,14 +846,25 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     vw[m] = args.qvel[m * N + e];
     om[m] = args.qvel[(3 + m) * N + e];
   }
+  // this wave's PD targets (waist + own chains) from one target block
+  auto load_ctrl = [&](const float* cp, float (&w3)[3], float (&ct)[CPW][MAXJ]) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) w3[m] = cp[(12 + m) * N + e];
+    static_for<CPW>([&](auto C) {
+      constexpr int c = decltype(C)::value;
+      const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
+#pragma unroll
+      for (int m = 0; m < MAXJ; ++m) ct[c][m] = cp[(h0 + (m < nj ? m : nj - 1)) * N + e];
+    });
+  };
   float wq[3], wqd[3], wct[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     wq[m] = args.qpos[(7 + 12 + m) * N + e];
     wqd[m] = args.qvel[(6 + 12 + m) * N + e];
-    wct[m] = args.ctrl[(12 + m) * N + e];
   }
   float cq[CPW][MAXJ], cqd[CPW][MAXJ], cct[CPW][MAXJ];
+  load_ctrl(args.ctrl, wct, cct);
   float anc[CPW][8];
   float mu = cf.mu;
   if (legs && args.mu) mu = args.mu[e];
@@ -859,11 +876,16 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
       const int mm = m < nj ? m : nj - 1;
       cq[c][m] = args.qpos[(7 + h0 + mm) * N + e];
       cqd[c][m] = args.qvel[(6 + h0 + mm) * N + e];
-      cct[c][m] = args.ctrl[(h0 + mm) * N + e];
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) anc[c][m] = legs ? args.anchor[(ch * 8 + m) * N + e] : ANCHOR_FREE;
   });
+  // target schedule: the next block's targets are loaded one substep before
+  // they apply, so their memory latency hides behind that substep
+  const bool sched = MODE == MODE_STEP && args.ctrl_stride > 0;
+  const float* cnext = args.ctrl;
+  int cpos = args.ctrl_phase;                    // substeps of the current block already stepped
+  float nwt[3], nct[CPW][MAXJ];
 
   float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[CPW][MAXJ];
   float fn[CPW][4], footp[CPW][3], footv[CPW][3];
@@ -883,6 +905,22 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   const float dt = cf.dt;
   for (int sub = 0; sub < args.nsub; ++sub) {
     const int buf = sub & 1;
+    if (sched) {                                 // uniform
+      if (cpos == args.ctrl_period) {            // block boundary: the prefetched targets apply
+#pragma unroll
+        for (int m = 0; m < 3; ++m) wct[m] = nwt[m];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c)
+#pragma unroll
+          for (int m = 0; m < MAXJ; ++m) cct[c][m] = nct[c][m];
+        cpos = 0;
+      }
+      if (cpos == args.ctrl_period - 1 && sub + 1 < args.nsub) {
+        cnext += args.ctrl_stride;
+        load_ctrl(cnext, nwt, nct);
+      }
+      ++cpos;
+    }
     // ---- pelvis kinematics (src/g1_core.h:30-37)
     float R0[9], p0[3], v0[6];
     {
@@ -1546,6 +1584,10 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
   return launch_step(a, flags, (hipStream_t)stream);
 }
 
+// Substeps per launch of g1_hip_step_schedule (a launch covers 4 ms of
+// simulated time per 1,000 substeps at dt 2e-3; 16,384 envs take ~9 ms).
+constexpr int kScheduleMaxSub = 1000;
+
 int g1_hip_step_schedule(int nenv, int nsteps, int period, int step0, float dt, float* qpos, float* qvel,
                          float* anchor, const float* ctrl_blocks, long long block_stride, float* foot_pos,
                          float* foot_vel, float* contact_forces, const float* mu, double* base_pose64,
@@ -1572,14 +1614,16 @@ int g1_hip_step_schedule(int nenv, int nsteps, int period, int step0, float dt, 
   a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
   a.qacc = nullptr; a.tau = nullptr; a.mu = mu; a.base64 = base_pose64;
   a.env_list = nullptr; a.list_count = nullptr; a.stride = nenv;
-  // one launch per target block touched: steps step0 + s share block
-  // (step0 + s) / period; each launch runs to the end of its block or of the
-  // schedule, so no launch crosses a target change
+  // launches of up to kScheduleMaxSub substeps, each switching target blocks
+  // itself: step step0 + s uses block (step0 + s) / period
+  a.ctrl_stride = block_stride;
+  a.ctrl_period = period;
   for (int s = 0; s < nsteps;) {
     const int abs_step = step0 + s;
-    const int n = std::min(period - abs_step % period, nsteps - s);
+    const int n = std::min(kScheduleMaxSub, nsteps - s);
     a.nsub = n;
     a.ctrl = ctrl_blocks + (size_t)(abs_step / period) * (size_t)block_stride;
+    a.ctrl_phase = abs_step % period;
     rc = launch_step(a, flags, (hipStream_t)stream);
     if (rc) return rc;
     s += n;

@@ -75,11 +75,13 @@ int g1_hip_step_ex(int nenv, int nsub, float dt, float* qpos, float* qvel, float
  * ctrl_blocks + ((step0 + s) / period) * block_stride ([29][nenv] each,
  * block_stride >= 29 * nenv floats apart) -- the targets of the reference's
  * benchmark loop (src/g1_kernel.cu:227-238) resampled every `period` steps.
- * The host enqueues one launch per target block touched (nsub = the steps of
- * the schedule in that block), so a caller stepping a long open-loop
- * schedule pays one call instead of one per block.  foot_pos / foot_vel /
- * contact_forces (nullable) hold the last launch's outputs.  Returns 0 or a
- * negative error code like g1_hip_step_ex. */
+ * The library enqueues launches of up to 1,000 steps that switch target
+ * blocks themselves (each block's targets loaded one substep before they
+ * apply), bit-identical to calling g1_hip_step_ex block by block; a long
+ * open-loop schedule pays one call and one state load/store per 1,000
+ * steps.  foot_pos / foot_vel / contact_forces (nullable) hold the outputs
+ * of the last step.  Returns 0 or a negative error code like
+ * g1_hip_step_ex. */
 int g1_hip_step_schedule(int nenv, int nsteps, int period, int step0, float dt, float* qpos, float* qvel,
                          float* anchor, const float* ctrl_blocks, long long block_stride, float* foot_pos,
                          float* foot_vel, float* contact_forces, const float* mu, double* base_pose64,

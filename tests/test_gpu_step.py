@@ -251,11 +251,13 @@ def test_reset_done_matches_reference_fixture(device, golden):
                 assert bool((qvel[:, torch.as_tensor(envs, device=device)] == 0).all())
 
 
-@pytest.mark.parametrize("nenv,step0,nsteps", [(4096, 3, 37), (40000, 20, 25)])
+@pytest.mark.parametrize("nenv,step0,nsteps", [(4096, 3, 37), (40000, 20, 25), (100, 7, 1205)])
 def test_step_schedule_matches_launches(device, nenv, step0, nsteps):
-    """g1_hip_step_schedule (one host call, one launch per target block)
-    gives the bits of g1_hip_step_ex launched block by block with the same
-    targets, at both work splits, with per-env friction and feet outputs."""
+    """g1_hip_step_schedule (one host call; launches of up to 1,000
+    substeps that switch target blocks themselves, the next block's targets
+    loaded a substep ahead) gives the bits of g1_hip_step_ex launched block
+    by block with the same targets, at both work splits, with per-env
+    friction and feet outputs; 1,205 steps take two launches."""
     lib = _lib.load()
     s = torch.cuda.current_stream(device).cuda_stream
     nblocks = (step0 + nsteps + 9) // 10
