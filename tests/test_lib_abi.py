@@ -139,3 +139,39 @@ def test_step_schedule_rejects_bad_arguments(args, msg):
                                   None, None, None, None, None, 0, None)
     assert rc == -1
     assert msg in _lib.last_error()
+
+
+def test_training_and_stepper_entry_points_validate_arguments():
+    """Host-side checks of the fused-training and multi-step entry points
+    (before any launch)."""
+    lib = _lib.load()
+    d = C.c_void_p(16)
+    # column partials belong to the SILU_BWD epilogue
+    assert lib.g1_mlp_gemm_ex(0, 0, 0, d, 128, d, 128, 128, 128, 128, 1, d, None, 128, None, None, d, None) == -1
+    assert "SILU_BWD" in _lib.last_error()
+    # SILU_BWD with partials may leave the fp32 output out, not H
+    assert lib.g1_mlp_gemm_ex(0, 0, 5, d, 128, d, 128, 128, 128, 128, 1, None, None, 128, None, d, d, None) == -1
+    # loss partials: the batch a multiple of 64, ld a multiple of 32
+    assert lib.g1_mlp_mse_grad_part(d, d, 100, 128, 35, d, d, d, None) == -1
+    assert lib.g1_mlp_mse_grad_part(d, d, 128, 100, 35, d, d, d, None) == -1
+    assert lib.g1_mlp_mse_grad_part(d, d, 128, 128, 129, d, d, d, None) == -1
+    # Adam over slices: slices closer than the tensor, no tensors
+    n = (C.c_int * 1)(100)
+    one = _lib.ptr_array([16])
+    assert lib.g1_mlp_adam_multi_sum(1, one, one, one, one, None, None, None, None, n, (C.c_int * 1)(4),
+                                     (C.c_longlong * 1)(99), None, d, 0.9, 0.999, 1e-8, None) == -1
+    assert lib.g1_mlp_adam_multi_sum(0, one, one, one, one, None, None, None, None, n, None, None, None, d, 0.9,
+                                     0.999, 1e-8, None) == -1
+    # sample-major gather: the row must hold the features and targets
+    assert lib.g1_mlp_gather_rows(d, d, 8, 106, 35, 10, d, d, d, d, d, 64, d, 128, None) == -1
+    assert lib.g1_mlp_gather_rows(d, d, 8, 106, 35, 10, d, d, d, d, d, 128, None, 128, None) == -1
+    # stepper steps: negative counts, no state
+    ws = _lib.ptr_array([16] * 5)
+    assert lib.g1_hip_surrogate_steps(8, -1, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d, d, d,
+                                      None, None, None, None, 0, None) == -1
+    assert lib.g1_hip_surrogate_steps(8, 2, 0, 2e-3, 16, 1e-3, None, d, d, d, None, d, d, ws, ws, d, d, d, d, d,
+                                      d, None, None, None, None, 0, None) == -1
+    assert "g1_hip_surrogate_steps" in _lib.last_error()
+    # zero steps is a no-op
+    assert lib.g1_hip_surrogate_steps(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d, d, d,
+                                      None, None, None, None, 0, None) == 0
