@@ -344,13 +344,16 @@ def test_adam_multi_matches_single_tensor_adam(device):
                                   B16[k].float().cpu().numpy().reshape(-1, c).T)
 
 
-def test_graph_replays_equal_eager_iterations(device):
+@pytest.mark.parametrize("B", [512, 2048])
+def test_graph_replays_equal_eager_iterations(device, B):
     """One training iteration captured as a HIP graph and replayed, with eager
     work and synchronisations between the replays, gives after every replay
-    the same weights, bit for bit, as eager iterations from the same start."""
+    the same weights, bit for bit, as eager iterations from the same start
+    (2,048 samples: 32 bias partials per layer, Adam's one-wave-per-element
+    sum)."""
     from g1sim_amd.surrogate import fit_normalization, generate_dataset
     ds = generate_dataset(device, nenv=1024, steps=12, seed=41)
-    B, iters = 512, 6
+    iters = 6
     nets = [MLP(device, seed=3) for _ in range(2)]
     for m in nets:
         fit_normalization(m, ds)
