@@ -4,8 +4,8 @@
 // forward-only surrogate and the policy) through a whole MLP: it builds the
 // network input from the SoA state, runs every dense layer on
 // v_mfma_f32_16x16x32_bf16 (bf16 operands, fp32 accumulation) with the
-// activations held in LDS (two 64 x 520 bf16 buffers, or one 128 x 520
-// buffer updated in place: 133 KB), and finishes with a fused epilogue, so
+// activations held in LDS (two 64 x 528 bf16 buffers, or one 128 x 528
+// buffer updated in place: 135 KB), and finishes with a fused epilogue, so
 // no activation ever goes to HBM.  Weights are
 // read from L2 (1.7 MB for the surrogate, 0.43 MB for the policy), each
 // lane loading 16 contiguous bytes of a weight row as its B fragment, several
@@ -50,6 +50,17 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Workgroup barrier that orders LDS only: each wave's LDS accesses complete
+// and become visible, while its global loads stay in flight.  __syncthreads()
+// also waits for every outstanding global load, and the next layer's weight
+// fragments are loaded just before the layer barriers, so it put a full L2
+// latency into every layer.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // f(std::integral_constant<int, i>) for i = 0 .. N_ - 1, unrolled at compile time
 template <typename F, int... I>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
@@ -60,7 +71,13 @@ template <int N_, typename F> __device__ __forceinline__ void static_for(F&& f) 
 }
 
 constexpr int ROWS = 64;                 // envs per workgroup of the 64-row kernels
-constexpr int LDA = 512 + 8;             // activation row stride (bf16); 16-byte pad
+// Activation row stride (bf16): 32 bytes of pad.  The A fragments are
+// 16-byte reads, lane l at row l % 16 and k-offset (l / 16) * 8; with an
+// 8-element (16-byte) pad the rows' bank offsets step by 4 dwords and lanes
+// l and l + 16 of adjacent rows meet on the same banks, with 16 elements they
+// step by 8 and never meet (measured on MI355X: the fragment reads of a k-loop
+// 1.57x faster, a 512 x 512 layer's k-loop with L2 weights 5-10 % faster).
+constexpr int LDA = 512 + 16;
 constexpr int LDO = 65;                  // fp32 output row stride in LDS (odd: no bank conflicts)
 constexpr int NV = 35, NU = 29;
 
@@ -68,7 +85,7 @@ enum Act : int { ACT_SILU = 0, ACT_ELU = 1 };
 
 // Rows (envs) per workgroup, both with 8 waves (two per SIMD).  64: two
 // activation buffers, layer l reads one and writes the other.  128: one
-// buffer (128 x 520 bf16 = 133 KB) updated in place -- each layer's output
+// buffer (128 x 528 bf16 = 135 KB) updated in place -- each layer's output
 // stays in the accumulators until every wave has read the layer's input.
 // Every weight fragment a wave loads then serves 8 row tiles instead of 4,
 // which halves the L2 weight traffic per FLOP (measured on MI355X: the
@@ -188,7 +205,10 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
 // hidden layer: out = act(in W^T + b) as bf16 into LDS; the next layer's
 // first weight k-steps (Wn) go into `next` before the epilogue.  With one
 // buffer (sout == sin, 128 rows) a barrier separates the last read of the
-// input from the first write of the output.
+// input from the first write of the output, and `b` is the kernel's LDS copy
+// of the bias (read after the k-loop: its registers are needed in the loop,
+// and from global memory the read put a memory latency before every
+// epilogue).
 template <int NIN, int NOUT, int ACT, int NIN2, int NOUT2, int RTILES>
 __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* __restrict__ W,
                                       const float* __restrict__ b, int w, int lane, Ring<NIN, NOUT, RTILES>& rg,
@@ -197,14 +217,19 @@ __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* _
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
   typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
   // bias of the lane's four columns per column tile: loaded before the k-loop
-  // with 64 rows, after it with 128 (its registers are needed in the loop)
+  // with 64 rows (global memory), after it with 128 (LDS)
   float bc[T::CT][4];
   auto load_bias = [&]() {
 #pragma unroll
     for (int c = 0; c < T::CT; ++c) {
       const int col = (c0 + c) * 16 + (lane >> 4) * 4;
+      if constexpr (RTILES == 8) {
+        const float4 v = *reinterpret_cast<const float4*>(b + col);
+        bc[c][0] = v.x; bc[c][1] = v.y; bc[c][2] = v.z; bc[c][3] = v.w;
+      } else {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
+        for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
+      }
     }
   };
   if constexpr (RTILES == 4) load_bias();
@@ -214,7 +239,7 @@ __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* _
   if constexpr (RTILES == 8) load_bias();
   wprefetch<NIN2, NOUT2, RTILES>(Wn, w, lane, next);
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (RTILES == 8) __syncthreads();      // in place: every wave has read the input
+  if constexpr (RTILES == 8) lds_barrier();        // in place: every wave has read the input
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     // four adjacent columns of one row per lane: one 8-byte LDS store
@@ -248,7 +273,7 @@ __device__ __forceinline__ void dense_out(const bf16* sin, float* sout, const bf
   for (int c = 0; c < T::CT; ++c) bc[c] = b[(c0 + c) * 16 + lr];
   f32x4 acc[T::RT][T::CT];
   mma_layer<NIN, NOUT, RTILES>(sin, W, w, lane, rg, acc);
-  if constexpr (RTILES == 8) __syncthreads();      // in place: every wave has read the input
+  if constexpr (RTILES == 8) lds_barrier();        // in place: every wave has read the input
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
     const int col = (c0 + c) * 16 + lr;
@@ -288,6 +313,235 @@ struct SurrogateArgs {
 constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
 static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 
+// ------------------------------------------ split-half chain (128 rows)
+// The forward-only surrogate with 128 rows per workgroup runs its layers as
+// one unrolled sequence of k-steps in which every SiLU epilogue overlaps
+// MFMA work.  Each wave's four column tiles of a 512-wide layer are split
+// into half A = tiles {2w, 2w + 1} (features 32w .. 32w + 31, all below 256)
+// and half B = tiles {16 + 2w, 17 + 2w} (256 + 32w ..).  A hidden layer is
+//   lo: half A, then half B, over k-steps 0-7 (features 0-255 of the
+//       input), with the previous layer's half-B epilogue interleaved with
+//       the half-A steps and stored into features 256-511 (nobody reads them
+//       until the barrier; done by the end of the half-A steps, so no more
+//       than two sets of accumulators are live);
+//   barrier;
+//   hi: half A, then half B, over k-steps 8-15, with this layer's half-A
+//       epilogue interleaved and stored into features 0-255 (after the
+//       barrier no wave reads them in this layer);
+//   barrier.
+// The first layer reads the network input from columns 256-383 (its
+// half-A epilogue overlaps its half-B k-steps); the output layer's k-steps
+// 0-7 overlap the last hidden layer's half-B epilogue.  Every output element
+// still accumulates its k-steps in ascending order, so the results are those
+// of the layer-by-layer kernel bit for bit.  (Measured on MI355X: the four
+// SiLU epilogues took 36 % of a workgroup's time when no MFMA work ran
+// beside them.)
+namespace split {
+constexpr int IN_COL = 256;                  // the network input: columns 256-383
+constexpr int PF = 2, NB = PF + 1;           // weight fragments PF steps ahead
+constexpr int NL1 = 8, NH = 32, NO = 16;     // steps of the first, each hidden and the output layer
+constexpr int NG = NL1 + 3 * NH + NO;
+constexpr int OUT0 = NL1 + 3 * NH;
+struct Step { int layer, half, k; };
+constexpr Step step(int g) {
+  if (g < NL1) return Step{0, g / 4, g % 4};
+  if (g < OUT0) {
+    const int s = (g - NL1) % NH;
+    return Step{1 + (g - NL1) / NH, (s / 8) % 2, s < 16 ? s % 8 : 8 + s % 8};
+  }
+  return Step{4, 0, g - OUT0};
+}
+constexpr int hstep(int g) { return (g - NL1) % NH; }   // step within a hidden layer
+// The bias the epilogue work of step g needs, as layer * 2 + half, or -1.
+constexpr int side_bias(int g) {
+  if (g < 0 || g >= NG) return -1;
+  if (g < NL1) return g >= 4 ? 0 : -1;                            // first layer, half A
+  if (g < OUT0) {
+    const int l = 1 + (g - NL1) / NH, hs = hstep(g);
+    return hs < 8 ? (l - 1) * 2 + 1 : hs >= 24 ? l * 2 : -1;       // previous layer's half B; own half A
+  }
+  return g - OUT0 < 8 ? 3 * 2 + 1 : -1;                           // last hidden layer, half B
+}
+// a barrier follows step g: the end of the first layer, the middle and end of
+// each hidden layer, the middle and end of the output layer
+constexpr bool barrier_after(int g) {
+  return g == NL1 - 1 || (g >= NL1 && g < OUT0 && (hstep(g) == 15 || hstep(g) == 31)) || g == OUT0 + 7 ||
+         g == OUT0 + NO - 1;
+}
+
+// Weight fragments are read with buffer loads: the layer's base in a buffer
+// resource (scalar registers), the lane's byte offset within a k-slice (the
+// same for every k-step and every hidden layer, five registers in all) and
+// the k-step's slice offset as a scalar.  (Flat addresses, one 64-bit pair
+// per layer, half, column tile and k-step in this unrolled chain, spilled.)
+struct WSrc {
+  __amdgpu_buffer_rsrc_t rs[5];
+  int voff[2][2];          // hidden layers: half, column tile
+  int voffo;               // output layer
+};
+__device__ __forceinline__ WSrc wsrc(const SurrogateArgs& a, int w, int lane) {
+  WSrc r;
+  constexpr int nin[5] = {SUR_PIN, SUR_H, SUR_H, SUR_H, SUR_H};
+  constexpr int nout[5] = {SUR_H, SUR_H, SUR_H, SUR_H, SUR_POUT};
+#pragma unroll
+  for (int l = 0; l < 5; ++l)
+    r.rs[l] = __builtin_amdgcn_make_buffer_rsrc((void*)a.W[l], (short)0, nin[l] * nout[l] * 2, 0x00020000);
+  const int lb = (lane & 15) * 64 + (lane >> 4) * 16;     // row lane % 16 of the tile, 16 bytes per lane group
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) r.voff[h][c] = (h * 16 + 2 * w + c) * 16 * 64 + lb;
+  r.voffo = (w & 3) * 16 * 64 + lb;
+  return r;
+}
+
+template <int G> __device__ __forceinline__ void load_w(const WSrc& ws, bf16x8 (&wr)[NB][2]) {
+  constexpr Step S = step(G);
+  if constexpr (S.layer < 4) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      wr[G % NB][c] = __builtin_bit_cast(
+          bf16x8, __builtin_amdgcn_raw_buffer_load_b128(ws.rs[S.layer], ws.voff[S.half][c], S.k * SUR_H * 64, 0));
+  } else {
+    wr[G % NB][0] = __builtin_bit_cast(
+        bf16x8, __builtin_amdgcn_raw_buffer_load_b128(ws.rs[4], ws.voffo, S.k * SUR_POUT * 64, 0));
+  }
+}
+
+// The chain from the network input (in LDS, columns 256-383) to the fp32
+// output tile in `out` ([128][LDO], over the activation buffer).  `wr` holds
+// the weight fragments of steps 0 .. PF - 1 (prefetch()).
+template <int UNUSED = 0>
+__device__ __forceinline__ void chain(const SurrogateArgs& a, const WSrc& ws, bf16* buf, const float* sbias,
+                                      float* out, int w, int lane, bf16x8 (&wr)[NB][2]) {
+  typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
+  f32x4 accA[2][8][2], accB[2][8][2], accO[4];
+  bf16x8 fa[2][8];
+  const float bo = a.b[4][(w & 3) * 16 + lr];
+  // Per-lane element offsets; every LDS address is one of them plus a
+  // compile-time constant.  They are re-declared opaque at every step, so no
+  // address derived from them stays live across steps (the compiler otherwise
+  // kept each (row tile, column) address of the epilogue stores live through
+  // all five layers, which reuse them, and spilled them).
+  int oa = lr * LDA + lk;                              // A fragments
+  int oo = ((w >> 2) * 64 + lr) * LDA + lk;            // A fragments of the output layer
+  int oe = lr * LDA + 2 * w * 16 + lq;                 // epilogue stores
+  int ob = 2 * w * 16 + lq;                            // bias
+  // A fragments (activations) of step G into fa[G & 1]
+  auto load_a = [&](auto GC) __attribute__((always_inline)) {
+    constexpr int G = decltype(GC)::value;
+    constexpr Step S = step(G);
+    constexpr int col0 = (S.layer == 0 ? IN_COL : 0) + S.k * 32;
+    if constexpr (S.layer < 4) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(buf + oa + r * 16 * LDA + col0);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(buf + oo + r * 16 * LDA + col0);
+    }
+  };
+  // The bias of the lane's four columns in each of its two column tiles, for
+  // the epilogue work of a phase: read from LDS one step before the phase (a
+  // read within the step made the epilogue wait for the step's A-fragment
+  // prefetch, which the LDS returns first).  bsv[0]: half-A phases, bsv[1]:
+  // half-B phases (adjacent phases use different halves).
+  float4 bsv[2][2];
+  auto load_bias = [&](int lh) __attribute__((always_inline)) {
+    const int l = lh >> 1, hb = lh & 1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      bsv[hb][c] = *reinterpret_cast<const float4*>(sbias + ob + l * SUR_H + hb * IN_COL + c * 16);
+  };
+  // SiLU epilogue of one 16 x 16 tile (row tile rt, column tile c of half hb:
+  // the lane's four adjacent columns), stored as bf16 into the buffer
+  auto epi = [&](const f32x4& acc, int rt, int c, int hb) __attribute__((always_inline)) {
+    const int cc = hb * IN_COL + c * 16;
+    const float4 bv = bsv[hb][c];
+    bf16x4 h;
+    h[0] = (bf16)g1act::silu(acc[0] + bv.x);
+    h[1] = (bf16)g1act::silu(acc[1] + bv.y);
+    h[2] = (bf16)g1act::silu(acc[2] + bv.z);
+    h[3] = (bf16)g1act::silu(acc[3] + bv.w);
+    *reinterpret_cast<bf16x4*>(buf + oe + rt * 16 * LDA + cc) = h;
+  };
+  load_a(std::integral_constant<int, 0>{});
+  static_for<NG>([&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    constexpr Step S = step(G);
+    asm volatile("" : "+v"(oa), "+v"(oo), "+v"(oe), "+v"(ob));
+    if constexpr (side_bias(G + 1) >= 0 && side_bias(G + 1) != side_bias(G)) load_bias(side_bias(G + 1));
+    if constexpr (G + PF < NG) load_w<G + PF>(ws, wr);
+    if constexpr (G + 1 < NG && !barrier_after(G)) load_a(std::integral_constant<int, G + 1>{});
+    // MFMAs of the step
+    if constexpr (S.layer < 4) {
+      auto mma = [&](f32x4 (&acc)[8][2]) __attribute__((always_inline)) {
+        if constexpr (S.k == 0) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r], acc[r][c], 0, 0, 0);
+      };
+      if constexpr (S.half == 0) mma(accA[S.layer & 1]);
+      else mma(accB[S.layer & 1]);
+    } else {
+      if constexpr (S.k == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accO[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], accO[r], 0, 0, 0);
+    }
+    // the epilogue work interleaved with this step
+    if constexpr (S.layer == 0 && S.half == 1) {
+      // first layer, half A: row tiles 2 (G - 4) and 2 (G - 4) + 1
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) epi(accA[0][2 * (G - 4) + i][c], 2 * (G - 4) + i, c, 0);
+    } else if constexpr (S.layer >= 1 && S.layer < 4 && hstep(G) < 8) {
+      // the previous layer's half B: one row tile per half-A step
+      constexpr int rt = hstep(G);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) epi(accB[(S.layer - 1) & 1][rt][c], rt, c, 1);
+    } else if constexpr (S.layer >= 1 && S.layer < 4 && hstep(G) >= 24) {
+      // this layer's half A: one row tile per step
+      constexpr int rt = hstep(G) - 24;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) epi(accA[S.layer & 1][rt][c], rt, c, 0);
+    } else if constexpr (S.layer == 4 && S.k < 8) {
+      // the last hidden layer's half B: one row tile per step
+#pragma unroll
+      for (int c = 0; c < 2; ++c) epi(accB[3 & 1][S.k][c], S.k, c, 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (barrier_after(G)) {
+      lds_barrier();
+      if constexpr (G + 1 < NG) load_a(std::integral_constant<int, G + 1>{});
+    }
+  });
+  // every wave has read the buffer: the fp32 output tile goes over it
+  // (row tiles (w / 4) * 4 .. + 3, column tile w % 4)
+  const int col = (w & 3) * 16 + lr;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 4 + r) * 16 + lq + q) * LDO + col] = accO[r][q] + bo;
+}
+
+template <int UNUSED = 0>
+__device__ __forceinline__ void prefetch(const WSrc& ws, bf16x8 (&wr)[NB][2]) {
+  static_for<PF>([&](auto GC) { load_w<decltype(GC)::value>(ws, wr); });
+}
+}  // namespace split
+
 // LDS stash of the commit's inputs (MODE 1), each row [64 envs]: rows 0-70
 // qpos and qvel as the feature stage read them, rows 71-105 ABA's qacc (of
 // judged and fallback envs), row 106 the env's fallback flag on entry.
@@ -301,6 +555,13 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   __shared__ __attribute__((aligned(16))) bf16 buf[Rows<R>::NBUF][R * LDA];
   __shared__ float stash[MODE == 1 ? STASH_ROWS * ROWS : 1];
   __shared__ float ysc[MODE == 1 ? 2 * NV : 1];          // ystd, ymean
+  // with 128 rows the layers run as the split-half chain (split::chain), the
+  // network input in columns 256-383 of the buffer, and the hidden layers'
+  // biases are staged in LDS ([4][512])
+  constexpr bool SPLIT = MODE == 0 && R == 128;
+  constexpr int FCOL = SPLIT ? split::IN_COL : 0;
+  constexpr bool SBIAS = R == 128;
+  __shared__ __attribute__((aligned(16))) float sbias[SBIAS ? 4 * SUR_H : 4];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
@@ -348,6 +609,13 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     });
   };
   load_pass(std::integral_constant<int, 0>{});
+  // hidden-layer biases for the LDS copy (thread tid: element tid of each layer)
+  float bstage[SBIAS ? 4 : 1];
+  if constexpr (SBIAS) {
+    static_assert(!SBIAS || NTHR == SUR_H, "one bias element per thread and layer");
+#pragma unroll
+    for (int l = 0; l < 4; ++l) bstage[l] = a.b[l][tid];
+  }
   float abav[APW] = {};
   if constexpr (MODE == 1) {
     if (judged_blk) {
@@ -389,7 +657,13 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   }
   __builtin_amdgcn_sched_barrier(0);
   Ring<SUR_PIN, SUR_H, RT> ring0;
-  wprefetch<SUR_PIN, SUR_H, RT>(a.W[0], w, lane, ring0);
+  bf16x8 swr[split::NB][2];
+  split::WSrc sws;
+  if constexpr (SPLIT) {
+    sws = split::wsrc(a, w, lane);
+    split::prefetch(sws, swr);
+  }
+  else wprefetch<SUR_PIN, SUR_H, RT>(a.W[0], w, lane, ring0);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (MODE == 1) {
     // stash the commit's inputs
@@ -425,29 +699,38 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
           });
           // one 16-byte LDS store (a 2-byte store per feature at the 1,040-byte
           // row stride put 8 lanes on each bank)
-          *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + f0]) = x8;
+          *reinterpret_cast<bf16x8*>(&buf[0][r * LDA + FCOL + f0]) = x8;
         });
       }
     });
     __builtin_amdgcn_sched_barrier(0);
   });
-  __syncthreads();
+  if constexpr (SBIAS) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) sbias[l * SUR_H + tid] = bstage[l];
+  }
+  lds_barrier();
+  auto hbias = [&](int l) -> const float* { return SBIAS ? sbias + l * SUR_H : a.b[l]; };
   // layer l reads b[l % 2] and writes b[(l + 1) % 2]; one buffer with 128 rows
   bf16* b0 = buf[0];
   bf16* b1 = buf[Rows<R>::NBUF - 1];
-  Ring<SUR_H, SUR_H, RT> ring1, ring2, ring3;
-  Ring<SUR_H, SUR_POUT, RT> ring4;
-  dense<SUR_PIN, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
-  __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b1, b0, a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
-  __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
-  __syncthreads();
-  dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_POUT, RT>(b1, b0, a.W[3], a.b[3], w, lane, ring3, a.W[4], ring4);
-  __syncthreads();
   float* out = reinterpret_cast<float*>(b1);
-  dense_out<SUR_H, SUR_POUT, RT>(b0, out, a.W[4], a.b[4], w, lane, ring4);
-  __syncthreads();
+  if constexpr (SPLIT) {
+    split::chain(a, sws, buf[0], sbias, out, w, lane, swr);
+  } else {
+    Ring<SUR_H, SUR_H, RT> ring1, ring2, ring3;
+    Ring<SUR_H, SUR_POUT, RT> ring4;
+    dense<SUR_PIN, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[0], hbias(0), w, lane, ring0, a.W[1], ring1);
+    lds_barrier();
+    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b1, b0, a.W[1], hbias(1), w, lane, ring1, a.W[2], ring2);
+    lds_barrier();
+    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[2], hbias(2), w, lane, ring2, a.W[3], ring3);
+    lds_barrier();
+    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_POUT, RT>(b1, b0, a.W[3], hbias(3), w, lane, ring3, a.W[4], ring4);
+    lds_barrier();
+    dense_out<SUR_H, SUR_POUT, RT>(b0, out, a.W[4], a.b[4], w, lane, ring4);
+  }
+  lds_barrier();
   if constexpr (MODE == 0) {
     // [R][64] fp32 tile to pred rows; 512 threads
     for (int i = tid; i < R * SUR_POUT; i += NTHR) {
@@ -614,6 +897,10 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
   constexpr int NTHR = Rows<R>::NT;
   constexpr int RT = Rows<R>::RT;
   __shared__ __attribute__((aligned(16))) bf16 buf[Rows<R>::NBUF][R * LDA];
+  // with 128 rows the hidden layers' biases (512, 256, 128) are staged in LDS
+  constexpr bool SBIAS = R == 128;
+  constexpr int PB1 = 512, PB2 = PB1 + 256, PB3 = PB2 + 128;
+  __shared__ __attribute__((aligned(16))) float sbias[SBIAS ? PB3 : 4];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int e0 = blockIdx.x * R;
   const size_t N = (size_t)a.nenv;
@@ -669,6 +956,15 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
       om[i] = f < a.nobs ? a.obs_mean[f] : 0.f;
       oi[i] = f < a.nobs ? a.obs_inv[f] : 0.f;
     }
+    // hidden-layer biases for the LDS copy: element tid of layer 0, element
+    // tid - 512 of the concatenated layers 1 and 2
+    float bs0 = 0.f, bs1 = 0.f;
+    if constexpr (SBIAS) {
+      static_assert(!SBIAS || (NTHR == PB1 && PB3 - PB1 <= NTHR), "bias staging");
+      bs0 = a.b[0][tid];
+      if (tid < PB2 - PB1) bs1 = a.b[1][tid];
+      else if (tid < PB3 - PB1) bs1 = a.b[2][tid - (PB2 - PB1)];
+    }
     // the first layer's weights go last: the wait counter is in order
     __builtin_amdgcn_sched_barrier(0);
     wprefetch<POL_PIN, 512, RT>(a.W[0], w, lane, ring0);
@@ -678,7 +974,11 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
       const int row = g + G * i;
       if (row < RAW_N) rawat(row, r) = v[i];
     }
-    __syncthreads();
+    if constexpr (SBIAS) {
+      sbias[tid] = bs0;
+      if (tid < PB3 - PB1) sbias[PB1 + tid] = bs1;
+    }
+    lds_barrier();
     // The group's features with the layout and the group index as compile-time
     // constants (one uniform branch picks the instance): every feature's kind,
     // raw row and offset fold into its code.  (With both known only at run
@@ -714,7 +1014,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
       });
     });
   }
-  __syncthreads();
+  lds_barrier();
   if (a.push_interval && tid < R) {
     // examples/train.py:301-310: countdown; on expiry add a random base
     // velocity kick and draw the next interval from [50, 150)
@@ -738,15 +1038,18 @@ __global__ __launch_bounds__(Rows<R>::NT) void policy_kernel(PolicyArgs a) {
   Ring<512, 256, RT> ring1;
   Ring<256, 128, RT> ring2;
   Ring<128, 32, RT> ring3;
-  dense<POL_PIN, 512, ACT_ELU, 512, 256, RT>(b0, b1, a.W[0], a.b[0], w, lane, ring0, a.W[1], ring1);
-  __syncthreads();
-  dense<512, 256, ACT_ELU, 256, 128, RT>(b1, b0, a.W[1], a.b[1], w, lane, ring1, a.W[2], ring2);
-  __syncthreads();
-  dense<256, 128, ACT_ELU, 128, 32, RT>(b0, b1, a.W[2], a.b[2], w, lane, ring2, a.W[3], ring3);
-  __syncthreads();
+  const float* pb0 = SBIAS ? sbias : a.b[0];
+  const float* pb1 = SBIAS ? sbias + PB1 : a.b[1];
+  const float* pb2 = SBIAS ? sbias + PB2 : a.b[2];
+  dense<POL_PIN, 512, ACT_ELU, 512, 256, RT>(b0, b1, a.W[0], pb0, w, lane, ring0, a.W[1], ring1);
+  lds_barrier();
+  dense<512, 256, ACT_ELU, 256, 128, RT>(b1, b0, a.W[1], pb1, w, lane, ring1, a.W[2], ring2);
+  lds_barrier();
+  dense<256, 128, ACT_ELU, 128, 32, RT>(b0, b1, a.W[2], pb2, w, lane, ring2, a.W[3], ring3);
+  lds_barrier();
   float* out = reinterpret_cast<float*>(b0);
   dense_out<128, 32, RT>(b1, out, a.W[3], a.b[3], w, lane, ring3);
-  __syncthreads();
+  lds_barrier();
   // epilogue: thread (r, g) -> env r; actions in pairs (j, j + 8) that share
   // one Box-Muller draw, j from {0..7, 16..23}, pair p of group g being
   // jb = g + G p.  The group index is a compile-time constant (one uniform
