@@ -278,6 +278,28 @@ def _fp32_heldout(mlp, ds, idx, tol):
             "fp32_weights_bf16_inputs": run(xz.to(torch.bfloat16).float(), False)}
 
 
+def _input_cell_sample(mlp, ds, idx, nsamp=4096):
+    """Held-out states for the bf16 input-cell floor (score_input_cell): the
+    physical inputs and ABA qacc of nsamp samples, and for each total-force
+    input the width, in N m or N, of the set of values that give the same
+    bf16 network input -- the fused kernel z-scores each input in float32,
+    (x - mean) * inv, and rounds it to bf16, so every x in that set is the
+    same input to any bf16 network."""
+    import torch
+    from g1sim_amd.surrogate import NIN
+    st, en = idx[:nsamp, 0].long(), idx[:nsamp, 1].long()
+    X = ds.X[st, :, en]
+    Y = ds.Y[st, :, en]
+    z = (X - mlp.xmean) * mlp.xinv                         # float32, as the kernel forms it
+    b = z.to(torch.bfloat16).float().abs().double()
+    # bf16 keeps 8 significant bits: the spacing at |b| in [2^e, 2^(e+1)) is
+    # 2^(e - 7); the zero cell (bf16 denormals) is taken as empty
+    ulp = torch.where(b > 0, torch.pow(2.0, torch.floor(torch.log2(b.clamp_min(1e-38))) - 7.0), torch.zeros_like(b))
+    width = ulp / mlp.xinv.double()
+    return {"X": X.double().cpu().numpy(), "Y": Y.double().cpu().numpy(),
+            "wtau": width[:, NIN - 35:].cpu().numpy()}
+
+
 def _stepper_run(dev, mlp, nenv, steps, warmup, tol, seed):
     """Surrogate + judge stepper on the randomized-actuation workload:
     (wall seconds of the timed steps, summary)."""
@@ -342,7 +364,9 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     acc = evaluate(mlp, ds, nsamp=65536, tol=1e-3)
-    floor = _fp32_heldout(mlp, ds, heldout_indices(ds, nsamp=65536), 1e-3)
+    hidx = heldout_indices(ds, nsamp=65536)
+    floor = _fp32_heldout(mlp, ds, hidx, 1e-3)
+    cell = _input_cell_sample(mlp, ds, hidx)
     del ds
     torch.cuda.empty_cache()
     it_ms = timing.get("avg_iteration_ms", float("nan"))
@@ -356,7 +380,7 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
                     "every env starts in fallback until a judge accepts it",
            "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps", "train_iters": train_iters,
            "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train, "train_roofline": train_roof,
-           "heldout_bf16_kernels": acc, "heldout_fp32_evaluation": floor}
+           "heldout_bf16_kernels": acc, "heldout_fp32_evaluation": floor, "_input_cells": cell}
     # the stepper at the headline size and at 131,072 envs; tolerance 1e-3,
     # then every judgement accepted (tolerance 1e30: the cost of the
     # surrogate path itself, independent of the network's accuracy)
@@ -537,6 +561,41 @@ def score_precision(samples):
                                          "max": float(d.max())}
     out["contact_stress_131072x1000"] = res
     return out
+
+
+def input_cell_floor(q, qacc, wtau, dt=DT):
+    """One state's bf16 input-cell floor (score_input_cell): qacc =
+    (M(q) + dt B(q))^-1 (tau - C) is affine in the total generalized force
+    tau (src/g1_core.h:176-207, B the implicit-damping diagonal kv + limit
+    kd), so over the box of tau values of widths wtau qacc_j spans
+    sum_k |A_jk| wtau_k with A = (M + dt B)^-1; returns the max over dofs of
+    half that span / max(|qacc_j|, 1), the judge's measure."""
+    from g1sim_amd.model import MODEL
+    from oracle import oracle as O
+    M = O.crba(q)
+    h = q[7:]
+    b = MODEL.act_kv + np.where((h < MODEL.jnt_lo) | (h > MODEL.jnt_hi), 5.0, 0.0)
+    M[6:, 6:] += np.diag(dt * b)
+    span = np.abs(np.linalg.inv(M)) @ wtau
+    return float((0.5 * span / np.maximum(np.abs(qacc), 1.0)).max())
+
+
+def score_input_cell(cell, tol=1e-3, dt=DT):
+    """The bf16 input-cell floor of the surrogate, scored with the FP64
+    oracle.  A network reading bf16 inputs returns one output for every state
+    whose inputs round to the same bf16 values, so over that set its error on
+    some dof is at least half the span of the true qacc (input_cell_floor).
+    Only the total force's rounding is counted (q and qd add more), so each
+    state's figure is a lower bound."""
+    X, Y, W = cell["X"], cell["Y"], cell["wtau"]
+    fl = np.array([input_cell_floor(X[i, :36], Y[i], W[i], dt) for i in range(X.shape[0])])
+    q = np.percentile
+    return {"samples": int(X.shape[0]), "tol": tol,
+            "what": "per held-out state, the least worst-case error (judge measure) of any network that reads "
+                    "bf16 inputs, over the states whose total-force inputs round to the same bf16 values; "
+                    "tau rounding only (a lower bound)",
+            "median": float(np.median(fl)), "p10": float(q(fl, 10)), "p90": float(q(fl, 90)),
+            "min": float(fl.min()), "frac_above_tol": float((fl > tol).mean())}
 
 
 def cpu_baseline(samples=None):
@@ -838,7 +897,10 @@ def main():
         out["surrogate"], mlp = surrogate_leg(dev, nenv, args.steps, args.warmup, args.train_iters)
         if not args.no_policy:
             out["policy_in_the_loop"] = policy_leg(dev, mlp)
+    cell = out["surrogate"].pop("_input_cells", None) if "surrogate" in out else None
     if not args.no_cpu_baseline:
+        if cell is not None:
+            out["surrogate"]["bf16_input_cell_floor"] = score_input_cell(cell)
         out["cpu_baseline"], prec = cpu_baseline(samples)
         if prec is not None:
             out["precision_vs_fp64"] = prec

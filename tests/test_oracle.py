@@ -345,3 +345,39 @@ def test_reset_env_structure():
     assert (np.abs(d) <= 0.3 + 1e-6).all() and np.abs(d).max() > 0.1
     qp3, _, _ = O.reset_env_f32(6, 42, 0.3, 0.0)
     assert not np.array_equal(qp2, qp3)
+
+
+def test_input_cell_span_matches_box_corners():
+    """bench.input_cell_floor: the span of the affine map tau -> qacc =
+    (M + dt B)^-1 (tau - C) over a box of tau widths is sum_k |A_jk| w_k;
+    checked against every corner of a 4-dof box (an affine map takes its
+    extremes at corners), with qacc from the fused-step identity checked in
+    test_fused_step_linear_system."""
+    import bench
+    from g1sim_amd.model import MODEL
+    qp0, qv0, an0, ct0 = W.randomized_actuation(2, seed=9, nblocks=1)
+    qp, qv, an = qp0[:, 0].astype(float), qv0[:, 0].astype(float), an0[:, 0].astype(float)
+    ctrl = ct0[0][:, 0].astype(float)
+    for _ in range(40):
+        qp, qv, an, _ = O.step(qp, qv, ctrl, an, 1, 2e-3)
+    out = O.step_full(qp, qv, ctrl, an, 1, 2e-3)
+    rng = np.random.default_rng(3)
+    dofs = [2, 9, 17, 30]
+    w = np.zeros(35)
+    w[dofs] = rng.uniform(0.01, 0.2, 4)
+    M = O.crba(qp)
+    b = MODEL.act_kv + np.where((qp[7:] < MODEL.jnt_lo) | (qp[7:] > MODEL.jnt_hi), 5.0, 0.0)
+    M[6:, 6:] += np.diag(2e-3 * b)
+    C = out["tau"] - M @ out["qacc"]
+    acc = []
+    for corner in range(16):
+        tau = out["tau"].copy()
+        for i, d in enumerate(dofs):
+            tau[d] += (0.5 if corner >> i & 1 else -0.5) * w[d]
+        acc.append(np.linalg.solve(M, tau - C))
+    acc = np.array(acc)
+    span = acc.max(axis=0) - acc.min(axis=0)
+    expect = (0.5 * span / np.maximum(np.abs(out["qacc"]), 1.0)).max()
+    got = bench.input_cell_floor(qp, out["qacc"], w, 2e-3)
+    assert abs(got - expect) <= 1e-9 * max(1.0, expect)
+    assert got > 0
