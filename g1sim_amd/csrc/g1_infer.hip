@@ -542,6 +542,194 @@ __device__ __forceinline__ void prefetch(const WSrc& ws, bf16x8 (&wr)[NB][2]) {
 }
 }  // namespace split
 
+// ----------------------------- split-half chain, two buffers (64 rows)
+// The same idea for 64 rows per workgroup (the surrogate below 32,768 envs
+// and the stepper's network + commit kernel, where a workgroup is one judge
+// block and the step time is one workgroup's latency).  Layer l reads buffer
+// l % 2 and writes buffer (l + 1) % 2, so an epilogue can store as soon as it
+// is computed.  A hidden layer is
+//   A-lo (half A, k-steps 0-7), the previous layer's half-B epilogue
+//        interleaved and stored into features 256-511 of this layer's input;
+//   barrier;
+//   A-hi (half A, k-steps 8-15);
+//   B (half B, k-steps 0-15), this layer's half-A epilogue interleaved (one
+//        tile every other step) and stored into the next layer's input;
+//   barrier.
+// The first layer runs A then B (its half-A epilogue in its B steps); the
+// output layer's k-steps 0-7 overlap the last half-B epilogue.  k-steps
+// still accumulate in ascending order: the results are those of the
+// layer-by-layer kernel bit for bit.
+namespace split2 {
+using split::Step;
+using split::NB;
+using split::PF;
+constexpr int NL1 = 8, NH = 32, NO = 16;
+constexpr int NG = NL1 + 3 * NH + NO;
+constexpr int OUT0 = NL1 + 3 * NH;
+constexpr Step step(int g) {
+  if (g < NL1) return Step{0, g / 4, g % 4};
+  if (g < OUT0) {
+    const int s = (g - NL1) % NH;
+    return Step{1 + (g - NL1) / NH, s < 16 ? 0 : 1, s < 16 ? s : s - 16};
+  }
+  return Step{4, 0, g - OUT0};
+}
+constexpr int hstep(int g) { return (g - NL1) % NH; }
+constexpr bool barrier_after(int g) {
+  return g == NL1 - 1 || (g >= NL1 && g < OUT0 && (hstep(g) == 7 || hstep(g) == 31)) || g == OUT0 + 7 ||
+         g == OUT0 + NO - 1;
+}
+// the bias (layer * 2 + half) the epilogue work of step g needs, or -1
+constexpr int side_bias(int g) {
+  if (g < 0 || g >= NG) return -1;
+  if (g < NL1) return g >= 4 ? 0 : -1;
+  if (g < OUT0) {
+    const int l = 1 + (g - NL1) / NH, hs = hstep(g);
+    return hs < 8 ? (l - 1) * 2 + 1 : hs >= 16 ? l * 2 : -1;
+  }
+  return g - OUT0 < 8 ? 3 * 2 + 1 : -1;
+}
+constexpr int BLA = 8;        // biases are read from global memory BLA steps ahead
+
+template <int G> __device__ __forceinline__ void load_w(const split::WSrc& ws, bf16x8 (&wr)[NB][2]) {
+  constexpr Step S = step(G);
+  if constexpr (S.layer < 4) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      wr[G % NB][c] = __builtin_bit_cast(
+          bf16x8, __builtin_amdgcn_raw_buffer_load_b128(ws.rs[S.layer], ws.voff[S.half][c], S.k * SUR_H * 64, 0));
+  } else {
+    wr[G % NB][0] = __builtin_bit_cast(
+        bf16x8, __builtin_amdgcn_raw_buffer_load_b128(ws.rs[4], ws.voffo, S.k * SUR_POUT * 64, 0));
+  }
+}
+
+template <int UNUSED = 0>
+__device__ __forceinline__ void prefetch(const split::WSrc& ws, bf16x8 (&wr)[NB][2]) {
+  static_for<PF>([&](auto GC) { split2::load_w<decltype(GC)::value>(ws, wr); });
+}
+
+// From the network input (buffer 0, columns 0-127) to the fp32 output tile in
+// `out` ([64][LDO], over buffer 1).  `wr` holds the weight fragments of steps
+// 0 .. PF - 1 (prefetch()).
+template <int UNUSED = 0>
+__device__ __forceinline__ void chain(const SurrogateArgs& a, const split::WSrc& ws, bf16* buf0, bf16* buf1,
+                                      float* out, int w, int lane, bf16x8 (&wr)[NB][2]) {
+  typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
+  f32x4 accA[2][4][2], accB[2][4][2], accO[2];
+  bf16x8 fa[2][4];
+  const float bo = a.b[4][(w & 3) * 16 + lr];
+  // per-lane element offsets, opaque at every step (as in split::chain)
+  int oa = lr * LDA + lk;                              // A fragments
+  int oo = ((w >> 2) * 32 + lr) * LDA + lk;            // A fragments of the output layer
+  int oe = lr * LDA + 2 * w * 16 + lq;                 // epilogue stores
+  int ob = 2 * w * 16 + lq;                            // bias
+  auto bufl = [&](int l) -> bf16* { return (l & 1) ? buf1 : buf0; };
+  auto load_a = [&](auto GC) __attribute__((always_inline)) {
+    constexpr int G = decltype(GC)::value;
+    constexpr Step S = step(G);
+    const bf16* src = bufl(S.layer) + S.k * 32;
+    if constexpr (S.layer < 4) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(src + oa + r * 16 * LDA);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(src + oo + r * 16 * LDA);
+    }
+  };
+  // bias of the lane's columns for a phase (bsv[half][tile]), read BLA steps ahead
+  float4 bsv[2][2];
+  auto load_bias = [&](int lh) __attribute__((always_inline)) {
+    const int l = lh >> 1, hb = lh & 1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) bsv[hb][c] = *reinterpret_cast<const float4*>(a.b[l] + ob + hb * 256 + c * 16);
+  };
+  // SiLU epilogue of one 16 x 16 tile of layer l (row tile rt, column tile c
+  // of half hb), stored as bf16 into layer l's output buffer
+  auto epi = [&](const f32x4& acc, int l, int rt, int c, int hb) __attribute__((always_inline)) {
+    const float4 bv = bsv[hb][c];
+    bf16x4 h;
+    h[0] = (bf16)g1act::silu(acc[0] + bv.x);
+    h[1] = (bf16)g1act::silu(acc[1] + bv.y);
+    h[2] = (bf16)g1act::silu(acc[2] + bv.z);
+    h[3] = (bf16)g1act::silu(acc[3] + bv.w);
+    *reinterpret_cast<bf16x4*>(bufl(l + 1) + oe + rt * 16 * LDA + hb * 256 + c * 16) = h;
+  };
+  static_for<BLA>([&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    if constexpr (side_bias(G) >= 0 && side_bias(G) != side_bias(G - 1)) load_bias(side_bias(G));
+  });
+  load_a(std::integral_constant<int, 0>{});
+  static_for<NG>([&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    constexpr Step S = step(G);
+    asm volatile("" : "+v"(oa), "+v"(oo), "+v"(oe), "+v"(ob));
+    if constexpr (side_bias(G + BLA) >= 0 && side_bias(G + BLA) != side_bias(G + BLA - 1))
+      load_bias(side_bias(G + BLA));
+    if constexpr (G + PF < NG) split2::load_w<G + PF>(ws, wr);
+    if constexpr (G + 1 < NG && !barrier_after(G)) load_a(std::integral_constant<int, G + 1>{});
+    if constexpr (S.layer < 4) {
+      auto mma = [&](f32x4 (&acc)[4][2]) __attribute__((always_inline)) {
+        if constexpr (S.k == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r], acc[r][c], 0, 0, 0);
+      };
+      if constexpr (S.half == 0) mma(accA[S.layer & 1]);
+      else mma(accB[S.layer & 1]);
+    } else {
+      if constexpr (S.k == 0) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) accO[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], accO[r], 0, 0, 0);
+    }
+    // the epilogue work interleaved with this step
+    if constexpr (S.layer == 0 && S.half == 1) {
+      // first layer, half A: two tiles per step
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        constexpr int t0 = 2 * (G - 4);
+        epi(accA[0][(t0 + i) / 2][(t0 + i) % 2], 0, (t0 + i) / 2, (t0 + i) % 2, 0);
+      }
+    } else if constexpr (S.layer >= 1 && S.layer < 4 && hstep(G) < 8) {
+      // the previous layer's half B: one tile per A-lo step
+      constexpr int t = hstep(G);
+      epi(accB[(S.layer - 1) & 1][t / 2][t % 2], S.layer - 1, t / 2, t % 2, 1);
+    } else if constexpr (S.layer >= 1 && S.layer < 4 && hstep(G) >= 16 && (hstep(G) & 1) == 0) {
+      // this layer's half A: one tile every other B step
+      constexpr int t = (hstep(G) - 16) / 2;
+      epi(accA[S.layer & 1][t / 2][t % 2], S.layer, t / 2, t % 2, 0);
+    } else if constexpr (S.layer == 4 && S.k < 8) {
+      // the last hidden layer's half B: one tile per step
+      epi(accB[3 & 1][S.k / 2][S.k % 2], 3, S.k / 2, S.k % 2, 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (barrier_after(G)) {
+      lds_barrier();
+      if constexpr (G + 1 < NG) load_a(std::integral_constant<int, G + 1>{});
+    }
+  });
+  // every wave has read buffer 1 (last by the fourth hidden layer): the fp32
+  // output tile goes over it (row tiles (w / 4) * 2 .. + 1, column tile w % 4)
+  const int col = (w & 3) * 16 + lr;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 2 + r) * 16 + lq + q) * LDO + col] = accO[r][q] + bo;
+}
+}  // namespace split2
+
 // LDS stash of the commit's inputs (MODE 1), each row [64 envs]: rows 0-70
 // qpos and qvel as the feature stage read them, rows 71-105 ABA's qacc (of
 // judged and fallback envs), row 106 the env's fallback flag on entry.
@@ -559,6 +747,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   // network input in columns 256-383 of the buffer, and the hidden layers'
   // biases are staged in LDS ([4][512])
   constexpr bool SPLIT = MODE == 0 && R == 128;
+  constexpr bool SPLIT2 = R == 64;
   constexpr int FCOL = SPLIT ? split::IN_COL : 0;
   constexpr bool SBIAS = R == 128;
   __shared__ __attribute__((aligned(16))) float sbias[SBIAS ? 4 * SUR_H : 4];
@@ -662,6 +851,9 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   if constexpr (SPLIT) {
     sws = split::wsrc(a, w, lane);
     split::prefetch(sws, swr);
+  } else if constexpr (SPLIT2) {
+    sws = split::wsrc(a, w, lane);
+    split2::prefetch(sws, swr);
   }
   else wprefetch<SUR_PIN, SUR_H, RT>(a.W[0], w, lane, ring0);
   __builtin_amdgcn_sched_barrier(0);
@@ -717,6 +909,8 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   float* out = reinterpret_cast<float*>(b1);
   if constexpr (SPLIT) {
     split::chain(a, sws, buf[0], sbias, out, w, lane, swr);
+  } else if constexpr (SPLIT2) {
+    split2::chain(a, sws, buf[0], buf[1], out, w, lane, swr);
   } else {
     Ring<SUR_H, SUR_H, RT> ring1, ring2, ring3;
     Ring<SUR_H, SUR_POUT, RT> ring4;
