@@ -308,9 +308,15 @@ struct SurrogateArgs {
   float* qacc_out;
   float* qpos_rw;
   float* qvel_rw;
+  // deferred judging (MODE 1, nullable): the record of a judged block whose
+  // live envs are all in fallback, [SNAP_ROWS][nenv] (network inputs, then
+  // ABA's qacc), and the env's pending-judgement flag
+  float* snap;
+  uint8_t* sflag;
 };
 
 constexpr int SUR_NIN = 106, SUR_PIN = 128, SUR_H = 512, SUR_POUT = 64;
+constexpr int SNAP_ROWS = SUR_NIN + 35;       // deferred-judgement record: inputs, ABA qacc
 static_assert(ROWS == g1k::JUDGE_BLOCK, "a network tile is one judge block");
 
 // ------------------------------------------ split-half chain (128 rows)
@@ -820,15 +826,42 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   if constexpr (MODE == 1) {
     // A tile that is not judged this step and whose live envs are all in
     // fallback takes ABA's qacc everywhere: commit without the network.
-    if (a.fallback && !judged_blk) {
+    // With deferred judging (a.snap) so does a judged tile whose live envs
+    // are all in fallback -- a fallback env commits ABA's qacc whatever the
+    // judgement -- after recording its network inputs and ABA's qacc for the
+    // batch judgement of the period (judge_batch_kernel), which decides
+    // whether the envs leave fallback.
+    if (a.fallback && (!judged_blk || a.snap)) {
       const int own = e < a.nenv && !fb;
       if (!__syncthreads_or(own)) {
+        if (judged_blk && e < a.nenv) {
+          // this thread's features (all of them in the one feature pass of
+          // a 64-env tile) and ABA rows
+          static_assert(NPASS == 1, "the record takes the features of the single pass");
+          static_for<G>([&](auto FG) {
+            if (fg == decltype(FG)::value) {
+              static_for<FPP>([&](auto I) {
+                constexpr int ii = decltype(I)::value;
+                constexpr int f = 8 * (decltype(FG)::value + G * (ii / 8)) + ii % 8;
+                if constexpr (f < SUR_NIN) a.snap[(size_t)f * N + e] = raw[ii];
+              });
+            }
+          });
+#pragma unroll
+          for (int i = 0; i < APW; ++i) {
+            const int j = fg + G * i;
+            if (j < NV) a.snap[(size_t)(SUR_NIN + j) * N + e] = abav[i];
+          }
+          if (fg == 0) a.sflag[e] = 1;
+        }
         if (w == 0) {
           float sur[NV];
 #pragma unroll
           for (int j = 0; j < NV; ++j) sur[j] = 0.f;
-          g1k::surrogate_commit(e < a.nenv, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback,
-                                a.stats, a.qacc_out, a.qpos_rw, a.qvel_rw);
+          // phase -1: no env of the tile is judged now (a recorded tile is
+          // judged with the batch)
+          g1k::surrogate_commit(e < a.nenv, e, N, judged_blk ? -1 : a.phase, a.period, a.tol, a.dt, sur,
+                                a.qacc_aba, a.fallback, a.stats, a.qacc_out, a.qpos_rw, a.qvel_rw);
         }
         return;
       }
@@ -1558,6 +1591,51 @@ __global__ __launch_bounds__(256) void rollout_init_kernel(int nenv, float lin, 
   if (push_interval) push_interval[e] = draw_int(seed, -1, e, K_INIT + 3, 50, 100);
 }
 
+// The batch judgement of deferred judging: env e with a record of this
+// period (sflag[e]) is judged on its recorded state -- the network's output
+// pred (the forward over the records) against the recorded ABA qacc, the
+// judge's measure and tolerance -- and leaves fallback if within it.  The
+// env has stayed in fallback since it was recorded (every env is judged once
+// per period; a reset clears its record), so the network's qacc is still
+// used only by envs whose last judgement is at most period - 1 steps old.
+// stats[4] / [5]: batch-judged / rejected envs, [2] the max judged error.
+__global__ __launch_bounds__(256) void judge_batch_kernel(int nenv, float tol, const float* __restrict__ pred,
+                                                          int ldp, const float* __restrict__ ymean,
+                                                          const float* __restrict__ ystd,
+                                                          const float* __restrict__ snap, uint8_t* sflag,
+                                                          uint8_t* fallback, unsigned* stats) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = (size_t)nenv;
+  const bool rec = e < nenv && sflag[e] != 0;
+  float err = 0.f;
+  if (rec) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const float sur = pred[(size_t)e * ldp + j] * ystd[j] + ymean[j];
+      const float ref = snap[(size_t)(SUR_NIN + j) * N + e];
+      err = fmaxf(err, fabsf(sur - ref) * g1k::frcp(fmaxf(fabsf(ref), 1.0f)));
+    }
+  }
+  const bool reject = rec && !(err <= tol);
+  if (rec) {
+    fallback[e] = reject ? 1 : 0;
+    sflag[e] = 0;
+  }
+  const unsigned n_judged = (unsigned)__popcll(__ballot(rec));
+  const unsigned n_reject = (unsigned)__popcll(__ballot(reject));
+  unsigned ebits = rec ? __float_as_uint(err) : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = (unsigned)__shfl_xor((int)ebits, off, 64);
+    ebits = o > ebits ? o : ebits;
+  }
+  if ((threadIdx.x & 63) == 0 && n_judged) {
+    atomicAdd(&stats[4], n_judged);
+    if (n_reject) atomicAdd(&stats[5], n_reject);
+    atomicMax(&stats[2], ebits);
+  }
+}
+
 }  // namespace g1inf
 
 // ================================================================== C-ABI
@@ -1622,6 +1700,48 @@ int g1_hip_surrogate_step(int nenv, float dt, int phase, int period, float tol, 
   return g1err::launch("g1_hip_surrogate_step");
 }
 
+int g1_hip_surrogate_step_deferred(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,
+                                   const float* tau, const float* xmean, const float* xinv, const void* const* W,
+                                   const float* const* b, const float* ymean, const float* ystd,
+                                   const float* qacc_aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                                   float* snap, uint8_t* sflag, void* stream) {
+  g1err::clear();
+  if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !(dt > 0.f))
+    return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
+  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba || !fallback || !stats || !snap ||
+      !sflag)
+    return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
+  SurrogateArgs a{};
+  if (!fill_surrogate_weights(a, W, b)) return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
+  if (nenv == 0) return 0;
+  a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
+  a.phase = phase; a.period = period; a.tol = tol; a.dt = dt; a.ymean = ymean; a.ystd = ystd;
+  a.qacc_aba = qacc_aba; a.fallback = fallback; a.stats = stats; a.qacc_out = qacc_out;
+  a.qpos_rw = qpos; a.qvel_rw = qvel; a.snap = snap; a.sflag = sflag;
+  hipLaunchKernelGGL((surrogate_kernel<1, ROWS>), dim3((nenv + ROWS - 1) / ROWS), dim3(Rows<ROWS>::NT), 0,
+                     (hipStream_t)stream, a);
+  return g1err::launch("g1_hip_surrogate_step_deferred");
+}
+
+int g1_hip_surrogate_judge_batch(int nenv, float tol, const float* xmean, const float* xinv, const void* const* W,
+                                 const float* const* b, const float* ymean, const float* ystd, const float* snap,
+                                 uint8_t* sflag, uint8_t* fallback, unsigned* stats, float* pred, int ldp,
+                                 void* stream) {
+  g1err::clear();
+  if (nenv < 0 || !xmean || !xinv || !ymean || !ystd || !snap || !sflag || !fallback || !stats || !pred ||
+      ldp < SUR_POUT)
+    return g1err::fail("g1_hip_surrogate_judge_batch: invalid arguments");
+  SurrogateArgs chk{};
+  if (!fill_surrogate_weights(chk, W, b)) return g1err::fail("g1_hip_surrogate_judge_batch: invalid arguments");
+  if (nenv == 0) return 0;
+  const size_t N = (size_t)nenv;
+  int rc = g1_hip_surrogate_forward(nenv, snap, snap + 36 * N, snap + 71 * N, xmean, xinv, W, b, pred, ldp, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(judge_batch_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv, tol,
+                     pred, ldp, ymean, ystd, snap, sflag, fallback, stats);
+  return g1err::launch("g1_hip_surrogate_judge_batch");
+}
+
 // nsteps stepper steps in one host call: per step the judge + forces kernel
 // (g1_hip_judge_forces, defined in g1_step.hip) and the network + commit
 // kernel, phase (t0 + s) % period, the same launches in the same order as
@@ -1644,6 +1764,37 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
     rc = g1_hip_surrogate_step(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean, ystd,
                                qacc_aba, fallback, stats, qacc_out, stream);
     if (rc) return rc;
+  }
+  return 0;
+}
+
+// The same with deferred judging: g1_hip_surrogate_step_deferred per step,
+// and the batch judgement (g1_hip_surrogate_judge_batch) after each step
+// whose phase is period - 1.
+int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
+                                    float* qvel, float* anchor, const float* ctrl, const float* mu,
+                                    const float* xmean, const float* xinv, const void* const* W,
+                                    const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
+                                    float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                                    float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
+                                    float* snap, uint8_t* sflag, float* pred, int ldp, void* stream) {
+  g1err::clear();
+  if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
+      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag || !pred || ldp < SUR_POUT)
+    return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
+  for (int s = 0; s < nsteps; ++s) {
+    const int phase = (int)(((long long)t0 + s) % period);
+    int rc = g1_hip_judge_forces(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_aba, tau,
+                                 contact_forces, foot_pos, foot_vel, judge_flags, stream);
+    if (rc) return rc;
+    rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean,
+                                        ystd, qacc_aba, fallback, stats, qacc_out, snap, sflag, stream);
+    if (rc) return rc;
+    if (phase == period - 1) {
+      rc = g1_hip_surrogate_judge_batch(nenv, tol, xmean, xinv, W, b, ymean, ystd, snap, sflag, fallback, stats,
+                                        pred, ldp, stream);
+      if (rc) return rc;
+    }
   }
   return 0;
 }

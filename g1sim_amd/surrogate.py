@@ -24,6 +24,7 @@ from .sim import check_soa, norm_device
 
 NIN, NOUT, HID, NHID = NQ + NV + NV, NV, 512, 4
 PIN, POUT = 128, 128
+SNAP_ROWS = NIN + NV          # deferred-judgement record: network inputs, ABA qacc
 EPI = dict(F32=0, F32_ATOMIC=1, BIAS_SILU=2, BIAS_SILU_H=3, BIAS_F32=4, SILU_BWD=5, F32_SPLIT=6)
 
 
@@ -461,12 +462,23 @@ class SurrogateStepper:
     until the judge has accepted the network on it; a rejected judgement puts
     it back.  Envs that the caller resets (mark_reset) start over in
     fallback.  So the network's qacc is integrated only for envs whose last
-    judgement, at most period - 1 steps earlier, was within the tolerance."""
+    judgement, at most period - 1 steps earlier, was within the tolerance.
+
+    Deferred judging (defer=True, the fused path): a judged block whose envs
+    are all in fallback is committed with ABA's qacc -- as a fallback env is
+    whatever the judgement -- and records its network inputs and ABA's qacc;
+    the records of a period are judged together by one network pass after
+    the period's last step (g1_hip_surrogate_judge_batch), and the envs within
+    the tolerance leave fallback then instead of the step after their
+    judgement.  Blocks with an env off fallback are judged at once, so the
+    guarantee above is unchanged, and a network that is never accepted costs
+    one network pass per period instead of one tile latency per step."""
 
     def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True,
-                 track_qacc=False):
+                 track_qacc=False, defer=True):
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
         self.fused = fused and mlp.fusable
+        self.defer = defer and self.fused
         self.lib = _lib.load()
         self.dev = norm_device(device)
         self.rows = (nenv + 127) // 128 * 128
@@ -474,7 +486,14 @@ class SurrogateStepper:
         self.tau = torch.zeros((NV, nenv), **f32)
         self.qacc_aba = torch.zeros((NV, nenv), **f32)
         self.fallback = torch.ones(nenv, dtype=torch.uint8, device=self.dev)
-        self.stats = torch.zeros(4, dtype=torch.int32, device=self.dev)
+        # [0] judged at once, [1] rejected at once, [2] max judged error (float
+        # bits), [3] fallback env-steps, [4] / [5] batch-judged / rejected
+        self.stats = torch.zeros(6, dtype=torch.int32, device=self.dev)
+        if self.defer:
+            self.snap = torch.zeros((SNAP_ROWS, nenv), **f32)
+            self.sflag = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
+            self.bpred = torch.zeros((max(nenv, 1), POUT), **f32)
+            self.dropped = torch.zeros((), dtype=torch.int64, device=self.dev)     # records dropped by resets
         self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
         self.pred = torch.zeros((self.rows, POUT), **f32)
         self.ystd = (1.0 / mlp.yinv).contiguous()
@@ -491,6 +510,10 @@ class SurrogateStepper:
         fallback until their next accepted judgement."""
         check_soa("reset", reset.view(1, -1), 1, self.nenv, self.dev, dtype=torch.uint8)
         self.fallback.bitwise_or_(reset)
+        if self.defer:
+            # a pending record would judge the state before the reset
+            self.dropped += (self.sflag.to(torch.int64) * (reset != 0).to(torch.int64)).sum()
+            self.sflag.mul_((reset == 0).to(torch.uint8))
 
     def step(self, qpos, qvel, anchor, ctrl, mu=None, prediction=None):
         """One step of every env.  prediction (optional): z-scored qacc rows
@@ -517,7 +540,19 @@ class SurrogateStepper:
                                        or not prediction.is_contiguous() or prediction.dim() != 2
                                        or prediction.shape[0] < n or prediction.shape[1] != POUT):
             raise ValueError(f"prediction must be a contiguous float32 (>= {n}, {POUT}) tensor on {self.dev}")
-        if self.fused and prediction is None:
+        if self.fused and prediction is None and self.defer:
+            W, b = m._wb()
+            ck(lib.g1_hip_surrogate_step_deferred(
+                n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
+                m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(), self.ystd.data_ptr(),
+                self.qacc_aba.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(), qacc_out,
+                self.snap.data_ptr(), self.sflag.data_ptr(), s), "surrogate_step_deferred")
+            if phase == self.period - 1:
+                ck(lib.g1_hip_surrogate_judge_batch(
+                    n, self.tol, m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
+                    self.ystd.data_ptr(), self.snap.data_ptr(), self.sflag.data_ptr(), self.fallback.data_ptr(),
+                    self.stats.data_ptr(), self.bpred.data_ptr(), POUT, s), "surrogate_judge_batch")
+        elif self.fused and prediction is None:
             W, b = m._wb()
             ck(lib.g1_hip_surrogate_step(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
                                          self.tau.data_ptr(), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b,
@@ -557,6 +592,16 @@ class SurrogateStepper:
         m = self.mlp
         W, b = m._wb()
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        if self.defer:
+            _lib.check(lib.g1_hip_surrogate_steps_deferred(
+                ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                ctrl.data_ptr(), ptr(mu), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
+                self.ystd.data_ptr(), self.qacc_aba.data_ptr(), self.tau.data_ptr(), self.fallback.data_ptr(),
+                self.stats.data_ptr(), ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
+                self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
+                self.bpred.data_ptr(), POUT, s), "surrogate_steps_deferred")
+            self.t += n
+            return
         _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),
                                               qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(), ptr(mu),
                                               m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
@@ -567,11 +612,17 @@ class SurrogateStepper:
         self.t += n
 
     def summary(self):
+        """judged / rejected count every judgement (at once and in batches);
+        aba_frac is the share of env-steps that integrated ABA's qacc (judged
+        at once, or in fallback -- a batch-judged env-step was in fallback)."""
         st = self.stats.cpu().numpy().astype(np.uint32)
-        judged, rejected, fb = int(st[0]), int(st[1]), int(st[3])
+        now, now_rej, fb, batch, batch_rej = int(st[0]), int(st[1]), int(st[3]), int(st[4]), int(st[5])
+        judged, rejected = now + batch, now_rej + batch_rej
         maxerr = float(np.array([st[2]], np.uint32).view(np.float32)[0])
         total = self.t * self.nenv
         return {"env_steps": total, "judged": judged, "rejected": rejected,
                 "reject_frac": rejected / max(judged, 1), "fallback_env_steps": fb,
-                "aba_frac": (judged + fb) / max(total, 1), "max_judged_rel_err": maxerr,
-                "fallback_envs_now": int(self.fallback.sum().item())}
+                "aba_frac": (now + fb) / max(total, 1), "max_judged_rel_err": maxerr,
+                "fallback_envs_now": int(self.fallback.sum().item()), "batch_judged": batch,
+                "pending_records": int(self.sflag.sum().item()) if self.defer else 0,
+                "dropped_records": int(self.dropped.item()) if self.defer else 0}

@@ -232,7 +232,8 @@ def test_trained_network_gain(device):
     assert hist[-1][1] < 0.2 * hist[0][1]                       # it learns
     assert after["median_rel_err"] < 0.2 * before["median_rel_err"]
     assert np.isfinite(ha["qpos"]).all()
-    assert summ["judged"] == 50 * n // 16
+    # every judged block is judged at once or recorded for its period's batch
+    assert summ["judged"] + summ["pending_records"] + summ["dropped_records"] == 50 * n // 16
 
 
 def _trained_small(device, seed=4):
@@ -271,14 +272,15 @@ def test_fused_forward_equals_layer_gemms(device):
 
 def test_fused_step_equals_unfused(device):
     """Surrogate steps through the fused kernel and through features + GEMMs +
-    integrate agree on state, fallback flags and statistics."""
+    integrate agree on state, fallback flags and statistics (immediate
+    judging in both: the unfused path has no deferred mode)."""
     mlp = _trained_small(device, seed=6)
     n = 3000
     qp, qv, an, cb = W.randomized_actuation(n, seed=26, nblocks=1)
     runs = []
     for fused in (True, False):
         st = GpuState(qp, qv, an, cb[0], device)
-        stepper = SurrogateStepper(mlp, n, device, period=16, tol=0.5, fused=fused)
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=0.5, fused=fused, defer=False)
         assert stepper.fused == fused
         for _ in range(5):
             stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
@@ -337,7 +339,8 @@ def test_fallback_tiles_skip_the_network(device):
     runs = []
     for fused in (True, False):
         st = GpuState(qp, qv, an, cb[0], device)
-        stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e30, fused=fused)
+        # immediate judging: the judged all-fallback block runs the network
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e30, fused=fused, defer=False)
         stepper.fallback.copy_(torch.as_tensor(fb0, device=device))
         stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
         torch.cuda.synchronize()
@@ -437,3 +440,73 @@ def test_steps_equal_repeated_step(device, tol):
     else:
         assert int(ea["fallback"].sum()) == n
     write_metrics(f"surrogate_steps_one_call_tol{tol:g}", {"stats": ea["stats"].tolist(), "fallback_env_steps": summ})
+
+
+def _stepper_run(device, mlp, n, steps, tol, defer, reset_at=None):
+    qp, qv, an, cb = W.randomized_actuation(n, seed=33, nblocks=1)
+    st = GpuState(qp, qv, an, cb[0], device)
+    stepper = SurrogateStepper(mlp, n, device, period=16, tol=tol, defer=defer, track_qacc=True)
+    fb = []
+    for k in range(steps):
+        if reset_at is not None and k == reset_at:
+            stepper.mark_reset(torch.ones(n, dtype=torch.uint8, device=device))
+        stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
+        fb.append(stepper.fallback.clone())
+    torch.cuda.synchronize()
+    return st, stepper, [f.cpu().numpy() for f in fb]
+
+
+@pytest.mark.parametrize("n", [5000, 40000])
+def test_deferred_judging_matches_immediate_when_never_accepted(device, n):
+    """A network the judge never accepts (random weights, tolerance 1e-3):
+    deferred and immediate judging integrate ABA everywhere, bit for bit,
+    and judge the same envs with the same errors (the batch pass and the
+    fused step's network give the same bits; 40,000 envs batch-judge with
+    128-env workgroups)."""
+    mlp = MLP(device, seed=4)
+    runs = [_stepper_run(device, mlp, n, 32, 1e-3, defer) for defer in (True, False)]
+    (sd, pd, _), (si, pi, _) = runs
+    hd, hi = sd.host(), si.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(hd[k], hi[k]), k
+    a, b = pd.summary(), pi.summary()
+    assert a["judged"] == b["judged"] == 32 * n // 16
+    assert a["rejected"] == b["rejected"] == a["judged"]
+    assert a["max_judged_rel_err"] == b["max_judged_rel_err"]
+    assert a["batch_judged"] == a["judged"] and a["pending_records"] == 0
+    assert a["aba_frac"] == b["aba_frac"] == 1.0
+
+
+def test_deferred_judging_accepts_at_the_period_end(device):
+    """Tolerance 1e30 (every judgement accepts): with deferred judging every
+    env stays with ABA through the first period (the state is plain ABA's bit
+    for bit), all leave fallback after its last step, and the next period's
+    judgements, of envs off fallback, are made at once."""
+    mlp = MLP(device, seed=4)
+    n = 4096
+    st, stepper, fb = _stepper_run(device, mlp, n, 16, 1e30, True)
+    ref = _stepper_run(device, mlp, n, 16, 1e-3, False)[0]          # never accepted: ABA
+    h, r = st.host(), ref.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(h[k], r[k]), k
+    assert all(int(f.sum()) == n for f in fb[:15]) and int(fb[15].sum()) == 0
+    s1 = stepper.summary()
+    assert s1["batch_judged"] == n and s1["rejected"] == 0 and s1["judged"] == n
+    for _ in range(16):
+        stepper.step(st.qpos, st.qvel, st.anchor, st.ctrl)
+    torch.cuda.synchronize()
+    s2 = stepper.summary()
+    assert s2["batch_judged"] == n and s2["judged"] == 2 * n and s2["pending_records"] == 0
+    assert int(stepper.fallback.sum().item()) == 0 and np.isfinite(st.host()["qpos"]).all()
+
+
+def test_reset_clears_pending_records(device):
+    """mark_reset mid-period: the records made before the reset are dropped
+    (they judged the pre-reset state), so those blocks stay in fallback; the
+    blocks judged after the reset are accepted at the period end."""
+    mlp = MLP(device, seed=4)
+    n = 4096
+    st, stepper, fb = _stepper_run(device, mlp, n, 16, 1e30, True, reset_at=8)
+    blk = (np.arange(n) // 64) % 16
+    assert np.array_equal(fb[15], (blk < 8).astype(np.uint8))
+    assert stepper.summary()["batch_judged"] == int((blk >= 8).sum())

@@ -175,3 +175,31 @@ def test_training_and_stepper_entry_points_validate_arguments():
     # zero steps is a no-op
     assert lib.g1_hip_surrogate_steps(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d, d, d,
                                       None, None, None, None, 0, None) == 0
+
+
+def test_deferred_judging_entry_points_validate_arguments():
+    """Host-side checks of the deferred-judging entry points (before any
+    launch): records, flags and the batch's prediction buffer are required."""
+    lib = _lib.load()
+    d = C.c_void_p(16)
+    ws = _lib.ptr_array([16] * 5)
+    # step: no record buffer, no flags
+    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
+                                              None, d, None) == -1
+    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
+                                              d, None, None) == -1
+    assert "g1_hip_surrogate_step_deferred" in _lib.last_error()
+    # phase outside the period
+    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 16, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
+                                              d, d, None) == -1
+    # batch: prediction rows narrower than the network's 64 outputs, no fallback flags
+    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, d, d, d, d, d, 32, None) == -1
+    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, d, d, None, d, d, 64, None) == -1
+    assert "g1_hip_surrogate_judge_batch" in _lib.last_error()
+    # steps: missing records; zero steps is a no-op
+    assert lib.g1_hip_surrogate_steps_deferred(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
+                                               d, d, None, None, None, None, 0, None, d, d, 64, None) == -1
+    assert lib.g1_hip_surrogate_steps_deferred(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
+                                               d, d, None, None, None, None, 0, d, d, d, 64, None) == 0
+    # zero envs is a no-op for each
+    assert lib.g1_hip_surrogate_judge_batch(0, 1e-3, d, d, ws, ws, d, d, d, d, d, d, d, 64, None) == 0
