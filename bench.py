@@ -645,11 +645,12 @@ def cpu_baseline(samples=None):
 
 
 # ================================================================ profiling
-def kernel_only(kind, nenv, steps, warmup):
+def kernel_only(kind, nenv, steps, warmup, tol=1e30):
     """One fused inference kernel alone, one surrogate stepper step (judge +
-    forces, network + commit; random weights, every judgement accepted after
-    the warmup), or one training iteration (graph replays), timed with HIP
-    events: for profiling runs."""
+    forces, network + commit; random weights; at the default tol every
+    judgement is accepted after the warmup, at 1e-3 every env stays in
+    fallback and its judgements are deferred), or one training iteration
+    (graph replays), timed with HIP events: for profiling runs."""
     import torch
     from g1sim_amd import workloads as W
     dev = torch.device("cuda:0")
@@ -672,13 +673,13 @@ def kernel_only(kind, nenv, steps, warmup):
         from g1sim_amd.surrogate import MLP, SurrogateStepper
         mlp = MLP(dev, seed=0)
         anchor, ctrl = torch.as_tensor(an, device=dev), torch.as_tensor(cb[0], device=dev)
-        st = SurrogateStepper(mlp, nenv, dev, period=16, tol=1e30)
+        st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
         fn = lambda: st.step(qpos, qvel, anchor, ctrl)  # noqa: E731
         for _ in range(max(warmup, 16)):
             fn()
         torch.cuda.synchronize()
         ms = _event_avg_ms(fn, steps, torch.cuda.current_stream(dev))
-        print(json.dumps({"kernel": "surrogate stepper step", "nenv": nenv, "steps": steps, "avg_step_ms": ms,
+        print(json.dumps({"kernel": "surrogate stepper step", "nenv": nenv, "tol": tol, "steps": steps, "avg_step_ms": ms,
                           "env_steps_per_s": nenv / (ms * 1e-3)}), flush=True)
         return
     if kind == "surrogate":
@@ -764,6 +765,8 @@ def main():
     ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper", "train"),
                     help="time one fused inference kernel, one surrogate stepper step or one training "
                          "iteration alone at --nenv (for profiling) and exit")
+    ap.add_argument("--tol", type=float, default=1e30,
+                    help="judge tolerance of --kernel stepper (default: every judgement accepted)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -775,7 +778,7 @@ def main():
         sys.exit(2)
 
     if args.kernel:
-        kernel_only(args.kernel, args.nenv, args.steps, args.warmup)
+        kernel_only(args.kernel, args.nenv, args.steps, args.warmup, args.tol)
         return
     if args.peaks:
         print(json.dumps(measure_peaks()), flush=True)

@@ -308,10 +308,12 @@ struct SurrogateArgs {
   float* qacc_out;
   float* qpos_rw;
   float* qvel_rw;
-  // deferred judging (MODE 1, nullable): the record of a judged block whose
-  // live envs are all in fallback, [SNAP_ROWS][nenv] (network inputs, then
-  // ABA's qacc), and the env's pending-judgement flag
-  float* snap;
+  // deferred judging (MODE 1): the blocks whose live envs are all in
+  // fallback were committed by the judge + forces kernel
+  // (g1_hip_judge_forces_commit), which also recorded the judged ones
+  int committed;
+  // batch judgement (MODE 2): the pending-record flags (the records are the
+  // network input, qacc_aba their ABA qacc rows)
   uint8_t* sflag;
 };
 
@@ -744,7 +746,7 @@ constexpr int STASH_ABA = 71, STASH_FB = STASH_ABA + NV, STASH_ROWS = STASH_FB +
 template <int MODE, int R>
 __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a) {
   constexpr int NTHR = Rows<R>::NT, NWAVE = Rows<R>::NW;
-  static_assert(MODE == 0 || R == ROWS, "the step (commit) kernel works on 64-env judge blocks");
+  static_assert(MODE != 1 || R == ROWS, "the step (commit) kernel works on 64-env judge blocks");
   constexpr int RT = Rows<R>::RT;
   __shared__ __attribute__((aligned(16))) bf16 buf[Rows<R>::NBUF][R * LDA];
   __shared__ float stash[MODE == 1 ? STASH_ROWS * ROWS : 1];
@@ -752,7 +754,7 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   // with 128 rows the layers run as the split-half chain (split::chain), the
   // network input in columns 256-383 of the buffer, and the hidden layers'
   // biases are staged in LDS ([4][512])
-  constexpr bool SPLIT = MODE == 0 && R == 128;
+  constexpr bool SPLIT = MODE != 1 && R == 128;
   constexpr bool SPLIT2 = R == 64;
   constexpr int FCOL = SPLIT ? split::IN_COL : 0;
   constexpr bool SBIAS = R == 128;
@@ -783,6 +785,12 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   // are no flags) and tested after the other loads are issued
   const bool has_fb = MODE == 1 && a.fallback != nullptr;
   const uint8_t fbyte = *(has_fb ? a.fallback + ec : reinterpret_cast<const uint8_t*>(a.qpos));
+  if constexpr (MODE == 1) {
+    // deferred judging: a tile whose live envs are all in fallback has been
+    // committed by the judge + forces kernel -- leave before any other load
+    // (with every env in fallback, every tile)
+    if (a.committed && !__syncthreads_or(e < a.nenv && fbyte == 0)) return;
+  }
   auto feature_src = [&](int f) -> const float* {
     return f < 36 ? a.qpos + (size_t)f * N : f < 71 ? a.qvel + (size_t)(f - 36) * N : a.tau + (size_t)(f - 71) * N;
   };
@@ -825,43 +833,20 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   const bool fb = has_fb && e < a.nenv && fbyte != 0;
   if constexpr (MODE == 1) {
     // A tile that is not judged this step and whose live envs are all in
-    // fallback takes ABA's qacc everywhere: commit without the network.
-    // With deferred judging (a.snap) so does a judged tile whose live envs
-    // are all in fallback -- a fallback env commits ABA's qacc whatever the
-    // judgement -- after recording its network inputs and ABA's qacc for the
-    // batch judgement of the period (judge_batch_kernel), which decides
-    // whether the envs leave fallback.
-    if (a.fallback && (!judged_blk || a.snap)) {
+    // fallback takes ABA's qacc everywhere: commit without the network.  With
+    // deferred judging (a.committed) every tile whose live envs are all in
+    // fallback, judged or not, has been committed (and a judged one recorded)
+    // by the judge + forces kernel: nothing to do.
+    if (a.fallback && (!judged_blk || a.committed)) {
       const int own = e < a.nenv && !fb;
       if (!__syncthreads_or(own)) {
-        if (judged_blk && e < a.nenv) {
-          // this thread's features (all of them in the one feature pass of
-          // a 64-env tile) and ABA rows
-          static_assert(NPASS == 1, "the record takes the features of the single pass");
-          static_for<G>([&](auto FG) {
-            if (fg == decltype(FG)::value) {
-              static_for<FPP>([&](auto I) {
-                constexpr int ii = decltype(I)::value;
-                constexpr int f = 8 * (decltype(FG)::value + G * (ii / 8)) + ii % 8;
-                if constexpr (f < SUR_NIN) a.snap[(size_t)f * N + e] = raw[ii];
-              });
-            }
-          });
-#pragma unroll
-          for (int i = 0; i < APW; ++i) {
-            const int j = fg + G * i;
-            if (j < NV) a.snap[(size_t)(SUR_NIN + j) * N + e] = abav[i];
-          }
-          if (fg == 0) a.sflag[e] = 1;
-        }
+        if (a.committed) return;
         if (w == 0) {
           float sur[NV];
 #pragma unroll
           for (int j = 0; j < NV; ++j) sur[j] = 0.f;
-          // phase -1: no env of the tile is judged now (a recorded tile is
-          // judged with the batch)
-          g1k::surrogate_commit(e < a.nenv, e, N, judged_blk ? -1 : a.phase, a.period, a.tol, a.dt, sur,
-                                a.qacc_aba, a.fallback, a.stats, a.qacc_out, a.qpos_rw, a.qvel_rw);
+          g1k::surrogate_commit(e < a.nenv, e, N, a.phase, a.period, a.tol, a.dt, sur, a.qacc_aba, a.fallback,
+                                a.stats, a.qacc_out, a.qpos_rw, a.qvel_rw);
         }
         return;
       }
@@ -963,6 +948,48 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     for (int i = tid; i < R * SUR_POUT; i += NTHR) {
       const int rr = i / SUR_POUT, c = i % SUR_POUT, er = e0 + rr;
       if (er < a.nenv) a.pred[(size_t)er * a.ldp + c] = out[rr * LDO + c];
+    }
+  } else if constexpr (MODE == 2) {
+    // The batch judgement of deferred judging: env e with a record of this
+    // period (sflag[e]) is judged on its recorded state -- the network's
+    // output against the recorded ABA qacc, the judge's measure and
+    // tolerance (g1k::surrogate_judge) -- and leaves fallback if within it.
+    // The env has stayed in fallback since it was recorded (every env is
+    // judged once per period; a reset clears its record), so the network's
+    // qacc is still used only by envs whose last judgement is at most
+    // period - 1 steps old.  stats[4] / [5]: batch-judged / rejected envs,
+    // [2] the max judged error.  One env per thread of the first R.
+    if (tid < R) {
+      const int er = e0 + tid;
+      const bool rec = er < a.nenv && a.sflag[er] != 0;
+      float err = 0.f;
+      if (rec) {
+        const float* orow = out + tid * LDO;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          const float sur = orow[j] * a.ystd[j] + a.ymean[j];
+          const float ref = a.qacc_aba[(size_t)j * N + er];
+          err = fmaxf(err, fabsf(sur - ref) * g1k::frcp(fmaxf(fabsf(ref), 1.0f)));
+        }
+      }
+      const bool reject = rec && !(err <= a.tol);
+      if (rec) {
+        a.fallback[er] = reject ? 1 : 0;
+        a.sflag[er] = 0;
+      }
+      const unsigned n_judged = (unsigned)__popcll(__ballot(rec));
+      const unsigned n_reject = (unsigned)__popcll(__ballot(reject));
+      unsigned ebits = rec ? __float_as_uint(err) : 0u;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = (unsigned)__shfl_xor((int)ebits, off, 64);
+        ebits = o > ebits ? o : ebits;
+      }
+      if (lane == 0 && n_judged) {
+        atomicAdd(&a.stats[4], n_judged);
+        if (n_reject) atomicAdd(&a.stats[5], n_reject);
+        atomicMax(&a.stats[2], ebits);
+      }
     }
   } else {
     // The commit (g1_commit.h), spread over the waves: wave 0 judges and
@@ -1591,51 +1618,6 @@ __global__ __launch_bounds__(256) void rollout_init_kernel(int nenv, float lin, 
   if (push_interval) push_interval[e] = draw_int(seed, -1, e, K_INIT + 3, 50, 100);
 }
 
-// The batch judgement of deferred judging: env e with a record of this
-// period (sflag[e]) is judged on its recorded state -- the network's output
-// pred (the forward over the records) against the recorded ABA qacc, the
-// judge's measure and tolerance -- and leaves fallback if within it.  The
-// env has stayed in fallback since it was recorded (every env is judged once
-// per period; a reset clears its record), so the network's qacc is still
-// used only by envs whose last judgement is at most period - 1 steps old.
-// stats[4] / [5]: batch-judged / rejected envs, [2] the max judged error.
-__global__ __launch_bounds__(256) void judge_batch_kernel(int nenv, float tol, const float* __restrict__ pred,
-                                                          int ldp, const float* __restrict__ ymean,
-                                                          const float* __restrict__ ystd,
-                                                          const float* __restrict__ snap, uint8_t* sflag,
-                                                          uint8_t* fallback, unsigned* stats) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t N = (size_t)nenv;
-  const bool rec = e < nenv && sflag[e] != 0;
-  float err = 0.f;
-  if (rec) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      const float sur = pred[(size_t)e * ldp + j] * ystd[j] + ymean[j];
-      const float ref = snap[(size_t)(SUR_NIN + j) * N + e];
-      err = fmaxf(err, fabsf(sur - ref) * g1k::frcp(fmaxf(fabsf(ref), 1.0f)));
-    }
-  }
-  const bool reject = rec && !(err <= tol);
-  if (rec) {
-    fallback[e] = reject ? 1 : 0;
-    sflag[e] = 0;
-  }
-  const unsigned n_judged = (unsigned)__popcll(__ballot(rec));
-  const unsigned n_reject = (unsigned)__popcll(__ballot(reject));
-  unsigned ebits = rec ? __float_as_uint(err) : 0u;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned o = (unsigned)__shfl_xor((int)ebits, off, 64);
-    ebits = o > ebits ? o : ebits;
-  }
-  if ((threadIdx.x & 63) == 0 && n_judged) {
-    atomicAdd(&stats[4], n_judged);
-    if (n_reject) atomicAdd(&stats[5], n_reject);
-    atomicMax(&stats[2], ebits);
-  }
-}
-
 }  // namespace g1inf
 
 // ================================================================== C-ABI
@@ -1704,12 +1686,11 @@ int g1_hip_surrogate_step_deferred(int nenv, float dt, int phase, int period, fl
                                    const float* tau, const float* xmean, const float* xinv, const void* const* W,
                                    const float* const* b, const float* ymean, const float* ystd,
                                    const float* qacc_aba, uint8_t* fallback, unsigned* stats, float* qacc_out,
-                                   float* snap, uint8_t* sflag, void* stream) {
+                                   void* stream) {
   g1err::clear();
   if (nenv < 0 || period <= 0 || phase < 0 || phase >= period || !(dt > 0.f))
     return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
-  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba || !fallback || !stats || !snap ||
-      !sflag)
+  if (!qpos || !qvel || !tau || !xmean || !xinv || !ymean || !ystd || !qacc_aba || !fallback || !stats)
     return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
   SurrogateArgs a{};
   if (!fill_surrogate_weights(a, W, b)) return g1err::fail("g1_hip_surrogate_step_deferred: invalid arguments");
@@ -1717,7 +1698,7 @@ int g1_hip_surrogate_step_deferred(int nenv, float dt, int phase, int period, fl
   a.nenv = nenv; a.qpos = qpos; a.qvel = qvel; a.tau = tau; a.xmean = xmean; a.xinv = xinv;
   a.phase = phase; a.period = period; a.tol = tol; a.dt = dt; a.ymean = ymean; a.ystd = ystd;
   a.qacc_aba = qacc_aba; a.fallback = fallback; a.stats = stats; a.qacc_out = qacc_out;
-  a.qpos_rw = qpos; a.qvel_rw = qvel; a.snap = snap; a.sflag = sflag;
+  a.qpos_rw = qpos; a.qvel_rw = qvel; a.committed = 1;
   hipLaunchKernelGGL((surrogate_kernel<1, ROWS>), dim3((nenv + ROWS - 1) / ROWS), dim3(Rows<ROWS>::NT), 0,
                      (hipStream_t)stream, a);
   return g1err::launch("g1_hip_surrogate_step_deferred");
@@ -1725,20 +1706,23 @@ int g1_hip_surrogate_step_deferred(int nenv, float dt, int phase, int period, fl
 
 int g1_hip_surrogate_judge_batch(int nenv, float tol, const float* xmean, const float* xinv, const void* const* W,
                                  const float* const* b, const float* ymean, const float* ystd, const float* snap,
-                                 uint8_t* sflag, uint8_t* fallback, unsigned* stats, float* pred, int ldp,
-                                 void* stream) {
+                                 uint8_t* sflag, uint8_t* fallback, unsigned* stats, void* stream) {
   g1err::clear();
-  if (nenv < 0 || !xmean || !xinv || !ymean || !ystd || !snap || !sflag || !fallback || !stats || !pred ||
-      ldp < SUR_POUT)
+  if (nenv < 0 || !xmean || !xinv || !ymean || !ystd || !snap || !sflag || !fallback || !stats)
     return g1err::fail("g1_hip_surrogate_judge_batch: invalid arguments");
-  SurrogateArgs chk{};
-  if (!fill_surrogate_weights(chk, W, b)) return g1err::fail("g1_hip_surrogate_judge_batch: invalid arguments");
+  SurrogateArgs a{};
+  if (!fill_surrogate_weights(a, W, b)) return g1err::fail("g1_hip_surrogate_judge_batch: invalid arguments");
   if (nenv == 0) return 0;
   const size_t N = (size_t)nenv;
-  int rc = g1_hip_surrogate_forward(nenv, snap, snap + 36 * N, snap + 71 * N, xmean, xinv, W, b, pred, ldp, stream);
-  if (rc) return rc;
-  hipLaunchKernelGGL(judge_batch_kernel, dim3((nenv + 255) / 256), dim3(256), 0, (hipStream_t)stream, nenv, tol,
-                     pred, ldp, ymean, ystd, snap, sflag, fallback, stats);
+  a.nenv = nenv; a.qpos = snap; a.qvel = snap + 36 * N; a.tau = snap + 71 * N; a.xmean = xmean; a.xinv = xinv;
+  a.tol = tol; a.ymean = ymean; a.ystd = ystd; a.qacc_aba = snap + 106 * N; a.fallback = fallback;
+  a.stats = stats; a.sflag = sflag;
+  if (nenv >= kRows128MinEnvs)
+    hipLaunchKernelGGL((surrogate_kernel<2, 128>), dim3((nenv + 127) / 128), dim3(Rows<128>::NT), 0,
+                       (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL((surrogate_kernel<2, 64>), dim3((nenv + 63) / 64), dim3(Rows<64>::NT), 0,
+                       (hipStream_t)stream, a);
   return g1err::launch("g1_hip_surrogate_judge_batch");
 }
 
@@ -1768,31 +1752,34 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
   return 0;
 }
 
-// The same with deferred judging: g1_hip_surrogate_step_deferred per step,
-// and the batch judgement (g1_hip_surrogate_judge_batch) after each step
-// whose phase is period - 1.
+// The same with deferred judging: per step g1_hip_judge_forces_commit (which
+// also commits, and records if judged, every block whose envs are all in
+// fallback) and g1_hip_surrogate_step_deferred (the other blocks), and the
+// batch judgement (g1_hip_surrogate_judge_batch) after each step whose phase
+// is period - 1.
 int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
                                     float* qvel, float* anchor, const float* ctrl, const float* mu,
                                     const float* xmean, const float* xinv, const void* const* W,
                                     const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
                                     float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
                                     float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
-                                    float* snap, uint8_t* sflag, float* pred, int ldp, void* stream) {
+                                    float* snap, uint8_t* sflag, void* stream) {
   g1err::clear();
   if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
-      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag || !pred || ldp < SUR_POUT)
+      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag)
     return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
   for (int s = 0; s < nsteps; ++s) {
     const int phase = (int)(((long long)t0 + s) % period);
-    int rc = g1_hip_judge_forces(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_aba, tau,
-                                 contact_forces, foot_pos, foot_vel, judge_flags, stream);
+    int rc = g1_hip_judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_aba,
+                                        tau, contact_forces, foot_pos, foot_vel, qacc_out, snap, sflag, stats,
+                                        judge_flags, stream);
     if (rc) return rc;
     rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean,
-                                        ystd, qacc_aba, fallback, stats, qacc_out, snap, sflag, stream);
+                                        ystd, qacc_aba, fallback, stats, qacc_out, stream);
     if (rc) return rc;
     if (phase == period - 1) {
       rc = g1_hip_surrogate_judge_batch(nenv, tol, xmean, xinv, W, b, ymean, ystd, snap, sflag, fallback, stats,
-                                        pred, ldp, stream);
+                                        stream);
       if (rc) return rc;
     }
   }

@@ -657,6 +657,18 @@ struct StepArgs {
   // its substeps done before the launch)
   long long ctrl_stride;
   int ctrl_period, ctrl_phase;
+  // Judge + forces with commit (commit_fallback, MODE_JUDGE_FORCES): a block
+  // whose live envs are all in fallback is also integrated and stored as by
+  // the plain step (a fallback env takes ABA's qacc whatever its
+  // judgement).  If that block is judged this step, its deferred-judgement
+  // record goes to snap ([141][nenv]: qpos 36, qvel 35 before the step,
+  // tau_total 35, qacc 35) with sflag[e] = 1.  qacc_track (nullable) receives
+  // the committed envs' qacc; stats[3] counts their env-steps.
+  int commit_fallback;
+  float* snap;
+  uint8_t* sflag;
+  float* qacc_track;
+  unsigned* stats;
 };
 // Kernel variants (separate template instances, so the plain step's code is
 // not touched by the judge modes).
@@ -784,7 +796,7 @@ G1K_DEV void dispatch_post(int chain, const float (&a0)[6], const float (&a15)[6
 // pelvis/waist code (the 4-wave split).
 template <bool F64_BASE, bool F64_CONTACT, int SPLIT, int ROLE, int MODE>
 G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PER_WG],
-                       float (&fcx)[2][2][6][ENVS_PER_WG], float* wsh) {
+                       float (&fcx)[2][2][6][ENVS_PER_WG], float* wsh, bool commit = false, bool record = false) {
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   constexpr bool JF = MODE == MODE_JUDGE_FORCES;
   const int lane = threadIdx.x & 63;
@@ -880,6 +892,36 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 #pragma unroll
     for (int m = 0; m < 8; ++m) anc[c][m] = legs ? args.anchor[(ch * 8 + m) * N + e] : ANCHOR_FREE;
   });
+  // deferred-judgement record, part 1: the state before the step, in the
+  // network's input order (qpos rows 0-35, qvel rows 36-70)
+  if constexpr (JF) {
+    if (record && valid) {
+      float* sp = args.snap;
+      if (wave == 0) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          sp[m * N + e] = (float)bp[m];
+          sp[(36 + m) * N + e] = vw[m];
+          sp[(36 + 3 + m) * N + e] = om[m];
+          sp[(7 + 12 + m) * N + e] = wq[m];
+          sp[(36 + 6 + 12 + m) * N + e] = wqd[m];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) sp[(3 + m) * N + e] = (float)bq[m];
+      }
+      static_for<CPW>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
+#pragma unroll
+        for (int m = 0; m < MAXJ; ++m) {
+          if (m < nj) {
+            sp[(7 + h0 + m) * N + e] = cq[c][m];
+            sp[(36 + 6 + h0 + m) * N + e] = cqd[c][m];
+          }
+        }
+      });
+    }
+  }
   // target schedule: the next block's targets are loaded one substep before
   // they apply, so their memory latency hides behind that substep
   const bool sched = MODE == MODE_STEP && args.ctrl_stride > 0;
@@ -1130,8 +1172,19 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     });
     return;
   }
+  // judge + forces with commit: the committed block's state, its tracked
+  // qacc, and (judged) the rest of its deferred-judgement record
+  float* trk = JF && commit ? args.qacc_track : nullptr;
+  float* rec = JF && record ? args.snap : nullptr;
+  if constexpr (JF) {
+    if (commit && wave == 0) {
+      // the active lanes are the valid ones (the others have returned)
+      const unsigned long long vm = __ballot(true);
+      if (lane == 0 && args.stats) atomicAdd(&args.stats[3], (unsigned)__popcll(vm));
+    }
+  }
   if (wave == 0) {
-    if constexpr (!JF) {
+    if (!JF || commit) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) args.qpos[m * N + e] = (float)bp[m];
 #pragma unroll
@@ -1150,17 +1203,38 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
         args.qvel[(6 + 12 + m) * N + e] = wqd[m];
       }
     }
-    if (want_tau) {
+    // (a committed block's qacc and tau are read by nobody: the network
+    // kernel leaves the block alone, the record has its own copy)
+    if (want_tau && !(JF && commit)) {
 #pragma unroll
       for (int m = 0; m < 6; ++m) args.tau[m * N + e] = tau_b[m];
 #pragma unroll
       for (int m = 0; m < 3; ++m) args.tau[(6 + 12 + m) * N + e] = tau_w[m];
     }
-    if (args.qacc) {
+    if (args.qacc && !(JF && commit)) {
 #pragma unroll
       for (int m = 0; m < 6; ++m) args.qacc[m * N + e] = qacc_base[m];
 #pragma unroll
       for (int m = 0; m < 3; ++m) args.qacc[(6 + 12 + m) * N + e] = qdd_w[m];
+    }
+    if (trk) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) trk[m * N + e] = qacc_base[m];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) trk[(6 + 12 + m) * N + e] = qdd_w[m];
+    }
+    if (rec) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        rec[(71 + m) * N + e] = tau_b[m];
+        rec[(106 + m) * N + e] = qacc_base[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        rec[(71 + 6 + 12 + m) * N + e] = tau_w[m];
+        rec[(106 + 6 + 12 + m) * N + e] = qdd_w[m];
+      }
+      args.sflag[e] = 1;
     }
   }
   static_for<CPW>([&](auto C) {
@@ -1169,12 +1243,17 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 #pragma unroll
     for (int m = 0; m < MAXJ; ++m) {
       if (m < nj) {
-        if constexpr (!JF) {
+        if (!JF || commit) {
           args.qpos[(7 + h0 + m) * N + e] = cq[c][m];
           args.qvel[(6 + h0 + m) * N + e] = cqd[c][m];
         }
-        if (args.qacc) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
-        if (want_tau) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
+        if (args.qacc && !(JF && commit)) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
+        if (want_tau && !(JF && commit)) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
+        if (trk) trk[(6 + h0 + m) * N + e] = qdd_c[c][m];
+        if (rec) {
+          rec[(71 + 6 + h0 + m) * N + e] = tau_c[c][m];
+          rec[(106 + 6 + h0 + m) * N + e] = qdd_c[c][m];
+        }
       }
     }
     if (legs) {
@@ -1346,22 +1425,28 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
   __shared__ float xch[2][SPLIT][XCH][ENVS_PER_WG];
   __shared__ float fcx[2][2][6][ENVS_PER_WG];   // legs' contact wrenches, pelvis frame (4-wave split)
   __shared__ float wsh[W_IN_LDS<SPLIT> ? 4 * MAXJ * 6 * ENVS_PER_WG : 1];
+  bool commit = false, record = false;
   if constexpr (MODE == MODE_JUDGE_FORCES) {
     const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
+    const bool live = e_raw < args.nenv;
     const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
-    const bool in = e_raw < args.nenv && (blk || (args.sel_fallback && args.sel_fallback[e_raw]));
+    const bool fb = live && args.sel_fallback && args.sel_fallback[e_raw];
+    const bool in = live && (blk || fb);
     if (__ballot(in) == 0ull) {        // uniform: every wave reads the same 64 flags
       forces_block(forces_args(args), __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), fcx[0][0]);
       return;
     }
+    // every live env of the block in fallback: commit here (uniform)
+    commit = args.commit_fallback && args.sel_fallback && __ballot(live && !fb) == 0ull;
+    record = commit && blk && args.snap;
   }
   if constexpr (SPLIT == 4) {
-    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx, wsh);
+    step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx, wsh, commit, record);
   } else {
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, MODE>(args, xch, fcx, wsh);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 0, MODE>(args, xch, fcx, wsh, commit, record);
     else
-      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, MODE>(args, xch, fcx, wsh);
+      step_body<F64_BASE, F64_CONTACT, SPLIT, 1, MODE>(args, xch, fcx, wsh, commit, record);
   }
 }
 
@@ -1748,6 +1833,41 @@ int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel
   a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
   a.stride = nenv;
   a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
+}
+
+// g1_hip_judge_forces, and a block whose live envs are all in fallback is
+// also committed (integrated and stored as by the plain step) with its
+// deferred-judgement record written if it is judged this step
+// (StepArgs::commit_fallback).
+int g1_hip_judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                               const float* mu, const uint8_t* fallback, int phase, int period, float* qacc_out,
+                               float* tau_out, float* contact_forces, float* foot_pos, float* foot_vel,
+                               float* qacc_track, float* snap, uint8_t* sflag, unsigned* stats, unsigned flags,
+                               void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (period <= 0 || phase < 0 || phase >= period || !qacc_out || !tau_out || !fallback || !snap || !sflag ||
+      !stats) {
+    set_err("judge_forces_commit: bad phase/period, or no qacc / tau / fallback / records / stats");
+    return -1;
+  }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("judge_forces_commit: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv; a.nsub = 1;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
+  a.ctrl = ctrl;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
+  a.stride = nenv;
+  a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
+  a.commit_fallback = 1; a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
   return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
 }
 

@@ -492,7 +492,6 @@ class SurrogateStepper:
         if self.defer:
             self.snap = torch.zeros((SNAP_ROWS, nenv), **f32)
             self.sflag = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
-            self.bpred = torch.zeros((max(nenv, 1), POUT), **f32)
             self.dropped = torch.zeros((), dtype=torch.int64, device=self.dev)     # records dropped by resets
         self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
         self.pred = torch.zeros((self.rows, POUT), **f32)
@@ -527,31 +526,45 @@ class SurrogateStepper:
             check_soa("mu", mu.view(1, -1), 1, n, self.dev)
         phase = self.t % self.period
         ck = _lib.check
-        # ABA qacc of the judged and fallback envs, and everyone's tau_total,
-        # contact update and feet, in one kernel
-        ck(lib.g1_hip_judge_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(), ctrl.data_ptr(),
-                                   None if mu is None else mu.data_ptr(), self.fallback.data_ptr(), phase,
-                                   self.period, self.qacc_aba.data_ptr(), self.tau.data_ptr(),
-                                   self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
-                                   self.foot_vel.data_ptr(), self.judge_flags, s), "judge_forces")
-        m = self.mlp
         qacc_out = None if self.qacc is None else self.qacc.data_ptr()
+        defer = self.defer and prediction is None
+        if defer:
+            # the same, and every block whose envs are all in fallback is
+            # committed (and recorded when judged) by this kernel
+            ck(lib.g1_hip_judge_forces_commit(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                              ctrl.data_ptr(), None if mu is None else mu.data_ptr(),
+                                              self.fallback.data_ptr(), phase, self.period,
+                                              self.qacc_aba.data_ptr(), self.tau.data_ptr(),
+                                              self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
+                                              self.foot_vel.data_ptr(), qacc_out, self.snap.data_ptr(),
+                                              self.sflag.data_ptr(), self.stats.data_ptr(), self.judge_flags, s),
+               "judge_forces_commit")
+        else:
+            # ABA qacc of the judged and fallback envs, and everyone's
+            # tau_total, contact update and feet, in one kernel
+            ck(lib.g1_hip_judge_forces(n, self.dt, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+                                       ctrl.data_ptr(), None if mu is None else mu.data_ptr(),
+                                       self.fallback.data_ptr(), phase, self.period, self.qacc_aba.data_ptr(),
+                                       self.tau.data_ptr(), self.contact_forces.data_ptr(),
+                                       self.foot_pos.data_ptr(), self.foot_vel.data_ptr(), self.judge_flags, s),
+               "judge_forces")
+        m = self.mlp
         if prediction is not None and (prediction.dtype != torch.float32 or prediction.device != self.dev
                                        or not prediction.is_contiguous() or prediction.dim() != 2
                                        or prediction.shape[0] < n or prediction.shape[1] != POUT):
             raise ValueError(f"prediction must be a contiguous float32 (>= {n}, {POUT}) tensor on {self.dev}")
-        if self.fused and prediction is None and self.defer:
+        if defer:
             W, b = m._wb()
             ck(lib.g1_hip_surrogate_step_deferred(
                 n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(), self.tau.data_ptr(),
                 m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(), self.ystd.data_ptr(),
-                self.qacc_aba.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(), qacc_out,
-                self.snap.data_ptr(), self.sflag.data_ptr(), s), "surrogate_step_deferred")
+                self.qacc_aba.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(), qacc_out, s),
+               "surrogate_step_deferred")
             if phase == self.period - 1:
                 ck(lib.g1_hip_surrogate_judge_batch(
                     n, self.tol, m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
                     self.ystd.data_ptr(), self.snap.data_ptr(), self.sflag.data_ptr(), self.fallback.data_ptr(),
-                    self.stats.data_ptr(), self.bpred.data_ptr(), POUT, s), "surrogate_judge_batch")
+                    self.stats.data_ptr(), s), "surrogate_judge_batch")
         elif self.fused and prediction is None:
             W, b = m._wb()
             ck(lib.g1_hip_surrogate_step(n, self.dt, phase, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(),
@@ -598,8 +611,8 @@ class SurrogateStepper:
                 ctrl.data_ptr(), ptr(mu), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
                 self.ystd.data_ptr(), self.qacc_aba.data_ptr(), self.tau.data_ptr(), self.fallback.data_ptr(),
                 self.stats.data_ptr(), ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
-                self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
-                self.bpred.data_ptr(), POUT, s), "surrogate_steps_deferred")
+                self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(), s),
+                "surrogate_steps_deferred")
             self.t += n
             return
         _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),

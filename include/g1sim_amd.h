@@ -295,21 +295,26 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
 /* Deferred judging (no reference counterpart; the judge of the north
  * star's surrogate + ABA-judge stepper).
  *
- * g1_hip_surrogate_step_deferred is g1_hip_surrogate_step, except for a
- * judged block whose envs are all in fallback.  Such a block is committed
- * with ABA's qacc, as a fallback env is whatever the judgement.  It records
- * its network inputs (rows 0-105: qpos 36, qvel 35, tau 35) and ABA's qacc
- * (rows 106-140) into snap [141][nenv] and sets sflag [nenv].
+ * g1_hip_judge_forces_commit is g1_hip_judge_forces plus a commit: a block of
+ * 64 envs whose live envs are all in fallback is integrated and stored in
+ * place, as by the plain step (a fallback env takes ABA's qacc whatever its
+ * judgement).  qacc_track (nullable) receives those envs' qacc, and stats[3]
+ * counts their env-steps.  If such a block is judged this step, it records
+ * its network inputs before the step (rows 0-105: qpos 36, qvel 35,
+ * tau_total 35) and ABA's qacc (rows 106-140) into snap [141][nenv], and
+ * sets sflag [nenv].
  *
- * g1_hip_surrogate_judge_batch runs the network on the records
- * (g1_hip_surrogate_forward into pred [nenv][ldp], ldp >= 64) and judges
- * each recorded env with the judge's measure and tolerance.  An env within
- * the tolerance leaves fallback, then its flag is cleared.  stats needs 6
- * entries: [4] / [5] count batch-judged / rejected envs, [2] takes their max
- * error.
+ * g1_hip_surrogate_step_deferred, called after it, is g1_hip_surrogate_step
+ * for the other blocks (it leaves the committed ones alone).
  *
- * g1_hip_surrogate_steps_deferred is g1_hip_surrogate_steps with these:
- * the batch judgement follows every step whose phase is period - 1.
+ * g1_hip_surrogate_judge_batch runs the network on the records (the fused
+ * forward kernel, its output kept in LDS) and judges each recorded env with
+ * the judge's measure and tolerance.  An env within the tolerance leaves
+ * fallback, then its flag is cleared.  stats needs 6 entries: [4] / [5]
+ * count batch-judged / rejected envs, [2] takes their max error.
+ *
+ * g1_hip_surrogate_steps_deferred runs these per step, with the batch
+ * judgement after every step whose phase is period - 1.
  *
  * Blocks with an env off fallback are judged at once.  So, as with
  * immediate judging, the network's qacc is integrated only for envs whose
@@ -317,16 +322,20 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
  * an env leaves fallback at the end of the period instead of the step after
  * its judgement.  Callers that reset envs clear their sflag (the record
  * would judge the pre-reset state). */
+int g1_hip_judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anchor,
+                               const float* ctrl, const float* mu, const uint8_t* fallback, int phase,
+                               int period, float* qacc_out, float* tau_out, float* contact_forces,
+                               float* foot_pos, float* foot_vel, float* qacc_track, float* snap,
+                               uint8_t* sflag, unsigned* stats, unsigned flags, void* stream);
 int g1_hip_surrogate_step_deferred(int nenv, float dt, int phase, int period, float tol, float* qpos,
                                    float* qvel, const float* tau, const float* xmean, const float* xinv,
                                    const void* const* W, const float* const* b, const float* ymean,
                                    const float* ystd, const float* qacc_aba, uint8_t* fallback,
-                                   unsigned* stats, float* qacc_out, float* snap, uint8_t* sflag,
-                                   void* stream);
+                                   unsigned* stats, float* qacc_out, void* stream);
 int g1_hip_surrogate_judge_batch(int nenv, float tol, const float* xmean, const float* xinv,
                                  const void* const* W, const float* const* b, const float* ymean,
                                  const float* ystd, const float* snap, uint8_t* sflag, uint8_t* fallback,
-                                 unsigned* stats, float* pred, int ldp, void* stream);
+                                 unsigned* stats, void* stream);
 int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol,
                                     float* qpos, float* qvel, float* anchor, const float* ctrl,
                                     const float* mu, const float* xmean, const float* xinv,
@@ -334,7 +343,7 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
                                     const float* ystd, float* qacc_aba, float* tau, uint8_t* fallback,
                                     unsigned* stats, float* qacc_out, float* contact_forces, float* foot_pos,
                                     float* foot_vel, unsigned judge_flags, float* snap, uint8_t* sflag,
-                                    float* pred, int ldp, void* stream);
+                                    void* stream);
 
 /* RL observation of every env (replaces the torch code of
  * src/g1sim.py:132-170).  layout 0: 99 values per env in the reference order

@@ -179,27 +179,38 @@ def test_training_and_stepper_entry_points_validate_arguments():
 
 def test_deferred_judging_entry_points_validate_arguments():
     """Host-side checks of the deferred-judging entry points (before any
-    launch): records, flags and the batch's prediction buffer are required."""
+    launch): fallback flags, records, statistics and the batch's prediction
+    buffer are required."""
     lib = _lib.load()
     d = C.c_void_p(16)
     ws = _lib.ptr_array([16] * 5)
-    # step: no record buffer, no flags
-    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
-                                              None, d, None) == -1
-    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
-                                              d, None, None) == -1
-    assert "g1_hip_surrogate_step_deferred" in _lib.last_error()
+    # judge + forces with commit: no fallback flags, no records, no statistics
+    assert lib.g1_hip_judge_forces_commit(8, 2e-3, d, d, d, d, None, None, 0, 16, d, d, None, None, None, None, d, d,
+                                          d, 0, None) == -1
+    assert lib.g1_hip_judge_forces_commit(8, 2e-3, d, d, d, d, None, d, 0, 16, d, d, None, None, None, None, None, d,
+                                          d, 0, None) == -1
+    assert lib.g1_hip_judge_forces_commit(8, 2e-3, d, d, d, d, None, d, 0, 16, d, d, None, None, None, None, d, d,
+                                          None, 0, None) == -1
+    assert "judge_forces_commit" in _lib.last_error()
     # phase outside the period
-    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 16, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, d, None,
-                                              d, d, None) == -1
-    # batch: prediction rows narrower than the network's 64 outputs, no fallback flags
-    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, d, d, d, d, d, 32, None) == -1
-    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, d, d, None, d, d, 64, None) == -1
+    assert lib.g1_hip_judge_forces_commit(8, 2e-3, d, d, d, d, None, d, 16, 16, d, d, None, None, None, None, d, d,
+                                          d, 0, None) == -1
+    # step: no fallback flags or statistics (the committed blocks are found from the flags)
+    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, None, d, None,
+                                              None) == -1
+    assert lib.g1_hip_surrogate_step_deferred(8, 2e-3, 0, 16, 1e-3, d, d, d, d, d, ws, ws, d, d, d, d, None, None,
+                                              None) == -1
+    assert "g1_hip_surrogate_step_deferred" in _lib.last_error()
+    # batch: no records, no fallback flags
+    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, None, d, d, d, None) == -1
+    assert lib.g1_hip_surrogate_judge_batch(8, 1e-3, d, d, ws, ws, d, d, d, d, None, d, None) == -1
     assert "g1_hip_surrogate_judge_batch" in _lib.last_error()
     # steps: missing records; zero steps is a no-op
     assert lib.g1_hip_surrogate_steps_deferred(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
-                                               d, d, None, None, None, None, 0, None, d, d, 64, None) == -1
+                                               d, d, None, None, None, None, 0, None, d, None) == -1
     assert lib.g1_hip_surrogate_steps_deferred(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
-                                               d, d, None, None, None, None, 0, d, d, d, 64, None) == 0
-    # zero envs is a no-op for each
-    assert lib.g1_hip_surrogate_judge_batch(0, 1e-3, d, d, ws, ws, d, d, d, d, d, d, d, 64, None) == 0
+                                               d, d, None, None, None, None, 0, d, d, None) == 0
+    # zero envs is a no-op
+    assert lib.g1_hip_surrogate_judge_batch(0, 1e-3, d, d, ws, ws, d, d, d, d, d, d, None) == 0
+    assert lib.g1_hip_judge_forces_commit(0, 2e-3, d, d, d, d, None, d, 0, 16, d, d, None, None, None, None, d, d, d,
+                                          0, None) == 0
