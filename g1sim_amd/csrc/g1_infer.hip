@@ -152,8 +152,9 @@ __device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int
       rg.f[k][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT, RTILES>(W, w, lane, k, c));
 }
 
-// acc = in[rows x NIN] (LDS) . W^T for this wave's tiles, with the first PF
-// k-steps of weights already in `rg`.
+// acc = init + in[rows x NIN] (LDS) . W^T for this wave's tiles (init[c]:
+// column tile c's start, the bias), with the first PF k-steps of weights
+// already in `rg`.
 //
 // TR = false: acc[r][c][q] is out[row (r0 + r) * 16 + (lane / 16) * 4 + q]
 // [col (c0 + c) * 16 + lane % 16]; TR = true (weights as the first MFMA
@@ -163,16 +164,13 @@ __device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int
 template <int NIN, int NOUT, int RTILES, bool TR = false>
 __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf16* __restrict__ W, int w,
                                           int lane, Ring<NIN, NOUT, RTILES>& rg,
-                                          f32x4 (&acc)[Tile<NOUT, RTILES>::RT][Tile<NOUT, RTILES>::CT]) {
+                                          f32x4 (&acc)[Tile<NOUT, RTILES>::RT][Tile<NOUT, RTILES>::CT],
+                                          const f32x4 (&init)[Tile<NOUT, RTILES>::CT]) {
   using T = Tile<NOUT, RTILES>;
   using R = Ring<NIN, NOUT, RTILES>;
   constexpr int KS = R::KS, PF = R::PF, NB = R::NB;
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   const int r0 = T::r0(w);
-#pragma unroll
-  for (int r = 0; r < T::RT; ++r)
-#pragma unroll
-    for (int c = 0; c < T::CT; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const bf16* ap = sin + (r0 * 16 + lr) * LDA + lk;
   // Software pipeline: weight fragments (L2) PF k-steps ahead in the ring,
   // activation fragments (LDS) one step ahead; a scheduling barrier per
@@ -195,20 +193,20 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
 #pragma unroll
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
-      for (int c = 0; c < T::CT; ++c)
-        acc[r][c] = TR ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(rg.f[k % NB][c], fa[k & 1][r], acc[r][c], 0, 0, 0)
-                       : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], rg.f[k % NB][c], acc[r][c], 0, 0, 0);
+      for (int c = 0; c < T::CT; ++c) {
+        const f32x4 cin = k == 0 ? init[c] : acc[r][c];
+        acc[r][c] = TR ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(rg.f[k % NB][c], fa[k & 1][r], cin, 0, 0, 0)
+                       : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k & 1][r], rg.f[k % NB][c], cin, 0, 0, 0);
+      }
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// hidden layer: out = act(in W^T + b) as bf16 into LDS; the next layer's
-// first weight k-steps (Wn) go into `next` before the epilogue.  With one
-// buffer (sout == sin, 128 rows) a barrier separates the last read of the
-// input from the first write of the output, and `b` is the kernel's LDS copy
-// of the bias (read after the k-loop: its registers are needed in the loop,
-// and from global memory the read put a memory latency before every
-// epilogue).
+// hidden layer: out = act(in W^T + b) as bf16 into LDS, the accumulators
+// starting at the bias; the next layer's first weight k-steps (Wn) go into
+// `next` before the epilogue.  With one buffer (sout == sin, 128 rows) a
+// barrier separates the last read of the input from the first write of the
+// output, and `b` is the kernel's LDS copy of the bias.
 template <int NIN, int NOUT, int ACT, int NIN2, int NOUT2, int RTILES>
 __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* __restrict__ W,
                                       const float* __restrict__ b, int w, int lane, Ring<NIN, NOUT, RTILES>& rg,
@@ -216,27 +214,21 @@ __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* _
   using T = Tile<NOUT, RTILES>;
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
   typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-  // bias of the lane's four columns per column tile: loaded before the k-loop
-  // with 64 rows (global memory), after it with 128 (LDS)
-  float bc[T::CT][4];
-  auto load_bias = [&]() {
+  // bias of the lane's four columns per column tile
+  f32x4 bc[T::CT];
 #pragma unroll
-    for (int c = 0; c < T::CT; ++c) {
-      const int col = (c0 + c) * 16 + (lane >> 4) * 4;
-      if constexpr (RTILES == 8) {
-        const float4 v = *reinterpret_cast<const float4*>(b + col);
-        bc[c][0] = v.x; bc[c][1] = v.y; bc[c][2] = v.z; bc[c][3] = v.w;
-      } else {
+  for (int c = 0; c < T::CT; ++c) {
+    const int col = (c0 + c) * 16 + (lane >> 4) * 4;
+    if constexpr (RTILES == 8) {
+      bc[c] = *reinterpret_cast<const f32x4*>(b + col);
+    } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
-      }
+      for (int q = 0; q < 4; ++q) bc[c][q] = b[col + q];
     }
-  };
-  if constexpr (RTILES == 4) load_bias();
+  }
   f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT, RTILES, true>(sin, W, w, lane, rg, acc);
+  mma_layer<NIN, NOUT, RTILES, true>(sin, W, w, lane, rg, acc, bc);
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (RTILES == 8) load_bias();
   wprefetch<NIN2, NOUT2, RTILES>(Wn, w, lane, next);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (RTILES == 8) lds_barrier();        // in place: every wave has read the input
@@ -250,7 +242,7 @@ __device__ __forceinline__ void dense(const bf16* sin, bf16* sout, const bf16* _
       bf16x4 h;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float z = acc[r][c][q] + bc[c][q];
+        const float z = acc[r][c][q];
         if constexpr (ACT == ACT_SILU) h[q] = (bf16)g1act::silu(z);
         else h[q] = (bf16)g1act::elu(z);
       }
@@ -268,11 +260,14 @@ __device__ __forceinline__ void dense_out(const bf16* sin, float* sout, const bf
   using T = Tile<NOUT, RTILES>;
   static_assert(NOUT <= LDO, "output tile wider than the LDS row");
   const int r0 = T::r0(w), c0 = T::c0(w), lr = lane & 15;
-  float bc[T::CT];
+  f32x4 bc[T::CT];
 #pragma unroll
-  for (int c = 0; c < T::CT; ++c) bc[c] = b[(c0 + c) * 16 + lr];
+  for (int c = 0; c < T::CT; ++c) {
+    const float v = b[(c0 + c) * 16 + lr];
+    bc[c] = f32x4{v, v, v, v};
+  }
   f32x4 acc[T::RT][T::CT];
-  mma_layer<NIN, NOUT, RTILES>(sin, W, w, lane, rg, acc);
+  mma_layer<NIN, NOUT, RTILES>(sin, W, w, lane, rg, acc, bc);
   if constexpr (RTILES == 8) lds_barrier();        // in place: every wave has read the input
 #pragma unroll
   for (int c = 0; c < T::CT; ++c) {
@@ -280,7 +275,7 @@ __device__ __forceinline__ void dense_out(const bf16* sin, float* sout, const bf
 #pragma unroll
     for (int r = 0; r < T::RT; ++r)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sout[((r0 + r) * 16 + (lane >> 4) * 4 + q) * LDO + col] = acc[r][c][q] + bc[c];
+      for (int q = 0; q < 4; ++q) sout[((r0 + r) * 16 + (lane >> 4) * 4 + q) * LDO + col] = acc[r][c][q];
   }
 }
 
@@ -360,15 +355,14 @@ constexpr Step step(int g) {
   return Step{4, 0, g - OUT0};
 }
 constexpr int hstep(int g) { return (g - NL1) % NH; }   // step within a hidden layer
-// The bias the epilogue work of step g needs, as layer * 2 + half, or -1.
-constexpr int side_bias(int g) {
-  if (g < 0 || g >= NG) return -1;
-  if (g < NL1) return g >= 4 ? 0 : -1;                            // first layer, half A
-  if (g < OUT0) {
-    const int l = 1 + (g - NL1) / NH, hs = hstep(g);
-    return hs < 8 ? (l - 1) * 2 + 1 : hs >= 24 ? l * 2 : -1;       // previous layer's half B; own half A
-  }
-  return g - OUT0 < 8 ? 3 * 2 + 1 : -1;                           // last hidden layer, half B
+// The accumulators of a half-layer start at its bias (k-step 0 adds the
+// first k-slice to it), as in the layer-by-layer GEMMs of training, so no
+// epilogue adds a bias.  init_bias(g): the hidden half-layer whose k-step 0
+// is step g, as layer * 2 + half, or -1.
+constexpr int init_bias(int g) {
+  if (g < 0 || g >= OUT0) return -1;
+  const Step s = step(g);
+  return s.k == 0 ? s.layer * 2 + s.half : -1;
 }
 // a barrier follows step g: the end of the first layer, the middle and end of
 // each hidden layer, the middle and end of the output layer
@@ -427,15 +421,22 @@ __device__ __forceinline__ void chain(const SurrogateArgs& a, const WSrc& ws, bf
   f32x4 accA[2][8][2], accB[2][8][2], accO[4];
   bf16x8 fa[2][8];
   const float bo = a.b[4][(w & 3) * 16 + lr];
-  // Per-lane element offsets; every LDS address is one of them plus a
-  // compile-time constant.  They are re-declared opaque at every step, so no
-  // address derived from them stays live across steps (the compiler otherwise
-  // kept each (row tile, column) address of the epilogue stores live through
-  // all five layers, which reuse them, and spilled them).
-  int oa = lr * LDA + lk;                              // A fragments
-  int oo = ((w >> 2) * 64 + lr) * LDA + lk;            // A fragments of the output layer
-  int oe = lr * LDA + 2 * w * 16 + lq;                 // epilogue stores
-  int ob = 2 * w * 16 + lq;                            // bias
+  // Per-lane LDS byte offsets; every LDS address is one of them plus a
+  // compile-time constant below 64 KB, which the ds instructions take as
+  // their immediate offset: row tiles 0-3 and 4-7 each have their own base
+  // (row tile 4 starts 67,584 bytes in), so a step computes no addresses.
+  // The offsets are re-declared opaque at every step, so no address derived
+  // from them stays live across steps (the compiler otherwise kept each (row
+  // tile, column) address of the epilogue stores live through all five
+  // layers, which reuse them, and spilled them).
+  constexpr int RT4 = 4 * 16 * LDA * 2;                // bytes from row tile 0 to row tile 4
+  constexpr int RTB = 16 * LDA * 2;                    // bytes per row tile
+  int oa0 = (lr * LDA + lk) * 2, oa1 = oa0 + RT4;      // A fragments
+  int oo = (((w >> 2) * 64 + lr) * LDA + lk) * 2;      // A fragments of the output layer
+  int oe0 = (lr * LDA + 2 * w * 16 + lq) * 2, oe1 = oe0 + RT4;   // epilogue stores
+  int ob = (2 * w * 16 + lq) * 4;                      // bias
+  char* bb = reinterpret_cast<char*>(buf);
+  const char* sb = reinterpret_cast<const char*>(sbias);
   // A fragments (activations) of step G into fa[G & 1]
   auto load_a = [&](auto GC) __attribute__((always_inline)) {
     constexpr int G = decltype(GC)::value;
@@ -443,70 +444,61 @@ __device__ __forceinline__ void chain(const SurrogateArgs& a, const WSrc& ws, bf
     constexpr int col0 = (S.layer == 0 ? IN_COL : 0) + S.k * 32;
     if constexpr (S.layer < 4) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(buf + oa + r * 16 * LDA + col0);
+      for (int r = 0; r < 8; ++r)
+        fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(bb + (r < 4 ? oa0 : oa1) + (r & 3) * RTB + col0 * 2);
     } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(buf + oo + r * 16 * LDA + col0);
+      for (int r = 0; r < 4; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(bb + oo + r * RTB + col0 * 2);
     }
   };
   // The bias of the lane's four columns in each of its two column tiles, for
-  // the epilogue work of a phase: read from LDS one step before the phase (a
-  // read within the step made the epilogue wait for the step's A-fragment
-  // prefetch, which the LDS returns first).  bsv[0]: half-A phases, bsv[1]:
-  // half-B phases (adjacent phases use different halves).
-  float4 bsv[2][2];
+  // k-step 0 of a half-layer: read from LDS one step before (bsv[half]).
+  f32x4 bsv[2][2];
   auto load_bias = [&](int lh) __attribute__((always_inline)) {
     const int l = lh >> 1, hb = lh & 1;
 #pragma unroll
     for (int c = 0; c < 2; ++c)
-      bsv[hb][c] = *reinterpret_cast<const float4*>(sbias + ob + l * SUR_H + hb * IN_COL + c * 16);
+      bsv[hb][c] = *reinterpret_cast<const f32x4*>(sb + ob + (l * SUR_H + hb * IN_COL + c * 16) * 4);
   };
+  const f32x4 bov = f32x4{bo, bo, bo, bo};
   // SiLU epilogue of one 16 x 16 tile (row tile rt, column tile c of half hb:
   // the lane's four adjacent columns), stored as bf16 into the buffer
   auto epi = [&](const f32x4& acc, int rt, int c, int hb) __attribute__((always_inline)) {
     const int cc = hb * IN_COL + c * 16;
-    const float4 bv = bsv[hb][c];
     bf16x4 h;
-    h[0] = (bf16)g1act::silu(acc[0] + bv.x);
-    h[1] = (bf16)g1act::silu(acc[1] + bv.y);
-    h[2] = (bf16)g1act::silu(acc[2] + bv.z);
-    h[3] = (bf16)g1act::silu(acc[3] + bv.w);
-    *reinterpret_cast<bf16x4*>(buf + oe + rt * 16 * LDA + cc) = h;
+    h[0] = (bf16)g1act::silu(acc[0]);
+    h[1] = (bf16)g1act::silu(acc[1]);
+    h[2] = (bf16)g1act::silu(acc[2]);
+    h[3] = (bf16)g1act::silu(acc[3]);
+    *reinterpret_cast<bf16x4*>(bb + (rt < 4 ? oe0 : oe1) + (rt & 3) * RTB + cc * 2) = h;
   };
+  load_bias(init_bias(0));
   load_a(std::integral_constant<int, 0>{});
   static_for<NG>([&](auto GC) {
     constexpr int G = decltype(GC)::value;
     constexpr Step S = step(G);
-    asm volatile("" : "+v"(oa), "+v"(oo), "+v"(oe), "+v"(ob));
-    if constexpr (side_bias(G + 1) >= 0 && side_bias(G + 1) != side_bias(G)) load_bias(side_bias(G + 1));
+    asm volatile("" : "+v"(oa0), "+v"(oa1), "+v"(oo), "+v"(oe0), "+v"(oe1), "+v"(ob));
     if constexpr (G + PF < NG) load_w<G + PF>(ws, wr);
     if constexpr (G + 1 < NG && !barrier_after(G)) load_a(std::integral_constant<int, G + 1>{});
-    // MFMAs of the step
+    // MFMAs of the step (k-step 0 starts from the bias)
     if constexpr (S.layer < 4) {
       auto mma = [&](f32x4 (&acc)[8][2]) __attribute__((always_inline)) {
-        if constexpr (S.k == 0) {
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
 #pragma unroll
         for (int r = 0; r < 8; ++r)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r],
+                                                                S.k == 0 ? bsv[S.half][c] : acc[r][c], 0, 0, 0);
       };
       if constexpr (S.half == 0) mma(accA[S.layer & 1]);
       else mma(accB[S.layer & 1]);
     } else {
-      if constexpr (S.k == 0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) accO[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], accO[r], 0, 0, 0);
+        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], S.k == 0 ? bov : accO[r], 0,
+                                                          0, 0);
     }
+    if constexpr (init_bias(G + 1) >= 0) load_bias(init_bias(G + 1));
     // the epilogue work interleaved with this step
     if constexpr (S.layer == 0 && S.half == 1) {
       // first layer, half A: row tiles 2 (G - 4) and 2 (G - 4) + 1
@@ -541,7 +533,7 @@ __device__ __forceinline__ void chain(const SurrogateArgs& a, const WSrc& ws, bf
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 4 + r) * 16 + lq + q) * LDO + col] = accO[r][q] + bo;
+    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 4 + r) * 16 + lq + q) * LDO + col] = accO[r][q];
 }
 
 template <int UNUSED = 0>
@@ -587,15 +579,12 @@ constexpr bool barrier_after(int g) {
   return g == NL1 - 1 || (g >= NL1 && g < OUT0 && (hstep(g) == 7 || hstep(g) == 31)) || g == OUT0 + 7 ||
          g == OUT0 + NO - 1;
 }
-// the bias (layer * 2 + half) the epilogue work of step g needs, or -1
-constexpr int side_bias(int g) {
-  if (g < 0 || g >= NG) return -1;
-  if (g < NL1) return g >= 4 ? 0 : -1;
-  if (g < OUT0) {
-    const int l = 1 + (g - NL1) / NH, hs = hstep(g);
-    return hs < 8 ? (l - 1) * 2 + 1 : hs >= 16 ? l * 2 : -1;
-  }
-  return g - OUT0 < 8 ? 3 * 2 + 1 : -1;
+// the hidden half-layer (layer * 2 + half) whose k-step 0 is step g, or -1:
+// its accumulators start at its bias, as in split::chain
+constexpr int init_bias(int g) {
+  if (g < 0 || g >= OUT0) return -1;
+  const Step s = step(g);
+  return s.k == 0 ? s.layer * 2 + s.half : -1;
 }
 constexpr int BLA = 8;        // biases are read from global memory BLA steps ahead
 
@@ -646,62 +635,56 @@ __device__ __forceinline__ void chain(const SurrogateArgs& a, const split::WSrc&
       for (int r = 0; r < 2; ++r) fa[G & 1][r] = *reinterpret_cast<const bf16x8*>(src + oo + r * 16 * LDA);
     }
   };
-  // bias of the lane's columns for a phase (bsv[half][tile]), read BLA steps ahead
-  float4 bsv[2][2];
+  // bias of the lane's columns for k-step 0 of a half-layer (bsv[half][tile]),
+  // read BLA steps ahead
+  f32x4 bsv[2][2];
   auto load_bias = [&](int lh) __attribute__((always_inline)) {
     const int l = lh >> 1, hb = lh & 1;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) bsv[hb][c] = *reinterpret_cast<const float4*>(a.b[l] + ob + hb * 256 + c * 16);
+    for (int c = 0; c < 2; ++c) bsv[hb][c] = *reinterpret_cast<const f32x4*>(a.b[l] + ob + hb * 256 + c * 16);
   };
+  const f32x4 bov = f32x4{bo, bo, bo, bo};
   // SiLU epilogue of one 16 x 16 tile of layer l (row tile rt, column tile c
   // of half hb), stored as bf16 into layer l's output buffer
   auto epi = [&](const f32x4& acc, int l, int rt, int c, int hb) __attribute__((always_inline)) {
-    const float4 bv = bsv[hb][c];
     bf16x4 h;
-    h[0] = (bf16)g1act::silu(acc[0] + bv.x);
-    h[1] = (bf16)g1act::silu(acc[1] + bv.y);
-    h[2] = (bf16)g1act::silu(acc[2] + bv.z);
-    h[3] = (bf16)g1act::silu(acc[3] + bv.w);
+    h[0] = (bf16)g1act::silu(acc[0]);
+    h[1] = (bf16)g1act::silu(acc[1]);
+    h[2] = (bf16)g1act::silu(acc[2]);
+    h[3] = (bf16)g1act::silu(acc[3]);
     *reinterpret_cast<bf16x4*>(bufl(l + 1) + oe + rt * 16 * LDA + hb * 256 + c * 16) = h;
   };
   static_for<BLA>([&](auto GC) {
     constexpr int G = decltype(GC)::value;
-    if constexpr (side_bias(G) >= 0 && side_bias(G) != side_bias(G - 1)) load_bias(side_bias(G));
+    if constexpr (init_bias(G) >= 0) load_bias(init_bias(G));
   });
   load_a(std::integral_constant<int, 0>{});
   static_for<NG>([&](auto GC) {
     constexpr int G = decltype(GC)::value;
     constexpr Step S = step(G);
     asm volatile("" : "+v"(oa), "+v"(oo), "+v"(oe), "+v"(ob));
-    if constexpr (side_bias(G + BLA) >= 0 && side_bias(G + BLA) != side_bias(G + BLA - 1))
-      load_bias(side_bias(G + BLA));
     if constexpr (G + PF < NG) split2::load_w<G + PF>(ws, wr);
     if constexpr (G + 1 < NG && !barrier_after(G)) load_a(std::integral_constant<int, G + 1>{});
+    // MFMAs of the step (k-step 0 starts from the bias)
     if constexpr (S.layer < 4) {
       auto mma = [&](f32x4 (&acc)[4][2]) __attribute__((always_inline)) {
-        if constexpr (S.k == 0) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[G % NB][c], fa[G & 1][r],
+                                                                S.k == 0 ? bsv[S.half][c] : acc[r][c], 0, 0, 0);
       };
       if constexpr (S.half == 0) mma(accA[S.layer & 1]);
       else mma(accB[S.layer & 1]);
     } else {
-      if constexpr (S.k == 0) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) accO[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
 #pragma unroll
       for (int r = 0; r < 2; ++r)
-        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], accO[r], 0, 0, 0);
+        accO[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[G & 1][r], wr[G % NB][0], S.k == 0 ? bov : accO[r], 0,
+                                                          0, 0);
     }
+    // (after the MFMAs: a half's next bias may replace the one this step used)
+    if constexpr (init_bias(G + BLA) >= 0) load_bias(init_bias(G + BLA));
     // the epilogue work interleaved with this step
     if constexpr (S.layer == 0 && S.half == 1) {
       // first layer, half A: two tiles per step
@@ -734,7 +717,7 @@ __device__ __forceinline__ void chain(const SurrogateArgs& a, const split::WSrc&
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 2 + r) * 16 + lq + q) * LDO + col] = accO[r][q] + bo;
+    for (int q = 0; q < 4; ++q) out[(((w >> 2) * 2 + r) * 16 + lq + q) * LDO + col] = accO[r][q];
 }
 }  // namespace split2
 
@@ -753,9 +736,9 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
   __shared__ float ysc[MODE == 1 ? 2 * NV : 1];          // ystd, ymean
   // with 128 rows the layers run as the split-half chain (split::chain), the
   // network input in columns 256-383 of the buffer, and the hidden layers'
-  // biases are staged in LDS ([4][512])
-  constexpr bool SPLIT = MODE != 1 && R == 128;
-  constexpr bool SPLIT2 = R == 64;
+  // biases are staged in LDS ([4][512]); with 64 rows as the two-buffer chain
+  // (split2::chain)
+  constexpr bool SPLIT = R == 128;
   constexpr int FCOL = SPLIT ? split::IN_COL : 0;
   constexpr bool SBIAS = R == 128;
   __shared__ __attribute__((aligned(16))) float sbias[SBIAS ? 4 * SUR_H : 4];
@@ -863,17 +846,10 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     if (tid < 2 * NV) ys = tid < NV ? a.ystd[tid] : a.ymean[tid - NV];
   }
   __builtin_amdgcn_sched_barrier(0);
-  Ring<SUR_PIN, SUR_H, RT> ring0;
   bf16x8 swr[split::NB][2];
-  split::WSrc sws;
-  if constexpr (SPLIT) {
-    sws = split::wsrc(a, w, lane);
-    split::prefetch(sws, swr);
-  } else if constexpr (SPLIT2) {
-    sws = split::wsrc(a, w, lane);
-    split2::prefetch(sws, swr);
-  }
-  else wprefetch<SUR_PIN, SUR_H, RT>(a.W[0], w, lane, ring0);
+  const split::WSrc sws = split::wsrc(a, w, lane);
+  if constexpr (SPLIT) split::prefetch(sws, swr);
+  else split2::prefetch(sws, swr);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (MODE == 1) {
     // stash the commit's inputs
@@ -920,28 +896,10 @@ __global__ __launch_bounds__(Rows<R>::NT) void surrogate_kernel(SurrogateArgs a)
     for (int l = 0; l < 4; ++l) sbias[l * SUR_H + tid] = bstage[l];
   }
   lds_barrier();
-  auto hbias = [&](int l) -> const float* { return SBIAS ? sbias + l * SUR_H : a.b[l]; };
-  // layer l reads b[l % 2] and writes b[(l + 1) % 2]; one buffer with 128 rows
-  bf16* b0 = buf[0];
-  bf16* b1 = buf[Rows<R>::NBUF - 1];
-  float* out = reinterpret_cast<float*>(b1);
-  if constexpr (SPLIT) {
-    split::chain(a, sws, buf[0], sbias, out, w, lane, swr);
-  } else if constexpr (SPLIT2) {
-    split2::chain(a, sws, buf[0], buf[1], out, w, lane, swr);
-  } else {
-    Ring<SUR_H, SUR_H, RT> ring1, ring2, ring3;
-    Ring<SUR_H, SUR_POUT, RT> ring4;
-    dense<SUR_PIN, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[0], hbias(0), w, lane, ring0, a.W[1], ring1);
-    lds_barrier();
-    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b1, b0, a.W[1], hbias(1), w, lane, ring1, a.W[2], ring2);
-    lds_barrier();
-    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_H, RT>(b0, b1, a.W[2], hbias(2), w, lane, ring2, a.W[3], ring3);
-    lds_barrier();
-    dense<SUR_H, SUR_H, ACT_SILU, SUR_H, SUR_POUT, RT>(b1, b0, a.W[3], hbias(3), w, lane, ring3, a.W[4], ring4);
-    lds_barrier();
-    dense_out<SUR_H, SUR_POUT, RT>(b0, out, a.W[4], a.b[4], w, lane, ring4);
-  }
+  // the fp32 output tile goes over the last buffer
+  float* out = reinterpret_cast<float*>(buf[Rows<R>::NBUF - 1]);
+  if constexpr (SPLIT) split::chain(a, sws, buf[0], sbias, out, w, lane, swr);
+  else split2::chain(a, sws, buf[0], buf[1], out, w, lane, swr);
   lds_barrier();
   if constexpr (MODE == 0) {
     // [R][64] fp32 tile to pred rows; 512 threads

@@ -48,6 +48,8 @@ enum Epi : int {
   EPI_BIAS_SILU = 2,   // Z (fp32) = acc + b; H (bf16) = silu(Z)     (training forward)
   EPI_BIAS_SILU_H = 3, // H (bf16) = silu(acc + b)                   (inference forward)
   EPI_BIAS_F32 = 4,    // C (fp32) = acc + b                          (output layer)
+  // (the three bias epilogues start the accumulators at b, so acc + b is the
+  // accumulator itself: the fused inference kernels do the same, bit for bit)
   EPI_SILU_BWD = 5,    // dZ = acc * silu'(Z): fp32 to C, bf16 to H   (data grad through SiLU)
   EPI_F32_SPLIT = 6,   // C[z] (fp32) = acc of r-slice z; C[z] at C + z * I * ldc (summed by g1_mlp_sum_splits)
 };
@@ -125,11 +127,16 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   const int rb = blockIdx.z * g.r_per_split;
   const int nsteps = g.r_per_split / BR;
 
+  // accumulators start at 0, or at the bias of the lane's column (lane % 16 of
+  // each tile; the bias epilogues run unsplit)
+  constexpr bool BIAS_INIT = EPI == EPI_BIAS_SILU || EPI == EPI_BIAS_SILU_H || EPI == EPI_BIAS_F32;
   f32x4 acc[F][F];
 #pragma unroll
-  for (int a = 0; a < F; ++a)
+  for (int b = 0; b < F; ++b) {
+    const float v = BIAS_INIT ? g.bias[j0 + wj * (T / 2) + b * 16 + (lane & 15)] : 0.f;
 #pragma unroll
-    for (int b = 0; b < F; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < F; ++a) acc[a][b] = f32x4{v, v, v, v};
+  }
 
   bf16x8 va[TL::CHUNKS], vb[TL::CHUNKS];
   load<T, A_T>(va, g.A, g.lda, i0, rb, tid);
@@ -190,14 +197,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         } else if constexpr (EPI == EPI_F32_SPLIT) {
           g.C[(size_t)blockIdx.z * g.I * g.ldc + o] = v;
         } else if constexpr (EPI == EPI_BIAS_SILU) {
-          const float z = v + g.bias[j];
-          g.C[o] = z;
-          g.H[o] = (bf16)g1act::silu(z);
+          g.C[o] = v;
+          g.H[o] = (bf16)g1act::silu(v);
         } else if constexpr (EPI == EPI_BIAS_SILU_H) {
-          const float z = v + g.bias[j];
-          g.H[o] = (bf16)g1act::silu(z);
+          g.H[o] = (bf16)g1act::silu(v);
         } else if constexpr (EPI == EPI_BIAS_F32) {
-          g.C[o] = v + g.bias[j];
+          g.C[o] = v;
         } else if constexpr (EPI == EPI_SILU_BWD) {
           const float d = v * g1act::silu_grad(g.Zin[o]);
           if (g.C) g.C[o] = d;
