@@ -108,7 +108,9 @@ def load():
     lib.g1_hip_surrogate_judge_batch.argtypes = [i, f, vp, vp, pa, pa, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_hip_surrogate_steps_deferred.restype = i
     lib.g1_hip_surrogate_steps_deferred.argtypes = [i, i, i, f, i, f, vp, vp, vp, vp, vp, vp, vp, pa, pa, vp, vp, vp,
-                                                    vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp]
+                                                    vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp, vp]
+    lib.g1_hip_fallback_run.restype = i
+    lib.g1_hip_fallback_run.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_get_obs.restype = i
     lib.g1_hip_get_obs.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
     lib.g1_policy_act.restype = i
@@ -143,7 +145,8 @@ def exported_symbols():
             "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
             "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_mse_grad_part",
             "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows", "g1_hip_surrogate_steps", "g1_hip_surrogate_step_deferred",
-            "g1_hip_surrogate_judge_batch", "g1_hip_surrogate_steps_deferred", "g1_hip_judge_forces_commit"]
+            "g1_hip_surrogate_judge_batch", "g1_hip_surrogate_steps_deferred", "g1_hip_judge_forces_commit",
+            "g1_hip_fallback_run"]
 
 
 def ptr_array(tensors_or_ptrs):

@@ -33,6 +33,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -42,6 +43,7 @@
 #include "g1_commit.h"
 #include "g1_act.h"
 #include "g1_err.h"
+#include "g1_launch.h"
 #include "../../include/g1sim_amd.h"
 
 namespace g1inf {
@@ -1715,31 +1717,51 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
 // fallback) and g1_hip_surrogate_step_deferred (the other blocks), and the
 // batch judgement (g1_hip_surrogate_judge_batch) after each step whose phase
 // is period - 1.
+//
+// With blk_run (one byte per block of 64 envs) the steps go by period
+// segments: an env leaves fallback only at a batch judgement, so a block
+// whose envs are all in fallback at a segment's start stays so until the
+// period ends, and every one of its steps until then is ABA's.  One
+// g1_hip_fallback_run launch steps all such blocks through the segment
+// (recording each at its judged step); the per-step launches then run the
+// other blocks only.  Same results as without blk_run: the state, records,
+// flags and statistics of the per-step path.
 int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
                                     float* qvel, float* anchor, const float* ctrl, const float* mu,
                                     const float* xmean, const float* xinv, const void* const* W,
                                     const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
                                     float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
                                     float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
-                                    float* snap, uint8_t* sflag, void* stream) {
+                                    float* snap, uint8_t* sflag, uint8_t* blk_run, void* stream) {
   g1err::clear();
   if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
       !qacc_aba || !tau || !stats || !fallback || !snap || !sflag)
     return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
-  for (int s = 0; s < nsteps; ++s) {
-    const int phase = (int)(((long long)t0 + s) % period);
-    int rc = g1_hip_judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_aba,
-                                        tau, contact_forces, foot_pos, foot_vel, qacc_out, snap, sflag, stats,
-                                        judge_flags, stream);
-    if (rc) return rc;
-    rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean,
-                                        ystd, qacc_aba, fallback, stats, qacc_out, stream);
-    if (rc) return rc;
-    if (phase == period - 1) {
-      rc = g1_hip_surrogate_judge_batch(nenv, tol, xmean, xinv, W, b, ymean, ystd, snap, sflag, fallback, stats,
-                                        stream);
+  for (int s = 0; s < nsteps;) {
+    const int phase0 = (int)(((long long)t0 + s) % period);
+    const int seg = blk_run ? std::min(nsteps - s, period - phase0) : 1;
+    if (blk_run) {
+      int rc = g1_hip_fallback_run(nenv, seg, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase0, period, blk_run,
+                                   qacc_out, contact_forces, foot_pos, foot_vel, snap, sflag, stats, judge_flags,
+                                   stream);
       if (rc) return rc;
     }
+    for (int j = 0; j < seg; ++j) {
+      const int phase = phase0 + j;
+      int rc = g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period,
+                                             qacc_aba, tau, contact_forces, foot_pos, foot_vel, qacc_out, snap,
+                                             sflag, stats, blk_run, judge_flags, stream);
+      if (rc) return rc;
+      rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean,
+                                          ystd, qacc_aba, fallback, stats, qacc_out, stream);
+      if (rc) return rc;
+      if (phase == period - 1) {
+        rc = g1_hip_surrogate_judge_batch(nenv, tol, xmean, xinv, W, b, ymean, ystd, snap, sflag, fallback, stats,
+                                          stream);
+        if (rc) return rc;
+      }
+    }
+    s += seg;
   }
   return 0;
 }

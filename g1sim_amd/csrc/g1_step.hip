@@ -43,6 +43,7 @@
 #include "g1_fastmath.h"
 #include "g1_commit.h"
 #include "g1_err.h"
+#include "g1_launch.h"
 #include "../../include/g1sim_amd.h"
 
 namespace g1k {
@@ -669,10 +670,19 @@ struct StepArgs {
   uint8_t* sflag;
   float* qacc_track;
   unsigned* stats;
+  // Fallback run (MODE_FALLBACK_RUN): a block whose live envs are all in
+  // fallback is stepped nsub times as by the plain step (every substep
+  // integrates ABA's qacc, as a fallback env's step does) and its record is
+  // written at the substep whose phase, (run_phase0 + substep) % sel_period,
+  // is the block's judge phase; blk_run[b] = 1 marks such a block, 0 any
+  // other (which the kernel leaves alone).  In MODE_JUDGE_FORCES a non-null
+  // blk_run makes the marked blocks return at once: the run has stepped them.
+  uint8_t* blk_run;
+  int run_phase0;
 };
 // Kernel variants (separate template instances, so the plain step's code is
 // not touched by the judge modes).
-constexpr int MODE_STEP = 0, MODE_JUDGE_BLOCKS = 1, MODE_JUDGE_FORCES = 2;
+constexpr int MODE_STEP = 0, MODE_JUDGE_BLOCKS = 1, MODE_JUDGE_FORCES = 2, MODE_FALLBACK_RUN = 3;
 static_assert(ENVS_PER_WG == JUDGE_BLOCK, "judge blocks are the step's workgroups");
 
 // Position of the k-th set bit (k from 0, k < popcount(m)) of m.
@@ -799,15 +809,16 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
                        float (&fcx)[2][2][6][ENVS_PER_WG], float* wsh, bool commit = false, bool record = false) {
   constexpr int CPW = 4 / SPLIT;                 // chains per wave
   constexpr bool JF = MODE == MODE_JUDGE_FORCES;
+  constexpr bool FR = MODE == MODE_FALLBACK_RUN;
   const int lane = threadIdx.x & 63;
   const int wave = ROLE >= 0 ? ROLE : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool judge = MODE == MODE_JUDGE_BLOCKS || (MODE == MODE_STEP && args.env_list != nullptr);
   bool valid;
   int e;
   size_t N;
-  if constexpr (JF) {
+  if constexpr (JF || FR) {
     // the whole block, in env order (the kernel has checked it holds a judged
-    // or fallback env)
+    // or fallback env, or that all its live envs are in fallback)
     const int e_raw = blockIdx.x * ENVS_PER_WG + lane;
     valid = e_raw < args.nenv;
     e = valid ? e_raw : args.nenv - 1;
@@ -836,6 +847,7 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     N = judge ? (size_t)args.stride : (size_t)nenv;
   }
   const Cfg cf = args.cfg;
+  const int e_ = e;
   const bool legs = wave * CPW < 2;            // this wave owns the leg chains
 
   // ---- load (all waves: base + waist; own chains)
@@ -894,33 +906,42 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   });
   // deferred-judgement record, part 1: the state before the step, in the
   // network's input order (qpos rows 0-35, qvel rows 36-70)
-  if constexpr (JF) {
-    if (record && valid) {
-      float* sp = args.snap;
-      if (wave == 0) {
+  // The record's addresses are computed where it is written (the env index
+  // goes through an empty asm): hoisted out of the fallback run's substep
+  // loop, its ~140 row addresses stayed live and the 2-wave instance spilled.
+  auto opaque = [](int v) __attribute__((always_inline)) {
+    asm volatile("" : "+v"(v));
+    return v;
+  };
+  auto record_state = [&]() __attribute__((always_inline)) {
+    float* sp = args.snap;
+    const int e = opaque(e_);
+    if (wave == 0) {
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          sp[m * N + e] = (float)bp[m];
-          sp[(36 + m) * N + e] = vw[m];
-          sp[(36 + 3 + m) * N + e] = om[m];
-          sp[(7 + 12 + m) * N + e] = wq[m];
-          sp[(36 + 6 + 12 + m) * N + e] = wqd[m];
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) sp[(3 + m) * N + e] = (float)bq[m];
+      for (int m = 0; m < 3; ++m) {
+        sp[m * N + e] = (float)bp[m];
+        sp[(36 + m) * N + e] = vw[m];
+        sp[(36 + 3 + m) * N + e] = om[m];
+        sp[(7 + 12 + m) * N + e] = wq[m];
+        sp[(36 + 6 + 12 + m) * N + e] = wqd[m];
       }
-      static_for<CPW>([&](auto C) {
-        constexpr int c = decltype(C)::value;
-        const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
 #pragma unroll
-        for (int m = 0; m < MAXJ; ++m) {
-          if (m < nj) {
-            sp[(7 + h0 + m) * N + e] = cq[c][m];
-            sp[(36 + 6 + h0 + m) * N + e] = cqd[c][m];
-          }
-        }
-      });
+      for (int m = 0; m < 4; ++m) sp[(3 + m) * N + e] = (float)bq[m];
     }
+    static_for<CPW>([&](auto C) {
+      constexpr int c = decltype(C)::value;
+      const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
+#pragma unroll
+      for (int m = 0; m < MAXJ; ++m) {
+        if (m < nj) {
+          sp[(7 + h0 + m) * N + e] = cq[c][m];
+          sp[(36 + 6 + h0 + m) * N + e] = cqd[c][m];
+        }
+      }
+    });
+  };
+  if constexpr (JF) {
+    if (record && valid) record_state();
   }
   // target schedule: the next block's targets are loaded one substep before
   // they apply, so their memory latency hides behind that substep
@@ -931,8 +952,10 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
 
   float qacc_base[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, qdd_w[3] = {0.f, 0.f, 0.f}, qdd_c[CPW][MAXJ];
   float fn[CPW][4], footp[CPW][3], footv[CPW][3];
-  // total generalized force of the last substep (servo + limits + J^T f)
-  const bool want_tau = args.tau != nullptr;
+  // total generalized force of the last substep (servo + limits + J^T f);
+  // the fallback run computes it on its record substep only
+  const bool tau_out = args.tau != nullptr;
+  bool want_tau = tau_out;
   const bool f64_solve = args.f64_solve;
   float tau_c[CPW][MAXJ], tau_w[3] = {0.f, 0.f, 0.f}, tau_b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -945,8 +968,17 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     for (int m = 0; m < 3; ++m) { footp[c][m] = 0.f; footv[c][m] = 0.f; }
   }
   const float dt = cf.dt;
+  int rec_phase = -1;
+  if constexpr (FR) rec_phase = (int)(blockIdx.x % (unsigned)args.sel_period);
   for (int sub = 0; sub < args.nsub; ++sub) {
     const int buf = sub & 1;
+    // fallback run: this substep is the block's judged step (uniform)
+    bool rec_now = false;
+    if constexpr (FR) {
+      rec_now = args.snap && (args.run_phase0 + sub) % args.sel_period == rec_phase;
+      want_tau = rec_now;
+      if (rec_now && valid) record_state();
+    }
     if (sched) {                                 // uniform
       if (cpos == args.ctrl_period) {            // block boundary: the prefetched targets apply
 #pragma unroll
@@ -1143,6 +1175,38 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
       qacc_base[3] = a0[0]; qacc_base[4] = a0[1]; qacc_base[5] = a0[2];
     }
 
+    if constexpr (FR) {
+      // fallback run, record part 2: this substep's tau_total and qacc
+      if (rec_now && valid) {
+        float* rp = args.snap;
+        const int e = opaque(e_);
+        if (wave == 0) {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) {
+            rp[(71 + m) * N + e] = tau_b[m];
+            rp[(106 + m) * N + e] = qacc_base[m];
+          }
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            rp[(71 + 6 + 12 + m) * N + e] = tau_w[m];
+            rp[(106 + 6 + 12 + m) * N + e] = qdd_w[m];
+          }
+          args.sflag[e] = 1;
+        }
+        static_for<CPW>([&](auto C) {
+          constexpr int c = decltype(C)::value;
+          const int ch = wave * CPW + c, h0 = chain_h0(ch), nj = chain_n(ch);
+#pragma unroll
+          for (int m = 0; m < MAXJ; ++m) {
+            if (m < nj) {
+              rp[(71 + 6 + h0 + m) * N + e] = tau_c[c][m];
+              rp[(106 + 6 + h0 + m) * N + e] = qdd_c[c][m];
+            }
+          }
+        });
+      }
+    }
+
     // ---- semi-implicit Euler (src/g1_core.h:212-222; quaternion exponential
     // src/spatial.h:73-79), the same bits as the surrogate's commit
     euler_base(dt, qacc_base, vw, om, bp, bq);
@@ -1174,13 +1238,13 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   }
   // judge + forces with commit: the committed block's state, its tracked
   // qacc, and (judged) the rest of its deferred-judgement record
-  float* trk = JF && commit ? args.qacc_track : nullptr;
+  float* trk = (JF && commit) || FR ? args.qacc_track : nullptr;
   float* rec = JF && record ? args.snap : nullptr;
-  if constexpr (JF) {
-    if (commit && wave == 0) {
+  if constexpr (JF || FR) {
+    if ((FR || commit) && wave == 0) {
       // the active lanes are the valid ones (the others have returned)
       const unsigned long long vm = __ballot(true);
-      if (lane == 0 && args.stats) atomicAdd(&args.stats[3], (unsigned)__popcll(vm));
+      if (lane == 0 && args.stats) atomicAdd(&args.stats[3], (unsigned)__popcll(vm) * (unsigned)args.nsub);
     }
   }
   if (wave == 0) {
@@ -1205,7 +1269,7 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
     }
     // (a committed block's qacc and tau are read by nobody: the network
     // kernel leaves the block alone, the record has its own copy)
-    if (want_tau && !(JF && commit)) {
+    if (tau_out && !(JF && commit)) {
 #pragma unroll
       for (int m = 0; m < 6; ++m) args.tau[m * N + e] = tau_b[m];
 #pragma unroll
@@ -1248,7 +1312,7 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
           args.qvel[(6 + h0 + m) * N + e] = cqd[c][m];
         }
         if (args.qacc && !(JF && commit)) args.qacc[(6 + h0 + m) * N + e] = qdd_c[c][m];
-        if (want_tau && !(JF && commit)) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
+        if (tau_out && !(JF && commit)) args.tau[(6 + h0 + m) * N + e] = tau_c[c][m];
         if (trk) trk[(6 + h0 + m) * N + e] = qdd_c[c][m];
         if (rec) {
           rec[(71 + 6 + h0 + m) * N + e] = tau_c[c][m];
@@ -1427,6 +1491,7 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
   __shared__ float wsh[W_IN_LDS<SPLIT> ? 4 * MAXJ * 6 * ENVS_PER_WG : 1];
   bool commit = false, record = false;
   if constexpr (MODE == MODE_JUDGE_FORCES) {
+    if (args.blk_run && args.blk_run[blockIdx.x]) return;     // stepped by this period's fallback run
     const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
     const bool live = e_raw < args.nenv;
     const bool blk = (int)(blockIdx.x % (unsigned)args.sel_period) == args.sel_phase;
@@ -1439,6 +1504,13 @@ __global__ __launch_bounds__(64 * SPLIT) void step_kernel(StepArgs args) {
     // every live env of the block in fallback: commit here (uniform)
     commit = args.commit_fallback && args.sel_fallback && __ballot(live && !fb) == 0ull;
     record = commit && blk && args.snap;
+  } else if constexpr (MODE == MODE_FALLBACK_RUN) {
+    const int e_raw = blockIdx.x * ENVS_PER_WG + (threadIdx.x & 63);
+    const bool live = e_raw < args.nenv;
+    const bool fb = live && args.sel_fallback[e_raw];
+    const bool run = __ballot(live && !fb) == 0ull;     // uniform: every live env in fallback
+    if (threadIdx.x == 0) args.blk_run[blockIdx.x] = run ? 1 : 0;
+    if (!run) return;
   }
   if constexpr (SPLIT == 4) {
     step_body<F64_BASE, F64_CONTACT, SPLIT, -1, MODE>(args, xch, fcx, wsh, commit, record);
@@ -1603,6 +1675,9 @@ KernelFn pick(bool f64b, bool f64c, int mode) {
   if (mode == MODE_JUDGE_FORCES)
     return f64c ? step_kernel<false, true, SPLIT, MODE_JUDGE_FORCES>
                 : step_kernel<false, false, SPLIT, MODE_JUDGE_FORCES>;
+  if (mode == MODE_FALLBACK_RUN)
+    return f64c ? step_kernel<false, true, SPLIT, MODE_FALLBACK_RUN>
+                : step_kernel<false, false, SPLIT, MODE_FALLBACK_RUN>;
   if (mode == MODE_JUDGE_BLOCKS)
     return f64c ? step_kernel<false, true, SPLIT, MODE_JUDGE_BLOCKS>
                 : step_kernel<false, false, SPLIT, MODE_JUDGE_BLOCKS>;
@@ -1638,6 +1713,45 @@ int validate(int nenv, int nsub, float dt, const void* qpos, const void* qvel, c
   return 0;
 }
 }  // namespace
+
+// g1_hip_judge_forces, and a block whose live envs are all in fallback is
+// also committed (integrated and stored as by the plain step) with its
+// deferred-judgement record written if it is judged this step
+// (StepArgs::commit_fallback).  blk_run (nullable): the blocks a fallback run
+// has already stepped, left alone.
+namespace g1k_host {
+int judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                        const float* mu, const uint8_t* fallback, int phase, int period, float* qacc_out,
+                        float* tau_out, float* contact_forces, float* foot_pos, float* foot_vel, float* qacc_track,
+                        float* snap, uint8_t* sflag, unsigned* stats, const uint8_t* blk_run, unsigned flags,
+                        void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (period <= 0 || phase < 0 || phase >= period || !qacc_out || !tau_out || !fallback || !snap || !sflag ||
+      !stats) {
+    set_err("judge_forces_commit: bad phase/period, or no qacc / tau / fallback / records / stats");
+    return -1;
+  }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("judge_forces_commit: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv; a.nsub = 1;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
+  a.ctrl = ctrl;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
+  a.stride = nenv;
+  a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
+  a.commit_fallback = 1; a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
+  a.blk_run = const_cast<uint8_t*>(blk_run);
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
+}
+}  // namespace g1k_host
 
 extern "C" {
 
@@ -1836,39 +1950,49 @@ int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel
   return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
 }
 
-// g1_hip_judge_forces, and a block whose live envs are all in fallback is
-// also committed (integrated and stored as by the plain step) with its
-// deferred-judgement record written if it is judged this step
-// (StepArgs::commit_fallback).
+// g1k_host::judge_forces_commit without a fallback run.
 int g1_hip_judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
                                const float* mu, const uint8_t* fallback, int phase, int period, float* qacc_out,
                                float* tau_out, float* contact_forces, float* foot_pos, float* foot_vel,
                                float* qacc_track, float* snap, uint8_t* sflag, unsigned* stats, unsigned flags,
                                void* stream) {
+  return g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period, qacc_out,
+                                       tau_out, contact_forces, foot_pos, foot_vel, qacc_track, snap, sflag, stats,
+                                       nullptr, flags, stream);
+}
+
+// The blocks whose live envs are all in fallback, nsub steps as the plain
+// step (MODE_FALLBACK_RUN), each writing its deferred-judgement record at
+// its judged step; blk_run[b] = 1 for those blocks, 0 for the others.
+int g1_hip_fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                        const float* mu, const uint8_t* fallback, int phase0, int period, uint8_t* blk_run,
+                        float* qacc_track, float* contact_forces, float* foot_pos, float* foot_vel, float* snap,
+                        uint8_t* sflag, unsigned* stats, unsigned flags, void* stream) {
   g_err[0] = 0;
-  int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
+  int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
   if (rc) return rc;
-  if (period <= 0 || phase < 0 || phase >= period || !qacc_out || !tau_out || !fallback || !snap || !sflag ||
-      !stats) {
-    set_err("judge_forces_commit: bad phase/period, or no qacc / tau / fallback / records / stats");
+  if (period <= 0 || phase0 < 0 || nsub > period - phase0 || !fallback || !blk_run || !snap || !sflag || !stats) {
+    set_err("fallback_run: bad phase/period (the run may not cross a period end), or no fallback / blk_run / "
+            "records / stats");
     return -1;
   }
   if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
-    set_err("judge_forces_commit: unsupported flags 0x%x", flags);
+    set_err("fallback_run: unsupported flags 0x%x", flags);
     return -1;
   }
-  if (nenv == 0) return 0;
+  if (nenv == 0 || nsub == 0) return 0;
   g1k::StepArgs a{};
-  a.nenv = nenv; a.nsub = 1;
+  a.nenv = nenv; a.nsub = nsub;
   a.cfg = g1k::default_cfg(dt);
   a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
   a.ctrl = ctrl;
   a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
-  a.qacc = qacc_out; a.tau = tau_out; a.mu = mu;
+  a.mu = mu;
   a.stride = nenv;
-  a.sel_fallback = fallback; a.sel_phase = phase; a.sel_period = period;
-  a.commit_fallback = 1; a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
-  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
+  a.sel_fallback = fallback; a.sel_phase = phase0; a.sel_period = period;
+  a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
+  a.blk_run = blk_run; a.run_phase0 = phase0;
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_FALLBACK_RUN);
 }
 
 int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,

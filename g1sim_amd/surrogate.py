@@ -472,13 +472,22 @@ class SurrogateStepper:
     the tolerance leave fallback then instead of the step after their
     judgement.  Blocks with an env off fallback are judged at once, so the
     guarantee above is unchanged, and a network that is never accepted costs
-    one network pass per period instead of one tile latency per step."""
+    one network pass per period instead of one tile latency per step.
+
+    Fallback runs (fallback_runs=True, deferred judging, steps() only): an
+    env leaves fallback only at a batch judgement, so a block whose envs are
+    all in fallback at a period segment's start is ABA's until the period
+    ends; steps() advances all such blocks through the segment with one
+    g1_hip_fallback_run launch (each recorded at its judged step) and runs
+    the per-step launches for the other blocks."""
 
     def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True,
-                 track_qacc=False, defer=True):
+                 track_qacc=False, defer=True, fallback_runs=True):
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
         self.fused = fused and mlp.fusable
         self.defer = defer and self.fused
+        # steps(): all-fallback blocks stepped by one launch per period segment
+        self.fallback_runs = fallback_runs and self.defer
         self.lib = _lib.load()
         self.dev = norm_device(device)
         self.rows = (nenv + 127) // 128 * 128
@@ -493,6 +502,8 @@ class SurrogateStepper:
             self.snap = torch.zeros((SNAP_ROWS, nenv), **f32)
             self.sflag = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
             self.dropped = torch.zeros((), dtype=torch.int64, device=self.dev)     # records dropped by resets
+            # blocks of 64 envs stepped by a fallback run in steps() (g1_hip_fallback_run)
+            self.blk_run = torch.zeros((nenv + 63) // 64, dtype=torch.uint8, device=self.dev)
         self.X16 = torch.zeros((self.rows, PIN), dtype=torch.bfloat16, device=self.dev)
         self.pred = torch.zeros((self.rows, POUT), **f32)
         self.ystd = (1.0 / mlp.yinv).contiguous()
@@ -611,8 +622,8 @@ class SurrogateStepper:
                 ctrl.data_ptr(), ptr(mu), m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(),
                 self.ystd.data_ptr(), self.qacc_aba.data_ptr(), self.tau.data_ptr(), self.fallback.data_ptr(),
                 self.stats.data_ptr(), ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
-                self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(), s),
-                "surrogate_steps_deferred")
+                self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
+                self.blk_run.data_ptr() if self.fallback_runs else None, s), "surrogate_steps_deferred")
             self.t += n
             return
         _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),

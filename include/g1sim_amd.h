@@ -314,7 +314,21 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
  * count batch-judged / rejected envs, [2] takes their max error.
  *
  * g1_hip_surrogate_steps_deferred runs these per step, with the batch
- * judgement after every step whose phase is period - 1.
+ * judgement after every step whose phase is period - 1.  With blk_run
+ * (uint8 [ceil(nenv / 64)], nullable) it goes by period segments: one
+ * g1_hip_fallback_run launch per segment steps every block whose envs are
+ * all in fallback to the segment's end (such a block stays in fallback until
+ * the period's batch judgement, so all its steps are ABA's), and the
+ * per-step launches leave those blocks alone.  The results are the per-step
+ * path's: state, records, flags and statistics.
+ *
+ * g1_hip_fallback_run: the blocks whose live envs are all in fallback take
+ * nsub steps as by the plain step, each writing its record (as
+ * g1_hip_judge_forces_commit does) at its judged step, phase (phase0 + s) %
+ * period; nsub <= period - phase0.  blk_run[b] = 1 marks the blocks stepped,
+ * 0 the others.  qacc_track, contact_forces and the feet (nullable) receive
+ * the last step's values of the stepped blocks; stats[3] counts their
+ * env-steps.
  *
  * Blocks with an env off fallback are judged at once.  So, as with
  * immediate judging, the network's qacc is integrated only for envs whose
@@ -343,7 +357,12 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
                                     const float* ystd, float* qacc_aba, float* tau, uint8_t* fallback,
                                     unsigned* stats, float* qacc_out, float* contact_forces, float* foot_pos,
                                     float* foot_vel, unsigned judge_flags, float* snap, uint8_t* sflag,
-                                    void* stream);
+                                    uint8_t* blk_run, void* stream);
+int g1_hip_fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor,
+                        const float* ctrl, const float* mu, const uint8_t* fallback, int phase0, int period,
+                        uint8_t* blk_run, float* qacc_track, float* contact_forces, float* foot_pos,
+                        float* foot_vel, float* snap, uint8_t* sflag, unsigned* stats, unsigned flags,
+                        void* stream);
 
 /* RL observation of every env (replaces the torch code of
  * src/g1sim.py:132-170).  layout 0: 99 values per env in the reference order
