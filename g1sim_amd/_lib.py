@@ -108,7 +108,7 @@ def load():
     lib.g1_hip_surrogate_judge_batch.argtypes = [i, f, vp, vp, pa, pa, vp, vp, vp, vp, vp, vp, vp]
     lib.g1_hip_surrogate_steps_deferred.restype = i
     lib.g1_hip_surrogate_steps_deferred.argtypes = [i, i, i, f, i, f, vp, vp, vp, vp, vp, vp, vp, pa, pa, vp, vp, vp,
-                                                    vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp, vp]
+                                                    vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp, C.POINTER(i), vp]
     lib.g1_hip_fallback_run.restype = i
     lib.g1_hip_fallback_run.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_get_obs.restype = i

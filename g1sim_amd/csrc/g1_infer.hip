@@ -1599,6 +1599,36 @@ bool fill_surrogate_weights(SurrogateArgs& a, const void* const* W, const float*
 }
 }  // namespace
 
+// Envs off fallback (fallback[e] == 0) into *count: one workgroup, 16 flags
+// per thread per pass over the first nvec * 16 flags (nvec = 0 unless the
+// array is 16-byte aligned), the rest bytewise.  A kernel, not a memset plus
+// atomics, so that it can sit in a captured graph.
+__global__ __launch_bounds__(1024) void count_off_fallback_kernel(int nenv, int nvec, const uint8_t* fallback,
+                                                                    unsigned* count) {
+  __shared__ unsigned part[16];
+  const int tid = threadIdx.x;
+  unsigned c = 0;
+  for (int i = tid; i < nvec; i += 1024) {
+    const uint4 v = reinterpret_cast<const uint4*>(fallback)[i];
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c += ((w[k] >> (8 * j)) & 0xffu) == 0u;
+  }
+  for (int e = nvec * 16 + tid; e < nenv; e += 1024) c += fallback[e] == 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((tid & 63) == 0) part[tid >> 6] = c;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned t = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += part[k];
+    *count = t;
+  }
+}
+
 extern "C" {
 
 int g1_hip_surrogate_forward(int nenv, const float* qpos, const float* qvel, const float* tau, const float* xmean,
@@ -1726,17 +1756,30 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
 // (recording each at its judged step); the per-step launches then run the
 // other blocks only.  Same results as without blk_run: the state, records,
 // flags and statistics of the per-step path.
+//
+// all_fallback (host int, in/out, with blk_run; nullable): 1 on entry says
+// that every env is in fallback (after the stepper's start, or a call that
+// returned 1 and only resets since).  While that holds the per-step launches
+// have nothing to do and are not issued.  After each batch judgement the
+// envs off fallback are counted into stats[6] and read back once the next
+// segment's fallback run is queued (one stream synchronisation per period);
+// a count of 0 keeps skipping.  On return it says whether every env is in
+// fallback after the last step.
 int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
                                     float* qvel, float* anchor, const float* ctrl, const float* mu,
                                     const float* xmean, const float* xinv, const void* const* W,
                                     const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
                                     float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
                                     float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
-                                    float* snap, uint8_t* sflag, uint8_t* blk_run, void* stream) {
+                                    float* snap, uint8_t* sflag, uint8_t* blk_run, int* all_fallback, void* stream) {
   g1err::clear();
   if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
-      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag)
+      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag || (all_fallback && !blk_run))
     return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
+  const hipStream_t st = (hipStream_t)stream;
+  bool skip = all_fallback && *all_fallback == 1;     // per-step launches have nothing to do
+  bool counted = false;                                // a count is queued in stats[6]
+  unsigned off = 0;
   for (int s = 0; s < nsteps;) {
     const int phase0 = (int)(((long long)t0 + s) % period);
     const int seg = blk_run ? std::min(nsteps - s, period - phase0) : 1;
@@ -1746,23 +1789,46 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
                                    stream);
       if (rc) return rc;
     }
+    if (counted) {
+      // the count of the last batch judgement (queued before this segment's run)
+      if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("g1_hip_surrogate_steps_deferred: sync failed");
+      skip = off == 0;
+      counted = false;
+    }
     for (int j = 0; j < seg; ++j) {
       const int phase = phase0 + j;
-      int rc = g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period,
-                                             qacc_aba, tau, contact_forces, foot_pos, foot_vel, qacc_out, snap,
-                                             sflag, stats, blk_run, judge_flags, stream);
-      if (rc) return rc;
-      rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b, ymean,
-                                          ystd, qacc_aba, fallback, stats, qacc_out, stream);
-      if (rc) return rc;
+      int rc;
+      if (!skip) {
+        rc = g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period,
+                                           qacc_aba, tau, contact_forces, foot_pos, foot_vel, qacc_out, snap,
+                                           sflag, stats, blk_run, judge_flags, stream);
+        if (rc) return rc;
+        rc = g1_hip_surrogate_step_deferred(nenv, dt, phase, period, tol, qpos, qvel, tau, xmean, xinv, W, b,
+                                            ymean, ystd, qacc_aba, fallback, stats, qacc_out, stream);
+        if (rc) return rc;
+      }
       if (phase == period - 1) {
         rc = g1_hip_surrogate_judge_batch(nenv, tol, xmean, xinv, W, b, ymean, ystd, snap, sflag, fallback, stats,
                                           stream);
         if (rc) return rc;
+        if (all_fallback) {
+          const int nvec = (reinterpret_cast<uintptr_t>(fallback) % 16 == 0) ? nenv / 16 : 0;
+          hipLaunchKernelGGL(count_off_fallback_kernel, dim3(1), dim3(1024), 0, st, nenv, nvec, fallback, stats + 6);
+          if (hipGetLastError() != hipSuccess)
+            return g1err::fail("g1_hip_surrogate_steps_deferred: count launch failed");
+          if (hipMemcpyAsync(&off, stats + 6, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess)
+            return g1err::fail("g1_hip_surrogate_steps_deferred: count copy failed");
+          counted = true;
+        }
       }
     }
     s += seg;
   }
+  if (counted) {
+    if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("g1_hip_surrogate_steps_deferred: sync failed");
+    skip = off == 0;
+  }
+  if (all_fallback) *all_fallback = skip ? 1 : 0;
   return 0;
 }
 

@@ -479,7 +479,12 @@ class SurrogateStepper:
     all in fallback at a period segment's start is ABA's until the period
     ends; steps() advances all such blocks through the segment with one
     g1_hip_fallback_run launch (each recorded at its judged step) and runs
-    the per-step launches for the other blocks."""
+    the per-step launches for the other blocks.  While every env is in
+    fallback (known at the start, then from a count after each batch
+    judgement, read once per period) those per-step launches have nothing to
+    do and are not issued.  `fallback` belongs to the stepper: code that
+    writes it directly sets `_all_fallback.value = 0` before the next
+    steps()."""
 
     def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True,
                  track_qacc=False, defer=True, fallback_runs=True):
@@ -496,8 +501,12 @@ class SurrogateStepper:
         self.qacc_aba = torch.zeros((NV, nenv), **f32)
         self.fallback = torch.ones(nenv, dtype=torch.uint8, device=self.dev)
         # [0] judged at once, [1] rejected at once, [2] max judged error (float
-        # bits), [3] fallback env-steps, [4] / [5] batch-judged / rejected
-        self.stats = torch.zeros(6, dtype=torch.int32, device=self.dev)
+        # bits), [3] fallback env-steps, [4] / [5] batch-judged / rejected,
+        # [6] envs off fallback after the last batch judgement of steps()
+        self.stats = torch.zeros(8, dtype=torch.int32, device=self.dev)
+        # known on the host: every env is in fallback (the fail-safe start);
+        # steps() keeps it up to date, step() and resets leave it unknown or true
+        self._all_fallback = C.c_int(1)
         if self.defer:
             self.snap = torch.zeros((SNAP_ROWS, nenv), **f32)
             self.sflag = torch.zeros(nenv, dtype=torch.uint8, device=self.dev)
@@ -537,6 +546,7 @@ class SurrogateStepper:
             check_soa("mu", mu.view(1, -1), 1, n, self.dev)
         phase = self.t % self.period
         ck = _lib.check
+        self._all_fallback.value = 0        # a batch judgement here may accept envs: unknown
         qacc_out = None if self.qacc is None else self.qacc.data_ptr()
         defer = self.defer and prediction is None
         if defer:
@@ -623,7 +633,8 @@ class SurrogateStepper:
                 self.ystd.data_ptr(), self.qacc_aba.data_ptr(), self.tau.data_ptr(), self.fallback.data_ptr(),
                 self.stats.data_ptr(), ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
                 self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
-                self.blk_run.data_ptr() if self.fallback_runs else None, s), "surrogate_steps_deferred")
+                self.blk_run.data_ptr() if self.fallback_runs else None,
+                C.byref(self._all_fallback) if self.fallback_runs else None, s), "surrogate_steps_deferred")
             self.t += n
             return
         _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),

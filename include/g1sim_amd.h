@@ -320,7 +320,14 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
  * all in fallback to the segment's end (such a block stays in fallback until
  * the period's batch judgement, so all its steps are ABA's), and the
  * per-step launches leave those blocks alone.  The results are the per-step
- * path's: state, records, flags and statistics.
+ * path's: state, records, flags and statistics.  all_fallback (host int,
+ * in/out, needs blk_run; nullable): 1 on entry says every env is in fallback
+ * (as at the stepper's start, with only resets since); the per-step
+ * launches, which would then have nothing to do, are not issued while that
+ * holds.  After each batch judgement the envs off fallback are counted into
+ * stats[6] (stats needs 7 entries) and read back, with one stream
+ * synchronisation per period, once the next segment's fallback run is
+ * queued; on return all_fallback says whether every env is in fallback.
  *
  * g1_hip_fallback_run: the blocks whose live envs are all in fallback take
  * nsub steps as by the plain step, each writing its record (as
@@ -357,7 +364,7 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
                                     const float* ystd, float* qacc_aba, float* tau, uint8_t* fallback,
                                     unsigned* stats, float* qacc_out, float* contact_forces, float* foot_pos,
                                     float* foot_vel, unsigned judge_flags, float* snap, uint8_t* sflag,
-                                    uint8_t* blk_run, void* stream);
+                                    uint8_t* blk_run, int* all_fallback, void* stream);
 int g1_hip_fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor,
                         const float* ctrl, const float* mu, const uint8_t* fallback, int phase0, int period,
                         uint8_t* blk_run, float* qacc_track, float* contact_forces, float* foot_pos,

@@ -207,9 +207,14 @@ def test_deferred_judging_entry_points_validate_arguments():
     assert "g1_hip_surrogate_judge_batch" in _lib.last_error()
     # steps: missing records; zero steps is a no-op
     assert lib.g1_hip_surrogate_steps_deferred(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
-                                               d, d, None, None, None, None, 0, None, d, None, None) == -1
+                                               d, d, None, None, None, None, 0, None, d, None, None, None) == -1
+    flag = C.c_int(1)
+    # the all-fallback flag needs the run flags
+    assert lib.g1_hip_surrogate_steps_deferred(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
+                                               d, d, None, None, None, None, 0, d, d, None, C.byref(flag),
+                                               None) == -1
     assert lib.g1_hip_surrogate_steps_deferred(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
-                                               d, d, None, None, None, None, 0, d, d, d, None) == 0
+                                               d, d, None, None, None, None, 0, d, d, d, None, None) == 0
     # fallback run: a run past the period's end, no run flags, no records; zero steps is a no-op
     assert lib.g1_hip_fallback_run(8, 5, 2e-3, d, d, d, d, None, d, 12, 16, d, None, None, None, None, d, d, d, 0,
                                    None) == -1
