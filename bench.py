@@ -646,11 +646,12 @@ def cpu_baseline(samples=None):
 
 # ================================================================ profiling
 def kernel_only(kind, nenv, steps, warmup, tol=1e30):
-    """One fused inference kernel alone, one surrogate stepper step (judge +
-    forces, network + commit; random weights; at the default tol every
-    judgement is accepted after the warmup, at 1e-3 every env stays in
-    fallback and its judgements are deferred), or one training iteration
-    (graph replays), timed with HIP events: for profiling runs."""
+    """One fused inference kernel alone, surrogate stepper steps (step():
+    judge + forces, network + commit; steps(): the same in calls of 10 steps,
+    with fallback runs; random weights; at the default tol every judgement is
+    accepted after the warmup, at 1e-3 every env stays in fallback and its
+    judgements are deferred), or one training iteration (graph replays),
+    timed with HIP events: for profiling runs."""
     import torch
     from g1sim_amd import workloads as W
     dev = torch.device("cuda:0")
@@ -669,17 +670,23 @@ def kernel_only(kind, nenv, steps, warmup, tol=1e30):
                           "avg_iteration_ms": it, "achieved_tflops": tf, "frac_bf16_peak": tf * 1e12 / PEAK_BF16}),
               flush=True)
         return
-    if kind == "stepper":
+    if kind in ("stepper", "stepper-steps"):
         from g1sim_amd.surrogate import MLP, SurrogateStepper
         mlp = MLP(dev, seed=0)
         anchor, ctrl = torch.as_tensor(an, device=dev), torch.as_tensor(cb[0], device=dev)
         st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol)
-        fn = lambda: st.step(qpos, qvel, anchor, ctrl)  # noqa: E731
-        for _ in range(max(warmup, 16)):
+        if kind == "stepper":
+            fn, per = (lambda: st.step(qpos, qvel, anchor, ctrl)), 1  # noqa: E731
+        else:
+            # steps() in calls of one PD-target block, as the bench's stepper legs
+            fn, per = (lambda: st.steps(qpos, qvel, anchor, ctrl, CTRL_PERIOD)), CTRL_PERIOD  # noqa: E731
+        for _ in range(max(warmup, 16) // per + 1):
             fn()
         torch.cuda.synchronize()
-        ms = _event_avg_ms(fn, steps, torch.cuda.current_stream(dev))
-        print(json.dumps({"kernel": "surrogate stepper step", "nenv": nenv, "tol": tol, "steps": steps, "avg_step_ms": ms,
+        calls = max(steps // per, 1)
+        ms = _event_avg_ms(fn, calls, torch.cuda.current_stream(dev)) / per
+        print(json.dumps({"kernel": "surrogate stepper " + ("step()" if per == 1 else "steps()"), "nenv": nenv,
+                          "tol": tol, "steps": calls * per, "avg_step_ms": ms,
                           "env_steps_per_s": nenv / (ms * 1e-3)}), flush=True)
         return
     if kind == "surrogate":
@@ -762,11 +769,11 @@ def main():
                     help="skip the 131,072-env FP32 ABA leg of the default run")
     ap.add_argument("--no-precision", action="store_true", help="skip the per-configuration precision samples")
     ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
-    ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper", "train"),
-                    help="time one fused inference kernel, one surrogate stepper step or one training "
-                         "iteration alone at --nenv (for profiling) and exit")
+    ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper", "stepper-steps", "train"),
+                    help="time one fused inference kernel, surrogate stepper steps (step() calls, or steps() "
+                         "calls of 10 steps) or one training iteration alone at --nenv (for profiling) and exit")
     ap.add_argument("--tol", type=float, default=1e30,
-                    help="judge tolerance of --kernel stepper (default: every judgement accepted)")
+                    help="judge tolerance of --kernel stepper / stepper-steps (default: every judgement accepted)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 256 MiB buffer once (reference kernel of known "
                          "traffic for the FETCH_SIZE / WRITE_SIZE counters)")

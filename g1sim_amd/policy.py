@@ -139,7 +139,8 @@ class PolicyRollout:
             from .surrogate import SurrogateStepper
             if surrogate is None:
                 raise ValueError("physics='surrogate' needs a trained surrogate.MLP")
-            self.stepper = SurrogateStepper(surrogate, nenv, device, dt=dt, period=period, tol=tol)
+            # no per-period count read-back: the loop never waits for the device
+            self.stepper = SurrogateStepper(surrogate, nenv, device, dt=dt, period=period, tol=tol, skip_idle=False)
             self.foot_pos, self.foot_vel = self.stepper.foot_pos, self.stepper.foot_vel
             self.contact_forces = self.stepper.contact_forces
         elif physics != "aba":

@@ -479,20 +479,24 @@ class SurrogateStepper:
     all in fallback at a period segment's start is ABA's until the period
     ends; steps() advances all such blocks through the segment with one
     g1_hip_fallback_run launch (each recorded at its judged step) and runs
-    the per-step launches for the other blocks.  While every env is in
-    fallback (known at the start, then from a count after each batch
-    judgement, read once per period) those per-step launches have nothing to
-    do and are not issued.  `fallback` belongs to the stepper: code that
-    writes it directly sets `_all_fallback.value = 0` before the next
-    steps()."""
+    the per-step launches for the other blocks.  With skip_idle, while every
+    env is in fallback (known at the start, then from a count after each
+    batch judgement, read with one stream synchronisation per period) those
+    per-step launches have nothing to do and are not issued; without it
+    steps() never waits for the device.  `fallback` belongs to the stepper:
+    code that writes it directly sets `_all_fallback.value = 0` before the
+    next steps()."""
 
     def __init__(self, mlp, nenv, device, dt=2e-3, period=16, tol=1e-3, judge_flags=0, fused=True,
-                 track_qacc=False, defer=True, fallback_runs=True):
+                 track_qacc=False, defer=True, fallback_runs=True, skip_idle=True):
         self.mlp, self.nenv, self.dt, self.period, self.tol = mlp, nenv, dt, period, tol
         self.fused = fused and mlp.fusable
         self.defer = defer and self.fused
-        # steps(): all-fallback blocks stepped by one launch per period segment
+        # steps(): all-fallback blocks stepped by one launch per period segment;
+        # skip_idle: the per-step launches are left out while every env is in
+        # fallback (one stream synchronisation per period reads the count)
         self.fallback_runs = fallback_runs and self.defer
+        self.skip_idle = skip_idle and self.fallback_runs
         self.lib = _lib.load()
         self.dev = norm_device(device)
         self.rows = (nenv + 127) // 128 * 128
@@ -634,7 +638,7 @@ class SurrogateStepper:
                 self.stats.data_ptr(), ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(),
                 self.foot_vel.data_ptr(), self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
                 self.blk_run.data_ptr() if self.fallback_runs else None,
-                C.byref(self._all_fallback) if self.fallback_runs else None, s), "surrogate_steps_deferred")
+                C.byref(self._all_fallback) if self.skip_idle else None, s), "surrogate_steps_deferred")
             self.t += n
             return
         _lib.check(lib.g1_hip_surrogate_steps(ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(),

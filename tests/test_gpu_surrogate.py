@@ -443,13 +443,14 @@ def test_steps_equal_repeated_step(device, tol):
     write_metrics(f"surrogate_steps_one_call_tol{tol:g}", {"stats": ea["stats"].tolist(), "fallback_env_steps": summ})
 
 
-def _steps_run(device, mlp, n, tol, fallback_runs, schedule, reset=None, reset_before=3):
+def _steps_run(device, mlp, n, tol, fallback_runs, skip_idle, schedule, reset=None, reset_before=3):
     """SurrogateStepper.steps over the call sizes in `schedule` (calls start
     and end mid-period), optionally resetting envs (mark_reset) before call
     number `reset_before`."""
     qp, qv, an, cb = W.randomized_actuation(n, seed=35, nblocks=1)
     st = GpuState(qp, qv, an, cb[0], device)
-    stepper = SurrogateStepper(mlp, n, device, period=16, tol=tol, track_qacc=True, fallback_runs=fallback_runs)
+    stepper = SurrogateStepper(mlp, n, device, period=16, tol=tol, track_qacc=True, fallback_runs=fallback_runs,
+                               skip_idle=skip_idle)
     for i, k in enumerate(schedule):
         if reset is not None and i == reset_before:
             stepper.mark_reset(reset)
@@ -464,13 +465,14 @@ def _steps_run(device, mlp, n, tol, fallback_runs, schedule, reset=None, reset_b
 @pytest.mark.parametrize("case", ["never_accepted", "accepted", "accepted_then_reset"])
 def test_fallback_runs_match_per_step(device, n, case):
     """steps() with fallback runs (one g1_hip_fallback_run launch per period
-    segment for the blocks whose envs are all in fallback) gives the bits of
-    the per-step path: state, feet, contact forces, tracked qacc, records,
-    flags and statistics.  Never accepted: every block runs; accepted (tol
-    1e30): the blocks leave fallback at the first period's end and the later
-    steps take the per-step path; then a reset of whole blocks and of single
-    envs makes blocks that run and blocks that mix fallback and network envs.
-    40,000 envs take the 2-wave split."""
+    segment for the blocks whose envs are all in fallback), with and without
+    the idle per-step launches left out, gives the bits of the per-step path:
+    state, feet, contact forces, tracked qacc, records, flags and statistics.
+    Never accepted: every block runs; accepted (tol 1e30): the blocks leave
+    fallback at the first period's end and the later steps take the per-step
+    path; then a reset of whole blocks and of single envs makes blocks that
+    run and blocks that mix fallback and network envs.  40,000 envs take the
+    2-wave split."""
     mlp = MLP(device, seed=4)
     tol = 1e-3 if case == "never_accepted" else 1e30
     reset = None
@@ -481,13 +483,16 @@ def test_fallback_runs_match_per_step(device, n, case):
         r[64 * 9:64 * 10] = 0                                # block 9: none
         reset = torch.as_tensor(r, device=device)
     schedule = [3, 10, 13, 21, 7, 16]                        # t = 3, 13, 26, 47, 54, 70
-    (ha, ea, sa), (hb, eb, sb) = (_steps_run(device, mlp, n, tol, runs, schedule, reset) for runs in (True, False))
-    for k in ("qpos", "qvel", "anchor", "foot_pos", "foot_vel", "fn"):
-        assert np.array_equal(ha[k], hb[k]), (k, float(np.abs(ha[k] - hb[k]).max()))
-    ea["stats"], eb["stats"] = ea["stats"][:6], eb["stats"][:6]     # [6]: the fallback runs' own count
-    for k in ea:
-        assert np.array_equal(ea[k], eb[k]), k
-    assert sa == sb
+    hb, eb, sb = _steps_run(device, mlp, n, tol, False, False, schedule, reset)        # per-step path
+    eb["stats"] = eb["stats"][:6]
+    for skip_idle in (True, False):
+        ha, ea, sa = _steps_run(device, mlp, n, tol, True, skip_idle, schedule, reset)
+        for k in ("qpos", "qvel", "anchor", "foot_pos", "foot_vel", "fn"):
+            assert np.array_equal(ha[k], hb[k]), (skip_idle, k, float(np.abs(ha[k] - hb[k]).max()))
+        ea["stats"] = ea["stats"][:6]          # [6]: the idle skip's own count of envs off fallback
+        for k in ea:
+            assert np.array_equal(ea[k], eb[k]), (skip_idle, k)
+        assert sa == sb, skip_idle
     if case == "never_accepted":
         assert sa["aba_frac"] == 1.0 and sa["fallback_envs_now"] == n
         assert sa["judged"] == sa["rejected"] == (70 // 16) * n       # four batch judgements, all rejected
