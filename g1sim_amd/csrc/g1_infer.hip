@@ -1778,8 +1778,18 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
     return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
   const hipStream_t st = (hipStream_t)stream;
   bool skip = all_fallback && *all_fallback == 1;     // per-step launches have nothing to do
-  bool counted = false;                                // a count is queued in stats[6]
-  unsigned off = 0;
+  // A count is queued in stats[6] with its copy into `off`: every return
+  // waits for the stream first, so the copy never lands in a dead frame.
+  struct Pending {
+    hipStream_t st;
+    bool counted = false;
+    unsigned off = 0;
+    ~Pending() {
+      if (counted) (void)hipStreamSynchronize(st);
+    }
+  } pend{st};
+  bool& counted = pend.counted;
+  unsigned& off = pend.off;
   for (int s = 0; s < nsteps;) {
     const int phase0 = (int)(((long long)t0 + s) % period);
     const int seg = blk_run ? std::min(nsteps - s, period - phase0) : 1;
@@ -1827,6 +1837,7 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
   if (counted) {
     if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("g1_hip_surrogate_steps_deferred: sync failed");
     skip = off == 0;
+    counted = false;
   }
   if (all_fallback) *all_fallback = skip ? 1 : 0;
   return 0;
