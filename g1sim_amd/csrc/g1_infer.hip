@@ -137,11 +137,16 @@ template <int NIN, int NOUT, int RTILES> struct Ring {
 // output row is one contiguous block, so the fragments of a k-step (16 rows
 // x 64 B per load instruction) are contiguous 1 KB runs (row-major
 // [NOUT][NIN] puts them NIN * 2 bytes apart and measured at 0.6x the rate on
-// MI355X).  Lane l reads row c * 16 + l % 16, k-elements (l / 16) * 8 + 0..7.
-template <int NOUT, int RTILES>
-__device__ __forceinline__ const bf16* wfrag(const bf16* W, int w, int lane, int k, int c) {
+// MI355X).  Lane l reads row c * 16 + l % 16, k-elements (l / 16) * 8 + 0..7,
+// with a buffer load: the layer in a buffer resource (scalar registers), the
+// lane's byte offset within a k-slice as the vector offset and the k-slice as
+// the scalar offset, so no load carries 64-bit address arithmetic.
+template <int NIN, int NOUT, int RTILES>
+__device__ __forceinline__ bf16x8 wload(const bf16* W, int w, int lane, int k, int c) {
   using T = Tile<NOUT, RTILES>;
-  return W + ((size_t)k * NOUT + (size_t)(T::c0(w) + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)W, (short)0, NIN * NOUT * 2, 0x00020000);
+  const int voff = (((T::c0(w) + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8) * 2;
+  return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, k * NOUT * 64, 0));
 }
 
 template <int NIN, int NOUT, int RTILES>
@@ -151,7 +156,7 @@ __device__ __forceinline__ void wprefetch(const bf16* __restrict__ W, int w, int
   for (int k = 0; k < R::PF; ++k)
 #pragma unroll
     for (int c = 0; c < Tile<NOUT, RTILES>::CT; ++c)
-      rg.f[k][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT, RTILES>(W, w, lane, k, c));
+      rg.f[k][c] = wload<NIN, NOUT, RTILES>(W, w, lane, k, c);
 }
 
 // acc = init + in[rows x NIN] (LDS) . W^T for this wave's tiles (init[c]:
@@ -185,7 +190,7 @@ __device__ __forceinline__ void mma_layer(const bf16* __restrict__ sin, const bf
     if (k + PF < KS) {
 #pragma unroll
       for (int c = 0; c < T::CT; ++c)
-        rg.f[(k + PF) % NB][c] = *reinterpret_cast<const bf16x8*>(wfrag<NOUT, RTILES>(W, w, lane, k + PF, c));
+        rg.f[(k + PF) % NB][c] = wload<NIN, NOUT, RTILES>(W, w, lane, k + PF, c);
     }
     if (k + 1 < KS) {
 #pragma unroll
