@@ -314,6 +314,10 @@ def _stepper_run(dev, mlp, nenv, steps, warmup, tol, seed, defer=True):
     st = SurrogateStepper(mlp, nenv, dev, period=16, tol=tol, defer=defer)
 
     def run(first_block, nsteps):
+        if defer:
+            # the whole target schedule in one host call
+            st.steps_schedule(qpos, qvel, anchor, ctrl, first_block * CTRL_PERIOD, nsteps, CTRL_PERIOD)
+            return
         # the steps of each target block in one host call
         for s0 in range(0, nsteps, CTRL_PERIOD):
             st.steps(qpos, qvel, anchor, ctrl[first_block + s0 // CTRL_PERIOD], min(CTRL_PERIOD, nsteps - s0))

@@ -109,6 +109,10 @@ def load():
     lib.g1_hip_surrogate_steps_deferred.restype = i
     lib.g1_hip_surrogate_steps_deferred.argtypes = [i, i, i, f, i, f, vp, vp, vp, vp, vp, vp, vp, pa, pa, vp, vp, vp,
                                                     vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp, C.POINTER(i), vp]
+    lib.g1_hip_surrogate_steps_schedule.restype = i
+    lib.g1_hip_surrogate_steps_schedule.argtypes = [i, i, i, f, i, f, vp, vp, vp, vp, C.c_longlong, i, i, vp, vp, vp,
+                                                    pa, pa, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u, vp, vp, vp,
+                                                    C.POINTER(i), vp]
     lib.g1_hip_fallback_run.restype = i
     lib.g1_hip_fallback_run.argtypes = [i, i, f, vp, vp, vp, vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, u, vp]
     lib.g1_hip_get_obs.restype = i
@@ -146,7 +150,7 @@ def exported_symbols():
             "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_mse_grad_part",
             "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows", "g1_hip_surrogate_steps", "g1_hip_surrogate_step_deferred",
             "g1_hip_surrogate_judge_batch", "g1_hip_surrogate_steps_deferred", "g1_hip_judge_forces_commit",
-            "g1_hip_fallback_run"]
+            "g1_hip_fallback_run", "g1_hip_surrogate_steps_schedule"]
 
 
 def ptr_array(tensors_or_ptrs):

@@ -1770,17 +1770,22 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
 // segment's fallback run is queued (one stream synchronisation per period);
 // a count of 0 keeps skipping.  On return it says whether every env is in
 // fallback after the last step.
-int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
-                                    float* qvel, float* anchor, const float* ctrl, const float* mu,
-                                    const float* xmean, const float* xinv, const void* const* W,
-                                    const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
-                                    float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
-                                    float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
-                                    float* snap, uint8_t* sflag, uint8_t* blk_run, int* all_fallback, void* stream) {
+static int steps_deferred(const char* who, int nenv, int nsteps, int t0, float dt, int period, float tol,
+                          float* qpos, float* qvel, float* anchor, const float* ctrl, long long ctrl_stride,
+                          int ctrl_period, int ctrl_step0, const float* mu, const float* xmean, const float* xinv,
+                          const void* const* W, const float* const* b, const float* ymean, const float* ystd,
+                          float* qacc_aba, float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                          float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags, float* snap,
+                          uint8_t* sflag, uint8_t* blk_run, int* all_fallback, void* stream) {
   g1err::clear();
   if (nenv < 0 || nsteps < 0 || t0 < 0 || period <= 0 || !(dt > 0.f) || !qpos || !qvel || !anchor || !ctrl ||
-      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag || (all_fallback && !blk_run))
-    return g1err::fail("g1_hip_surrogate_steps_deferred: invalid arguments");
+      !qacc_aba || !tau || !stats || !fallback || !snap || !sflag || (all_fallback && !blk_run) ||
+      ctrl_stride < 0 || (ctrl_stride > 0 && (ctrl_stride < 29LL * nenv || ctrl_period <= 0 || ctrl_step0 < 0)))
+    return g1err::fail(who);
+  // the targets of step s of this call (fixed without a schedule)
+  auto ctrl_at = [&](int s) {
+    return ctrl_stride > 0 ? ctrl + (size_t)((ctrl_step0 + s) / ctrl_period) * (size_t)ctrl_stride : ctrl;
+  };
   const hipStream_t st = (hipStream_t)stream;
   bool skip = all_fallback && *all_fallback == 1;     // per-step launches have nothing to do
   // A count is queued in stats[6] with its copy into `off`: every return
@@ -1799,14 +1804,15 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
     const int phase0 = (int)(((long long)t0 + s) % period);
     const int seg = blk_run ? std::min(nsteps - s, period - phase0) : 1;
     if (blk_run) {
-      int rc = g1_hip_fallback_run(nenv, seg, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase0, period, blk_run,
-                                   qacc_out, contact_forces, foot_pos, foot_vel, snap, sflag, stats, judge_flags,
-                                   stream);
+      int rc = g1k_host::fallback_run(nenv, seg, dt, qpos, qvel, anchor, ctrl_at(s), ctrl_stride, ctrl_period,
+                                      ctrl_stride > 0 ? (ctrl_step0 + s) % ctrl_period : 0, mu, fallback, phase0,
+                                      period, blk_run, qacc_out, contact_forces, foot_pos, foot_vel, snap, sflag,
+                                      stats, judge_flags, stream);
       if (rc) return rc;
     }
     if (counted) {
       // the count of the last batch judgement (queued before this segment's run)
-      if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("g1_hip_surrogate_steps_deferred: sync failed");
+      if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("surrogate steps: sync failed");
       skip = off == 0;
       counted = false;
     }
@@ -1814,7 +1820,7 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
       const int phase = phase0 + j;
       int rc;
       if (!skip) {
-        rc = g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl, mu, fallback, phase, period,
+        rc = g1k_host::judge_forces_commit(nenv, dt, qpos, qvel, anchor, ctrl_at(s + j), mu, fallback, phase, period,
                                            qacc_aba, tau, contact_forces, foot_pos, foot_vel, qacc_out, snap,
                                            sflag, stats, blk_run, judge_flags, stream);
         if (rc) return rc;
@@ -1830,9 +1836,9 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
           const int nvec = (reinterpret_cast<uintptr_t>(fallback) % 16 == 0) ? nenv / 16 : 0;
           hipLaunchKernelGGL(count_off_fallback_kernel, dim3(1), dim3(1024), 0, st, nenv, nvec, fallback, stats + 6);
           if (hipGetLastError() != hipSuccess)
-            return g1err::fail("g1_hip_surrogate_steps_deferred: count launch failed");
+            return g1err::fail("surrogate steps: count launch failed");
           if (hipMemcpyAsync(&off, stats + 6, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess)
-            return g1err::fail("g1_hip_surrogate_steps_deferred: count copy failed");
+            return g1err::fail("surrogate steps: count copy failed");
           counted = true;
         }
       }
@@ -1840,12 +1846,47 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
     s += seg;
   }
   if (counted) {
-    if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("g1_hip_surrogate_steps_deferred: sync failed");
+    if (hipStreamSynchronize(st) != hipSuccess) return g1err::fail("surrogate steps: sync failed");
     skip = off == 0;
     counted = false;
   }
   if (all_fallback) *all_fallback = skip ? 1 : 0;
   return 0;
+}
+
+int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
+                                    float* qvel, float* anchor, const float* ctrl, const float* mu,
+                                    const float* xmean, const float* xinv, const void* const* W,
+                                    const float* const* b, const float* ymean, const float* ystd, float* qacc_aba,
+                                    float* tau, uint8_t* fallback, unsigned* stats, float* qacc_out,
+                                    float* contact_forces, float* foot_pos, float* foot_vel, unsigned judge_flags,
+                                    float* snap, uint8_t* sflag, uint8_t* blk_run, int* all_fallback, void* stream) {
+  return steps_deferred("g1_hip_surrogate_steps_deferred: invalid arguments", nenv, nsteps, t0, dt, period, tol,
+                        qpos, qvel, anchor, ctrl, 0, 0, 0, mu, xmean, xinv, W, b, ymean, ystd, qacc_aba, tau,
+                        fallback, stats, qacc_out, contact_forces, foot_pos, foot_vel, judge_flags, snap, sflag,
+                        blk_run, all_fallback, stream);
+}
+
+// The same with a PD-target schedule: step s of the call uses the block
+// ctrl_blocks + ((ctrl_step0 + s) / ctrl_period) block_stride, as
+// g1_hip_step_schedule; with blk_run the fallback runs switch blocks
+// themselves and span whole period segments.
+int g1_hip_surrogate_steps_schedule(int nenv, int nsteps, int t0, float dt, int period, float tol, float* qpos,
+                                    float* qvel, float* anchor, const float* ctrl_blocks, long long block_stride,
+                                    int ctrl_period, int ctrl_step0, const float* mu, const float* xmean,
+                                    const float* xinv, const void* const* W, const float* const* b,
+                                    const float* ymean, const float* ystd, float* qacc_aba, float* tau,
+                                    uint8_t* fallback, unsigned* stats, float* qacc_out, float* contact_forces,
+                                    float* foot_pos, float* foot_vel, unsigned judge_flags, float* snap,
+                                    uint8_t* sflag, uint8_t* blk_run, int* all_fallback, void* stream) {
+  if (block_stride <= 0) {
+    g1err::clear();
+    return g1err::fail("g1_hip_surrogate_steps_schedule: invalid arguments");
+  }
+  return steps_deferred("g1_hip_surrogate_steps_schedule: invalid arguments", nenv, nsteps, t0, dt, period, tol,
+                        qpos, qvel, anchor, ctrl_blocks, block_stride, ctrl_period, ctrl_step0, mu, xmean, xinv, W,
+                        b, ymean, ystd, qacc_aba, tau, fallback, stats, qacc_out, contact_forces, foot_pos, foot_vel,
+                        judge_flags, snap, sflag, blk_run, all_fallback, stream);
 }
 
 int g1_policy_act(int nenv, int layout, const float* qpos, float* qvel, const float* commands,

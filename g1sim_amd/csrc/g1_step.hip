@@ -945,7 +945,7 @@ G1K_DEV void step_body(const StepArgs& args, float (&xch)[2][SPLIT][XCH][ENVS_PE
   }
   // target schedule: the next block's targets are loaded one substep before
   // they apply, so their memory latency hides behind that substep
-  const bool sched = MODE == MODE_STEP && args.ctrl_stride > 0;
+  const bool sched = (MODE == MODE_STEP || MODE == MODE_FALLBACK_RUN) && args.ctrl_stride > 0;
   const float* cnext = args.ctrl;
   int cpos = args.ctrl_phase;                    // substeps of the current block already stepped
   float nwt[3], nct[CPW][MAXJ];
@@ -1751,6 +1751,46 @@ int judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anc
   a.blk_run = const_cast<uint8_t*>(blk_run);
   return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_JUDGE_FORCES);
 }
+// g1_hip_fallback_run with a PD-target schedule (ctrl_stride > 0): substep s
+// of the run uses the block ctrl + ((ctrl_phase + s) / ctrl_period)
+// ctrl_stride, ctrl being the block of substep 0 and ctrl_phase < ctrl_period
+// the substeps of that block taken before the run.
+int fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor, const float* ctrl,
+                 long long ctrl_stride, int ctrl_period, int ctrl_phase, const float* mu, const uint8_t* fallback,
+                 int phase0, int period, uint8_t* blk_run, float* qacc_track, float* contact_forces, float* foot_pos,
+                 float* foot_vel, float* snap, uint8_t* sflag, unsigned* stats, unsigned flags, void* stream) {
+  g_err[0] = 0;
+  int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
+  if (rc) return rc;
+  if (period <= 0 || phase0 < 0 || nsub > period - phase0 || !fallback || !blk_run || !snap || !sflag || !stats) {
+    set_err("fallback_run: bad phase/period (the run may not cross a period end), or no fallback / blk_run / "
+            "records / stats");
+    return -1;
+  }
+  if (ctrl_stride < 0 || (ctrl_stride > 0 && (ctrl_stride < 29LL * nenv || ctrl_period <= 0 || ctrl_phase < 0 ||
+                                             ctrl_phase >= ctrl_period))) {
+    set_err("fallback_run: bad target schedule");
+    return -1;
+  }
+  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+    set_err("fallback_run: unsupported flags 0x%x", flags);
+    return -1;
+  }
+  if (nenv == 0 || nsub == 0) return 0;
+  g1k::StepArgs a{};
+  a.nenv = nenv; a.nsub = nsub;
+  a.cfg = g1k::default_cfg(dt);
+  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
+  a.ctrl = ctrl;
+  a.ctrl_stride = ctrl_stride; a.ctrl_period = ctrl_period; a.ctrl_phase = ctrl_phase;
+  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
+  a.mu = mu;
+  a.stride = nenv;
+  a.sel_fallback = fallback; a.sel_phase = phase0; a.sel_period = period;
+  a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
+  a.blk_run = blk_run; a.run_phase0 = phase0;
+  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_FALLBACK_RUN);
+}
 }  // namespace g1k_host
 
 extern "C" {
@@ -1968,31 +2008,9 @@ int g1_hip_fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, 
                         const float* mu, const uint8_t* fallback, int phase0, int period, uint8_t* blk_run,
                         float* qacc_track, float* contact_forces, float* foot_pos, float* foot_vel, float* snap,
                         uint8_t* sflag, unsigned* stats, unsigned flags, void* stream) {
-  g_err[0] = 0;
-  int rc = validate(nenv, nsub, dt, qpos, qvel, anchor, ctrl);
-  if (rc) return rc;
-  if (period <= 0 || phase0 < 0 || nsub > period - phase0 || !fallback || !blk_run || !snap || !sflag || !stats) {
-    set_err("fallback_run: bad phase/period (the run may not cross a period end), or no fallback / blk_run / "
-            "records / stats");
-    return -1;
-  }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
-    set_err("fallback_run: unsupported flags 0x%x", flags);
-    return -1;
-  }
-  if (nenv == 0 || nsub == 0) return 0;
-  g1k::StepArgs a{};
-  a.nenv = nenv; a.nsub = nsub;
-  a.cfg = g1k::default_cfg(dt);
-  a.qpos = qpos; a.qvel = qvel; a.anchor = anchor;
-  a.ctrl = ctrl;
-  a.foot_pos = foot_pos; a.foot_vel = foot_vel; a.contact_forces = contact_forces;
-  a.mu = mu;
-  a.stride = nenv;
-  a.sel_fallback = fallback; a.sel_phase = phase0; a.sel_period = period;
-  a.snap = snap; a.sflag = sflag; a.qacc_track = qacc_track; a.stats = stats;
-  a.blk_run = blk_run; a.run_phase0 = phase0;
-  return launch_step(a, flags, (hipStream_t)stream, g1k::MODE_FALLBACK_RUN);
+  return g1k_host::fallback_run(nenv, nsub, dt, qpos, qvel, anchor, ctrl, 0, 0, 0, mu, fallback, phase0, period,
+                                blk_run, qacc_track, contact_forces, foot_pos, foot_vel, snap, sflag, stats, flags,
+                                stream);
 }
 
 int g1_hip_surrogate_integrate(int nenv, float dt, int phase, int period, float tol, float* qpos, float* qvel,

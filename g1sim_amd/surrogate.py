@@ -650,6 +650,40 @@ class SurrogateStepper:
                                               self.foot_vel.data_ptr(), self.judge_flags, s), "surrogate_steps")
         self.t += n
 
+    def steps_schedule(self, qpos, qvel, anchor, ctrl_blocks, step0, n, period, mu=None):
+        """n steps with a PD-target schedule (g1_hip_surrogate_steps_schedule):
+        step s uses ctrl_blocks[(step0 + s) // period] (float32
+        [nblocks][29][nenv]); the same results as one steps() call per target
+        block, with fallback runs that switch blocks inside the kernel."""
+        if not self.defer:
+            raise ValueError("steps_schedule needs deferred judging")
+        if n < 0 or step0 < 0 or period <= 0:
+            raise ValueError("n >= 0, step0 >= 0 and period > 0 required")
+        ne = self.nenv
+        for name, t, rows in (("qpos", qpos, NQ), ("qvel", qvel, NV), ("anchor", anchor, 16)):
+            check_soa(name, t, rows, ne, self.dev)
+        if (ctrl_blocks.dtype != torch.float32 or ctrl_blocks.device != self.dev or ctrl_blocks.dim() != 3
+                or tuple(ctrl_blocks.shape[1:]) != (NV - 6, ne) or not ctrl_blocks.is_contiguous()
+                or (step0 + max(n, 1) - 1) // period >= ctrl_blocks.shape[0]):
+            raise ValueError(f"ctrl_blocks must be a contiguous float32 [nblocks][29][{ne}] tensor on {self.dev} "
+                             f"covering steps {step0} .. {step0 + n - 1}")
+        if mu is not None:
+            check_soa("mu", mu.view(1, -1), 1, ne, self.dev)
+        lib, s = self.lib, torch.cuda.current_stream(self.dev).cuda_stream
+        m = self.mlp
+        W, b = m._wb()
+        ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        _lib.check(lib.g1_hip_surrogate_steps_schedule(
+            ne, n, self.t, self.dt, self.period, self.tol, qpos.data_ptr(), qvel.data_ptr(), anchor.data_ptr(),
+            ctrl_blocks.data_ptr(), ctrl_blocks.shape[1] * ctrl_blocks.shape[2], period, step0, ptr(mu),
+            m.xmean.data_ptr(), m.xinv.data_ptr(), W, b, m.ymean.data_ptr(), self.ystd.data_ptr(),
+            self.qacc_aba.data_ptr(), self.tau.data_ptr(), self.fallback.data_ptr(), self.stats.data_ptr(),
+            ptr(self.qacc), self.contact_forces.data_ptr(), self.foot_pos.data_ptr(), self.foot_vel.data_ptr(),
+            self.judge_flags, self.snap.data_ptr(), self.sflag.data_ptr(),
+            self.blk_run.data_ptr() if self.fallback_runs else None,
+            C.byref(self._all_fallback) if self.skip_idle else None, s), "surrogate_steps_schedule")
+        self.t += n
+
     def summary(self):
         """judged / rejected count every judgement (at once and in batches);
         aba_frac is the share of env-steps that integrated ABA's qacc (judged

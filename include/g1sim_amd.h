@@ -365,6 +365,21 @@ int g1_hip_surrogate_steps_deferred(int nenv, int nsteps, int t0, float dt, int 
                                     unsigned* stats, float* qacc_out, float* contact_forces, float* foot_pos,
                                     float* foot_vel, unsigned judge_flags, float* snap, uint8_t* sflag,
                                     uint8_t* blk_run, int* all_fallback, void* stream);
+/* g1_hip_surrogate_steps_deferred with a PD-target schedule: step s of the
+ * call uses the block ctrl_blocks + ((ctrl_step0 + s) / ctrl_period) *
+ * block_stride ([29][nenv] each, block_stride >= 29 * nenv floats apart), as
+ * g1_hip_step_schedule; with blk_run the fallback runs switch target blocks
+ * themselves and span whole period segments.  Same results as one
+ * g1_hip_surrogate_steps_deferred call per target block. */
+int g1_hip_surrogate_steps_schedule(int nenv, int nsteps, int t0, float dt, int period, float tol,
+                                    float* qpos, float* qvel, float* anchor, const float* ctrl_blocks,
+                                    long long block_stride, int ctrl_period, int ctrl_step0, const float* mu,
+                                    const float* xmean, const float* xinv, const void* const* W,
+                                    const float* const* b, const float* ymean, const float* ystd,
+                                    float* qacc_aba, float* tau, uint8_t* fallback, unsigned* stats,
+                                    float* qacc_out, float* contact_forces, float* foot_pos, float* foot_vel,
+                                    unsigned judge_flags, float* snap, uint8_t* sflag, uint8_t* blk_run,
+                                    int* all_fallback, void* stream);
 int g1_hip_fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* anchor,
                         const float* ctrl, const float* mu, const uint8_t* fallback, int phase0, int period,
                         uint8_t* blk_run, float* qacc_track, float* contact_forces, float* foot_pos,

@@ -521,6 +521,47 @@ def test_fallback_runs_with_islands_and_friction(device, n):
     assert sa == sb and sa["aba_frac"] == 1.0 and sa["judged"] == 2 * n
 
 
+@pytest.mark.parametrize("n", [5000, 40000])
+@pytest.mark.parametrize("tol", [1e-3, 1e30])
+def test_steps_schedule_matches_block_calls(device, n, tol):
+    """steps_schedule (PD targets resampled every 10 steps, one host call;
+    fallback runs switch target blocks inside the kernel) gives the bits of
+    one steps() call per target block: state, feet, contact forces, tracked
+    qacc, records, flags and statistics.  The calls start mid-block and
+    mid-period."""
+    mlp = MLP(device, seed=4)
+    qp, qv, an, cb = W.randomized_actuation(n, seed=36, nblocks=7)
+    ctrl = torch.as_tensor(cb, device=device)
+    runs = []
+    for sched in (True, False):
+        st = GpuState(qp, qv, an, cb[0], device)
+        stepper = SurrogateStepper(mlp, n, device, period=16, tol=tol, track_qacc=True)
+        for step0, k in ((0, 7), (7, 38), (45, 20)):
+            if sched:
+                stepper.steps_schedule(st.qpos, st.qvel, st.anchor, ctrl, step0, k, 10)
+            else:
+                s0 = step0
+                while s0 < step0 + k:
+                    m = min(10 - s0 % 10, step0 + k - s0)
+                    stepper.steps(st.qpos, st.qvel, st.anchor, ctrl[s0 // 10], m)
+                    s0 += m
+        torch.cuda.synchronize()
+        extra = {k: getattr(stepper, k).cpu().numpy() for k in ("fallback", "qacc", "foot_pos", "foot_vel",
+                                                                "contact_forces", "snap", "sflag")}
+        extra["stats"] = stepper.stats.cpu().numpy()[:6]
+        runs.append((st.host(), extra, stepper.summary()))
+    (ha, ea, sa), (hb, eb, sb) = runs
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(ha[k], hb[k]), (k, float(np.abs(ha[k] - hb[k]).max()))
+    for k in ea:
+        assert np.array_equal(ea[k], eb[k]), k
+    assert sa == sb
+    if tol == 1e-3:
+        assert sa["judged"] == sa["batch_judged"] == (65 // 16) * n     # four batch judgements, all rejected
+    else:
+        assert sa["fallback_envs_now"] == 0
+
+
 def _stepper_run(device, mlp, n, steps, tol, defer, reset_at=None):
     qp, qv, an, cb = W.randomized_actuation(n, seed=33, nblocks=1)
     st = GpuState(qp, qv, an, cb[0], device)

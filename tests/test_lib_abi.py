@@ -215,6 +215,13 @@ def test_deferred_judging_entry_points_validate_arguments():
                                                None) == -1
     assert lib.g1_hip_surrogate_steps_deferred(8, 0, 0, 2e-3, 16, 1e-3, d, d, d, d, None, d, d, ws, ws, d, d, d, d,
                                                d, d, None, None, None, None, 0, d, d, d, None, None) == 0
+    # schedule: no stride, or a stride below one target block
+    assert lib.g1_hip_surrogate_steps_schedule(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, 0, 10, 0, None, d, d, ws, ws, d,
+                                               d, d, d, d, d, None, None, None, None, 0, d, d, d, None, None) == -1
+    assert lib.g1_hip_surrogate_steps_schedule(8, 2, 0, 2e-3, 16, 1e-3, d, d, d, d, 100, 10, 0, None, d, d, ws, ws,
+                                               d, d, d, d, d, d, None, None, None, None, 0, d, d, d, None,
+                                               None) == -1
+    assert "g1_hip_surrogate_steps_schedule" in _lib.last_error()
     # fallback run: a run past the period's end, no run flags, no records; zero steps is a no-op
     assert lib.g1_hip_fallback_run(8, 5, 2e-3, d, d, d, d, None, d, 12, 16, d, None, None, None, None, d, d, d, 0,
                                    None) == -1
