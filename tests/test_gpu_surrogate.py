@@ -562,6 +562,33 @@ def test_steps_schedule_matches_block_calls(device, n, tol):
         assert sa["fallback_envs_now"] == 0
 
 
+@pytest.mark.parametrize("n", [5000, 40000])
+def test_fallback_runs_are_plain_aba(device, n):
+    """A network the judge never accepts: steps_schedule (fallback runs,
+    target blocks switched in the kernel, batch judgements between periods)
+    leaves exactly the state of plain ABA (g1_hip_step_schedule) over 45
+    steps from the same start -- the fallback run computes the plain step's
+    bits."""
+    mlp = MLP(device, seed=4)
+    qp, qv, an, cb = W.randomized_actuation(n, seed=37, nblocks=5)
+    ctrl = torch.as_tensor(cb, device=device)
+    sur = GpuState(qp, qv, an, cb[0], device)
+    aba = GpuState(qp, qv, an, cb[0], device)
+    stepper = SurrogateStepper(mlp, n, device, period=16, tol=1e-3)
+    stepper.steps_schedule(sur.qpos, sur.qvel, sur.anchor, ctrl, 0, 45, 10)
+    lib = _lib.load()
+    s = torch.cuda.current_stream(device).cuda_stream
+    _lib.check(lib.g1_hip_step_schedule(n, 45, 10, 0, 2e-3, aba.qpos.data_ptr(), aba.qvel.data_ptr(),
+                                        aba.anchor.data_ptr(), ctrl.data_ptr(), ctrl.shape[1] * ctrl.shape[2],
+                                        None, None, None, None, None, 0, s), "step_schedule")
+    torch.cuda.synchronize()
+    ha, hb = sur.host(), aba.host()
+    for k in ("qpos", "qvel", "anchor"):
+        assert np.array_equal(ha[k], hb[k]), (k, float(np.abs(ha[k] - hb[k]).max()))
+    summ = stepper.summary()
+    assert summ["aba_frac"] == 1.0 and summ["judged"] == summ["rejected"] == 2 * n
+
+
 def _stepper_run(device, mlp, n, steps, tol, defer, reset_at=None):
     qp, qv, an, cb = W.randomized_actuation(n, seed=33, nblocks=1)
     st = GpuState(qp, qv, an, cb[0], device)
