@@ -383,7 +383,8 @@ def surrogate_leg(dev, nenv, steps, warmup, train_iters):
 
     res = {"judge": "rotating 1/16 of envs per step (blocks of 64), per-env fallback at rel. qacc err > 1e-3; "
                     "every env starts in fallback until a judge accepts it; judged blocks whose envs are all in "
-                    "fallback are judged in one network pass per period (deferred judging)",
+                    "fallback are judged in one network pass per period (deferred judging), and such blocks take "
+                    "each period segment's steps in one fallback-run launch (plain ABA's bits)",
            "dataset": "20M ABA transitions: 16,384 envs x 1,221 steps", "train_iters": train_iters,
            "train_final_mse_z": hist[-1][1], "dataset_and_train_s": t_train, "train_roofline": train_roof,
            "heldout_bf16_kernels": acc, "heldout_fp32_evaluation": floor, "_input_cells": cell}
