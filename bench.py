@@ -717,6 +717,31 @@ def kernel_only(kind, nenv, steps, warmup, tol=1e30):
                       "achieved_tflops": tf, "frac_bf16_peak": tf * 1e12 / PEAK_BF16}), flush=True)
 
 
+def launch_sweep(nenv):
+    """Scheduled-launch time against launch length (1 to 250 substeps) at
+    nenv, HIP events around 20 back-to-back launches, best of two passes,
+    and a least-squares split into a per-launch and a per-substep cost."""
+    import torch
+    from g1sim_amd import _lib, workloads as W
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    qp, qv, an, cb = W.randomized_actuation(nenv, seed=3, nblocks=30)
+    state = tuple(torch.as_tensor(x, device=dev) for x in (qp, qv, an))
+    ctrl = torch.as_tensor(cb, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    schedule_call(lib, nenv, state, ctrl, 0, 250, sptr=stream.cuda_stream)()       # clock up
+    torch.cuda.synchronize()
+    lengths = (1, 2, 5, 10, 20, 40, 100, 250)
+    best = {k: float("inf") for k in lengths}
+    for _ in range(2):
+        for k in lengths:
+            run = schedule_call(lib, nenv, state, ctrl, 0, k, sptr=stream.cuda_stream)
+            best[k] = min(best[k], _event_avg_ms(run, 20, stream) * 1e3)
+    per_sub, fixed = np.polyfit(np.array(lengths, float), np.array([best[k] for k in lengths]), 1)
+    print(json.dumps({"nenv": nenv, "us_per_launch": best, "fixed_us": fixed, "per_substep_us": per_sub}),
+          flush=True)
+
+
 def measure_peaks():
     """Measured machine peaks on this device: non-packed FP32 FMA, dense BF16
     MFMA (16x16x32, register operands), and HBM copy bandwidth."""
@@ -773,6 +798,8 @@ def main():
                     help="skip the 131,072-env FP32 ABA leg of the default run")
     ap.add_argument("--no-precision", action="store_true", help="skip the per-configuration precision samples")
     ap.add_argument("--peaks", action="store_true", help="measure FP32 FMA, BF16 MFMA and HBM copy peaks and exit")
+    ap.add_argument("--launch-sweep", action="store_true",
+                    help="time scheduled step launches of 1 to 250 substeps at --nenv and exit")
     ap.add_argument("--kernel", choices=("surrogate", "policy", "stepper", "stepper-steps", "train"),
                     help="time one fused inference kernel, surrogate stepper steps (step() calls, or steps() "
                          "calls of 10 steps) or one training iteration alone at --nenv (for profiling) and exit")
@@ -788,6 +815,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.launch_sweep:
+        launch_sweep(args.nenv)
+        return
     if args.kernel:
         kernel_only(args.kernel, args.nenv, args.steps, args.warmup, args.tol)
         return
