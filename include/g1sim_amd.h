@@ -333,7 +333,8 @@ int g1_hip_surrogate_steps(int nenv, int nsteps, int t0, float dt, int period, f
  * nsub steps as by the plain step, each writing its record (as
  * g1_hip_judge_forces_commit does) at its judged step, phase (phase0 + s) %
  * period; nsub <= period - phase0.  blk_run[b] = 1 marks the blocks stepped,
- * 0 the others.  qacc_track, contact_forces and the feet (nullable) receive
+ * 0 the others (nsub = 0 or nenv = 0 launches nothing and leaves blk_run as
+ * it was).  qacc_track, contact_forces and the feet (nullable) receive
  * the last step's values of the stepped blocks; stats[3] counts their
  * env-steps.
  *
