@@ -451,9 +451,10 @@ __global__ void colsum_final_kernel(const float* part, int nblk, int ld, float* 
 // the learning rate and bias corrections read from sched.
 constexpr int ADAM_MAX = 16;
 constexpr int ADAM_WIDE = 16;    // more gradient slices than this: one wave per element
+constexpr int ADAM_INFLIGHT = 8; // gradient slices whose loads are issued together (the rest one at a time)
 struct AdamSet {
   int ntensor;
-  int offset[ADAM_MAX + 1];      // first thread of each tensor (a wave-aligned run for a wide tensor)
+  int offset[ADAM_MAX + 1];      // first thread of each tensor's run (a multiple of 64)
   int n[ADAM_MAX];               // elements
   int nsum[ADAM_MAX];            // gradient = sum of nsum slices g + z * gstride, z ascending
   long long gstride[ADAM_MAX];
@@ -468,52 +469,80 @@ struct AdamSet {
   float wd[ADAM_MAX];
 };
 
+// the mask byte of a tensor without a mask
+__device__ const uint8_t adam_unmasked = 1;
+
 __global__ __launch_bounds__(256) void adam_multi_kernel(AdamSet s, const float* sched, float b1, float b2,
                                                          float eps) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.offset[s.ntensor]) return;
+  // Every tensor's run of threads starts at a multiple of 64, so a wave
+  // lies in one tensor: its index comes from the wave's first thread, and
+  // the tensor's fields are scalar loads.
+  const int i0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+  if (i0 >= s.offset[s.ntensor]) return;
   int k = 0;
-  while (k + 1 < s.ntensor && i >= s.offset[k + 1]) ++k;
+  while (k + 1 < s.ntensor && i0 >= s.offset[k + 1]) ++k;
+  const int i = i0 + (threadIdx.x & 63);
   const int nk = s.nsum[k];
   const float* gk = s.g[k];
   const long long gs = s.gstride[k];
-  int j;
+  const bool wide = nk > ADAM_WIDE;
+  // a wide tensor has one wave per element (its run is whole waves long),
+  // the others one thread
+  const int t = i - s.offset[k];
+  const int j = wide ? t >> 6 : t;
+  if (j >= s.n[k]) return;                     // gap before the next wave-aligned run
+  // the element's state and mask are loaded with its gradient slices (one
+  // round trip)
+  float* p = s.p[k];
+  const uint8_t* mask = s.mask[k];
+  const float pj = p[j], mj = s.m[k][j], vj = s.v[k][j];
+  const uint8_t mv = *(mask ? mask + j : &adam_unmasked);     // no branch: the load stays in flight
   float gsum;
-  if (nk > ADAM_WIDE) {
-    // one wave per element (the run is wave-aligned and whole waves long):
-    // lane l sums slices l, l + 64, ... in order, then a butterfly over the
-    // wave; lane 0 carries on with the element
-    const int t = i - s.offset[k], l = t & 63;
-    j = t >> 6;
+  if (wide) {
+    // lane l sums slices l, l + 64, ... in order (two loads in flight per
+    // trip), then a butterfly over the wave; lane 0 carries on
+    const int l = t & 63;
     float acc = 0.f;
-    for (int z = l; z < nk; z += 64) acc += gk[(size_t)z * gs + j];
+    for (int z = l; z < nk; z += 128) {
+      const float g0 = gk[(size_t)z * gs + j];
+      const float g1 = z + 64 < nk ? gk[(size_t)(z + 64) * gs + j] : 0.f;
+      acc += g0;
+      if (z + 64 < nk) acc += g1;
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (l != 0) return;
     gsum = acc;
   } else {
-    j = i - s.offset[k];
-    if (j >= s.n[k]) return;                   // gap before a wave-aligned run
-    gsum = gk[j];
-    for (int z = 1; z < nk; ++z) gsum += gk[(size_t)z * gs + j];
+    // the first ADAM_INFLIGHT slices are all loaded before the first add (z
+    // ascending, as the one-at-a-time sum)
+    float gz[ADAM_INFLIGHT];
+#pragma unroll
+    for (int z = 0; z < ADAM_INFLIGHT; ++z) gz[z] = z < nk ? gk[(size_t)z * gs + j] : 0.f;
+    gsum = gz[0];
+#pragma unroll
+    for (int z = 1; z < ADAM_INFLIGHT; ++z)
+      if (z < nk) gsum += gz[z];
+    for (int z = ADAM_INFLIGHT; z < nk; ++z) gsum += gk[(size_t)z * gs + j];
   }
   bf16* p16 = s.p16[k];
   bf16* p16t = s.p16t[k];
   const int tj = p16t ? (j % s.cols[k]) * (s.n[k] / s.cols[k]) + j / s.cols[k] : 0;
-  if (s.mask[k] && !s.mask[k][j]) {
+  if (mv == 0) {
     if (p16) p16[j] = (bf16)0.f;
     if (p16t) p16t[tj] = (bf16)0.f;
     return;
   }
   const float lr = sched[0], bc1 = sched[1], bc2 = sched[2];
-  float* p = s.p[k];
-  const float gi = gsum + s.wd[k] * p[j];
-  const float mi = b1 * s.m[k][j] + (1.f - b1) * gi;
-  const float vi = b2 * s.v[k][j] + (1.f - b2) * gi * gi;
+  // each fused multiply-add written out, so the rounding does not depend
+  // on which product the compiler contracts
+  const float gi = __builtin_fmaf(s.wd[k], pj, gsum);
+  const float mi = __builtin_fmaf(b1, mj, (1.f - b1) * gi);
+  const float vi = __builtin_fmaf((1.f - b2) * gi, gi, b2 * vj);
   s.m[k][j] = mi;
   s.v[k][j] = vi;
   const float upd = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-  const float pi = p[j] - upd;
+  const float pi = pj - upd;
   p[j] = pi;
   if (p16) p16[j] = (bf16)pi;
   if (p16t) p16t[tj] = (bf16)pi;
@@ -754,7 +783,7 @@ int g1_mlp_adam_multi_sum(int ntensor, float* const* p, float* const* m, float* 
       return g1err::fail("g1_mlp_adam_multi: tensor %d: %d gradient slices at stride %lld (>= %d elements)", k,
                          s.nsum[k], s.gstride[k], n[k]);
     const bool wide = s.nsum[k] > ADAM_WIDE;
-    if (wide) total = (total + 63) / 64 * 64;       // a whole number of waves per element
+    total = (total + 63) / 64 * 64;                 // every run starts a wave (a wide one: a wave per element)
     s.offset[k] = (int)total;
     total += wide ? 64LL * n[k] : (long long)n[k];
     s.p[k] = p[k]; s.m[k] = m[k]; s.v[k] = v[k]; s.g[k] = grad[k];

@@ -471,21 +471,26 @@ def test_mse_grad_partials_match_atomic_version(device):
 
 def test_adam_over_gradient_slices(device):
     """g1_mlp_adam_multi_sum: a gradient given as up to 16 slices gives the
-    bits of g1_mlp_sum_splits followed by the one-slice step; 128 slices (one
-    wave per element) match the step on the float64 sum of the slices."""
+    bits of g1_mlp_sum_splits followed by the one-slice step; 128 or 200
+    slices (one wave per element) match the step on the float64 sum of the
+    slices."""
     lib = _lib.load()
     rng = np.random.default_rng(13)
     s = _stream(device)
     mk = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=device)  # noqa: E731
-    sizes, nslices = [3000, 512, 35], [8, 128, 128]
+    # 8 and 12 slices (all loads in flight, then 4 more one at a time), 1
+    # slice; 128 and 200 slices (one wave per element; 200 leaves lanes with
+    # an odd count)
+    sizes, nslices = [3000, 512, 35, 700, 257, 77], [8, 128, 128, 12, 1, 200]
+    nt = len(sizes)
     slices = [rng.normal(size=(k, n)).astype(np.float32) for n, k in zip(sizes, nslices)]
     p0 = [rng.normal(size=n).astype(np.float32) for n in sizes]
     masks = [mk((rng.random(n) < 0.9).astype(np.uint8)) for n in sizes]
     sched = mk(np.array([1e-3, 1 - 0.9, 1 - 0.999, 0], np.float32))
-    n_arr = (C.c_int * 3)(*sizes)
+    n_arr = (C.c_int * nt)(*sizes)
     # reference: summed gradients (sum_splits for the 8-slice tensor, float64 for the wide ones), one-slice step
     summed = []
-    for k in range(3):
+    for k in range(nt):
         if nslices[k] <= 16:
             out = torch.zeros(sizes[k], device=device)
             assert lib.g1_mlp_sum_splits(mk(slices[k]).data_ptr(), nslices[k], sizes[k], out.data_ptr(), s) == 0
@@ -500,22 +505,24 @@ def test_adam_over_gradient_slices(device):
         P16 = [torch.zeros(n, dtype=torch.bfloat16, device=device) for n in sizes]
         if use_slices:
             G = [mk(x) for x in slices]
-            nsum = (C.c_int * 3)(*nslices)
-            gstride = (C.c_longlong * 3)(*sizes)
-            rc = lib.g1_mlp_adam_multi_sum(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
+            nsum = (C.c_int * nt)(*nslices)
+            gstride = (C.c_longlong * nt)(*sizes)
+            rc = lib.g1_mlp_adam_multi_sum(nt, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
                                            _lib.ptr_array(G), _lib.ptr_array(P16), None, None,
                                            _lib.ptr_array(masks), n_arr, nsum, gstride, None, sched.data_ptr(), 0.9,
                                            0.999, 1e-8, s)
         else:
-            rc = lib.g1_mlp_adam_multi(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
+            rc = lib.g1_mlp_adam_multi(nt, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
                                        _lib.ptr_array(summed), _lib.ptr_array(P16), None, None,
                                        _lib.ptr_array(masks), n_arr, None, sched.data_ptr(), 0.9, 0.999, 1e-8, s)
         assert rc == 0, _lib.last_error()
         torch.cuda.synchronize()
         runs.append(([x.cpu().numpy() for x in P], [x.cpu().numpy() for x in M]))
     (pa, ma), (pb, mb) = runs
-    assert np.array_equal(pa[0], pb[0]) and np.array_equal(ma[0], mb[0])          # up to 16 slices: same bits
-    for k in (1, 2):
+    for k in range(nt):
+        if nslices[k] <= 16:                                                      # up to 16 slices: same bits
+            assert np.array_equal(pa[k], pb[k]) and np.array_equal(ma[k], mb[k]), k
+    for k in (k for k in range(nt) if nslices[k] > 16):
         np.testing.assert_allclose(mb[k], ma[k], rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(pb[k], pa[k], rtol=1e-6, atol=1e-7)
         mk_ = masks[k].cpu().numpy() == 0
@@ -524,4 +531,4 @@ def test_adam_over_gradient_slices(device):
     bad = (C.c_longlong * 3)(sizes[0] - 1, sizes[1], sizes[2])
     assert lib.g1_mlp_adam_multi_sum(3, _lib.ptr_array(P), _lib.ptr_array(M), _lib.ptr_array(V),
                                      _lib.ptr_array([mk(x) for x in slices]), None, None, None, None, n_arr,
-                                     (C.c_int * 3)(*nslices), bad, None, sched.data_ptr(), 0.9, 0.999, 1e-8, s) == -1
+                                     (C.c_int * 3)(*nslices[:3]), bad, None, sched.data_ptr(), 0.9, 0.999, 1e-8, s) == -1
