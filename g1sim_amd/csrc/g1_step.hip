@@ -1702,6 +1702,13 @@ int launch_step(g1k::StepArgs a, unsigned flags, hipStream_t stream, int mode = 
   return 0;
 }
 
+// flags of the surrogate judge and fallback-run launches: the float64
+// islands of the solve and the contacts, and at most one wave split
+bool bad_judge_flags(unsigned flags) {
+  return (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) ||
+         ((flags & G1_FLAG_WAVES4) && (flags & G1_FLAG_WAVES2));
+}
+
 int validate(int nenv, int nsub, float dt, const void* qpos, const void* qvel, const void* anchor,
              const void* ctrl) {
   if (nenv < 0 || nsub < 0) { set_err("nenv (%d) and nsub (%d) must be >= 0", nenv, nsub); return -1; }
@@ -1733,7 +1740,7 @@ int judge_forces_commit(int nenv, float dt, float* qpos, float* qvel, float* anc
     set_err("judge_forces_commit: bad phase/period, or no qacc / tau / fallback / records / stats");
     return -1;
   }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+  if (bad_judge_flags(flags)) {
     set_err("judge_forces_commit: unsupported flags 0x%x", flags);
     return -1;
   }
@@ -1772,7 +1779,7 @@ int fallback_run(int nenv, int nsub, float dt, float* qpos, float* qvel, float* 
     set_err("fallback_run: bad target schedule");
     return -1;
   }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+  if (bad_judge_flags(flags)) {
     set_err("fallback_run: unsupported flags 0x%x", flags);
     return -1;
   }
@@ -1919,7 +1926,7 @@ int g1_hip_step_judge(int nenv, float dt, const float* qpos, const float* qvel, 
   int rc = validate(nenv, 1, dt, qpos, qvel, anchor, ctrl);
   if (rc) return rc;
   if (!list || !count || !qacc_out) { set_err("step_judge: list, count and qacc are required"); return -1; }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+  if (bad_judge_flags(flags)) {
     set_err("step_judge: unsupported flags 0x%x", flags);
     return -1;
   }
@@ -1946,7 +1953,7 @@ int g1_hip_step_judge_blocks(int nenv, float dt, const float* qpos, const float*
     set_err("step_judge_blocks: bad phase/period or no qacc");
     return -1;
   }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+  if (bad_judge_flags(flags)) {
     set_err("step_judge_blocks: unsupported flags 0x%x", flags);
     return -1;
   }
@@ -1973,7 +1980,7 @@ int g1_hip_judge_forces(int nenv, float dt, const float* qpos, const float* qvel
     set_err("judge_forces: bad phase/period, or no qacc / tau");
     return -1;
   }
-  if (flags & ~(G1_FLAG_F64_SOLVE | G1_FLAG_F64_CONTACT | G1_FLAG_WAVES4 | G1_FLAG_WAVES2)) {
+  if (bad_judge_flags(flags)) {
     set_err("judge_forces: unsupported flags 0x%x", flags);
     return -1;
   }

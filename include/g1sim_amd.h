@@ -49,7 +49,8 @@ void g1_cuda_reset_done(int nenv, float* qpos, float* qvel, float* anchor, const
                                   penetration depth and contact-point velocity */
 #define G1_FLAG_WAVES4    8u   /* force 4 waves (one limb chain each) per 64 envs */
 #define G1_FLAG_WAVES2   16u   /* force 2 waves (legs, arms) per 64 envs; by
-                                  default 2 from 32,768 envs, 4 below */
+                                  default 2 from 32,768 envs, 4 below (setting
+                                  both wave flags is an invalid argument) */
 
 /* g1_cuda_step with a runtime dt and extra outputs.  Returns 0 or a negative
  * error code (-1 invalid argument, -2 launch failure; see g1_hip_last_error).

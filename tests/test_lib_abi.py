@@ -232,6 +232,14 @@ def test_deferred_judging_entry_points_validate_arguments():
                                    None) == -1
     assert lib.g1_hip_fallback_run(8, 0, 2e-3, d, d, d, d, None, d, 12, 16, d, None, None, None, None, d, d, d, 0,
                                    None) == 0
+    # both wave splits at once
+    both = _lib.G1_FLAG_WAVES4 | _lib.G1_FLAG_WAVES2
+    assert lib.g1_hip_judge_forces_commit(8, 2e-3, d, d, d, d, None, d, 0, 16, d, d, None, None, None, None, d, d,
+                                          d, both, None) == -1
+    assert "flags" in _lib.last_error()
+    assert lib.g1_hip_fallback_run(8, 4, 2e-3, d, d, d, d, None, d, 12, 16, d, None, None, None, None, d, d, d, both,
+                                   None) == -1
+    assert "flags" in _lib.last_error()
     # zero envs is a no-op
     assert lib.g1_hip_surrogate_judge_batch(0, 1e-3, d, d, ws, ws, d, d, d, d, d, d, None) == 0
     assert lib.g1_hip_judge_forces_commit(0, 2e-3, d, d, d, d, None, d, 0, 16, d, d, None, None, None, None, d, d, d,
