@@ -46,6 +46,8 @@ def load():
     lib.g1_mlp_gemm.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp]
     lib.g1_mlp_gemm_ex.restype = i
     lib.g1_mlp_gemm_ex.argtypes = [i, i, i, vp, i, vp, i, i, i, i, i, vp, vp, i, vp, vp, vp, vp]
+    lib.g1_mlp_backward_layer.restype = i
+    lib.g1_mlp_backward_layer.argtypes = [vp, i, vp, i, vp, i, i, vp, vp, i, vp, vp, vp]
     lib.g1_mlp_mse_grad_part.restype = i
     lib.g1_mlp_mse_grad_part.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
     lib.g1_mlp_mse_grad.restype = i
@@ -147,7 +149,7 @@ def exported_symbols():
             "g1_hip_surrogate_integrate", "g1_hip_surrogate_forward", "g1_hip_surrogate_step", "g1_hip_get_obs",
             "g1_policy_act", "g1_rollout_post", "g1_rollout_init", "g1_policy_obs_norm_update", "g1_hip_peak_fp32",
             "g1_hip_peak_bf16", "g1_mlp_colsum_det", "g1_mlp_train_schedule", "g1_mlp_sample", "g1_mlp_adam_multi",
-            "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_mse_grad_part",
+            "g1_policy_obs_norm_update_noalloc", "g1_hip_step_schedule", "g1_mlp_gemm_ex", "g1_mlp_backward_layer", "g1_mlp_mse_grad_part",
             "g1_mlp_adam_multi_sum", "g1_mlp_gather_rows", "g1_hip_surrogate_steps", "g1_hip_surrogate_step_deferred",
             "g1_hip_surrogate_judge_batch", "g1_hip_surrogate_steps_deferred", "g1_hip_judge_forces_commit",
             "g1_hip_fallback_run", "g1_hip_surrogate_steps_schedule"]

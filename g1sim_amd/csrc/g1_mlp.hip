@@ -115,16 +115,17 @@ __device__ __forceinline__ void put(bf16 (*s)[LDR], const bf16x8 (&v)[Tiling<T>:
   }
 }
 
+// One workgroup's output tile (bx, by) of r-slice bz, with the operand
+// tiles staged in sA / sB.
 template <bool A_T, bool B_T, int EPI, int T>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, int bx, int by, int bz, bf16 (*sA)[T][LDR],
+                                          bf16 (*sB)[T][LDR]) {
   using TL = Tiling<T>;
   constexpr int F = TL::F;
-  __shared__ __attribute__((aligned(16))) bf16 sA[2][T][LDR];
-  __shared__ __attribute__((aligned(16))) bf16 sB[2][T][LDR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave >> 1, wj = wave & 1;
-  const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
-  const int rb = blockIdx.z * g.r_per_split;
+  const int i0 = bx * T, j0 = by * T;
+  const int rb = bz * g.r_per_split;
   const int nsteps = g.r_per_split / BR;
 
   // accumulators start at 0, or at the bias of the lane's column (lane % 16 of
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         } else if constexpr (EPI == EPI_F32_ATOMIC) {
           atomicAdd(&g.C[o], v);
         } else if constexpr (EPI == EPI_F32_SPLIT) {
-          g.C[(size_t)blockIdx.z * g.I * g.ldc + o] = v;
+          g.C[(size_t)bz * g.I * g.ldc + o] = v;
         } else if constexpr (EPI == EPI_BIAS_SILU) {
           g.C[o] = v;
           g.H[o] = (bf16)g1act::silu(v);
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         if (lane < 16) {
 #pragma unroll
           for (int b = 0; b < F; ++b)
-            g.colpart[(size_t)(blockIdx.x * 2 + wi) * g.ldc + j0 + wj * (T / 2) + b * 16 + lr] = csum[b];
+            g.colpart[(size_t)(bx * 2 + wi) * g.ldc + j0 + wj * (T / 2) + b * 16 + lr] = csum[b];
         }
       } else {
         // the workgroup's 64 rows are one block: the waves of a column half
@@ -237,9 +238,33 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
           for (int b = 0; b < F; ++b) red[wi * T + wj * (T / 2) + b * 16 + lr] = csum[b];
         }
         __syncthreads();
-        if (tid < T) g.colpart[(size_t)blockIdx.x * g.ldc + j0 + tid] = red[tid] + red[T + tid];
+        if (tid < T) g.colpart[(size_t)bx * g.ldc + j0 + tid] = red[tid] + red[T + tid];
       }
     }
+  }
+}
+
+template <bool A_T, bool B_T, int EPI, int T>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][T][LDR];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][T][LDR];
+  gemm_tile<A_T, B_T, EPI, T>(g, blockIdx.x, blockIdx.y, blockIdx.z, sA, sB);
+}
+
+// Two independent GEMMs with one tile size in one grid: blocks [0, n0) are
+// the first's, the rest the second's, each in the order its own launch
+// would give them (x fastest, then y, then r-slice).
+template <bool A0, bool B0, int E0, bool A1, bool B1, int E1, int T>
+__global__ __launch_bounds__(256) void gemm_pair_kernel(GemmArgs g0, GemmArgs g1, int nx0, int ny0, int n0, int nx1,
+                                                        int ny1) {
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][T][LDR];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][T][LDR];
+  const int b = blockIdx.x;
+  if (b < n0) {
+    gemm_tile<A0, B0, E0, T>(g0, b % nx0, (b / nx0) % ny0, b / (nx0 * ny0), sA, sB);
+  } else {
+    const int c = b - n0;
+    gemm_tile<A1, B1, E1, T>(g1, c % nx1, (c / nx1) % ny1, c / (nx1 * ny1), sA, sB);
   }
 }
 
@@ -638,10 +663,12 @@ __global__ __launch_bounds__(256) void features_kernel(int nenv, int nrows, cons
 // ================================================================== C-ABI
 namespace {
 using namespace g1mlp;
+// 128-tiles when they give at least two workgroups per CU of a 256-CU chip
+int tile_of(const GemmArgs& a, int split) { return (a.I / 128) * (a.J / 128) * split >= 512 ? 128 : 64; }
+
 template <bool AT, bool BT, int E>
 int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
-  // 128-tiles when they give at least two workgroups per CU of a 256-CU chip
-  if ((a.I / 128) * (a.J / 128) * split >= 512) {
+  if (tile_of(a, split) == 128) {
     hipLaunchKernelGGL((gemm_kernel<AT, BT, E, 128>), dim3(a.I / 128, a.J / 128, split), dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL((gemm_kernel<AT, BT, E, 64>), dim3(a.I / 64, a.J / 64, split), dim3(256), 0, s, a);
@@ -653,10 +680,9 @@ int launch_gemm(const GemmArgs& a, int split, hipStream_t s) {
 extern "C" {
 
 // C-ABI of the MLP kernels; the contract of each entry point is documented with its declaration.
-int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
-                   int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, float* colpart,
-                   void* stream) {
-  g1err::clear();
+// The arguments of one GEMM, checked, into *out (0), or an error (-1).
+static int make_gemm(int epi, const void* A, int lda, const void* B, int ldb, int I, int J, int R, int split, float* C,
+                     void* H, int ldc, const float* bias, const float* Zin, float* colpart, GemmArgs* out) {
   if (I <= 0 || J <= 0 || R <= 0 || split <= 0 || I % BI || J % BJ || R % (BR * split))
     return g1err::fail("g1_mlp_gemm: shape %d x %d x %d (split %d) is not a multiple of the %d x %d x %d tile",
                        I, J, R, split, BI, BJ, BR);
@@ -670,7 +696,16 @@ int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, co
     return g1err::fail("g1_mlp_gemm: epilogue %d needs bias / Zin", epi);
   if (split > 1 && epi != EPI_F32_ATOMIC && epi != EPI_F32_SPLIT)
     return g1err::fail("g1_mlp_gemm: split-R needs the F32_ATOMIC or F32_SPLIT epilogue");
-  GemmArgs g{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin, colpart};
+  *out = GemmArgs{(const bf16*)A, (const bf16*)B, lda, ldb, I, J, R, R / split, C, (bf16*)H, ldc, bias, Zin, colpart};
+  return 0;
+}
+
+int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,
+                   int R, int split, float* C, void* H, int ldc, const float* bias, const float* Zin, float* colpart,
+                   void* stream) {
+  g1err::clear();
+  GemmArgs g;
+  if (make_gemm(epi, A, lda, B, ldb, I, J, R, split, C, H, ldc, bias, Zin, colpart, &g)) return -1;
   hipStream_t s = (hipStream_t)stream;
   const int key = (a_trans ? 1 : 0) | (b_trans ? 2 : 0);
   switch (key * 8 + epi) {
@@ -686,6 +721,35 @@ int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, co
     case 3 * 8 + EPI_F32: return launch_gemm<true, true, EPI_F32>(g, split, s);
     default: return g1err::fail("g1_mlp_gemm: invalid arguments");
   }
+}
+
+int g1_mlp_backward_layer(const void* d16, int rows, const void* hin, int cols, const void* WT16, int B, int split,
+                          float* gW_split, void* dZ16, int ld_dz, const float* Z, float* colpart, void* stream) {
+  g1err::clear();
+  GemmArgs gw, gd;
+  // weight gradient: dW = d^T h over the batch, in r-slices
+  if (make_gemm(EPI_F32_SPLIT, d16, rows, hin, cols, rows, cols, B, split, gW_split, nullptr, cols, nullptr, nullptr,
+                nullptr, &gw))
+    return -1;
+  // data gradient through SiLU': dZ = (d W) silu'(Z), with the bias-gradient column partials
+  if (make_gemm(EPI_SILU_BWD, d16, rows, WT16, rows, B, cols, rows, 1, nullptr, dZ16, ld_dz, nullptr, Z, colpart, &gd))
+    return -1;
+  hipStream_t s = (hipStream_t)stream;
+  const int tw = tile_of(gw, split), td = tile_of(gd, 1);
+  if (tw != td) {
+    // tile sizes differ: two launches
+    const int rc = launch_gemm<true, true, EPI_F32_SPLIT>(gw, split, s);
+    return rc ? rc : launch_gemm<false, false, EPI_SILU_BWD>(gd, 1, s);
+  }
+  const int nx0 = gw.I / tw, ny0 = gw.J / tw, n0 = nx0 * ny0 * split, nx1 = gd.I / td, ny1 = gd.J / td;
+  const int n = n0 + nx1 * ny1;
+  if (tw == 128)
+    hipLaunchKernelGGL((gemm_pair_kernel<true, true, EPI_F32_SPLIT, false, false, EPI_SILU_BWD, 128>), dim3(n),
+                       dim3(256), 0, s, gw, gd, nx0, ny0, n0, nx1, ny1);
+  else
+    hipLaunchKernelGGL((gemm_pair_kernel<true, true, EPI_F32_SPLIT, false, false, EPI_SILU_BWD, 64>), dim3(n),
+                       dim3(256), 0, s, gw, gd, nx0, ny0, n0, nx1, ny1);
+  return g1err::launch("g1_mlp_backward_layer");
 }
 
 int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb, int I, int J,

@@ -169,17 +169,22 @@ class MLP:
         for l in range(L - 1, -1, -1):
             rows, cols, _ = self.dims[l]
             hin = X16 if l == 0 else ws["H16"][l - 1]
-            # dW = d^T h: reduction over the batch, split across gridDim.z into
-            # r-slices (plain stores)
-            self.gemm(1, 1, "F32_SPLIT", d16, rows, hin, cols, rows, cols, B, C=ws["gW_split"][l], ldc=cols,
-                      split=split)
             if l > 0:
                 prow = self.dims[l - 1][0]
-                # dH_{l-1} = d W_l through SiLU'(Z_{l-1}), as bf16, and the
+                # one launch: dW = d^T h (reduction over the batch in r-slices)
+                # and dH_{l-1} = d W_l through SiLU'(Z_{l-1}) as bf16, with the
                 # column partials of the layer l-1 bias gradient
-                self.gemm(0, 0, "SILU_BWD", d16, rows, self.W16T[l], rows, B, cols, rows,
-                          H=ws["dZ16"][l - 1], ldc=prow, Z=ws["Z"][l - 1], colpart=ws["gb_part"][l - 1])
+                rc = self.lib.g1_mlp_backward_layer(d16.data_ptr(), rows, hin.data_ptr(), cols,
+                                                    self.W16T[l].data_ptr(), B, split, ws["gW_split"][l].data_ptr(),
+                                                    ws["dZ16"][l - 1].data_ptr(), prow, ws["Z"][l - 1].data_ptr(),
+                                                    ws["gb_part"][l - 1].data_ptr(), self.stream)
+                self._check(rc, "backward_layer")
                 d16 = ws["dZ16"][l - 1]
+            else:
+                # dW = d^T h: reduction over the batch, split across gridDim.z
+                # into r-slices (plain stores)
+                self.gemm(1, 1, "F32_SPLIT", d16, rows, hin, cols, rows, cols, B, C=ws["gW_split"][l], ldc=cols,
+                          split=split)
 
     def gradients(self, B):
         """Sums the slices backward() left into ws['gW'] and ws['gb'] (in the

@@ -119,6 +119,19 @@ int g1_mlp_gemm(int a_trans, int b_trans, int epi, const void* A, int lda, const
 int g1_mlp_gemm_ex(int a_trans, int b_trans, int epi, const void* A, int lda, const void* B, int ldb,
                    int I, int J, int R, int split, float* C, void* H, int ldc, const float* bias,
                    const float* Zin, float* colpart, void* stream);
+/* One hidden layer's backward pass in one launch, the two GEMMs that read
+ * the layer's output gradient d16 [B][rows] (bf16) side by side in one grid:
+ * the weight gradient, g1_mlp_gemm_ex(1, 1, G1_MLP_EPI_F32_SPLIT, d16, rows,
+ * hin, cols, rows, cols, B, split, gW_split, NULL, cols, ...) with hin
+ * [B][cols] the layer's input, and the data gradient through SiLU',
+ * g1_mlp_gemm_ex(0, 0, G1_MLP_EPI_SILU_BWD, d16, rows, WT16, rows, B, cols,
+ * rows, 1, NULL, dZ16, ld_dz, NULL, Z, colpart) with WT16 [cols][rows] the
+ * transposed bf16 weights, Z [B][ld_dz] the input's pre-activation and
+ * colpart (required) the bias-gradient partials of the layer below.  Same
+ * outputs, bit for bit, as the two calls; the weight gradient's workgroups
+ * would fill only part of the chip alone.  Returns 0, -1 or -2. */
+int g1_mlp_backward_layer(const void* d16, int rows, const void* hin, int cols, const void* WT16, int B, int split,
+                          float* gW_split, void* dZ16, int ld_dz, const float* Z, float* colpart, void* stream);
 /* dY = 2 (pred - target) / (B * n_out) on columns < n_out, 0 beyond, as
  * bf16 (dY16) and fp32 (dY32, nullable); *sse += sum of squared errors when
  * sse is non-null.  pred, target: fp32 [B][ld]. */

@@ -404,6 +404,42 @@ def test_gemm_128_tiles(device, a_t, b_t, epi):
         assert (np.abs(C - d) <= tol * 1.2 + 1e-6).all()
 
 
+@pytest.mark.parametrize("B,rows,cols,split", [(8192, 512, 512, 8), (1024, 128, 128, 1), (2048, 128, 512, 2),
+                                                (16384, 512, 512, 32), (32768, 512, 512, 8)])
+def test_backward_layer_matches_two_gemms(device, B, rows, cols, split):
+    """g1_mlp_backward_layer gives the bits of its two g1_mlp_gemm_ex calls:
+    one grid of 64 x 64 tiles (the training shape, 8,192 samples, and two
+    small ones), one of 128 x 128 tiles (16,384 samples in 32 r-slices), and
+    two launches where the tile sizes differ (32,768 samples)."""
+    lib = _lib.load()
+    rng = np.random.default_rng(B + rows + cols + split)
+    s = _stream(device)
+    bf = lambda *shape: torch.as_tensor(rng.normal(size=shape).astype(np.float32), device=device).to(torch.bfloat16)  # noqa: E731
+    d16, hin, WT16 = bf(B, rows), bf(B, cols), bf(cols, rows)
+    Z = torch.as_tensor(rng.normal(size=(B, cols)).astype(np.float32), device=device)
+    outs = []
+    for fused, fill in ((False, 0.0), (True, 7.0)):
+        gW = torch.full((split, rows * cols), fill, device=device)
+        dZ = torch.full((B, cols), fill, device=device).to(torch.bfloat16)
+        cp = torch.full((B // 64, cols), fill, device=device)
+        if fused:
+            rc = lib.g1_mlp_backward_layer(d16.data_ptr(), rows, hin.data_ptr(), cols, WT16.data_ptr(), B, split,
+                                           gW.data_ptr(), dZ.data_ptr(), cols, Z.data_ptr(), cp.data_ptr(), s)
+            assert rc == 0, _lib.last_error()
+        else:
+            assert lib.g1_mlp_gemm_ex(1, 1, EPI["F32_SPLIT"], d16.data_ptr(), rows, hin.data_ptr(), cols, rows, cols,
+                                      B, split, gW.data_ptr(), None, cols, None, None, None, s) == 0
+            assert lib.g1_mlp_gemm_ex(0, 0, EPI["SILU_BWD"], d16.data_ptr(), rows, WT16.data_ptr(), rows, B, cols,
+                                      rows, 1, None, dZ.data_ptr(), cols, None, Z.data_ptr(), cp.data_ptr(), s) == 0
+        torch.cuda.synchronize()
+        outs.append((gW.cpu(), dZ.float().cpu(), cp.cpu()))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    # the data gradient's bias partials are required (the fused form stores no fp32 gradient)
+    assert lib.g1_mlp_backward_layer(d16.data_ptr(), rows, hin.data_ptr(), cols, WT16.data_ptr(), B, split,
+                                     gW.data_ptr(), dZ.data_ptr(), cols, Z.data_ptr(), None, s) == -1
+
+
 @pytest.mark.parametrize("I,J,R", [(256, 384, 256), (8192, 1024, 64)])
 def test_silu_bwd_column_partials(device, I, J, R):
     """The SILU_BWD epilogue's bias-gradient partials: column sums of its fp32
